@@ -1,0 +1,163 @@
+/*
+ * kmc.h -- C ABI of the MI355X (gfx950) matrix-centric k-means engine.
+ *
+ * Plain C: opaque handle, plain pointers and sizes, status codes, no HIP or
+ * torch types. Host buffers are owned by the caller; device buffers are owned
+ * by the handle and freed by kmc_destroy. The library never calls exit().
+ *
+ * Each entry point names the reference interface it replaces. Paths are
+ * relative to the reference repository (HicrestLaboratory/Matrix-Centric-K-Means).
+ */
+#ifndef KMC_H
+#define KMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 1..4 keep the meaning of the reference CLI exit codes
+ * (src/include/utils.hpp:34-37: EXIT_ARGS, EXIT_CUDA_DEV, EXIT_POINT_IOB,
+ * EXIT_INVALID_INFILE); the reference exits on every runtime error
+ * (src/cuda_utils.cu:10-41), the library returns the codes from 5 on. */
+typedef enum {
+  KMC_OK = 0,
+  KMC_ERR_ARG = 1,
+  KMC_ERR_NO_DEVICE = 2,
+  KMC_ERR_OUT_OF_BOUNDS = 3,
+  KMC_ERR_INFILE = 4,
+  KMC_ERR_HIP = 5,
+  KMC_ERR_OOM = 6,
+  KMC_ERR_STATE = 7,
+  KMC_ERR_UNSUPPORTED = 8
+} kmc_status;
+
+/* Kernel functions: Kmeans::Kernel of src/kmeans.cuh:56-61 (linear,
+ * polynomial (x.y+1)^2, sigmoid tanh(x.y+1); src/kernels/kernels.cuh:43-118).
+ * KMC_KERNEL_RBF (exp(-gamma ||x-y||^2)) has no reference counterpart. */
+typedef enum {
+  KMC_KERNEL_LINEAR = 0,
+  KMC_KERNEL_POLYNOMIAL = 1,
+  KMC_KERNEL_SIGMOID = 2,
+  KMC_KERNEL_RBF = 3
+} kmc_kernel;
+
+/* Arithmetic of the distance GEMM X C^T. Every form finishes with the FP32
+ * margin-refinement pass, so labels equal those of the FP32 direct form
+ * sum_j (x_j - c_j)^2 (comparisons/CPU_Kmeans/src/kmeans.cpp:121-131). */
+typedef enum {
+  KMC_PREC_FP32 = 0,   /* v_mfma_f32_32x32x2_f32                          */
+  KMC_PREC_BF16X3 = 1, /* hi+lo split bf16, 3 x v_mfma_f32_32x32x16_bf16  */
+  KMC_PREC_BF16 = 2    /* single bf16 product, wide refinement band       */
+} kmc_precision;
+
+/* Centroid update summation order. */
+typedef enum {
+  /* per cluster, float sums in point-index order: bit-identical to
+   * comparisons/CPU_Kmeans/src/kmeans.cpp:166-180 */
+  KMC_UPDATE_ORDERED = 0
+} kmc_update_mode;
+
+/* kmc_run flags */
+#define KMC_RUN_CHECK_CONVERGED 0x1u /* stop at the first iteration >= 2 whose
+                                        centroid shift test passes            */
+#define KMC_RUN_TIME_KERNELS 0x2u    /* record HIP events around the distance
+                                        kernel of every iteration             */
+
+typedef struct {
+  size_t n;          /* points                                   */
+  uint32_t d;        /* dimensions                               */
+  uint32_t k;        /* clusters                                 */
+  float tol;         /* centroid-shift tolerance (L2, per centroid) */
+  int32_t precision; /* kmc_precision                            */
+  int32_t update;    /* kmc_update_mode                          */
+  int32_t device;    /* HIP device ordinal                       */
+  int32_t reserved;
+} kmc_config;
+
+typedef struct {
+  uint64_t iterations;   /* iterations run                                   */
+  uint64_t converged_at; /* first iteration passing the shift test, or maxiter
+                            (the return value of Kmeans::run,
+                            src/kmeans.cuh:128-134)                          */
+  double inertia;        /* FP64 sum_i ||x_i - c_{l_i}||^2 of the last assign */
+  double score;          /* reference "Score" = inertia - sum_i ||x_i||^2
+                            (src/kmeans.cu:1053-1061 omits ||x||^2)          */
+  double max_shift;      /* FP64 max_c ||c_new - c_old|| of the last update  */
+  uint64_t refined_two;  /* points refined on two candidates (last assign)   */
+  uint64_t refined_full; /* points refined on all k centroids (last assign)  */
+  double ms_total;       /* wall time of the iteration loop, ms              */
+  double ms_assign;      /* sum of distance-kernel times (KMC_RUN_TIME_KERNELS) */
+  uint64_t assign_launches; /* distance-kernel launches timed                */
+} kmc_result;
+
+typedef struct kmc_handle kmc_handle;
+
+/* Replaces the Kmeans constructor's set-up (src/kmeans.cu:71-305): device
+ * selection, handle and stream creation. */
+kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out);
+
+/* Replaces the point packing + H2D copy of src/kmeans.cu:94-99,125-126.
+ * X is host row-major n x d with leading dimension ld >= d (floats). Also
+ * builds the per-point norms and, for the bf16 forms, the hi/lo operands. */
+kmc_status kmc_set_points(kmc_handle* h, const float* X, size_t ld);
+
+/* Same, from a device pointer (no PCIe copy). */
+kmc_status kmc_set_points_device(kmc_handle* h, const float* dX, size_t ld);
+
+/* Explicit initial centroids, host k x d row-major (the reference CPU
+ * k-means seeds with k distinct input rows, comparisons/CPU_Kmeans/src/
+ * kmeans.cpp:84-119). */
+kmc_status kmc_set_centroids(kmc_handle* h, const float* C0);
+
+/* Initial labels (host, n values in [0,k)); the initial centroids become the
+ * label means. Replaces init_centroids_rand (src/kmeans.cu:365-444), which
+ * uses labels i % k. */
+kmc_status kmc_set_labels(kmc_handle* h, const int32_t* labels0);
+
+/* Replaces Kmeans::run (src/kmeans.cu:772-1122): up to maxiter Lloyd
+ * iterations (assign -> update -> shift test). flags: KMC_RUN_*. */
+kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags,
+                   kmc_result* res);
+
+/* Replaces the label write-back (src/kmeans.cu:1084-1103). */
+kmc_status kmc_get_labels(kmc_handle* h, int32_t* labels);
+
+/* Current centroids, host k x d. */
+kmc_status kmc_get_centroids(kmc_handle* h, float* C);
+
+/* Stage hook: one assignment of every point against host centroids C (k x d):
+ * the fused distance GEMM + top-3 + margin band, then the refinement pass.
+ * Replaces compute_distances + argmin (src/kmeans.cu:846-966;
+ * comparisons/CPU_Kmeans/src/kmeans.cpp:121-154). labels: host n. */
+kmc_status kmc_assign(kmc_handle* h, const float* C, int32_t* labels,
+                      kmc_result* res);
+
+/* Stage hook: centroid update from host labels and the centroids those labels
+ * were assigned against (C_old, for the inertia). Writes C_new (host k x d)
+ * and counts (host k). Replaces compute_centroids_spmm
+ * (src/kernels/centroids.cu:177-223) and the CPU compute_centroids
+ * (comparisons/CPU_Kmeans/src/kmeans.cpp:166-180). */
+kmc_status kmc_update(kmc_handle* h, const int32_t* labels, const float* C_old,
+                      float* C_new, uint32_t* counts, kmc_result* res);
+
+/* Release every device buffer (src/kmeans.cu:307-359). */
+kmc_status kmc_destroy(kmc_handle* h);
+
+/* Message of the last error on this handle (never NULL). */
+const char* kmc_last_error(const kmc_handle* h);
+
+/* Static description of a status code. */
+const char* kmc_status_string(kmc_status s);
+
+/* Number of visible HIP devices (0 when none); replaces the device query of
+ * src/main.cu:40-52. */
+int kmc_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KMC_H */

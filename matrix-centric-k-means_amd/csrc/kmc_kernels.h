@@ -1,0 +1,135 @@
+// kmc_kernels.h -- parameter blocks and launchers of the Lloyd kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace kmc {
+
+enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_BF16 = 2 };
+
+// Device-resident loop state (one per handle).
+struct DevState {
+  unsigned long long iter;          // iterations completed
+  unsigned long long maxiter;       // stop after this many
+  unsigned long long converged_at;  // first iteration passing the shift test
+  int done;                         // set: every kernel returns at once
+  int pad;
+  double inertia;                   // FP64 inertia of the last assignment
+  double max_shift2;                // FP64 max_c ||c_new - c_old||^2
+  unsigned long long refined_two;
+  unsigned long long refined_all;
+  double sum_x2;                    // FP64 sum_i ||x_i||^2 (set-up)
+};
+
+struct PrepParams {
+  const float* C;  // K x DP master centroids
+  float* Cw;       // Kp x DP  (-2c, fp32 form)
+  uint16_t* Chi;   // Kp x DP  (bf16 forms)
+  uint16_t* Clo;
+  float* cn;       // Kp, ||c||^2 (+inf for padding rows)
+  unsigned* qcnt;  // refinement queue counters, cleared here
+  const int* done;
+  int K, Kp, DP, prec;
+};
+
+struct AssignParams {
+  const float* X;       // n x DP
+  const uint16_t* Xhi;  // n x DP
+  const uint16_t* Xlo;
+  const float* xn2;     // n, ||x||^2
+  long long n;
+  const float* Cw;
+  const uint16_t* Chi;
+  const uint16_t* Clo;
+  const float* cn;
+  int K, Kp, kchunk;
+  float alpha, beta, gref;  // margin-band constants
+  int* labels;
+  int* q2;        // (point, cand1, cand2) triples
+  int* qa;        // points needing all-k refinement
+  unsigned* qcnt; // [0] = q2 entries, [1] = qa entries
+  const int* done;
+};
+
+struct RefineParams {
+  const float* X;
+  const float* C;
+  const int* q2;
+  const int* qa;
+  const unsigned* qcnt;
+  int* labels;
+  const int* done;
+  int K, D, DP;
+};
+
+struct SortParams {
+  const int* labels;
+  int* bcount;   // K x nblk (cluster-major) -> exclusive offsets
+  int* count;    // K cluster sizes
+  int* start;    // K exclusive scan of count
+  int* members;  // n point ids sorted by (label, id)
+  const int* done;
+  long long n;
+  int K, CH, nblk;
+};
+
+struct UpdateParams {
+  const float* X;
+  const float* C;   // centroids the labels were assigned against
+  float* Cnext;
+  const int* members;
+  const int* start;
+  const int* count;
+  double* ipart;    // K * ceil(D/64) partial inertias
+  const int* done;
+  int K, D, DP;
+};
+
+struct FinishParams {
+  float* C;
+  const float* Cnext;
+  int* conv;
+  double* shift2;
+  const int* done;
+  float tol;
+  int K, D, DP;
+};
+
+struct StatsParams {
+  const double* ipart;
+  int nparts;
+  const double* shift2;
+  const int* conv;
+  const unsigned* qcnt;
+  DevState* state;
+  const int* done;
+  int K, check;
+};
+
+struct LoadParams {
+  const float* src;
+  size_t ld;
+  long long n;
+  int D, DP;
+  float* X;
+  uint16_t* Xhi;
+  uint16_t* Xlo;
+  float* xn2;
+  double* xn2_64;
+};
+
+hipError_t launch_prep(const PrepParams& P, hipStream_t s);
+hipError_t launch_assign(int DP, int prec, const AssignParams& P, int grid, size_t lds,
+                         hipStream_t s);
+size_t assign_lds(int DP, int prec, int kchunk);
+const void* assign_kernel_ptr(int DP, int prec);
+hipError_t launch_refine(const RefineParams& P, int grid2, int gridA, hipStream_t s);
+hipError_t launch_sort(const SortParams& P, hipStream_t s);
+hipError_t launch_update(const UpdateParams& P, hipStream_t s);
+hipError_t launch_finish(const FinishParams& F, const StatsParams& S, hipStream_t s);
+hipError_t launch_load_points(const LoadParams& P, hipStream_t s);
+hipError_t launch_sum_f64(const double* v, long long n, double* out, hipStream_t s);
+hipError_t launch_fill_i32(int* p, long long n, int v, hipStream_t s);
+
+}  // namespace kmc

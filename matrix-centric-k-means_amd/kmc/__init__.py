@@ -1,0 +1,207 @@
+"""Python host binding of the gfx950 k-means engine (C ABI in include/kmc.h).
+
+The engine is the HIP library ``lib/libkmc.so`` next to this package; there is
+no CPU fallback. Importing works without a GPU (so that the library's exports
+can be checked); creating an engine without a HIP device raises KmcError.
+
+The class mirrors the reference engine ``Kmeans`` (src/kmeans.cuh:117-139):
+construct with (n, d, k, tol), give points and initial centroids, ``run``
+returns the convergence iteration, ``labels()`` is the label write-back
+(src/kmeans.cu:1084-1103) and ``score`` the reference's objective
+(sum of min distances without ||x||^2, src/kmeans.cu:1053-1061).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libkmc.so")
+
+OK, ERR_ARG, ERR_NO_DEVICE, ERR_OUT_OF_BOUNDS, ERR_INFILE, ERR_HIP, ERR_OOM, \
+    ERR_STATE, ERR_UNSUPPORTED = range(9)
+
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16": 2}
+RUN_CHECK_CONVERGED = 0x1
+RUN_TIME_KERNELS = 0x2
+
+
+class KmcError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("kmc status %d (%s): %s" % (status, _status_text(status), msg))
+        self.status = status
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_size_t), ("d", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("tol", ctypes.c_float), ("precision", ctypes.c_int32),
+                ("update", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("converged_at", ctypes.c_uint64),
+                ("inertia", ctypes.c_double), ("score", ctypes.c_double),
+                ("max_shift", ctypes.c_double), ("refined_two", ctypes.c_uint64),
+                ("refined_full", ctypes.c_uint64), ("ms_total", ctypes.c_double),
+                ("ms_assign", ctypes.c_double), ("assign_launches", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libkmc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise KmcError(ERR_STATE, "%s not built: run matrix-centric-k-means_amd/build.py" % path)
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    sigs = {
+        "kmc_create": (ctypes.c_int, [P(_Config), P(vp)]),
+        "kmc_destroy": (ctypes.c_int, [vp]),
+        "kmc_set_points": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+        "kmc_set_points_device": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+        "kmc_set_centroids": (ctypes.c_int, [vp, vp]),
+        "kmc_set_labels": (ctypes.c_int, [vp, vp]),
+        "kmc_run": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32, P(_Result)]),
+        "kmc_get_labels": (ctypes.c_int, [vp, vp]),
+        "kmc_get_centroids": (ctypes.c_int, [vp, vp]),
+        "kmc_assign": (ctypes.c_int, [vp, vp, vp, P(_Result)]),
+        "kmc_update": (ctypes.c_int, [vp, vp, vp, vp, vp, P(_Result)]),
+        "kmc_last_error": (ctypes.c_char_p, [vp]),
+        "kmc_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "kmc_device_count": (ctypes.c_int, []),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _status_text(s):
+    try:
+        return load_library().kmc_status_string(s).decode()
+    except Exception:  # noqa: BLE001 - only used to format an error
+        return "?"
+
+
+def device_count():
+    return load_library().kmc_device_count()
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+    return a
+
+
+class Engine:
+    """One k-means problem resident on one GPU."""
+
+    def __init__(self, n, d, k, tol=1e-4, precision="fp32", device=0):
+        self._lib = load_library()
+        self.n, self.d, self.k = int(n), int(d), int(k)
+        self.precision = precision
+        cfg = _Config(self.n, self.d, self.k, float(tol), PRECISIONS[precision], 0, int(device), 0)
+        h = ctypes.c_void_p()
+        st = self._lib.kmc_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st != OK:
+            raise KmcError(st, "kmc_create(n=%d, d=%d, k=%d, %s)" % (n, d, k, precision))
+        self._h = h
+
+    def _check(self, st, what):
+        if st != OK:
+            msg = self._lib.kmc_last_error(self._h)
+            raise KmcError(st, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.kmc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_points(self, X):
+        X = _f32(X, (self.n, self.d))
+        self._check(self._lib.kmc_set_points(self._h, X.ctypes.data, self.d), "set_points")
+
+    def set_points_device(self, ptr, ld=None):
+        self._check(self._lib.kmc_set_points_device(self._h, ctypes.c_void_p(ptr),
+                                                    self.d if ld is None else ld),
+                    "set_points_device")
+
+    def set_centroids(self, C0):
+        C0 = _f32(C0, (self.k, self.d))
+        self._check(self._lib.kmc_set_centroids(self._h, C0.ctypes.data), "set_centroids")
+
+    def set_labels(self, labels):
+        lab = np.ascontiguousarray(labels, dtype=np.int32)
+        self._check(self._lib.kmc_set_labels(self._h, lab.ctypes.data), "set_labels")
+
+    def run(self, maxiter, check_converged=True, time_kernels=False):
+        flags = (RUN_CHECK_CONVERGED if check_converged else 0) | \
+                (RUN_TIME_KERNELS if time_kernels else 0)
+        r = _Result()
+        self._check(self._lib.kmc_run(self._h, int(maxiter), flags, ctypes.byref(r)), "run")
+        return r.as_dict()
+
+    def labels(self):
+        out = np.empty(self.n, dtype=np.int32)
+        self._check(self._lib.kmc_get_labels(self._h, out.ctypes.data), "get_labels")
+        return out
+
+    def centroids(self):
+        out = np.empty((self.k, self.d), dtype=np.float32)
+        self._check(self._lib.kmc_get_centroids(self._h, out.ctypes.data), "get_centroids")
+        return out
+
+    def assign(self, C):
+        C = _f32(C, (self.k, self.d))
+        out = np.empty(self.n, dtype=np.int32)
+        r = _Result()
+        self._check(self._lib.kmc_assign(self._h, C.ctypes.data, out.ctypes.data,
+                                         ctypes.byref(r)), "assign")
+        return out, r.as_dict()
+
+    def update(self, labels, C_old):
+        lab = np.ascontiguousarray(labels, dtype=np.int32)
+        C_old = _f32(C_old, (self.k, self.d))
+        C_new = np.empty((self.k, self.d), dtype=np.float32)
+        counts = np.empty(self.k, dtype=np.uint32)
+        r = _Result()
+        self._check(self._lib.kmc_update(self._h, lab.ctypes.data, C_old.ctypes.data,
+                                         C_new.ctypes.data, counts.ctypes.data,
+                                         ctypes.byref(r)), "update")
+        return C_new, counts, r.as_dict()
+
+
+def fit(X, C0, maxiter, tol=1e-4, precision="fp32", check_converged=True):
+    """Run Lloyd from explicit centroids; returns (labels, centroids, result)."""
+    X = _f32(X)
+    n, d = X.shape
+    C0 = _f32(C0)
+    with Engine(n, d, C0.shape[0], tol=tol, precision=precision) as e:
+        e.set_points(X)
+        e.set_centroids(C0)
+        res = e.run(maxiter, check_converged=check_converged)
+        return e.labels(), e.centroids(), res

@@ -1,0 +1,342 @@
+// kmc_oracle.cpp -- CPU oracle for the matrix-centric k-means hot path.
+//
+// TEST INFRASTRUCTURE ONLY. Nothing in the product links, loads or calls this
+// file: it is the checker that tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py compare the HIP path against.
+//
+// Three restatements live here, each citing the reference lines it follows:
+//   1. orc_cpu_*  : the reference CPU k-means, float arithmetic in the same
+//                   operation order (comparisons/CPU_Kmeans/src/kmeans.cpp).
+//   2. orc_f64_*  : the same Lloyd algorithm in FP64 (the accuracy yardstick
+//                   for inertia and centroids).
+//   3. orc_kk_*   : kernel k-means in the Popcorn formulation of the GPU
+//                   reference (src/kmeans.cu, src/kernels/distances.cu,
+//                   src/kernels/kernels.cuh), computed in FP64.
+//
+// How each function is pinned (checks live in tests/test_oracle.py):
+//   orc_cpu_init_centroids, orc_cpu_distances, orc_cpu_argmin,
+//   orc_cpu_centroids, orc_cpu_converged, orc_cpu_lloyd
+//       -> bit-exact against the compiled reference CPU k-means (oracle/_ref)
+//          and the committed golden fixtures made from it; distances also
+//          against the worked examples 16/36/20/3 of the reference's report.
+//   orc_f64_*  -> closed forms (mean of one-hot groups, brute-force argmin on
+//                 tiny inputs), the identity inertia = sum ||x||^2 - ..., and
+//                 Lloyd monotonicity of inertia.
+//   orc_kk_kernel_matrix -> numpy FP64 GEMM X X^T and the kernel closed forms.
+//   orc_kk_distances     -> the reference unit test's O(n^3) host formula
+//                 (tests/unit_kernels.cu:365-395) restated independently in
+//                 the test, and the linear-kernel identity
+//                 d_K(x_i, mu_c) = ||x_i - mu_c||^2.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <vector>
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// 1. Reference CPU k-means, float semantics
+// ---------------------------------------------------------------------------
+
+// comparisons/CPU_Kmeans/src/kmeans.cpp:84-119 (and the seeding at :22-29):
+// mt19937 seeded through seed_seq{seed}; uniform_int_distribution<int>(0,n-1)
+// draws row indices; a row is accepted only if it differs by value from every
+// row accepted so far. Returns 0, or -1 if fewer than k distinct rows exist.
+int orc_cpu_init_centroids(const float* X, size_t n, uint32_t d, uint32_t k,
+                           int seed, float* C0) {
+  std::seed_seq ss{seed};
+  std::mt19937 gen(ss);
+  std::uniform_int_distribution<int> pick(0, (int)n - 1);
+  std::vector<size_t> used;
+  // Bound the number of draws so that a data set with < k distinct rows
+  // cannot loop forever (the reference would).
+  const uint64_t max_draws = 64ull * n + 1024ull * k;
+  uint64_t draws = 0;
+  while (used.size() < k) {
+    if (draws++ > max_draws) return -1;
+    size_t r = (size_t)pick(gen);
+    bool dup = false;
+    for (size_t u : used) {
+      // element-wise float ==, as Point::operator== (src/include/point.hpp:66-72)
+      bool eq = true;
+      for (uint32_t j = 0; j < d; ++j)
+        if (X[u * d + j] != X[r * d + j]) { eq = false; break; }
+      if (eq) { dup = true; break; }
+    }
+    if (!dup) {
+      std::memcpy(C0 + used.size() * d, X + r * d, sizeof(float) * d);
+      used.push_back(r);
+    }
+  }
+  return 0;
+}
+
+// comparisons/CPU_Kmeans/src/kmeans.cpp:121-131: direct form, float, the sum
+// over dimensions runs in index order, one rounding per subtraction, product
+// and addition (no fused multiply-add: the Makefile builds with
+// -ffp-contract=off and without -ffast-math).
+void orc_cpu_distances(const float* X, size_t n, uint32_t d, const float* C,
+                       uint32_t k, float* D) {
+  for (size_t i = 0; i < n; ++i)
+    for (uint32_t c = 0; c < k; ++c) {
+      float acc = 0.0f;
+      for (uint32_t j = 0; j < d; ++j) {
+        float t = X[i * d + j] - C[(size_t)c * d + j];
+        float sq = t * t;
+        acc = acc + sq;
+      }
+      D[i * k + c] = acc;
+    }
+}
+
+// comparisons/CPU_Kmeans/src/kmeans.cpp:141-154: first index of the minimum
+// (strict <, starting from +inf), plus the cluster histogram.
+void orc_cpu_argmin(const float* D, size_t n, uint32_t k, uint32_t* labels,
+                    uint32_t* counts) {
+  std::memset(counts, 0, sizeof(uint32_t) * k);
+  for (size_t i = 0; i < n; ++i) {
+    float best = std::numeric_limits<float>::infinity();
+    uint32_t idx = 0;
+    for (uint32_t c = 0; c < k; ++c)
+      if (D[i * k + c] < best) { best = D[i * k + c]; idx = c; }
+    labels[i] = idx;
+    counts[idx]++;
+  }
+}
+
+// comparisons/CPU_Kmeans/src/kmeans.cpp:166-180: float sums in point order,
+// then multiply by scale = (float)(1.0 / (double)max(count,1)). An empty
+// cluster therefore becomes the zero vector.
+void orc_cpu_centroids(const float* X, size_t n, uint32_t d,
+                       const uint32_t* labels, const uint32_t* counts,
+                       uint32_t k, float* C) {
+  std::vector<float> acc((size_t)k * d, 0.0f);
+  for (size_t i = 0; i < n; ++i) {
+    float* a = acc.data() + (size_t)labels[i] * d;
+    for (uint32_t j = 0; j < d; ++j) {
+      a[j] = a[j] + X[i * d + j];
+    }
+  }
+  for (uint32_t c = 0; c < k; ++c) {
+    uint32_t cnt = counts[c] > 1 ? counts[c] : 1;
+    float scale = (float)(1.0 / (double)cnt);
+    for (uint32_t j = 0; j < d; ++j) {
+      C[(size_t)c * d + j] = acc[(size_t)c * d + j] * scale;
+    }
+  }
+}
+
+// comparisons/CPU_Kmeans/src/kmeans.cpp:225-234: converged iff for every
+// centroid sqrt(sum_j (c_j - last_j)^2) <= tol, the sum in float, in order.
+int orc_cpu_converged(const float* C, const float* Clast, uint32_t k,
+                      uint32_t d, float tol) {
+  for (uint32_t c = 0; c < k; ++c) {
+    float s = 0.0f;
+    for (uint32_t j = 0; j < d; ++j) {
+      float t = C[(size_t)c * d + j] - Clast[(size_t)c * d + j];
+      float sq = t * t;
+      s = s + sq;
+    }
+    if (std::sqrt((float)s) > tol) return 0;
+  }
+  return 1;
+}
+
+// comparisons/CPU_Kmeans/src/kmeans.cpp:192-223: the Lloyd loop. C holds the
+// initial centroids on entry and the final ones on exit. Returns the
+// iteration at which the shift test first passed (tested from iteration 2
+// on), or maxiter. iters_run (optional) receives the number of iterations run.
+uint64_t orc_cpu_lloyd(const float* X, size_t n, uint32_t d, uint32_t k,
+                       float tol, uint64_t maxiter, float* C, uint32_t* labels,
+                       uint32_t* counts, uint64_t* iters_run) {
+  std::vector<float> D(n * (size_t)k);
+  std::vector<float> last(C, C + (size_t)k * d);
+  uint64_t converged = maxiter, iter = 0;
+  while (iter++ < maxiter) {
+    orc_cpu_distances(X, n, d, C, k, D.data());
+    orc_cpu_argmin(D.data(), n, k, labels, counts);
+    orc_cpu_centroids(X, n, d, labels, counts, k, C);
+    if (iter > 1 && orc_cpu_converged(C, last.data(), k, d, tol)) {
+      converged = iter;
+      break;
+    }
+    std::memcpy(last.data(), C, sizeof(float) * (size_t)k * d);
+  }
+  if (iters_run) *iters_run = iter > maxiter ? maxiter : iter;
+  return converged;
+}
+
+// ---------------------------------------------------------------------------
+// 2. FP64 Lloyd (same assign -> update -> shift order)
+// ---------------------------------------------------------------------------
+
+// ||x_i||^2 in FP64.
+void orc_f64_row_norms(const float* X, size_t n, uint32_t d, double* out) {
+  for (size_t i = 0; i < n; ++i) {
+    double s = 0;
+    for (uint32_t j = 0; j < d; ++j) s += (double)X[i * d + j] * X[i * d + j];
+    out[i] = s;
+  }
+}
+
+// Exact-as-FP64 squared distances ||x_i - c||^2 (direct form).
+void orc_f64_distances(const float* X, size_t n, uint32_t d, const float* C,
+                       uint32_t k, double* D) {
+  for (size_t i = 0; i < n; ++i)
+    for (uint32_t c = 0; c < k; ++c) {
+      double s = 0;
+      for (uint32_t j = 0; j < d; ++j) {
+        double t = (double)X[i * d + j] - (double)C[(size_t)c * d + j];
+        s += t * t;
+      }
+      D[i * k + c] = s;
+    }
+}
+
+// Inertia sum_i ||x_i - C[label_i]||^2 in FP64.
+double orc_f64_inertia(const float* X, size_t n, uint32_t d, const float* C,
+                       const uint32_t* labels) {
+  double tot = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const float* c = C + (size_t)labels[i] * d;
+    double s = 0;
+    for (uint32_t j = 0; j < d; ++j) {
+      double t = (double)X[i * d + j] - (double)c[j];
+      s += t * t;
+    }
+    tot += s;
+  }
+  return tot;
+}
+
+// Per-cluster means in FP64 (empty cluster -> zero vector, as the reference
+// CPU k-means), rounded once to float.
+void orc_f64_centroids(const float* X, size_t n, uint32_t d,
+                       const uint32_t* labels, uint32_t k, float* C,
+                       uint32_t* counts) {
+  std::vector<double> acc((size_t)k * d, 0.0);
+  std::vector<uint32_t> cnt(k, 0);
+  for (size_t i = 0; i < n; ++i) {
+    cnt[labels[i]]++;
+    for (uint32_t j = 0; j < d; ++j)
+      acc[(size_t)labels[i] * d + j] += X[i * d + j];
+  }
+  for (uint32_t c = 0; c < k; ++c) {
+    double inv = 1.0 / (double)(cnt[c] > 1 ? cnt[c] : 1);
+    for (uint32_t j = 0; j < d; ++j)
+      C[(size_t)c * d + j] = (float)(acc[(size_t)c * d + j] * inv);
+    if (counts) counts[c] = cnt[c];
+  }
+}
+
+// One FP64 assignment: labels = first-min of exact FP64 distances; returns the
+// FP64 inertia of that assignment. margin_out (optional) receives, per point,
+// the gap between the second-smallest and the smallest FP64 distance.
+double orc_f64_assign(const float* X, size_t n, uint32_t d, const float* C,
+                      uint32_t k, uint32_t* labels, double* margin_out) {
+  double tot = 0;
+  for (size_t i = 0; i < n; ++i) {
+    double b1 = std::numeric_limits<double>::infinity(), b2 = b1;
+    uint32_t idx = 0;
+    for (uint32_t c = 0; c < k; ++c) {
+      double s = 0;
+      for (uint32_t j = 0; j < d; ++j) {
+        double t = (double)X[i * d + j] - (double)C[(size_t)c * d + j];
+        s += t * t;
+      }
+      if (s < b1) { b2 = b1; b1 = s; idx = c; }
+      else if (s < b2) { b2 = s; }
+    }
+    labels[i] = idx;
+    if (margin_out) margin_out[i] = b2 - b1;
+    tot += b1;
+  }
+  return tot;
+}
+
+// ---------------------------------------------------------------------------
+// 3. Kernel k-means (Popcorn formulation), FP64
+// ---------------------------------------------------------------------------
+
+// Kernel functions of src/kernels/kernels.cuh:49-52 (linear), :74-75
+// (sigmoid tanh(v+1)), :100-101 (polynomial (v+1)^2), before the reference's
+// -2 scaling; kind 3 is the Gaussian RBF exp(-gamma*||x-y||^2), which the
+// reference does not have. K is n x n row-major, FP64.
+void orc_kk_kernel_matrix(const float* X, size_t n, uint32_t d, int kind,
+                          double gamma, double* K) {
+  std::vector<double> nrm(n);
+  orc_f64_row_norms(X, n, d, nrm.data());
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = 0; j <= i; ++j) {
+      double v = 0;
+      for (uint32_t t = 0; t < d; ++t) v += (double)X[i * d + t] * X[j * d + t];
+      double f;
+      switch (kind) {
+        case 0: f = v; break;
+        case 1: f = (v + 1.0) * (v + 1.0); break;
+        case 2: f = std::tanh(v + 1.0); break;
+        default: {
+          double sq = nrm[i] + nrm[j] - 2.0 * v;
+          if (i == j) sq = 0.0;
+          f = std::exp(-gamma * (sq < 0 ? 0 : sq));
+        }
+      }
+      K[i * n + j] = f;
+      K[j * n + i] = f;
+    }
+}
+
+// Kernel-space distances for the labels given (src/kernels/distances.cu:895-1002
+// with B = -2K): D[i,c] = ||mu_c||_K^2 - 2 <phi(x_i), mu_c>, where
+// <phi(x_i), mu_c> = sum_{j in c} K[i,j] / |c| and
+// ||mu_c||_K^2 = sum_{i,j in c} K[i,j] / |c|^2. An empty cluster (and, as in
+// the reference's add_norm_mtx, a cluster whose norm is exactly 0) gets +inf.
+// With add_diag != 0 the K[i,i] term is added, giving the full distance
+// ||phi(x_i) - mu_c||^2 (the reference omits it; its argmin is unchanged).
+void orc_kk_distances(const double* K, size_t n, const uint32_t* labels,
+                      uint32_t k, int add_diag, double* D) {
+  std::vector<double> cnt(k, 0.0), cn(k, 0.0);
+  for (size_t i = 0; i < n; ++i) cnt[labels[i]] += 1.0;
+  std::vector<double> S(n * (size_t)k, 0.0);  // S[i,c] = sum_{j in c} K[i,j]
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = 0; j < n; ++j) S[i * k + labels[j]] += K[i * n + j];
+  for (size_t i = 0; i < n; ++i) cn[labels[i]] += S[i * k + labels[i]];
+  for (uint32_t c = 0; c < k; ++c)
+    cn[c] = cnt[c] > 0 ? cn[c] / (cnt[c] * cnt[c]) : 0.0;
+  const double inf = std::numeric_limits<double>::infinity();
+  for (size_t i = 0; i < n; ++i)
+    for (uint32_t c = 0; c < k; ++c) {
+      if (cnt[c] == 0 || cn[c] == 0.0) { D[i * k + c] = inf; continue; }
+      double v = cn[c] - 2.0 * S[i * k + c] / cnt[c];
+      if (add_diag) v += K[i * n + i];
+      D[i * k + c] = v;
+    }
+}
+
+// Kernel k-means Lloyd in FP64 from initial labels: each iteration assigns
+// every point to the first minimum of orc_kk_distances and stops early when
+// no label changes. Returns the number of iterations run; labels in/out.
+uint64_t orc_kk_lloyd(const double* K, size_t n, uint32_t k, uint64_t maxiter,
+                      uint32_t* labels) {
+  std::vector<double> D(n * (size_t)k);
+  uint64_t it = 0;
+  while (it < maxiter) {
+    ++it;
+    orc_kk_distances(K, n, labels, k, 1, D.data());
+    bool changed = false;
+    for (size_t i = 0; i < n; ++i) {
+      double b = std::numeric_limits<double>::infinity();
+      uint32_t idx = labels[i];
+      for (uint32_t c = 0; c < k; ++c)
+        if (D[i * k + c] < b) { b = D[i * k + c]; idx = c; }
+      if (idx != labels[i]) changed = true;
+      labels[i] = idx;
+    }
+    if (!changed) break;
+  }
+  return it;
+}
+
+}  // extern "C"

@@ -1,0 +1,216 @@
+"""ctypes loader for the CPU oracle (oracle/libkmc_oracle.so) and the compiled
+reference CPU k-means (oracle/_ref/libkmc_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker. The product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libkmc_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libkmc_ref.so")
+
+_P = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+
+
+def build():
+    """Compile the oracle (and the reference, where its sources are present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _ptr(a):
+    return a.ctypes.data
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_cpu_init_centroids.restype = ctypes.c_int
+        L.orc_cpu_init_centroids.argtypes = [_P, _sz, _u32, _u32, ctypes.c_int, _P]
+        L.orc_cpu_distances.argtypes = [_P, _sz, _u32, _P, _u32, _P]
+        L.orc_cpu_argmin.argtypes = [_P, _sz, _u32, _P, _P]
+        L.orc_cpu_centroids.argtypes = [_P, _sz, _u32, _P, _P, _u32, _P]
+        L.orc_cpu_converged.restype = ctypes.c_int
+        L.orc_cpu_converged.argtypes = [_P, _P, _u32, _u32, ctypes.c_float]
+        L.orc_cpu_lloyd.restype = _u64
+        L.orc_cpu_lloyd.argtypes = [_P, _sz, _u32, _u32, ctypes.c_float, _u64, _P, _P, _P, _P]
+        L.orc_f64_row_norms.argtypes = [_P, _sz, _u32, _P]
+        L.orc_f64_distances.argtypes = [_P, _sz, _u32, _P, _u32, _P]
+        L.orc_f64_inertia.restype = ctypes.c_double
+        L.orc_f64_inertia.argtypes = [_P, _sz, _u32, _P, _P]
+        L.orc_f64_centroids.argtypes = [_P, _sz, _u32, _P, _u32, _P, _P]
+        L.orc_f64_assign.restype = ctypes.c_double
+        L.orc_f64_assign.argtypes = [_P, _sz, _u32, _P, _u32, _P, _P]
+        L.orc_kk_kernel_matrix.argtypes = [_P, _sz, _u32, ctypes.c_int, ctypes.c_double, _P]
+        L.orc_kk_distances.argtypes = [_P, _sz, _P, _u32, ctypes.c_int, _P]
+        L.orc_kk_lloyd.restype = _u64
+        L.orc_kk_lloyd.argtypes = [_P, _sz, _u32, _u64, _P]
+        self.L = L
+
+    # -- reference CPU k-means restatement (float) --
+    def init_centroids(self, X, k, seed):
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.empty((k, X.shape[1]), np.float32)
+        if self.L.orc_cpu_init_centroids(_ptr(X), X.shape[0], X.shape[1], k, seed, _ptr(C)):
+            raise ValueError("fewer than k distinct rows")
+        return C
+
+    def distances(self, X, C):
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.ascontiguousarray(C, np.float32)
+        D = np.empty((X.shape[0], C.shape[0]), np.float32)
+        self.L.orc_cpu_distances(_ptr(X), X.shape[0], X.shape[1], _ptr(C), C.shape[0], _ptr(D))
+        return D
+
+    def argmin(self, D):
+        D = np.ascontiguousarray(D, np.float32)
+        lab = np.empty(D.shape[0], np.uint32)
+        cnt = np.empty(D.shape[1], np.uint32)
+        self.L.orc_cpu_argmin(_ptr(D), D.shape[0], D.shape[1], _ptr(lab), _ptr(cnt))
+        return lab, cnt
+
+    def centroids(self, X, labels, k):
+        X = np.ascontiguousarray(X, np.float32)
+        lab = np.ascontiguousarray(labels, np.uint32)
+        cnt = np.bincount(lab, minlength=k).astype(np.uint32)
+        C = np.empty((k, X.shape[1]), np.float32)
+        self.L.orc_cpu_centroids(_ptr(X), X.shape[0], X.shape[1], _ptr(lab), _ptr(cnt), k, _ptr(C))
+        return C, cnt
+
+    def converged(self, C, Clast, tol):
+        C = np.ascontiguousarray(C, np.float32)
+        Clast = np.ascontiguousarray(Clast, np.float32)
+        return bool(self.L.orc_cpu_converged(_ptr(C), _ptr(Clast), C.shape[0], C.shape[1],
+                                             ctypes.c_float(tol)))
+
+    def lloyd(self, X, C0, tol, maxiter):
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.array(C0, dtype=np.float32, copy=True)
+        n, d = X.shape
+        k = C.shape[0]
+        lab = np.empty(n, np.uint32)
+        cnt = np.empty(k, np.uint32)
+        ran = np.zeros(1, np.uint64)
+        conv = self.L.orc_cpu_lloyd(_ptr(X), n, d, k, ctypes.c_float(tol), maxiter, _ptr(C),
+                                    _ptr(lab), _ptr(cnt), _ptr(ran))
+        return int(conv), lab, C, cnt, int(ran[0])
+
+    # -- FP64 --
+    def f64_distances(self, X, C):
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.ascontiguousarray(C, np.float32)
+        D = np.empty((X.shape[0], C.shape[0]), np.float64)
+        self.L.orc_f64_distances(_ptr(X), X.shape[0], X.shape[1], _ptr(C), C.shape[0], _ptr(D))
+        return D
+
+    def f64_inertia(self, X, C, labels):
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.ascontiguousarray(C, np.float32)
+        lab = np.ascontiguousarray(labels, np.uint32)
+        return self.L.orc_f64_inertia(_ptr(X), X.shape[0], X.shape[1], _ptr(C), _ptr(lab))
+
+    def f64_centroids(self, X, labels, k):
+        X = np.ascontiguousarray(X, np.float32)
+        lab = np.ascontiguousarray(labels, np.uint32)
+        C = np.empty((k, X.shape[1]), np.float32)
+        cnt = np.empty(k, np.uint32)
+        self.L.orc_f64_centroids(_ptr(X), X.shape[0], X.shape[1], _ptr(lab), k, _ptr(C), _ptr(cnt))
+        return C, cnt
+
+    def f64_assign(self, X, C):
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.ascontiguousarray(C, np.float32)
+        lab = np.empty(X.shape[0], np.uint32)
+        margin = np.empty(X.shape[0], np.float64)
+        inert = self.L.orc_f64_assign(_ptr(X), X.shape[0], X.shape[1], _ptr(C), C.shape[0],
+                                      _ptr(lab), _ptr(margin))
+        return lab, margin, inert
+
+    # -- kernel k-means (FP64) --
+    def kernel_matrix(self, X, kind, gamma=0.0):
+        X = np.ascontiguousarray(X, np.float32)
+        n = X.shape[0]
+        K = np.empty((n, n), np.float64)
+        self.L.orc_kk_kernel_matrix(_ptr(X), n, X.shape[1], kind, gamma, _ptr(K))
+        return K
+
+    def kk_distances(self, K, labels, k, add_diag=True):
+        K = np.ascontiguousarray(K, np.float64)
+        lab = np.ascontiguousarray(labels, np.uint32)
+        D = np.empty((K.shape[0], k), np.float64)
+        self.L.orc_kk_distances(_ptr(K), K.shape[0], _ptr(lab), k, int(add_diag), _ptr(D))
+        return D
+
+    def kk_lloyd(self, K, labels0, k, maxiter):
+        K = np.ascontiguousarray(K, np.float64)
+        lab = np.array(labels0, dtype=np.uint32, copy=True)
+        it = self.L.orc_kk_lloyd(_ptr(K), K.shape[0], k, maxiter, _ptr(lab))
+        return int(it), lab
+
+
+class Reference:
+    """The reference CPU k-means, compiled from its own sources (oracle/_ref)."""
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = ctypes.CDLL(path)
+        L.kref_run.restype = _u64
+        L.kref_run.argtypes = [_P, _sz, _u32, _u32, ctypes.c_float, ctypes.c_int, _u64, _P, _P, _P]
+        L.kref_run_from.restype = _u64
+        L.kref_run_from.argtypes = [_P, _sz, _u32, _u32, ctypes.c_float, _u64, _P, _P]
+        L.kref_step.argtypes = [_P, _sz, _u32, _u32, _P, _P, _P, _P, _P]
+        L.kref_converged.restype = ctypes.c_int
+        L.kref_converged.argtypes = [_P, _sz, _u32, _u32, ctypes.c_float, _P, _P]
+        self.L = L
+
+    @staticmethod
+    def available(path=REF_SO):
+        return os.path.exists(path)
+
+    def run(self, X, k, tol, seed, maxiter):
+        X = np.ascontiguousarray(X, np.float32)
+        n, d = X.shape
+        C0 = np.empty((k, d), np.float32)
+        C = np.empty((k, d), np.float32)
+        lab = np.empty(n, np.uint32)
+        conv = self.L.kref_run(_ptr(X), n, d, k, ctypes.c_float(tol), seed, maxiter, _ptr(C0),
+                               _ptr(C), _ptr(lab))
+        return int(conv), C0, C, lab
+
+    def run_from(self, X, C0, tol, maxiter):
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.array(C0, dtype=np.float32, copy=True)
+        n, d = X.shape
+        lab = np.empty(n, np.uint32)
+        conv = self.L.kref_run_from(_ptr(X), n, d, C.shape[0], ctypes.c_float(tol), maxiter,
+                                    _ptr(C), _ptr(lab))
+        return int(conv), C, lab
+
+    def step(self, X, C):
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.ascontiguousarray(C, np.float32)
+        n, d = X.shape
+        k = C.shape[0]
+        D = np.empty((n, k), np.float32)
+        lab = np.empty(n, np.uint32)
+        cnt = np.empty(k, np.uint32)
+        Cn = np.empty((k, d), np.float32)
+        self.L.kref_step(_ptr(X), n, d, k, _ptr(C), _ptr(D), _ptr(lab), _ptr(cnt), _ptr(Cn))
+        return D, lab, cnt, Cn
+
+    def converged(self, X, C, Clast, tol):
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.ascontiguousarray(C, np.float32)
+        Clast = np.ascontiguousarray(Clast, np.float32)
+        return bool(self.L.kref_converged(_ptr(X), X.shape[0], X.shape[1], C.shape[0],
+                                          ctypes.c_float(tol), _ptr(C), _ptr(Clast)))

@@ -1,0 +1,59 @@
+"""The C-ABI library loads on a host without a GPU and exports every entry
+point that include/kmc.h declares; host-side argument checks and error paths
+(no compute is launched)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "kmc.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kmc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for want in ("kmc_create", "kmc_set_points", "kmc_run", "kmc_get_labels",
+                 "kmc_assign", "kmc_update", "kmc_destroy"):
+        assert want in names
+
+
+def test_library_exports_every_declared_symbol(kmc_mod):
+    lib = ctypes.CDLL(kmc_mod.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_status_strings(kmc_mod):
+    lib = kmc_mod.load_library()
+    assert lib.kmc_status_string(0) == b"ok"
+    assert lib.kmc_status_string(8) == b"unsupported configuration"
+    assert lib.kmc_status_string(99) == b"unknown status"
+
+
+def test_create_rejects_bad_config(kmc_mod):
+    lib = kmc_mod.load_library()
+    h = ctypes.c_void_p()
+    assert lib.kmc_create(None, ctypes.byref(h)) == kmc_mod.ERR_ARG
+    for n, d, k, prec in [(0, 4, 2, 0), (10, 0, 2, 0), (10, 4, 0, 0), (10, 4, 11, 0),
+                          (10, 4, 2, 7)]:
+        cfg = kmc_mod._Config(n, d, k, 1e-4, prec, 0, 0, 0)
+        assert lib.kmc_create(ctypes.byref(cfg), ctypes.byref(h)) == kmc_mod.ERR_ARG
+    cfg = kmc_mod._Config(10, 4, 2, -1.0, 0, 0, 0, 0)
+    assert lib.kmc_create(ctypes.byref(cfg), ctypes.byref(h)) == kmc_mod.ERR_ARG
+    cfg = kmc_mod._Config(100, 300, 2, 1e-4, 0, 0, 0, 0)  # d beyond the fused kernel
+    assert lib.kmc_create(ctypes.byref(cfg), ctypes.byref(h)) == kmc_mod.ERR_UNSUPPORTED
+
+
+def test_no_device_is_reported(kmc_mod):
+    if kmc_mod.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(kmc_mod.KmcError) as ei:
+        kmc_mod.Engine(100, 8, 4)
+    assert ei.value.status == kmc_mod.ERR_NO_DEVICE

@@ -1,0 +1,187 @@
+"""Pins the CPU oracle (oracle/kmc_oracle.cpp) before it is trusted as the
+checker of the HIP path.
+
+  * bit-exact against the golden fixtures written from the compiled reference
+    CPU k-means (tests/golden/make_golden.py), and against the compiled
+    reference itself (oracle/_ref) on fresh random inputs where it is built;
+  * the worked distance examples of the reference's report
+    (files/GPU_Computing_Report.pdf p.6);
+  * FP64 parts against closed forms, brute force and Lloyd monotonicity;
+  * kernel k-means against the reference unit test's O(n^3) host formula
+    (tests/unit_kernels.cu:365-395) and the linear-kernel identity.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+
+def test_report_known_answers(oracle):
+    # files/GPU_Computing_Report.pdf p.6: 1-D c=7, p=3 -> 16, p=1 -> 36;
+    # 2-D c=(1,4), p=(5,2) -> 20; 3-D c=(5,2,3), p=(4,3,2) -> 3
+    cases = [([[3.0], [1.0]], [[7.0]], [16.0, 36.0]),
+             ([[5.0, 2.0]], [[1.0, 4.0]], [20.0]),
+             ([[4.0, 3.0, 2.0]], [[5.0, 2.0, 3.0]], [3.0])]
+    for X, C, want in cases:
+        X = np.array(X, np.float32)
+        C = np.array(C, np.float32)
+        assert oracle.distances(X, C)[:, 0].tolist() == want
+        assert oracle.f64_distances(X, C)[:, 0].tolist() == want
+
+
+def test_steps_match_golden(oracle):
+    g = load_golden("ref_steps.npz")
+    for i in range(int(g["ncases"])):
+        X, C = g["%d_X" % i], g["%d_C" % i]
+        D = oracle.distances(X, C)
+        assert np.array_equal(D, g["%d_D" % i]), i
+        lab, cnt = oracle.argmin(D)
+        assert np.array_equal(lab, g["%d_labels" % i]), i
+        assert np.array_equal(cnt, g["%d_counts" % i]), i
+        Cn, _ = oracle.centroids(X, lab, C.shape[0])
+        assert np.array_equal(Cn, g["%d_Cnew" % i]), i
+
+
+def test_golden_pins_summation_order(oracle):
+    """A plausible mistake -- pairwise (numpy) instead of sequential float
+    sums -- must fail the golden distances, else the fixture pins nothing."""
+    g = load_golden("ref_steps.npz")
+    differs = 0
+    for i in range(int(g["ncases"])):
+        X, C = g["%d_X" % i], g["%d_C" % i]
+        diff = X[:, None, :] - C[None, :, :]
+        wrong = np.sum(diff * diff, axis=2, dtype=np.float32)
+        differs += int(not np.array_equal(wrong, g["%d_D" % i]))
+    assert differs > 0
+
+
+@pytest.mark.parametrize("name", ["ref_c1_blobs.npz", "ref_digits.npz"])
+def test_full_runs_match_golden(oracle, name):
+    g = load_golden(name)
+    X = g["X"]
+    C0 = oracle.init_centroids(X, int(g["k"]), int(g["seed"]))
+    assert np.array_equal(C0, g["C0"])
+    conv, lab, C, _, _ = oracle.lloyd(X, C0, float(g["tol"]), int(g["maxiter"]))
+    assert conv == int(g["converged"])
+    assert np.array_equal(lab, g["labels"])
+    assert np.array_equal(C, g["C"])
+
+
+def test_runs_from_centroids_match_golden(oracle):
+    g = load_golden("ref_runs.npz")
+    for i in range(int(g["ncases"])):
+        conv, lab, C, _, _ = oracle.lloyd(g["%d_X" % i], g["%d_C0" % i], float(g["%d_tol" % i]),
+                                          int(g["%d_maxiter" % i]))
+        assert conv == int(g["%d_converged" % i]), i
+        assert np.array_equal(lab, g["%d_labels" % i]), i
+        assert np.array_equal(C, g["%d_C" % i]), i
+
+
+def test_against_compiled_reference(oracle):
+    from oracle import Reference
+    if not Reference.available():
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    R = Reference()
+    rng = np.random.default_rng(7)
+    for n, d, k in [(40, 3, 5), (300, 11, 9), (700, 40, 23)]:
+        X = rng.normal(0, 10, size=(n, d)).astype(np.float32)
+        C = X[rng.choice(n, k, replace=False)]
+        D, lab, cnt, Cn = R.step(X, C)
+        D2 = oracle.distances(X, C)
+        lab2, cnt2 = oracle.argmin(D2)
+        Cn2, _ = oracle.centroids(X, lab2, k)
+        assert np.array_equal(D, D2) and np.array_equal(lab, lab2)
+        assert np.array_equal(cnt, cnt2) and np.array_equal(Cn, Cn2)
+        for tol in (0.0, 1e-3, 1.0):
+            conv, Cr, lr = R.run_from(X, C, tol, 40)
+            conv2, lo, Co, _, _ = oracle.lloyd(X, C, tol, 40)
+            assert conv == conv2 and np.array_equal(lr, lo) and np.array_equal(Cr, Co)
+        # shift test exactly at the tolerance
+        C2 = C.copy()
+        C2[0, 0] += np.float32(0.5)
+        for tol in (0.5, np.nextafter(np.float32(0.5), np.float32(0)), 0.25):
+            assert R.converged(X, C2, C, tol) == oracle.converged(C2, C, tol)
+
+
+def test_empty_cluster_becomes_zero(oracle):
+    X = np.array([[0, 0], [0, 1], [10, 10]], np.float32)
+    C = np.array([[0, 0.5], [10, 10], [100, 100]], np.float32)
+    lab, cnt = oracle.argmin(oracle.distances(X, C))
+    Cn, _ = oracle.centroids(X, lab, 3)
+    assert cnt.tolist() == [2, 1, 0]
+    assert Cn[2].tolist() == [0.0, 0.0]
+
+
+def test_f64_closed_forms(oracle):
+    rng = np.random.default_rng(3)
+    X = rng.normal(0, 5, size=(500, 13)).astype(np.float32)
+    C = X[:7] + 0.25
+    D = oracle.f64_distances(X, C)
+    # brute force argmin in numpy vs oracle assign
+    lab, margin, inert = oracle.f64_assign(X, C)
+    assert np.array_equal(lab, np.argmin(D, axis=1))
+    srt = np.sort(D, axis=1)
+    assert np.allclose(margin, srt[:, 1] - srt[:, 0], rtol=0, atol=1e-9)
+    # inertia identity: direct form vs expanded GEMM form in FP64
+    Xd, Cd = X.astype(np.float64), C.astype(np.float64)
+    expanded = (Xd ** 2).sum() - 2 * np.sum(Xd * Cd[lab]) + (Cd[lab] ** 2).sum()
+    assert abs(inert - expanded) <= 1e-9 * inert
+    assert abs(oracle.f64_inertia(X, C, lab) - inert) <= 1e-12 * inert
+    # means of the one-hot groups
+    Cm, cnt = oracle.f64_centroids(X, lab, 7)
+    for c in range(7):
+        if cnt[c]:
+            assert np.allclose(Cm[c], Xd[lab == c].mean(axis=0).astype(np.float32), rtol=1e-6,
+                               atol=1e-6)
+
+
+def test_f64_lloyd_inertia_monotone(oracle):
+    from kmc import datagen
+    X, _, _ = datagen.blobs(2000, 8, 12, 5, -3, 3, 0.5, 1.5)
+    C = datagen.init_rows(X, 12, 9)
+    last = np.inf
+    for _ in range(15):
+        lab, _, inert = oracle.f64_assign(X, C)
+        assert inert <= last * (1 + 1e-12)
+        last = inert
+        C, _ = oracle.f64_centroids(X, lab, 12)
+
+
+def _unit_test_formula(K, labels, k):
+    """tests/unit_kernels.cu:365-395, restated: K_ii - 2 sum_{j in c} K_ij/|c|
+    + sum_{j,l in c} K_jl / |c|^2 by explicit loops over clusters."""
+    n = K.shape[0]
+    cnt = np.bincount(labels, minlength=k).astype(np.float64)
+    out = np.zeros((n, k))
+    for c in range(k):
+        idx = np.nonzero(labels == c)[0]
+        cross = K[:, idx].sum(axis=1) / cnt[c]
+        within = K[np.ix_(idx, idx)].sum() / cnt[c] ** 2
+        out[:, c] = np.diag(K) - 2 * cross + within
+    return out
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_kernel_kmeans_oracle(oracle, kind):
+    rng = np.random.default_rng(11 + kind)
+    n, d, k = 60, 5, 4
+    X = rng.uniform(0, 1, size=(n, d)).astype(np.float32)
+    gamma = 1.0 / d
+    K = oracle.kernel_matrix(X, kind, gamma)
+    G = X.astype(np.float64) @ X.astype(np.float64).T
+    sq = np.maximum((G.diagonal()[:, None] + G.diagonal()[None, :] - 2 * G), 0)
+    np.fill_diagonal(sq, 0)
+    want = {0: G, 1: (G + 1) ** 2, 2: np.tanh(G + 1), 3: np.exp(-gamma * sq)}[kind]
+    assert np.allclose(K, want, rtol=1e-12, atol=1e-12)
+    labels = (np.arange(n) % k).astype(np.uint32)
+    D = oracle.kk_distances(K, labels, k, add_diag=True)
+    assert np.allclose(D, _unit_test_formula(K, labels, k), rtol=1e-10, atol=1e-10)
+    # without the diagonal (the reference's form) the argmin is unchanged
+    D0 = oracle.kk_distances(K, labels, k, add_diag=False)
+    assert np.array_equal(np.argmin(D, 1), np.argmin(D0, 1))
+    if kind == 0:
+        # linear kernel: kernel-space distance == ||x - mean||^2
+        Xd = X.astype(np.float64)
+        mu = np.stack([Xd[labels == c].mean(0) for c in range(k)])
+        direct = ((Xd[:, None, :] - mu[None]) ** 2).sum(-1)
+        assert np.allclose(D, direct, rtol=1e-9, atol=1e-9)
