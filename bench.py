@@ -1,0 +1,216 @@
+"""Lloyd-iteration throughput of the gfx950 k-means engine on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--prec bf16x3]
+
+Workload (the large plain-vs-BF16 configuration): N=1,000,000 points, D=128,
+K=256 Gaussian blobs (centres U[-10,10]^128, per-blob sigma U[0.5,2.0]),
+float32, seeded; 256 distinct seed-chosen rows as initial centroids; no
+convergence test (tol=0, every iteration runs). A step is one Lloyd
+iteration: centroid operands + fused MFMA distance/argmin + margin
+refinement + stable counting sort + ordered centroid update + FP64 inertia
+and shift. Points are resident in HBM before the timed region.
+
+The headline form (``--prec``, default bf16x3: hi+lo split bf16 MFMA) and the
+plain FP32 form run from the same initial centroids; the line reports the
+headline's iterations/s as ``value`` and the FP32 form beside it, checks that
+both forms end with bit-identical centroids and labels, and measures the
+inertia error of the headline form against an FP64 oracle.
+
+The roofline object covers the dominant kernel (the fused distance kernel):
+achieved = 2*N*K*D algorithmic flops per launch / mean launch time, the time
+taken live with HIP events recorded around every launch on the engine's
+stream during the timed region.
+
+The CPU baseline is the reference CPU k-means (comparisons/CPU_Kmeans,
+compiled from its own sources into oracle/_ref) timed single-threaded on this
+host on a bounded sample (first 100,000 points, 2 iterations), scaled to the
+full N; falls back to the oracle restatement when oracle/_ref is absent.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "matrix-centric-k-means_amd"))
+
+N, D, K = 1_000_000, 128, 256
+DATA_SEED, INIT_SEED = 20261203, 7
+PEAK = {"fp32": 157.3e12, "bf16x3": 2.5e15, "bf16": 2.5e15}
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(kmc, X, C0, prec, steps, warmup, sync):
+    with kmc.Engine(N, D, K, tol=0.0, precision=prec) as e:
+        e.set_points(X)
+        e.set_centroids(C0)
+        if warmup:
+            e.run(warmup, check_converged=False)
+        sync()
+        t0 = time.perf_counter()
+        res = e.run(steps, check_converged=False, time_kernels=True)
+        sync()
+        wall = time.perf_counter() - t0
+        C_end = e.centroids()
+        lab_end = e.labels()
+        # inertia of the final centroids through the engine's own assign+update
+        lab_f, _ = e.assign(C_end)
+        _, _, upd = e.update(lab_f, C_end)
+    return {"wall_s": wall, "res": res, "C_end": C_end, "labels_end": lab_end,
+            "labels_final": lab_f, "inertia_gpu": upd["inertia"]}
+
+
+def cpu_baseline(X, C0, n_sample=100_000, iters=2):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, Reference
+    Xs = np.ascontiguousarray(X[:n_sample])
+    if Reference.available():
+        secs = Reference().time_run(Xs, C0, iters)
+        kind = "reference"
+    else:
+        O = Oracle()
+        t0 = time.perf_counter()
+        O.lloyd(Xs, C0, 0.0, iters)
+        secs = time.perf_counter() - t0
+        kind = "port"
+    per_iter_full = secs / iters * (N / n_sample)
+    return {"value": 1.0 / per_iter_full, "unit": "iter/s", "cores": 1, "kind": kind,
+            "sample": "first %d of the %d points, D=%d, K=%d, %d iterations in %.2f s "
+                      "(single thread, g++ -O2), scaled linearly in N" %
+                      (n_sample, N, D, K, iters, secs)}
+
+
+def traffic_from_profiles(prec):
+    """HBM bytes per launch of the distance kernel from committed rocprofv3
+    --pmc summaries (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", "assign_c2_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get(prec, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def profile_copy(nbytes):
+    """A plain device copy of known size, for calibrating the PMC byte counters."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "launches": 3}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "bf16", "fp32"])
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison form")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--profile-copy", type=int, default=0,
+                    help="only run a device copy of this many bytes (PMC calibration)")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        sys.exit("bench.py measures one GPU: this build has no multi-GPU path "
+                 "(--gpus %d requested)" % a.gpus)
+    if a.profile_copy:
+        profile_copy(a.profile_copy)
+        return
+
+    import torch
+    import kmc
+    from kmc import datagen
+
+    def sync():
+        torch.cuda.synchronize()
+
+    if not torch.cuda.is_available() or kmc.device_count() < 1:
+        sys.exit("bench.py: no GPU visible")
+
+    log("generating N=%d D=%d K=%d blobs" % (N, D, K))
+    X, _, _ = datagen.large_blobs(N, D, K, DATA_SEED)
+    C0 = datagen.init_rows(X, K, INIT_SEED)
+
+    log("timing %s: warmup %d, steps %d" % (a.prec, a.warmup, a.steps))
+    head = run_form(kmc, X, C0, a.prec, a.steps, a.warmup, sync)
+    fp32 = None
+    if not a.no_fp32 and a.prec != "fp32":
+        log("timing fp32: warmup %d, steps %d" % (a.warmup, a.steps))
+        fp32 = run_form(kmc, X, C0, "fp32", a.steps, a.warmup, sync)
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    O = Oracle()
+    C_end = head["C_end"]
+    inert64 = O.f64_inertia(X, C_end, head["labels_final"].astype(np.uint32))
+    inert_err = abs(head["inertia_gpu"] - inert64) / inert64
+
+    r = head["res"]
+    iters = int(r["iterations"])
+    ips = iters / head["wall_s"]
+    launches = int(r["assign_launches"])
+    t_assign = r["ms_assign"] / 1e3 / max(1, launches)
+    flops = 2.0 * N * K * D
+    achieved = flops / t_assign
+    traffic = traffic_from_profiles(a.prec)
+
+    out = {
+        "metric": "Lloyd iter/s, N=1M D=128 K=256: BF16 MFMA vs FP32 at <=1e-5 inertia err",
+        "value": round(ips, 3),
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(head["wall_s"] * 1e3 / max(1, iters), 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16x3": "bf16", "bf16": "bf16", "fp32": "fp32"}[a.prec],
+        "data": "synthetic",
+        "config": {"workload": "Lloyd k-means, Gaussian blobs N=1000000 D=128 K=256, "
+                               "maxiter=50 tol=0",
+                   "n": N, "d": D, "k": K, "form": a.prec, "seed": DATA_SEED,
+                   "init": "256 distinct seed-chosen rows (seed %d)" % INIT_SEED,
+                   "update": "ordered (reference summation order)"},
+        "roofline": {"bound": "mfma", "kernel": "k_assign<128,%s>" % a.prec,
+                     "achieved": round(achieved / 1e12, 3), "peak": PEAK[a.prec] / 1e12,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK[a.prec], 4),
+                     "traffic": traffic,
+                     "algorithmic_flops_per_launch": flops,
+                     "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches},
+        "inertia_rel_err_vs_fp64": inert_err,
+        "inertia_fp64": inert64,
+        "refined_points_last_iter": {"two_candidates": int(r["refined_two"]),
+                                     "all_k": int(r["refined_full"])},
+    }
+    if fp32 is not None:
+        f = fp32["res"]
+        f_ips = int(f["iterations"]) / fp32["wall_s"]
+        f_t = f["ms_assign"] / 1e3 / max(1, int(f["assign_launches"]))
+        out["fp32"] = {"value": round(f_ips, 3), "ms_per_step": round(fp32["wall_s"] * 1e3 /
+                                                                      max(1, f["iterations"]), 5),
+                       "assign_mean_launch_ms": round(f_t * 1e3, 5),
+                       "assign_tflops": round(flops / f_t / 1e12, 3),
+                       "assign_frac_fp32_peak": round(flops / f_t / PEAK["fp32"], 4)}
+        out["speedup_vs_fp32"] = round(ips / f_ips, 3)
+        out["identical_to_fp32"] = bool(np.array_equal(fp32["C_end"], C_end) and
+                                        np.array_equal(fp32["labels_end"], head["labels_end"]))
+    if not a.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(X, C0)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,17 @@ typedef enum {
                                         centroid shift test passes            */
 #define KMC_RUN_TIME_KERNELS 0x2u    /* record HIP events around the distance
                                         kernel of every iteration             */
+#define KMC_RUN_CONVERGE_SCORE 0x4u  /* with KMC_RUN_CHECK_CONVERGED: stop when
+                                        |score_t - score_{t-1}| < tol instead
+                                        (the reference GPU engine's test,
+                                        src/kmeans.cu:1064-1073)              */
+#define KMC_RUN_EXCLUDE_EMPTY 0x8u   /* a cluster left empty by an assignment
+                                        is never chosen again (the reference
+                                        GPU engine gives it distance +inf,
+                                        src/kernels/distances.cu:588-598); by
+                                        default it becomes the zero vector and
+                                        stays selectable, as in the reference
+                                        CPU k-means                           */
 
 typedef struct {
   size_t n;          /* points                                   */
@@ -155,6 +166,77 @@ const char* kmc_status_string(kmc_status s);
 /* Number of visible HIP devices (0 when none); replaces the device query of
  * src/main.cu:40-52. */
 int kmc_device_count(void);
+
+/* Device memory in use (total - free) on a device; replaces the
+ * MEMORY FOOTPRINT report of src/kmeans.cu:1094-1099. */
+kmc_status kmc_memory_footprint(int device, size_t* used_bytes);
+
+/* ------------------------------------------------------------------------
+ * Kernel k-means (the reference's Popcorn path, src/kmeans.cu:71-305,772-1122)
+ * ------------------------------------------------------------------------ */
+
+/* Storage type of the n x n kernel matrix. */
+typedef enum { KMC_STORE_FP32 = 0, KMC_STORE_BF16 = 1 } kmc_kstore;
+
+#define KMC_RUN_STOP_STABLE 0x10u /* with KMC_RUN_CHECK_CONVERGED (kernel
+                                     k-means): stop when no label changes,
+                                     instead of the score test              */
+
+typedef struct {
+  size_t n;
+  uint32_t d;
+  uint32_t k;        /* clusters, at most 64                         */
+  float tol;         /* score tolerance (src/kmeans.cu:1068-1073)    */
+  int32_t kernel;    /* kmc_kernel                                   */
+  int32_t store;     /* kmc_kstore                                   */
+  float gamma;       /* RBF width; <= 0 selects 1/d                  */
+  int32_t device;
+} kmc_kk_config;
+
+typedef struct {
+  uint64_t iterations;
+  uint64_t converged_at;
+  double objective;  /* FP64 sum_i ||phi(x_i) - mu_{l_i}||^2 of the last assign */
+  double score;      /* reference "Score": objective - sum_i K_ii       */
+  uint64_t changed;  /* labels changed by the last assignment          */
+  double ms_total;   /* iteration loop, ms                             */
+  double ms_build;   /* kernel-matrix build at kmc_kk_set_points, ms   */
+  double ms_spmm;    /* sum of K-streaming kernel times (KMC_RUN_TIME_KERNELS) */
+  uint64_t spmm_launches;
+} kmc_kk_result;
+
+typedef struct kmc_kk_handle kmc_kk_handle;
+
+/* Replaces the Kmeans constructor's kernel-matrix path (src/kmeans.cu:71-305). */
+kmc_status kmc_kk_create(const kmc_kk_config* cfg, kmc_kk_handle** out);
+
+/* Copies X (host n x d, leading dimension ld) and builds K = f(X X^T):
+ * replaces init_kernel_mtx / _gemm / _syrk + Kernel::function
+ * (src/kernels/kernels.cuh:43-118,567-718; src/kmeans.cu:158-171). */
+kmc_status kmc_kk_set_points(kmc_kk_handle* h, const float* X, size_t ld);
+
+/* Initial labels (host n); replaces init_centroids_rand (src/kmeans.cu:365-444). */
+kmc_status kmc_kk_set_labels(kmc_kk_handle* h, const int32_t* labels0);
+
+/* Replaces Kmeans::run on the kernel path (src/kmeans.cu:772-1122).
+ * flags: KMC_RUN_CHECK_CONVERGED (score test), KMC_RUN_STOP_STABLE,
+ * KMC_RUN_TIME_KERNELS. */
+kmc_status kmc_kk_run(kmc_kk_handle* h, uint64_t maxiter, uint32_t flags, kmc_kk_result* res);
+
+kmc_status kmc_kk_get_labels(kmc_kk_handle* h, int32_t* labels);
+
+/* Stage hook: kernel-space distances for host labels, host n x k, as the
+ * reference materialises them (no K_ii term): replaces
+ * compute_distances_popcorn_spmv (src/kernels/distances.cu:895-1002). */
+kmc_status kmc_kk_distances(kmc_kk_handle* h, const int32_t* labels, float* D);
+
+/* Rows [row0, row0+nrows) of K as stored (bf16 widened to float), host
+ * nrows x n; replaces the LOG_KERNEL dump of src/kmeans.cu:173-194. */
+kmc_status kmc_kk_get_kernel_rows(kmc_kk_handle* h, size_t row0, size_t nrows, float* out);
+
+const char* kmc_kk_last_error(const kmc_kk_handle* h);
+
+kmc_status kmc_kk_destroy(kmc_kk_handle* h);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,7 @@ struct kmc_handle {
   int nparts = 0;
   int* conv = nullptr;
   double* shift2 = nullptr;
+  int* alive = nullptr;
   DevState* state = nullptr;
 
   // launch configuration
@@ -154,7 +155,9 @@ kmc_status alloc_all(kmc_handle* h) {
   KMC_CK(dalloc(&h->ipart, (size_t)h->nparts));
   KMC_CK(dalloc(&h->conv, K));
   KMC_CK(dalloc(&h->shift2, K));
+  KMC_CK(dalloc(&h->alive, K));
   KMC_CK(dalloc(&h->state, 1));
+  KMC_CK(kmc::launch_fill_i32(h->alive, (long long)K, 1, h->stream));
   KMC_CK(hipMemsetAsync(h->C, 0, K * DP * sizeof(float), h->stream));
   KMC_CK(hipMemsetAsync(h->Cnext, 0, K * DP * sizeof(float), h->stream));
   KMC_CK(hipMemsetAsync(h->state, 0, sizeof(DevState), h->stream));
@@ -167,7 +170,7 @@ void free_all(kmc_handle* h) {
   void* ptrs[] = {h->X, h->Xhi, h->Xlo, h->xn2, h->C, h->Cnext, h->Cw, h->Chi,
                   h->Clo, h->cn, h->labels, h->q2, h->qa, h->qcnt, h->bcount,
                   h->count, h->start, h->members, h->ipart, h->conv, h->shift2,
-                  h->state};
+                  h->alive, h->state};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -197,6 +200,7 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1) {
   pp.Clo = h->Clo;
   pp.cn = h->cn;
   pp.qcnt = h->qcnt;
+  pp.alive = h->alive;
   pp.done = done_ptr(h);
   pp.K = h->K;
   pp.Kp = h->Kp;
@@ -235,6 +239,7 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1) {
   rp.q2 = h->q2;
   rp.qa = h->qa;
   rp.qcnt = h->qcnt;
+  rp.alive = h->alive;
   rp.labels = h->labels;
   rp.done = done_ptr(h);
   rp.K = h->K;
@@ -246,7 +251,7 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1) {
   return KMC_OK;
 }
 
-kmc_status enqueue_update(kmc_handle* h, int check) {
+kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty) {
   kmc::SortParams sp{};
   sp.labels = h->labels;
   sp.bcount = h->bcount;
@@ -279,11 +284,14 @@ kmc_status enqueue_update(kmc_handle* h, int check) {
   fp.Cnext = h->Cnext;
   fp.conv = h->conv;
   fp.shift2 = h->shift2;
+  fp.count = h->count;
+  fp.alive = h->alive;
   fp.done = done_ptr(h);
   fp.tol = h->cfg.tol;
   fp.K = h->K;
   fp.D = h->D;
   fp.DP = h->DP;
+  fp.exclude_empty = exclude_empty;
   kmc::StatsParams st{};
   st.ipart = h->ipart;
   st.nparts = h->nparts;
@@ -292,8 +300,9 @@ kmc_status enqueue_update(kmc_handle* h, int check) {
   st.qcnt = h->qcnt;
   st.state = h->state;
   st.done = done_ptr(h);
+  st.tol = h->cfg.tol;
   st.K = h->K;
-  st.check = check;
+  st.mode = mode;
   KMC_CK(kmc::launch_finish(fp, st, h->stream));
   return KMC_OK;
 }
@@ -355,6 +364,19 @@ int kmc_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
+}
+
+kmc_status kmc_memory_footprint(int device, size_t* used_bytes) {
+  if (!used_bytes) return KMC_ERR_ARG;
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess) return KMC_ERR_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return KMC_ERR_ARG;
+  size_t fr = 0, tot = 0;
+  const hipError_t e = hipMemGetInfo(&fr, &tot);
+  (void)hipSetDevice(cur);
+  if (e != hipSuccess) return KMC_ERR_HIP;
+  *used_bytes = tot - fr;
+  return KMC_OK;
 }
 
 kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
@@ -477,6 +499,7 @@ kmc_status kmc_set_centroids(kmc_handle* h, const float* C0) {
   KMC_CK(hipSetDevice(h->dev));
   KMC_CK(hipMemcpy2DAsync(h->C, h->DP * sizeof(float), C0, h->D * sizeof(float),
                           h->D * sizeof(float), h->K, hipMemcpyHostToDevice, h->stream));
+  KMC_CK(kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream));
   KMC_CK(hipStreamSynchronize(h->stream));
   h->have_centroids = true;
   return KMC_OK;
@@ -493,7 +516,10 @@ kmc_status kmc_set_labels(kmc_handle* h, const int32_t* labels0) {
                         h->stream));
   kmc_status st = reset_state(h, 1);
   if (st != KMC_OK) return st;
-  st = enqueue_update(h, 0);  // C <- means of the label groups
+  KMC_CK(kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream));
+  // C <- means of the label groups; clusters empty under these labels are
+  // marked for KMC_RUN_EXCLUDE_EMPTY
+  st = enqueue_update(h, kmc::CONV_NONE, 1);
   if (st != KMC_OK) return st;
   KMC_CK(hipStreamSynchronize(h->stream));
   h->have_centroids = true;
@@ -514,8 +540,12 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
     if (st == KMC_OK) fill_result(h, s, res);
     return st;
   }
+  const bool score_mode = (flags & KMC_RUN_CONVERGE_SCORE) != 0;
   const bool check = (flags & KMC_RUN_CHECK_CONVERGED) != 0;
   const bool timing = (flags & KMC_RUN_TIME_KERNELS) != 0;
+  const bool exclude = (flags & KMC_RUN_EXCLUDE_EMPTY) != 0;
+  const int mode = !check ? kmc::CONV_NONE : (score_mode ? kmc::CONV_SCORE : kmc::CONV_SHIFT);
+  if (!exclude) KMC_CK(kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream));
   const uint64_t kMaxTimed = 4096;
   const uint64_t ntimed = timing ? std::min<uint64_t>(maxiter, kMaxTimed) : 0;
   while (h->events.size() < 2 * ntimed) {
@@ -536,7 +566,7 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
       hipEvent_t e0 = gi < ntimed ? h->events[2 * gi] : nullptr;
       hipEvent_t e1 = gi < ntimed ? h->events[2 * gi + 1] : nullptr;
       st = enqueue_assign(h, e0, e1);
-      if (st == KMC_OK) st = enqueue_update(h, check ? 1 : 0);
+      if (st == KMC_OK) st = enqueue_update(h, mode, exclude ? 1 : 0);
       if (st != KMC_OK) break;
     }
     if (st != KMC_OK) break;
@@ -629,7 +659,7 @@ kmc_status kmc_update(kmc_handle* h, const int32_t* labels, const float* C_old, 
                         h->stream));
   st = reset_state(h, 1);
   if (st != KMC_OK) return st;
-  st = enqueue_update(h, 0);
+  st = enqueue_update(h, kmc::CONV_NONE, 0);
   if (st != KMC_OK) return st;
   if (counts)
     KMC_CK(hipMemcpyAsync(counts, h->count, h->K * sizeof(uint32_t), hipMemcpyDeviceToHost,

@@ -20,7 +20,11 @@ struct DevState {
   unsigned long long refined_two;
   unsigned long long refined_all;
   double sum_x2;                    // FP64 sum_i ||x_i||^2 (set-up)
+  double last_score;                // score of the previous iteration
 };
+
+// convergence tests of the k_stats kernel
+enum { CONV_NONE = 0, CONV_SHIFT = 1, CONV_SCORE = 2, CONV_STABLE = 3 };
 
 struct PrepParams {
   const float* C;  // K x DP master centroids
@@ -29,6 +33,7 @@ struct PrepParams {
   uint16_t* Clo;
   float* cn;       // Kp, ||c||^2 (+inf for padding rows)
   unsigned* qcnt;  // refinement queue counters, cleared here
+  const int* alive;  // K: 0 = cluster excluded from assignment (+inf)
   const int* done;
   int K, Kp, DP, prec;
 };
@@ -58,6 +63,7 @@ struct RefineParams {
   const int* q2;
   const int* qa;
   const unsigned* qcnt;
+  const int* alive;
   int* labels;
   const int* done;
   int K, D, DP;
@@ -91,9 +97,11 @@ struct FinishParams {
   const float* Cnext;
   int* conv;
   double* shift2;
+  const int* count;  // cluster sizes of the assignment just used
+  int* alive;        // updated when exclude_empty is set
   const int* done;
   float tol;
-  int K, D, DP;
+  int K, D, DP, exclude_empty;
 };
 
 struct StatsParams {
@@ -104,7 +112,8 @@ struct StatsParams {
   const unsigned* qcnt;
   DevState* state;
   const int* done;
-  int K, check;
+  float tol;
+  int K, mode;  // mode: CONV_*
 };
 
 struct LoadParams {

@@ -1,0 +1,387 @@
+// kmc_kkmeans.hip -- gfx950 kernels of kernel k-means (Popcorn formulation).
+//
+// Set-up:
+//   k_kk_norms   ||x_i||^2 (for the RBF kernel)
+//   k_kk_build   K = f(X X^T) on MFMA (fp32 32x32x2), kernel function fused in
+//                the epilogue, stored fp32 or bf16, row stride Np (zero pad)
+// Per iteration (labels -> labels):
+//   k_kk_onehot  H^T[c][j] = (label_j == c), fp32 or bf16 (exact 0/1)
+//   k_kk_count   cluster sizes
+//   k_kk_spmm    S = K H on MFMA (16x16x4 f32 or 16x16x32 bf16): every row of
+//                K streamed from HBM once; per-block partial sums of
+//                S[i, label_i] for the cluster self-terms
+//   k_kk_gsum    g_c = sum_{i in c} S[i,c] (fixed order, FP64)
+//   k_kk_assign  d(i,c) = K_ii - 2 S[i,c]/|c| + g_c/|c|^2, first minimum,
+//                empty cluster (or g_c == 0) -> +inf; FP64 objective
+//   k_kk_stats   iteration counter, score test, stop flag
+#include "kmc_device.h"
+#include "kmc_kk.h"
+
+namespace kmc {
+
+__global__ __launch_bounds__(256) void k_kk_norms(KKParams P) {
+  const int lane = threadIdx.x & 63;
+  const long long i = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= P.n) return;
+  float s = 0.0f;
+  for (int j = lane; j < P.d; j += 64) {
+    const float v = P.X[(size_t)i * P.ldx + j];
+    s = fmaf(v, v, s);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) P.xn2[i] = s;
+}
+
+__device__ __forceinline__ float kernel_fn(int kind, float v, float xi2, float xj2, float gamma,
+                                           bool diag) {
+  switch (kind) {
+    case KK_LINEAR: return v;
+    case KK_POLY: { const float t = v + 1.0f; return t * t; }
+    case KK_SIGMOID: return tanhf(v + 1.0f);
+    default: {
+      const float sq = diag ? 0.0f : fmaxf(xi2 + xj2 - 2.0f * v, 0.0f);
+      return expf(-gamma * sq);
+    }
+  }
+}
+
+// 64x64 output tile per workgroup, 4 waves of 32x32; X staged through LDS in
+// k-chunks of 32 (rows padded by 4 floats against bank conflicts).
+template <int STORE>
+__global__ __launch_bounds__(256) void k_kk_build(KKParams P) {
+  constexpr int TK = 32, RS = TK + 4;
+  __shared__ __attribute__((aligned(16))) float As[64 * RS];
+  __shared__ __attribute__((aligned(16))) float Bs[64 * RS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long ti = blockIdx.y, tj = blockIdx.x;
+  const long long r0 = ti * 64, c0 = tj * 64;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int p = lane & 31, h = lane >> 5;
+  kmc_f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int k0 = 0; k0 < P.d; k0 += TK) {
+    __syncthreads();
+    for (int v = threadIdx.x; v < 64 * TK; v += 256) {
+      const int r = v / TK, q = v - r * TK;
+      const long long gi = r0 + r, gj = c0 + r;
+      const int kk = k0 + q;
+      As[r * RS + q] = (gi < P.n && kk < P.d) ? P.X[(size_t)gi * P.ldx + kk] : 0.0f;
+      Bs[r * RS + q] = (gj < P.n && kk < P.d) ? P.X[(size_t)gj * P.ldx + kk] : 0.0f;
+    }
+    __syncthreads();
+    // A: rows of this wave's i-block, B: rows of its j-block; lane half h
+    // takes k = 2*step + h
+#pragma unroll
+    for (int s = 0; s < TK / 2; ++s) {
+      const float a = As[(wr + p) * RS + 2 * s + h];
+      const float b = Bs[(wc + p) * RS + 2 * s + h];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+  }
+  // acc[r]: row (i) = 8*(r>>2) + 4*h + (r&3) of the A block, column (j) = p
+  const long long gj = c0 + wc + p;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const long long gi = r0 + wr + 8 * (r >> 2) + 4 * h + (r & 3);
+    if (gi >= P.Np || gj >= P.Np) continue;
+    float v = 0.0f;
+    if (gi < P.n && gj < P.n) {
+      const float xi2 = P.kind == KK_RBF ? P.xn2[gi] : 0.0f;
+      const float xj2 = P.kind == KK_RBF ? P.xn2[gj] : 0.0f;
+      v = kernel_fn(P.kind, acc[r], xi2, xj2, P.gamma, gi == gj);
+    }
+    const size_t off = (size_t)gi * P.Np + gj;
+    if (STORE == KS_FP32) reinterpret_cast<float*>(P.Kmat)[off] = v;
+    else reinterpret_cast<uint16_t*>(P.Kmat)[off] = bf16_rne(v);
+  }
+}
+
+// Diagonal of K (as stored), FP64 sum of it goes to the state.
+template <int STORE>
+__global__ __launch_bounds__(256) void k_kk_diag(KKParams P) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const size_t off = (size_t)i * P.Np + i;
+  P.kdiag[i] = STORE == KS_FP32 ? reinterpret_cast<const float*>(P.Kmat)[off]
+                                : bf16_to_f32(reinterpret_cast<const uint16_t*>(P.Kmat)[off]);
+}
+
+template <int STORE>
+__global__ __launch_bounds__(256) void k_kk_onehot(KKParams P) {
+  if (*P.done) return;
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= P.Np) return;
+  const int lab = j < P.n ? P.labels[j] : -1;
+  for (int c = 0; c < P.Kp; ++c) {
+    const bool on = (lab == c);
+    const size_t off = (size_t)c * P.Np + j;
+    if (STORE == KS_FP32) reinterpret_cast<float*>(P.HT)[off] = on ? 1.0f : 0.0f;
+    else reinterpret_cast<uint16_t*>(P.HT)[off] = on ? (uint16_t)0x3f80 : (uint16_t)0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_kk_count(KKParams P) {
+  if (*P.done) return;
+  __shared__ int cnt_s[KK_MAX_K];
+  for (int c = threadIdx.x; c < P.K; c += 256) cnt_s[c] = 0;
+  __syncthreads();
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.n;
+       i += (long long)gridDim.x * 256)
+    atomicAdd(&cnt_s[P.labels[i]], 1);
+  __syncthreads();
+  for (int c = threadIdx.x; c < P.K; c += 256)
+    if (cnt_s[c]) atomicAdd(&P.count[c], cnt_s[c]);
+}
+
+// One wave per 16-row block of K. MFMA 16x16: A = 16 rows x k columns of K,
+// B = k columns x 16 clusters of H; D[row][cluster]. Lane l: row/cluster
+// (l & 15), k-group g = l >> 4.
+//   fp32 (16x16x4): a 16-byte load K[row][j0 + 16q + 4g + 0..3] feeds steps
+//                   4q..4q+3 (column j0 + 16q + 4g + t at step t)
+//   bf16 (16x16x32): K[row][j0 + 8g + 0..7] per step, 32 columns per step
+// acc[r] of lane l holds S[row 4g + r][cluster l & 15].
+template <int STORE, int NB>
+__global__ __launch_bounds__(256) void k_kk_spmm(KKParams P) {
+  if (*P.done) return;
+  const int lane = threadIdx.x & 63;
+  const long long blk = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long nblk = (P.n + 15) / 16;
+  if (blk >= nblk) return;
+  const int m = lane & 15, g = lane >> 4;
+  const long long row = blk * 16 + m;  // < Np (K rows are padded to Np)
+  kmc_f32x4 acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = kmc_f32x4{0.f, 0.f, 0.f, 0.f};
+  if (STORE == KS_FP32) {
+    const float* krow = reinterpret_cast<const float*>(P.Kmat) + (size_t)row * P.Np + 4 * g;
+    const float* H = reinterpret_cast<const float*>(P.HT) + 4 * g;
+    for (long long j0 = 0; j0 < P.Np; j0 += 16) {
+      const float4 a = *reinterpret_cast<const float4*>(krow + j0);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const float4 hb =
+            *reinterpret_cast<const float4*>(H + (size_t)(16 * b + m) * P.Np + j0);
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, hb.x, acc[b], 0, 0, 0);
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, hb.y, acc[b], 0, 0, 0);
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, hb.z, acc[b], 0, 0, 0);
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, hb.w, acc[b], 0, 0, 0);
+      }
+    }
+  } else {
+    const uint16_t* krow = reinterpret_cast<const uint16_t*>(P.Kmat) + (size_t)row * P.Np + 8 * g;
+    const uint16_t* H = reinterpret_cast<const uint16_t*>(P.HT) + 8 * g;
+    for (long long j0 = 0; j0 < P.Np; j0 += 32) {
+      const uint4 a = *reinterpret_cast<const uint4*>(krow + j0);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const uint4 hb = *reinterpret_cast<const uint4*>(H + (size_t)(16 * b + m) * P.Np + j0);
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(kmc_bf16x8, a),
+                                                        __builtin_bit_cast(kmc_bf16x8, hb),
+                                                        acc[b], 0, 0, 0);
+      }
+    }
+  }
+  // write S and the partial self-terms of this block
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int c = 16 * b + m;
+    float part = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const long long i = blk * 16 + 4 * g + r;
+      if (i < P.n) {
+        P.S[(size_t)i * P.Kp + c] = acc[b][r];
+        if (P.labels[i] == c) part += acc[b][r];
+      }
+    }
+    // fixed-order combine of the 4 row groups
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    if (g == 0) P.gpart[(size_t)blk * P.Kp + c] = part;
+  }
+}
+
+// g_c = sum over row blocks (in order) of the partial self-terms, FP64.
+__global__ __launch_bounds__(256) void k_kk_gsum(KKParams P) {
+  if (*P.done) return;
+  const int c = blockIdx.x;
+  __shared__ double sd[256];
+  const long long nblk = (P.n + 15) / 16;
+  double s = 0.0;
+  for (long long b = threadIdx.x; b < nblk; b += 256) s += (double)P.gpart[(size_t)b * P.Kp + c];
+  sd[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sd[threadIdx.x] += sd[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) P.g[c] = sd[0];
+}
+
+// One thread per point.
+__global__ __launch_bounds__(256) void k_kk_assign(KKParams P) {
+  if (*P.done) return;
+  __shared__ float cnorm_s[KK_MAX_K], inv_s[KK_MAX_K];
+  __shared__ double red_s[256];
+  __shared__ int chg_s[256];
+  for (int c = threadIdx.x; c < P.K; c += 256) {
+    const int n_c = P.count[c];
+    const double gc = P.g[c];
+    // empty cluster, or a zero centroid norm, is never chosen
+    // (src/kernels/distances.cu:588-598)
+    const bool dead = (n_c == 0) || (gc == 0.0);
+    cnorm_s[c] = dead ? __builtin_inff() : (float)(gc / ((double)n_c * (double)n_c));
+    inv_s[c] = n_c ? (float)(2.0 / (double)n_c) : 0.0f;
+  }
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  double obj = 0.0;
+  int changed = 0;
+  if (i < P.n) {
+    const float* s = P.S + (size_t)i * P.Kp;
+    float best = __builtin_inff();
+    int bi = 0;
+    for (int c = 0; c < P.K; ++c) {
+      const float v = cnorm_s[c] - inv_s[c] * s[c];
+      if (v < best) { best = v; bi = c; }
+    }
+    obj = (double)P.kdiag[i] + (double)best;
+    changed = (bi != P.labels[i]);
+    P.labels[i] = bi;
+  }
+  red_s[threadIdx.x] = obj;
+  chg_s[threadIdx.x] = changed;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      red_s[threadIdx.x] += red_s[threadIdx.x + o];
+      chg_s[threadIdx.x] += chg_s[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    P.opart[blockIdx.x] = red_s[0];
+    P.cpart[blockIdx.x] = chg_s[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_kk_stats(KKParams P) {
+  if (*P.done) return;
+  __shared__ double sd[256];
+  __shared__ long long sc[256];
+  double s = 0.0;
+  long long ch = 0;
+  for (int b = threadIdx.x; b < P.nassign; b += 256) {
+    s += P.opart[b];
+    ch += P.cpart[b];
+  }
+  sd[threadIdx.x] = s;
+  sc[threadIdx.x] = ch;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sd[threadIdx.x] += sd[threadIdx.x + o];
+      sc[threadIdx.x] += sc[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    KKState* st = P.state;
+    const unsigned long long it = st->iter + 1;
+    st->iter = it;
+    st->objective = sd[0];
+    st->changed = (unsigned long long)sc[0];
+    const double score = sd[0] - st->sum_diag;
+    if (P.mode == CONV_SCORE && it >= 2 && it < st->maxiter &&
+        fabsf((float)score - (float)st->last_score) < P.tol) {
+      st->converged_at = it;
+      st->done = 1;
+    }
+    if (P.mode == CONV_STABLE && sc[0] == 0) {
+      st->converged_at = it;
+      st->done = 1;
+    }
+    st->last_score = score;
+    if (it >= st->maxiter) st->done = 1;
+  }
+}
+
+// Distances as the reference materialises them (src/kernels/distances.cu:
+// 895-1002, without the K_ii term): D[i,c] = g_c/|c|^2 - 2 S[i,c]/|c|,
+// +inf for an empty cluster or a zero norm.
+__global__ __launch_bounds__(256) void k_kk_dist_out(KKParams P) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.n) return;
+  for (int c = 0; c < P.K; ++c) {
+    const int n_c = P.count[c];
+    const double gc = P.g[c];
+    float v = __builtin_inff();
+    if (n_c != 0 && gc != 0.0)
+      v = (float)(gc / ((double)n_c * (double)n_c)) -
+          (float)(2.0 / (double)n_c) * P.S[(size_t)i * P.Kp + c];
+    P.Dout[(size_t)i * P.K + c] = v;
+  }
+}
+
+__global__ void k_kk_zero(KKParams P) {
+  if (*P.done) return;
+  for (int c = threadIdx.x; c < P.K; c += blockDim.x) P.count[c] = 0;
+}
+
+// ---------------------------------------------------------------------------
+hipError_t kk_launch_setup(const KKParams& P, hipStream_t s) {
+  hipLaunchKernelGGL(k_kk_norms, dim3((unsigned)((P.n + 3) / 4)), dim3(256), 0, s, P);
+  const unsigned T = (unsigned)((P.Np + 63) / 64);
+  if (P.store == KS_FP32) {
+    hipLaunchKernelGGL(k_kk_build<KS_FP32>, dim3(T, T), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_kk_diag<KS_FP32>, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, s, P);
+  } else {
+    hipLaunchKernelGGL(k_kk_build<KS_BF16>, dim3(T, T), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_kk_diag<KS_BF16>, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, s, P);
+  }
+  return hipGetLastError();
+}
+
+template <int STORE>
+static void launch_spmm_t(const KKParams& P, unsigned grid, hipStream_t s) {
+  switch (P.Kp / 16) {
+    case 1: hipLaunchKernelGGL((k_kk_spmm<STORE, 1>), dim3(grid), dim3(256), 0, s, P); break;
+    case 2: hipLaunchKernelGGL((k_kk_spmm<STORE, 2>), dim3(grid), dim3(256), 0, s, P); break;
+    default: hipLaunchKernelGGL((k_kk_spmm<STORE, 4>), dim3(grid), dim3(256), 0, s, P); break;
+  }
+}
+
+hipError_t kk_launch_distances(const KKParams& P, hipStream_t s) {
+  const unsigned gj = (unsigned)((P.Np + 255) / 256);
+  if (P.store == KS_FP32) hipLaunchKernelGGL(k_kk_onehot<KS_FP32>, dim3(gj), dim3(256), 0, s, P);
+  else hipLaunchKernelGGL(k_kk_onehot<KS_BF16>, dim3(gj), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_kk_zero, dim3(1), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_kk_count, dim3(64), dim3(256), 0, s, P);
+  const unsigned gs = (unsigned)(((P.n + 15) / 16 + 3) / 4);
+  if (P.store == KS_FP32) launch_spmm_t<KS_FP32>(P, gs, s);
+  else launch_spmm_t<KS_BF16>(P, gs, s);
+  hipLaunchKernelGGL(k_kk_gsum, dim3(P.K), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_kk_dist_out, dim3(P.nassign), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+hipError_t kk_launch_iter(const KKParams& P, hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+  const unsigned gj = (unsigned)((P.Np + 255) / 256);
+  if (P.store == KS_FP32) hipLaunchKernelGGL(k_kk_onehot<KS_FP32>, dim3(gj), dim3(256), 0, s, P);
+  else hipLaunchKernelGGL(k_kk_onehot<KS_BF16>, dim3(gj), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_kk_zero, dim3(1), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_kk_count, dim3(64), dim3(256), 0, s, P);
+  const unsigned gs = (unsigned)(((P.n + 15) / 16 + 3) / 4);
+  if (e0) (void)hipEventRecord(e0, s);
+  if (P.store == KS_FP32) launch_spmm_t<KS_FP32>(P, gs, s);
+  else launch_spmm_t<KS_BF16>(P, gs, s);
+  if (e1) (void)hipEventRecord(e1, s);
+  hipLaunchKernelGGL(k_kk_gsum, dim3(P.K), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_kk_assign, dim3(P.nassign), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_kk_stats, dim3(1), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+}  // namespace kmc

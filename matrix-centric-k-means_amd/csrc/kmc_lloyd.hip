@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams P) {
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0) P.cn[c] = (c < P.K) ? s : __builtin_inff();
+  if (lane == 0) P.cn[c] = (c < P.K && P.alive[c]) ? s : __builtin_inff();
 }
 
 // ---------------------------------------------------------------------------
@@ -142,7 +142,10 @@ __global__ __launch_bounds__(256) void k_assign(AssignParams P) {
 
   // max ||c|| over the real centroids (for the error bound)
   float cm = 0.0f;
-  for (int c = threadIdx.x; c < P.K; c += blockDim.x) cm = fmaxf(cm, P.cn[c]);
+  for (int c = threadIdx.x; c < P.K; c += blockDim.x) {
+    const float v = P.cn[c];
+    if (v < __builtin_inff()) cm = fmaxf(cm, v);  // excluded clusters are +inf
+  }
   cm = wave_max_f32(cm);
   if (lane == 0) red_s[wave] = cm;
   __syncthreads();
@@ -338,6 +341,7 @@ __global__ __launch_bounds__(256) void k_refine_all(RefineParams P) {
     float best = __builtin_inff();
     int bi = 0x7fffffff;
     for (int c = lane; c < P.K; c += 64) {
+      if (!P.alive[c]) continue;
       const float r = direct_dist_f32(x, P.C + (size_t)c * P.DP, P.D);
       if (r < best) { best = r; bi = c; }
     }
@@ -529,6 +533,7 @@ __global__ __launch_bounds__(256) void k_finish(FinishParams P) {
   P.conv[c] = (__fsqrt_rn(s) > P.tol) ? 0 : 1;
   P.shift2[c] = s64;
   for (int j = 0; j < P.D; ++j) cr[j] = nr[j];
+  if (P.exclude_empty && P.count[c] == 0) P.alive[c] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -566,10 +571,20 @@ __global__ __launch_bounds__(256) void k_stats(StatsParams P) {
     s->max_shift2 = sm[0];
     s->refined_two = P.qcnt[0];
     s->refined_all = P.qcnt[1];
-    if (P.check && it >= 2 && sc[0]) {
+    const double score = sd[0] - s->sum_x2;
+    if (P.mode == CONV_SHIFT && it >= 2 && sc[0]) {
+      // comparisons/CPU_Kmeans/src/kmeans.cpp:205-208: tested from iteration 2
       s->converged_at = it;
       s->done = 1;
     }
+    if (P.mode == CONV_SCORE && it >= 2 && it < s->maxiter &&
+        fabsf((float)score - (float)s->last_score) < P.tol) {
+      // src/kmeans.cu:1064-1073: float scores, tested from iteration 2,
+      // never at the last iteration
+      s->converged_at = it;
+      s->done = 1;
+    }
+    s->last_score = score;
     if (it >= s->maxiter) s->done = 1;
   }
 }

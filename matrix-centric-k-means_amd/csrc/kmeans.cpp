@@ -1,0 +1,138 @@
+// kmeans.cpp -- the drop-in Kmeans class (include/kmeans.hpp) over the C ABI.
+#include "kmeans.hpp"
+
+#include <random>
+#include <string>
+#include <vector>
+
+namespace kmc {
+
+bool query_device(int ordinal, DeviceInfo* info) {
+  const int count = kmc_device_count();
+  if (info) {
+    info->ordinal = ordinal;
+    info->count = count;
+  }
+  return ordinal >= 0 && ordinal < count;
+}
+
+}  // namespace kmc
+
+namespace {
+
+[[noreturn]] void raise(kmc_handle* h, kmc_status s, const char* what) {
+  std::string msg = std::string(what) + ": " + kmc_status_string(s);
+  if (h) {
+    const char* e = kmc_last_error(h);
+    if (e && *e) msg += std::string(" (") + e + ")";
+  }
+  throw kmc::Error(s, msg);
+}
+
+void check(kmc_handle* h, kmc_status s, const char* what) {
+  if (s != KMC_OK) raise(h, s, what);
+}
+
+}  // namespace
+
+Kmeans::Kmeans(const size_t _n, const uint32_t _d, const uint32_t _k, const float _tol,
+               const int* seed, Point<DATA_TYPE>** _points, kmc::DeviceInfo* _deviceProps,
+               InitMethod _initMethod, Kernel _kernel, int _level)
+    : Kmeans(_n, _d, _k, _tol, seed, _points, _deviceProps, _initMethod, _kernel, _level,
+             Options()) {}
+
+Kmeans::Kmeans(const size_t _n, const uint32_t _d, const uint32_t _k, const float _tol,
+               const int* seed, Point<DATA_TYPE>** _points, kmc::DeviceInfo* _deviceProps,
+               InitMethod _initMethod, Kernel _kernel, int _level, const Options& _opts)
+    : n(_n), d(_d), k(_k), tol(_tol), points(_points), deviceProps(_deviceProps),
+      initMethod(_initMethod), kernel(_kernel), level(_level), opts(_opts) {
+  init(seed);
+}
+
+void Kmeans::init(const int* seed) {
+  if (!points || n == 0 || d == 0 || k == 0 || k > n)
+    throw kmc::Error(KMC_ERR_ARG, "Kmeans: invalid n/d/k or points");
+  if (level < 0 || level > 4) throw kmc::Error(KMC_ERR_ARG, "Kmeans: level must be 0..4");
+  if (kernel != Kernel::linear)
+    throw kmc::Error(KMC_ERR_UNSUPPORTED,
+                     "Kmeans: kernel k-means runs through kmc_kernel_* (see kmc.h)");
+  if (initMethod == InitMethod::plus_plus)
+    throw kmc::Error(KMC_ERR_UNSUPPORTED, "Kmeans: k-means++ initialisation is not built");
+
+  kmc_config cfg{};
+  cfg.n = n;
+  cfg.d = d;
+  cfg.k = k;
+  cfg.tol = tol;
+  cfg.precision = opts.precision == Precision::fp32     ? KMC_PREC_FP32
+                  : opts.precision == Precision::bf16x3 ? KMC_PREC_BF16X3
+                                                        : KMC_PREC_BF16;
+  cfg.update = KMC_UPDATE_ORDERED;
+  cfg.device = deviceProps ? deviceProps->ordinal : 0;
+  check(nullptr, kmc_create(&cfg, &h), "kmc_create");
+
+  // Pack the borrowed points (src/kmeans.cu:94-99).
+  std::vector<float> X(n * (size_t)d);
+  for (size_t i = 0; i < n; ++i)
+    for (uint32_t j = 0; j < d; ++j) X[i * d + j] = points[i]->get(j);
+  check(h, kmc_set_points(h, X.data(), d), "kmc_set_points");
+
+  if (opts.reference_gpu) {
+    // init_centroids_rand (src/kmeans.cu:365-444): labels i % k, the
+    // initial centroids are the label means.
+    std::vector<int32_t> lab(n);
+    for (size_t i = 0; i < n; ++i) lab[i] = (int32_t)(i % k);
+    check(h, kmc_set_labels(h, lab.data()), "kmc_set_labels");
+  } else {
+    // k distinct rows drawn with mt19937(seed_seq{seed}) as the reference
+    // CPU k-means does (comparisons/CPU_Kmeans/src/kmeans.cpp:57-64,84-119).
+    std::mt19937 gen;
+    if (seed) {
+      std::seed_seq ss{*seed};
+      gen.seed(ss);
+    } else {
+      std::random_device rd;
+      gen.seed(rd());
+    }
+    std::uniform_int_distribution<int> pick(0, (int)n - 1);
+    std::vector<size_t> used;
+    std::vector<float> C0((size_t)k * d);
+    uint64_t draws = 0;
+    const uint64_t max_draws = 64ull * n + 1024ull * k;
+    while (used.size() < k) {
+      if (draws++ > max_draws) throw kmc::Error(KMC_ERR_ARG, "Kmeans: fewer than k distinct points");
+      const size_t r = (size_t)pick(gen);
+      bool dup = false;
+      for (size_t u : used)
+        if (*points[u] == *points[r]) { dup = true; break; }
+      if (dup) continue;
+      for (uint32_t j = 0; j < d; ++j) C0[used.size() * d + j] = X[r * d + j];
+      used.push_back(r);
+    }
+    check(h, kmc_set_centroids(h, C0.data()), "kmc_set_centroids");
+  }
+}
+
+Kmeans::~Kmeans() {
+  if (h) kmc_destroy(h);
+}
+
+uint64_t Kmeans::run(uint64_t maxiter, bool check_converged) {
+  uint32_t flags = check_converged ? KMC_RUN_CHECK_CONVERGED : 0u;
+  if (opts.reference_gpu) flags |= KMC_RUN_CONVERGE_SCORE | KMC_RUN_EXCLUDE_EMPTY;
+  kmc_result res{};
+  check(h, kmc_run(h, maxiter, flags, &res), "kmc_run");
+  score = (float)res.score;
+  inertia = res.inertia;
+  iterations = res.iterations;
+  std::vector<int32_t> lab(n);
+  check(h, kmc_get_labels(h, lab.data()), "kmc_get_labels");
+  for (size_t i = 0; i < n; ++i) points[i]->setCluster((unsigned int)lab[i]);
+  return res.converged_at;
+}
+
+std::vector<float> Kmeans::get_centroids() const {
+  std::vector<float> C((size_t)k * d);
+  check(h, kmc_get_centroids(h, C.data()), "kmc_get_centroids");
+  return C;
+}

@@ -50,6 +50,29 @@ class _Result(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class _KKConfig(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_size_t), ("d", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("tol", ctypes.c_float), ("kernel", ctypes.c_int32), ("store", ctypes.c_int32),
+                ("gamma", ctypes.c_float), ("device", ctypes.c_int32)]
+
+
+class _KKResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("converged_at", ctypes.c_uint64),
+                ("objective", ctypes.c_double), ("score", ctypes.c_double),
+                ("changed", ctypes.c_uint64), ("ms_total", ctypes.c_double),
+                ("ms_build", ctypes.c_double), ("ms_spmm", ctypes.c_double),
+                ("spmm_launches", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+KERNELS = {"linear": 0, "polynomial": 1, "sigmoid": 2, "rbf": 3}
+STORES = {"fp32": 0, "bf16": 1}
+RUN_CONVERGE_SCORE = 0x4
+RUN_EXCLUDE_EMPTY = 0x8
+RUN_STOP_STABLE = 0x10
+
 _lib = None
 
 
@@ -78,6 +101,16 @@ def load_library(path=LIB_PATH):
         "kmc_last_error": (ctypes.c_char_p, [vp]),
         "kmc_status_string": (ctypes.c_char_p, [ctypes.c_int]),
         "kmc_device_count": (ctypes.c_int, []),
+        "kmc_memory_footprint": (ctypes.c_int, [ctypes.c_int, P(ctypes.c_size_t)]),
+        "kmc_kk_create": (ctypes.c_int, [P(_KKConfig), P(vp)]),
+        "kmc_kk_destroy": (ctypes.c_int, [vp]),
+        "kmc_kk_set_points": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+        "kmc_kk_set_labels": (ctypes.c_int, [vp, vp]),
+        "kmc_kk_run": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32, P(_KKResult)]),
+        "kmc_kk_get_labels": (ctypes.c_int, [vp, vp]),
+        "kmc_kk_distances": (ctypes.c_int, [vp, vp, vp]),
+        "kmc_kk_get_kernel_rows": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.c_size_t, vp]),
+        "kmc_kk_last_error": (ctypes.c_char_p, [vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -205,3 +238,76 @@ def fit(X, C0, maxiter, tol=1e-4, precision="fp32", check_converged=True):
         e.set_centroids(C0)
         res = e.run(maxiter, check_converged=check_converged)
         return e.labels(), e.centroids(), res
+
+
+class KernelEngine:
+    """Kernel k-means on one GPU (the reference's Popcorn path): the n x n
+    kernel matrix K = f(X X^T) is built once on the GPU and streamed every
+    iteration (src/kmeans.cu:71-305, 772-1122)."""
+
+    def __init__(self, n, d, k, kernel="polynomial", store="fp32", gamma=0.0, tol=1e-4,
+                 device=0):
+        self._lib = load_library()
+        self.n, self.d, self.k = int(n), int(d), int(k)
+        cfg = _KKConfig(self.n, self.d, self.k, float(tol), KERNELS[kernel], STORES[store],
+                        float(gamma), int(device))
+        h = ctypes.c_void_p()
+        st = self._lib.kmc_kk_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st != OK:
+            raise KmcError(st, "kmc_kk_create(n=%d, d=%d, k=%d, %s, %s)" % (n, d, k, kernel, store))
+        self._h = h
+
+    def _check(self, st, what):
+        if st != OK:
+            msg = self._lib.kmc_kk_last_error(self._h)
+            raise KmcError(st, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.kmc_kk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_points(self, X):
+        X = _f32(X, (self.n, self.d))
+        self._check(self._lib.kmc_kk_set_points(self._h, X.ctypes.data, self.d), "kk_set_points")
+
+    def set_labels(self, labels):
+        lab = np.ascontiguousarray(labels, dtype=np.int32)
+        self._check(self._lib.kmc_kk_set_labels(self._h, lab.ctypes.data), "kk_set_labels")
+
+    def run(self, maxiter, check_converged=False, stop_stable=False, time_kernels=False):
+        flags = (RUN_CHECK_CONVERGED if check_converged else 0) | \
+                (RUN_STOP_STABLE if stop_stable else 0) | (RUN_TIME_KERNELS if time_kernels else 0)
+        r = _KKResult()
+        self._check(self._lib.kmc_kk_run(self._h, int(maxiter), flags, ctypes.byref(r)), "kk_run")
+        return r.as_dict()
+
+    def labels(self):
+        out = np.empty(self.n, dtype=np.int32)
+        self._check(self._lib.kmc_kk_get_labels(self._h, out.ctypes.data), "kk_get_labels")
+        return out
+
+    def distances(self, labels):
+        lab = np.ascontiguousarray(labels, dtype=np.int32)
+        D = np.empty((self.n, self.k), dtype=np.float32)
+        self._check(self._lib.kmc_kk_distances(self._h, lab.ctypes.data, D.ctypes.data),
+                    "kk_distances")
+        return D
+
+    def kernel_rows(self, row0, nrows):
+        out = np.empty((nrows, self.n), dtype=np.float32)
+        self._check(self._lib.kmc_kk_get_kernel_rows(self._h, row0, nrows, out.ctypes.data),
+                    "kk_get_kernel_rows")
+        return out

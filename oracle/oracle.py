@@ -171,6 +171,8 @@ class Reference:
         L.kref_step.argtypes = [_P, _sz, _u32, _u32, _P, _P, _P, _P, _P]
         L.kref_converged.restype = ctypes.c_int
         L.kref_converged.argtypes = [_P, _sz, _u32, _u32, ctypes.c_float, _P, _P]
+        L.kref_time_run.restype = ctypes.c_double
+        L.kref_time_run.argtypes = [_P, _sz, _u32, _u32, _u64, _P, _P]
         self.L = L
 
     @staticmethod
@@ -207,6 +209,14 @@ class Reference:
         Cn = np.empty((k, d), np.float32)
         self.L.kref_step(_ptr(X), n, d, k, _ptr(C), _ptr(D), _ptr(lab), _ptr(cnt), _ptr(Cn))
         return D, lab, cnt, Cn
+
+    def time_run(self, X, C0, maxiter):
+        """Seconds spent in the reference's Kmeans::run(maxiter) alone."""
+        X = np.ascontiguousarray(X, np.float32)
+        C0 = np.ascontiguousarray(C0, np.float32)
+        conv = np.zeros(1, np.uint64)
+        return self.L.kref_time_run(_ptr(X), X.shape[0], X.shape[1], C0.shape[0], maxiter,
+                                    _ptr(C0), _ptr(conv))
 
     def converged(self, X, C, Clast, tol):
         X = np.ascontiguousarray(X, np.float32)

@@ -97,3 +97,25 @@ int kref_converged(const float* X, size_t n, uint32_t d, uint32_t k, float tol,
 }
 
 }  // extern "C"
+
+#include <chrono>
+
+extern "C" {
+
+// Times Kmeans::run(maxiter) alone (as comparisons/CPU_Kmeans/src/main.cpp:
+// 284-290 does), starting from caller-given centroids; returns seconds.
+double kref_time_run(const float* X, size_t n, uint32_t d, uint32_t k,
+                     uint64_t maxiter, const float* C0, uint64_t* converged) {
+  Points pts(X, n, d);
+  int seed = 1;
+  Kmeans km(n, d, k, 0.0f, &seed, pts.p.data());
+  std::memcpy(km.h_centroids, C0, sizeof(float) * (size_t)k * d);
+  std::memcpy(km.h_last_centroids, C0, sizeof(float) * (size_t)k * d);
+  const auto t0 = std::chrono::high_resolution_clock::now();
+  const uint64_t conv = km.run(maxiter);
+  const auto t1 = std::chrono::high_resolution_clock::now();
+  if (converged) *converged = conv;
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+"""GPU parity of kernel k-means (the reference's Popcorn path) through the C ABI.
+
+The reference's GPU arithmetic (cuBLAS GEMM/SYRK, cuSPARSE SpMM/SpMV, RAFT
+argmin) cannot run here, so the oracle is the FP64 restatement
+(oracle/kmc_oracle.cpp, orc_kk_*), pinned in tests/test_oracle.py against the
+reference unit test's own host formula. Tolerances:
+  kernel matrix   fp32 store: |K - K64| <= 2e-6 * (|K64| + scale)
+                  bf16 store: |K - K64| <= 2**-8 * |K64| + 2e-6 * scale
+  distances       |D - D64| <= tol_rel * (max|K| * 4)
+  labels          identical to the FP64 argmin wherever the FP64 top-2 margin
+                  exceeds twice the distance tolerance
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = ["linear", "polynomial", "sigmoid", "rbf"]
+KIND = {"linear": 0, "polynomial": 1, "sigmoid": 2, "rbf": 3}
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _data(n, d, k, seed):
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0, 1, size=(k, d))
+    lab = rng.integers(0, k, size=n)
+    X = np.clip(centres[lab] + 0.1 * rng.standard_normal((n, d)), 0, 1).astype(np.float32)
+    X[rng.random((n, d)) < 0.5] = 0.0
+    return X
+
+
+@pytest.mark.parametrize("store", ["fp32", "bf16"])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_kernel_matrix_and_distances(kmc_mod, oracle, kernel, store):
+    n, d, k = 333, 37, 5
+    X = _data(n, d, k, 1 + KIND[kernel])
+    gamma = 1.0 / d
+    _say("kk", kernel, store, n, d, k)
+    K64 = oracle.kernel_matrix(X, KIND[kernel], gamma)
+    scale = np.abs(K64).max()
+    with kmc_mod.KernelEngine(n, d, k, kernel=kernel, store=store) as e:
+        e.set_points(X)
+        Kg = e.kernel_rows(0, n)
+        if store == "fp32":
+            tol = 2e-6 * (np.abs(K64) + scale)
+        else:
+            tol = 2.0 ** -8 * np.abs(K64) + 2e-6 * scale
+        assert np.all(np.abs(Kg - K64) <= tol), np.max(np.abs(Kg - K64) - tol)
+        labels = (np.arange(n) % k).astype(np.int32)
+        labels[labels == k - 1] = 0  # cluster k-1 empty: +inf everywhere
+        D = e.distances(labels)
+        D64 = oracle.kk_distances(K64.astype(np.float64), labels.astype(np.uint32), k,
+                                  add_diag=False)
+        assert np.all(np.isinf(D[:, k - 1])) and np.all(np.isinf(D64[:, k - 1]))
+        fin = np.isfinite(D64)
+        dtol = (1e-5 if store == "fp32" else 2e-2) * 4 * scale
+        assert np.all(np.abs(D[fin] - D64[fin]) <= dtol), np.max(np.abs(D[fin] - D64[fin]))
+
+
+@pytest.mark.parametrize("store", ["fp32", "bf16"])
+@pytest.mark.parametrize("kernel", ["polynomial", "rbf"])
+def test_one_iteration_labels(kmc_mod, oracle, kernel, store):
+    n, d, k = 1000, 24, 7
+    X = _data(n, d, k, 11)
+    K64 = oracle.kernel_matrix(X, KIND[kernel], 1.0 / d)
+    scale = np.abs(K64).max()
+    lab0 = (np.arange(n) % k).astype(np.int32)
+    _say("kk one iteration", kernel, store)
+    with kmc_mod.KernelEngine(n, d, k, kernel=kernel, store=store) as e:
+        e.set_points(X)
+        e.set_labels(lab0)
+        e.run(1)
+        lab = e.labels()
+    D64 = oracle.kk_distances(K64, lab0.astype(np.uint32), k, add_diag=True)
+    want = np.argmin(D64, axis=1)
+    srt = np.sort(D64, axis=1)
+    margin = srt[:, 1] - srt[:, 0]
+    dtol = (1e-5 if store == "fp32" else 2e-2) * 4 * scale
+    safe = margin > 2 * dtol
+    assert np.array_equal(lab[safe], want[safe])
+    assert safe.mean() > 0.5
+
+
+def test_linear_kernel_is_one_lloyd_step(kmc_mod, oracle):
+    """With the linear kernel, one kernel k-means iteration from labels L is a
+    Lloyd assignment against the means of L."""
+    n, d, k = 2000, 16, 9
+    from kmc import datagen
+    X, _, _ = datagen.blobs(n, d, k, 5, -4, 4, 0.5, 1.5)
+    lab0 = (np.arange(n) % k).astype(np.int32)
+    C, _ = oracle.f64_centroids(X, lab0.astype(np.uint32), k)
+    lab_ll, margin, _ = oracle.f64_assign(X, C)
+    with kmc_mod.KernelEngine(n, d, k, kernel="linear", store="fp32") as e:
+        e.set_points(X)
+        e.set_labels(lab0)
+        e.run(1)
+        lab = e.labels()
+    safe = margin > 1e-2
+    assert np.array_equal(lab[safe], lab_ll[safe].astype(np.int32))
+
+
+@pytest.mark.parametrize("store", ["fp32", "bf16"])
+def test_run_to_stability_matches_fp64(kmc_mod, oracle, store):
+    n, d, k = 800, 20, 6
+    X = _data(n, d, k, 21)
+    K64 = oracle.kernel_matrix(X, 1, 0.0)
+    lab0 = (np.arange(n) % k).astype(np.int32)
+    it64, lab64 = oracle.kk_lloyd(K64, lab0.astype(np.uint32), k, 100)
+    _say("kk run", store)
+    with kmc_mod.KernelEngine(n, d, k, kernel="polynomial", store=store) as e:
+        e.set_points(X)
+        e.set_labels(lab0)
+        res = e.run(100, check_converged=True, stop_stable=True)
+        lab = e.labels()
+    agree = np.mean(lab == lab64.astype(np.int32))
+    assert agree >= (0.999 if store == "fp32" else 0.98), (agree, res, it64)
+    D64 = oracle.kk_distances(K64, lab.astype(np.uint32), k, add_diag=True)
+    obj64 = D64[np.arange(n), lab].sum()
+    assert abs(res["objective"] - obj64) <= (1e-5 if store == "fp32" else 1e-2) * abs(obj64)
