@@ -47,15 +47,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(kmc, X, C0, prec, steps, warmup, sync):
-    with kmc.Engine(N, D, K, tol=0.0, precision=prec) as e:
+def run_form(kmc, X, C0, prec, steps, warmup, sync, update, graph):
+    with kmc.Engine(N, D, K, tol=0.0, precision=prec, update=update) as e:
         e.set_points(X)
         e.set_centroids(C0)
+        tuning = e.autotune()
+        log("%s autotune: variant %d, ms %s" % (prec, tuning["chosen"],
+                                               [round(v, 4) for v in tuning["ms"]]))
         if warmup:
-            e.run(warmup, check_converged=False)
+            e.run(warmup, check_converged=False, graph=graph)
+        # capture the timed chunk once (the capture is set-up, not a step):
+        # run it, then restore the starting centroids and replay it timed
+        C_start = e.centroids()
+        if graph:
+            e.run(steps, check_converged=False, time_kernels=True, graph=True)
+            e.set_centroids(C_start)
         sync()
         t0 = time.perf_counter()
-        res = e.run(steps, check_converged=False, time_kernels=True)
+        res = e.run(steps, check_converged=False, time_kernels=True, graph=graph)
         sync()
         wall = time.perf_counter() - t0
         C_end = e.centroids()
@@ -64,7 +73,7 @@ def run_form(kmc, X, C0, prec, steps, warmup, sync):
         lab_f, _ = e.assign(C_end)
         _, _, upd = e.update(lab_f, C_end)
     return {"wall_s": wall, "res": res, "C_end": C_end, "labels_end": lab_end,
-            "labels_final": lab_f, "inertia_gpu": upd["inertia"]}
+            "labels_final": lab_f, "inertia_gpu": upd["inertia"], "tuning": tuning}
 
 
 def cpu_baseline(X, C0, n_sample=100_000, iters=2):
@@ -115,6 +124,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "bf16", "fp32"])
+    ap.add_argument("--update", default="ordered", choices=["ordered", "tree"],
+                    help="centroid update: reference summation order, or FP64 tree")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--profile-copy", type=int, default=0,
@@ -143,11 +156,11 @@ def main():
     C0 = datagen.init_rows(X, K, INIT_SEED)
 
     log("timing %s: warmup %d, steps %d" % (a.prec, a.warmup, a.steps))
-    head = run_form(kmc, X, C0, a.prec, a.steps, a.warmup, sync)
+    head = run_form(kmc, X, C0, a.prec, a.steps, a.warmup, sync, a.update, not a.eager)
     fp32 = None
     if not a.no_fp32 and a.prec != "fp32":
         log("timing fp32: warmup %d, steps %d" % (a.warmup, a.steps))
-        fp32 = run_form(kmc, X, C0, "fp32", a.steps, a.warmup, sync)
+        fp32 = run_form(kmc, X, C0, "fp32", a.steps, a.warmup, sync, a.update, not a.eager)
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
@@ -182,13 +195,17 @@ def main():
                                "maxiter=50 tol=0",
                    "n": N, "d": D, "k": K, "form": a.prec, "seed": DATA_SEED,
                    "init": "256 distinct seed-chosen rows (seed %d)" % INIT_SEED,
-                   "update": "ordered (reference summation order)"},
+                   "update": {"ordered": "ordered (reference summation order)",
+                              "tree": "tree (FP64 chunk partials)"}[a.update]},
         "roofline": {"bound": "mfma", "kernel": "k_assign<128,%s>" % a.prec,
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK[a.prec] / 1e12,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK[a.prec], 4),
                      "traffic": traffic,
                      "algorithmic_flops_per_launch": flops,
                      "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches},
+        "assign_variant": {"chosen": head["tuning"]["chosen"],
+                           "threads": head["tuning"]["threads"][head["tuning"]["chosen"]],
+                           "autotune_ms": [round(v, 4) for v in head["tuning"]["ms"]]},
         "inertia_rel_err_vs_fp64": inert_err,
         "inertia_fp64": inert64,
         "refined_points_last_iter": {"two_candidates": int(r["refined_two"]),

@@ -57,7 +57,11 @@ typedef enum {
 typedef enum {
   /* per cluster, float sums in point-index order: bit-identical to
    * comparisons/CPU_Kmeans/src/kmeans.cpp:166-180 */
-  KMC_UPDATE_ORDERED = 0
+  KMC_UPDATE_ORDERED = 0,
+  /* per cluster, FP64 partial sums over fixed chunks of the sorted member
+   * list, combined in chunk order and rounded once: deterministic, at least
+   * as accurate, every CU busy; not bit-identical to the float sums */
+  KMC_UPDATE_TREE = 1
 } kmc_update_mode;
 
 /* kmc_run flags */
@@ -69,6 +73,10 @@ typedef enum {
                                         |score_t - score_{t-1}| < tol instead
                                         (the reference GPU engine's test,
                                         src/kmeans.cu:1064-1073)              */
+#define KMC_RUN_GRAPH 0x20u          /* capture the iterations between host
+                                        convergence checks into a HIP graph
+                                        (kept on the handle and replayed by
+                                        later runs of the same shape)        */
 #define KMC_RUN_EXCLUDE_EMPTY 0x8u   /* a cluster left empty by an assignment
                                         is never chosen again (the reference
                                         GPU engine gives it distance +inf,
@@ -153,6 +161,24 @@ kmc_status kmc_assign(kmc_handle* h, const float* C, int32_t* labels,
  * (comparisons/CPU_Kmeans/src/kmeans.cpp:166-180). */
 kmc_status kmc_update(kmc_handle* h, const int32_t* labels, const float* C_old,
                       float* C_new, uint32_t* counts, kmc_result* res);
+
+/* Launch-variant autotuner for the fused distance kernel (no counterpart in
+ * the reference, whose stages are fixed library calls): times every variant
+ * (waves per workgroup, register prefetch) on the handle's points and
+ * current centroids and keeps the fastest for later kmc_run/kmc_assign. */
+#define KMC_MAX_VARIANTS 8
+typedef struct {
+  int32_t nvariants;
+  int32_t chosen;
+  float ms[KMC_MAX_VARIANTS];     /* mean kernel time per variant */
+  int32_t threads[KMC_MAX_VARIANTS];
+  int32_t tile[KMC_MAX_VARIANTS]; /* points per workgroup tile */
+} kmc_tuning;
+
+kmc_status kmc_autotune(kmc_handle* h, kmc_tuning* out);
+
+/* Select a launch variant of the fused distance kernel directly. */
+kmc_status kmc_set_assign_variant(kmc_handle* h, int variant);
 
 /* Release every device buffer (src/kmeans.cu:307-359). */
 kmc_status kmc_destroy(kmc_handle* h);

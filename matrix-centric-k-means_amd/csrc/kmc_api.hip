@@ -34,19 +34,24 @@ struct kmc_handle {
   uint16_t* Chi = nullptr;
   uint16_t* Clo = nullptr;
   float* cn = nullptr;
+  float* CT = nullptr;
   // assignment
   int* labels = nullptr;
   int* q2 = nullptr;
   int* qa = nullptr;
-  unsigned* qcnt = nullptr;
+  unsigned* q2cnt = nullptr;
+  unsigned* qacnt = nullptr;
+  long long qstride = 0;
   // sort / update
-  int CH = 0, nblk = 0;
+  int CH = 0, nblk = 0, lbits = 0;
   int* bcount = nullptr;
   int* count = nullptr;
   int* start = nullptr;
   int* members = nullptr;
   double* ipart = nullptr;
+  double* tpart = nullptr;
   int nparts = 0;
+  int update_mode = 0;
   int* conv = nullptr;
   double* shift2 = nullptr;
   int* alive = nullptr;
@@ -56,10 +61,23 @@ struct kmc_handle {
   int kchunk = 0;
   size_t assign_lds = 0;
   int assign_grid = 0;
+  int variant = 0;  // k_assign launch variant (kmc_autotune)
+  int vgrid[8] = {0};
+  long long vqstride[8] = {0};
+  int max_regions = 0;
   float alpha = 0, beta = 0, gref = 0;
+  int idx_bits = 0;
 
   bool have_points = false, have_centroids = false;
   std::vector<hipEvent_t> events;
+
+  // captured iteration chunks (KMC_RUN_GRAPH), reused by later runs
+  struct GraphEntry {
+    uint64_t iters, ev_off;
+    int mode, exclude, variant, timed;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
 };
 
 namespace {
@@ -104,11 +122,24 @@ int pick_dp(int d) {
 // gamma_m = m u / (1 - m u), the classical bound for m rounded operations.
 double gam(double m, double u) { return m * u / (1.0 - m * u); }
 
-// Margin-band constants. For every centroid j the MFMA value b_j satisfies
-// |b_j - (||c_j||^2 - 2 x.c_j)| <= alpha ||x|| cmax + beta cmax^2, and the
-// reference's direct float form r_j satisfies |r_j - d_j| <= gref d_j. The
-// constants carry a factor 2 of headroom over the derivation below.
-void band_constants(int prec, int D, float* alpha, float* beta, float* gref) {
+// Margin-band constants. The MFMA value of centroid j is
+//   b_j = fl( fl(||c_j||^2 + ||x||^2) + sum_k x_k (-2 c_jk) ),
+// the full squared distance d_j with its centroid index packed into the low
+// mantissa bits. Its error splits into a part common to all centroids of the
+// point (the rounding of ||x||^2), which cancels in every margin, and a part
+// eta_j with
+//   |eta_j| <= (u + gA)(||c||^2 + ||x||^2 + 2||x||c) + gD c^2
+//              + 2 (t_in + gA (1 + t_in)) ||x|| c          (operands, products)
+// where c = cmax, u = 2^-24, gA = accumulation bound, gD = gamma_D, t_in =
+// operand rounding of the bf16 forms. With ||x||^2 <= 2 d_j + 2 c^2:
+//   |eta_j| <= alpha ||x|| c + beta c^2 + g1 d_j
+//   alpha = 2(u + gA) + 2(t_in + gA(1 + t_in)),  beta = 3(u + gA) + gD,
+//   g1 = 2(u + gA).
+// The reference's direct float form r_j satisfies |r_j - d_j| <= g_{D+3} d_j,
+// and the packed index moves b_j by < 2^(idx_bits-23) b_j. All terms relative
+// to the distance go into gref; the band test evaluates them at the upper
+// bound b + E of each distance. Every constant carries a factor 2 of headroom.
+void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, float* gref) {
   const double u = std::ldexp(1.0, -24);  // fp32 unit roundoff
   const double ub = std::ldexp(1.0, -9);  // bf16 unit roundoff
   double t_in, gA;
@@ -122,13 +153,14 @@ void band_constants(int prec, int D, float* alpha, float* beta, float* gref) {
     t_in = 2.0 * ub + ub * ub;
     gA = gam(D + 1, u);
   }
-  const double gD = gam(D, u);  // ||c||^2 in fp32
-  // dot term: |x.(-2c)| error <= 2 (t_in + gA (1 + t_in)) ||x|| ||c||
-  const double a = 2.0 * (t_in + gA * (1.0 + t_in));
-  const double b = gD + gA * (1.0 + gD);
+  const double gD = gam(D, u);
+  const double pk = std::ldexp(1.0, idx_bits - 23);
+  const double a = 2.0 * (u + gA) + 2.0 * (t_in + gA * (1.0 + t_in));
+  const double b = 3.0 * (u + gA) + gD;
+  const double g = gam(D + 3, u) + pk + 2.0 * (u + gA);
   *alpha = (float)(2.0 * a);
   *beta = (float)(2.0 * b);
-  *gref = (float)(2.0 * gam(D + 3, u));
+  *gref = (float)(2.0 * g);
 }
 
 kmc_status alloc_all(kmc_handle* h) {
@@ -144,15 +176,22 @@ kmc_status alloc_all(kmc_handle* h) {
   else KMC_CK(dalloc(&h->Chi, Kp * DP));
   if (h->prec == kmc::PREC_BF16X3) KMC_CK(dalloc(&h->Clo, Kp * DP));
   KMC_CK(dalloc(&h->cn, Kp));
+  KMC_CK(dalloc(&h->CT, Kp * DP));
   KMC_CK(dalloc(&h->labels, n));
-  KMC_CK(dalloc(&h->q2, 3 * n));
-  KMC_CK(dalloc(&h->qa, n));
-  KMC_CK(dalloc(&h->qcnt, 2));
+  size_t qtot = 0;
+  for (int v = 0; v < kmc::assign_variants(); ++v)
+    qtot = std::max(qtot, (size_t)h->vgrid[v] * (size_t)h->vqstride[v]);
+  KMC_CK(dalloc(&h->q2, 3 * qtot));
+  KMC_CK(dalloc(&h->qa, qtot));
+  KMC_CK(dalloc(&h->q2cnt, (size_t)h->max_regions));
+  KMC_CK(dalloc(&h->qacnt, (size_t)h->max_regions));
   KMC_CK(dalloc(&h->bcount, K * (size_t)h->nblk));
   KMC_CK(dalloc(&h->count, K));
   KMC_CK(dalloc(&h->start, K));
   KMC_CK(dalloc(&h->members, n));
   KMC_CK(dalloc(&h->ipart, (size_t)h->nparts));
+  if (h->update_mode == kmc::UPDATE_TREE)
+    KMC_CK(dalloc(&h->tpart, ((n + kmc::kTreeRows - 1) / kmc::kTreeRows + K) * DP));
   KMC_CK(dalloc(&h->conv, K));
   KMC_CK(dalloc(&h->shift2, K));
   KMC_CK(dalloc(&h->alive, K));
@@ -161,16 +200,18 @@ kmc_status alloc_all(kmc_handle* h) {
   KMC_CK(hipMemsetAsync(h->C, 0, K * DP * sizeof(float), h->stream));
   KMC_CK(hipMemsetAsync(h->Cnext, 0, K * DP * sizeof(float), h->stream));
   KMC_CK(hipMemsetAsync(h->state, 0, sizeof(DevState), h->stream));
-  KMC_CK(hipMemsetAsync(h->qcnt, 0, 2 * sizeof(unsigned), h->stream));
+  KMC_CK(hipMemsetAsync(h->q2cnt, 0, h->max_regions * sizeof(unsigned), h->stream));
+  KMC_CK(hipMemsetAsync(h->qacnt, 0, h->max_regions * sizeof(unsigned), h->stream));
   KMC_CK(hipStreamSynchronize(h->stream));
   return KMC_OK;
 }
 
 void free_all(kmc_handle* h) {
   void* ptrs[] = {h->X, h->Xhi, h->Xlo, h->xn2, h->C, h->Cnext, h->Cw, h->Chi,
-                  h->Clo, h->cn, h->labels, h->q2, h->qa, h->qcnt, h->bcount,
-                  h->count, h->start, h->members, h->ipart, h->conv, h->shift2,
-                  h->alive, h->state};
+                  h->Clo, h->cn, h->CT, h->labels, h->q2, h->qa, h->q2cnt, h->qacnt,
+                  h->bcount,
+                  h->count, h->start, h->members, h->ipart, h->tpart, h->conv,
+                  h->shift2, h->alive, h->state};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -192,6 +233,26 @@ kmc_status reset_state(kmc_handle* h, uint64_t maxiter) {
   return KMC_OK;
 }
 
+// Record an event on the handle's stream. Under stream capture an event
+// recorded with hipEventRecord comes back unusable, so the record node is
+// added to the capturing graph explicitly and made the stream's dependency.
+hipError_t record_event(hipStream_t s, hipEvent_t e) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t err = hipStreamIsCapturing(s, &cs);
+  if (err != hipSuccess) return err;
+  if (cs != hipStreamCaptureStatusActive) return hipEventRecord(e, s);
+  unsigned long long id = 0;
+  hipGraph_t g = nullptr;
+  const hipGraphNode_t* deps = nullptr;
+  size_t nd = 0;
+  err = hipStreamGetCaptureInfo_v2(s, &cs, &id, &g, &deps, &nd);
+  if (err != hipSuccess) return err;
+  hipGraphNode_t node;
+  err = hipGraphAddEventRecordNode(&node, g, deps, nd, e);
+  if (err != hipSuccess) return err;
+  return hipStreamUpdateCaptureDependencies(s, &node, 1, hipStreamSetCaptureDependencies);
+}
+
 kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1) {
   kmc::PrepParams pp{};
   pp.C = h->C;
@@ -199,7 +260,7 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1) {
   pp.Chi = h->Chi;
   pp.Clo = h->Clo;
   pp.cn = h->cn;
-  pp.qcnt = h->qcnt;
+  pp.CT = h->CT;
   pp.alive = h->alive;
   pp.done = done_ptr(h);
   pp.K = h->K;
@@ -221,33 +282,40 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1) {
   ap.K = h->K;
   ap.Kp = h->Kp;
   ap.kchunk = h->kchunk;
+  ap.idx_mask = (1u << h->idx_bits) - 1u;
   ap.alpha = h->alpha;
   ap.beta = h->beta;
   ap.gref = h->gref;
   ap.labels = h->labels;
   ap.q2 = h->q2;
   ap.qa = h->qa;
-  ap.qcnt = h->qcnt;
+  ap.q2cnt = h->q2cnt;
+  ap.qacnt = h->qacnt;
+  ap.qstride = h->qstride;
   ap.done = done_ptr(h);
-  if (e0) KMC_CK(hipEventRecord(e0, h->stream));
-  KMC_CK(kmc::launch_assign(h->DP, h->prec, ap, h->assign_grid, h->assign_lds, h->stream));
-  if (e1) KMC_CK(hipEventRecord(e1, h->stream));
+  if (e0) KMC_CK(record_event(h->stream, e0));
+  KMC_CK(kmc::launch_assign(h->DP, h->prec, h->variant, ap, h->assign_grid, h->assign_lds,
+                            h->stream));
+  if (e1) KMC_CK(record_event(h->stream, e1));
 
   kmc::RefineParams rp{};
   rp.X = h->X;
   rp.C = h->C;
+  rp.CT = h->CT;
   rp.q2 = h->q2;
   rp.qa = h->qa;
-  rp.qcnt = h->qcnt;
+  rp.q2cnt = h->q2cnt;
+  rp.qacnt = h->qacnt;
+  rp.qstride = h->qstride;
+  rp.nregions = h->assign_grid;
   rp.alive = h->alive;
   rp.labels = h->labels;
   rp.done = done_ptr(h);
   rp.K = h->K;
+  rp.Kp = h->Kp;
   rp.D = h->D;
   rp.DP = h->DP;
-  const int g2 = std::max(1, std::min(h->cus * 4, (int)((h->n + 255) / 256)));
-  const int gA = std::max(1, std::min(h->cus * 2, (int)((h->n + 3) / 4)));
-  KMC_CK(kmc::launch_refine(rp, g2, gA, h->stream));
+  KMC_CK(kmc::launch_refine(rp, h->stream));
   return KMC_OK;
 }
 
@@ -263,25 +331,32 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty) {
   sp.K = h->K;
   sp.CH = h->CH;
   sp.nblk = h->nblk;
+  sp.lbits = h->lbits;
   KMC_CK(kmc::launch_sort(sp, h->stream));
 
   kmc::UpdateParams up{};
   up.X = h->X;
   up.C = h->C;
   up.Cnext = h->Cnext;
+  up.labels = h->labels;
   up.members = h->members;
   up.start = h->start;
   up.count = h->count;
   up.ipart = h->ipart;
+  up.tpart = h->tpart;
   up.done = done_ptr(h);
+  up.n = (long long)h->n;
   up.K = h->K;
   up.D = h->D;
   up.DP = h->DP;
-  KMC_CK(kmc::launch_update(up, h->stream));
+  KMC_CK(kmc::launch_update(up, h->update_mode, h->stream));
 
   kmc::FinishParams fp{};
   fp.C = h->C;
   fp.Cnext = h->Cnext;
+  fp.tpart = h->tpart;
+  fp.start = h->start;
+  fp.tree = h->update_mode == kmc::UPDATE_TREE ? 1 : 0;
   fp.conv = h->conv;
   fp.shift2 = h->shift2;
   fp.count = h->count;
@@ -297,13 +372,65 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty) {
   st.nparts = h->nparts;
   st.shift2 = h->shift2;
   st.conv = h->conv;
-  st.qcnt = h->qcnt;
+  st.q2cnt = h->q2cnt;
+  st.qacnt = h->qacnt;
+  st.nregions = h->assign_grid;
   st.state = h->state;
   st.done = done_ptr(h);
   st.tol = h->cfg.tol;
   st.K = h->K;
   st.mode = mode;
   KMC_CK(kmc::launch_finish(fp, st, h->stream));
+  return KMC_OK;
+}
+
+// m iterations starting at global iteration `first`; events around the
+// distance kernel for iterations below ntimed.
+kmc_status enqueue_chunk(kmc_handle* h, uint64_t first, uint64_t m, int mode, int exclude,
+                         uint64_t ntimed) {
+  for (uint64_t it = 0; it < m; ++it) {
+    const uint64_t gi = first + it;
+    hipEvent_t e0 = gi < ntimed ? h->events[2 * gi] : nullptr;
+    hipEvent_t e1 = gi < ntimed ? h->events[2 * gi + 1] : nullptr;
+    kmc_status st = enqueue_assign(h, e0, e1);
+    if (st == KMC_OK) st = enqueue_update(h, mode, exclude);
+    if (st != KMC_OK) return st;
+  }
+  return KMC_OK;
+}
+
+// Launch m iterations as one graph; the graph is captured on first use and
+// kept for later runs with the same shape. Returns KMC_ERR_UNSUPPORTED when
+// the stream cannot be captured (the caller then launches eagerly).
+kmc_status launch_chunk_graph(kmc_handle* h, uint64_t first, uint64_t m, int mode, int exclude,
+                              uint64_t ntimed) {
+  const int timed = first < ntimed ? 1 : 0;
+  for (auto& g : h->graphs) {
+    if (g.iters == m && g.ev_off == (timed ? first : 0) && g.mode == mode &&
+        g.exclude == exclude && g.variant == h->variant && g.timed == timed) {
+      KMC_CK(hipGraphLaunch(g.exec, h->stream));
+      return KMC_OK;
+    }
+  }
+  if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return KMC_ERR_UNSUPPORTED;
+  kmc_status st = enqueue_chunk(h, first, m, mode, exclude, timed ? ntimed : 0);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+  if (st != KMC_OK || ec != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return KMC_ERR_UNSUPPORTED;
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    (void)hipGetLastError();
+    return KMC_ERR_UNSUPPORTED;
+  }
+  h->graphs.push_back({m, timed ? first : 0, mode, exclude, h->variant, timed, exec});
+  KMC_CK(hipGraphLaunch(exec, h->stream));
   return KMC_OK;
 }
 
@@ -327,7 +454,7 @@ void fill_result(const kmc_handle* h, const DevState& s, kmc_result* r) {
 
 kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
   double* xn64 = nullptr;
-  KMC_CK(dalloc(&xn64, h->n));
+  KMC_CK(dalloc(&xn64, h->n + 256));
   kmc::LoadParams lp{};
   lp.src = dsrc;
   lp.ld = ld;
@@ -340,7 +467,8 @@ kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
   lp.xn2 = h->xn2;
   lp.xn2_64 = xn64;
   hipError_t e = kmc::launch_load_points(lp, h->stream);
-  if (e == hipSuccess) e = kmc::launch_sum_f64(xn64, (long long)h->n, &h->state->sum_x2, h->stream);
+  if (e == hipSuccess)
+    e = kmc::launch_sum_f64(xn64, (long long)h->n, &h->state->sum_x2, xn64 + h->n, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(xn64);
   KMC_CK(e);
@@ -387,7 +515,7 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
   if (cfg->n > (size_t)0x7fffffff) return KMC_ERR_UNSUPPORTED;
   if (cfg->k > 16384u) return KMC_ERR_UNSUPPORTED;
   if (cfg->precision < 0 || cfg->precision > 2) return KMC_ERR_ARG;
-  if (cfg->update != KMC_UPDATE_ORDERED) return KMC_ERR_ARG;
+  if (cfg->update != KMC_UPDATE_ORDERED && cfg->update != KMC_UPDATE_TREE) return KMC_ERR_ARG;
   if (!(cfg->tol >= 0.0f)) return KMC_ERR_ARG;
   const int DP = pick_dp((int)cfg->d);
   if (DP == 0) return KMC_ERR_UNSUPPORTED;
@@ -403,8 +531,9 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
   h->D = (int)cfg->d;
   h->K = (int)cfg->k;
   h->DP = DP;
-  h->Kp = (h->K + 31) / 32 * 32;
+  h->Kp = (h->K + 63) / 64 * 64;  // centroid blocks are processed in pairs
   h->prec = cfg->precision;
+  h->update_mode = cfg->update == KMC_UPDATE_TREE ? kmc::UPDATE_TREE : kmc::UPDATE_ORDERED;
   kmc_status st = KMC_OK;
   do {
     if (hipSetDevice(h->dev) != hipSuccess) { st = fail(h, KMC_ERR_HIP, "hipSetDevice"); break; }
@@ -422,32 +551,51 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     int ch = ((h->K + 255) / 256) * 256 * 4;
     h->CH = std::max(1024, std::min(16384, ch));
     h->nblk = (int)((h->n + h->CH - 1) / h->CH);
-    h->nparts = h->K * ((h->D + 63) / 64);
+    h->lbits = 0;
+    while ((1 << h->lbits) < h->K) ++h->lbits;
+    const long long S = (h->D + 63) / 64;
+    h->nparts = h->update_mode == kmc::UPDATE_TREE
+                    ? (int)(((long long)h->n + kmc::kTreeRows - 1) / kmc::kTreeRows)
+                    : (int)(h->K * S);
 
     // assign launch shape: all centroids resident in LDS when they fit
     const size_t kResidentMax = 144 * 1024, kChunkMax = 64 * 1024;
     h->kchunk = h->Kp;
     h->assign_lds = kmc::assign_lds(DP, h->prec, h->kchunk);
     if (h->assign_lds > kResidentMax) {
-      int kc = 32;
-      while (kc + 32 <= h->Kp && kmc::assign_lds(DP, h->prec, kc + 32) <= kChunkMax) kc += 32;
+      int kc = 64;
+      while (kc + 64 <= h->Kp && kmc::assign_lds(DP, h->prec, kc + 64) <= kChunkMax) kc += 64;
       h->kchunk = kc;
       h->assign_lds = kmc::assign_lds(DP, h->prec, kc);
     }
-    const void* fn = kmc::assign_kernel_ptr(DP, h->prec);
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)h->assign_lds) != hipSuccess) {
-      st = fail(h, KMC_ERR_HIP, "hipFuncSetAttribute(assign)");
-      break;
+    bool ok = true;
+    for (int v = 0; v < kmc::assign_variants() && ok; ++v) {
+      const void* fn = kmc::assign_kernel_ptr(DP, h->prec, v);
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)h->assign_lds) != hipSuccess) {
+        st = fail(h, KMC_ERR_HIP, "hipFuncSetAttribute(assign)");
+        ok = false;
+        break;
+      }
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kmc::assign_threads(v),
+                                                       h->assign_lds) != hipSuccess ||
+          per_cu < 1)
+        per_cu = 1;
+      const int tile = kmc::assign_tile(v);
+      const long long tiles = (long long)((h->n + tile - 1) / tile);
+      h->vgrid[v] = (int)std::min<long long>(tiles, (long long)per_cu * h->cus);
+      // each assign workgroup owns a queue region large enough for all its points
+      h->vqstride[v] = (tiles + h->vgrid[v] - 1) / h->vgrid[v] * tile;
+      h->max_regions = std::max(h->max_regions, h->vgrid[v]);
     }
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, h->assign_lds) !=
-            hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    const long long tiles = (long long)((h->n + 127) / 128);
-    h->assign_grid = (int)std::min<long long>(tiles, (long long)per_cu * h->cus);
-    band_constants(h->prec, h->D, &h->alpha, &h->beta, &h->gref);
+    if (!ok) break;
+    h->variant = 0;
+    h->assign_grid = h->vgrid[0];
+    h->qstride = h->vqstride[0];
+    h->idx_bits = 0;
+    while ((1 << h->idx_bits) < h->Kp) ++h->idx_bits;
+    band_constants(h->prec, h->D, h->idx_bits, &h->alpha, &h->beta, &h->gref);
     st = alloc_all(h);
   } while (0);
   if (st != KMC_OK) {
@@ -465,6 +613,7 @@ kmc_status kmc_destroy(kmc_handle* h) {
   if (!h) return KMC_ERR_ARG;
   (void)hipSetDevice(h->dev);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
   for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
   free_all(h);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -559,16 +708,18 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
   KMC_CK(hipEventRecord(t0, h->stream));
   uint64_t launched = 0, chunk = 1;
   DevState s{};
+  bool use_graph = (flags & KMC_RUN_GRAPH) != 0;
   while (launched < maxiter) {
     const uint64_t m = check ? std::min<uint64_t>(chunk, maxiter - launched) : maxiter - launched;
-    for (uint64_t it = 0; it < m; ++it) {
-      const uint64_t gi = launched + it;
-      hipEvent_t e0 = gi < ntimed ? h->events[2 * gi] : nullptr;
-      hipEvent_t e1 = gi < ntimed ? h->events[2 * gi + 1] : nullptr;
-      st = enqueue_assign(h, e0, e1);
-      if (st == KMC_OK) st = enqueue_update(h, mode, exclude ? 1 : 0);
+    if (use_graph) {
+      st = launch_chunk_graph(h, launched, m, mode, exclude ? 1 : 0, ntimed);
+      if (st == KMC_ERR_UNSUPPORTED) {
+        use_graph = false;  // capture refused: launch eagerly from here on
+        st = KMC_OK;
+      }
       if (st != KMC_OK) break;
     }
+    if (!use_graph) st = enqueue_chunk(h, launched, m, mode, exclude ? 1 : 0, ntimed);
     if (st != KMC_OK) break;
     launched += m;
     if (check) {
@@ -606,6 +757,57 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
   return st;
 }
 
+kmc_status kmc_set_assign_variant(kmc_handle* h, int variant) {
+  if (!h || variant < 0 || variant >= kmc::assign_variants())
+    return fail(h, KMC_ERR_ARG, "kmc_set_assign_variant: no such variant");
+  h->variant = variant;
+  h->assign_grid = h->vgrid[variant];
+  h->qstride = h->vqstride[variant];
+  return KMC_OK;
+}
+
+kmc_status kmc_autotune(kmc_handle* h, kmc_tuning* out) {
+  if (!h) return KMC_ERR_ARG;
+  if (!h->have_points || !h->have_centroids)
+    return fail(h, KMC_ERR_STATE, "kmc_autotune: points and centroids must be set");
+  KMC_CK(hipSetDevice(h->dev));
+  kmc_status st = reset_state(h, 1);
+  if (st != KMC_OK) return st;
+  hipEvent_t e[4];
+  for (int i = 0; i < 4; ++i) KMC_CK(hipEventCreate(&e[i]));
+  const int nv = std::min(kmc::assign_variants(), KMC_MAX_VARIANTS);
+  float best = 1e30f;
+  int best_v = 0;
+  kmc_tuning t{};
+  t.nvariants = nv;
+  for (int v = 0; v < nv; ++v) {
+    st = kmc_set_assign_variant(h, v);
+    if (st != KMC_OK) break;
+    // one untimed launch, then two timed ones
+    st = enqueue_assign(h, nullptr, nullptr);
+    if (st == KMC_OK) st = enqueue_assign(h, e[0], e[1]);
+    if (st == KMC_OK) st = enqueue_assign(h, e[2], e[3]);
+    if (st != KMC_OK) break;
+    KMC_CK(hipEventSynchronize(e[3]));
+    float a = 0.f, b = 0.f;
+    KMC_CK(hipEventElapsedTime(&a, e[0], e[1]));
+    KMC_CK(hipEventElapsedTime(&b, e[2], e[3]));
+    t.ms[v] = 0.5f * (a + b);
+    t.threads[v] = kmc::assign_threads(v);
+    t.tile[v] = kmc::assign_tile(v);
+    if (t.ms[v] < best) {
+      best = t.ms[v];
+      best_v = v;
+    }
+  }
+  for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+  if (st != KMC_OK) return st;
+  t.chosen = best_v;
+  st = kmc_set_assign_variant(h, best_v);
+  if (out) *out = t;
+  return st;
+}
+
 kmc_status kmc_get_labels(kmc_handle* h, int32_t* labels) {
   if (!h || !labels) return fail(h, KMC_ERR_ARG, "kmc_get_labels: bad argument");
   KMC_CK(hipSetDevice(h->dev));
@@ -634,13 +836,16 @@ kmc_status kmc_assign(kmc_handle* h, const float* C, int32_t* labels, kmc_result
   if (st != KMC_OK) return st;
   st = enqueue_assign(h, nullptr, nullptr);
   if (st != KMC_OK) return st;
-  unsigned q[2] = {0, 0};
-  KMC_CK(hipMemcpyAsync(q, h->qcnt, sizeof(q), hipMemcpyDeviceToHost, h->stream));
+  std::vector<unsigned> q2(h->assign_grid), qa(h->assign_grid);  // regions of this variant
+  KMC_CK(hipMemcpyAsync(q2.data(), h->q2cnt, q2.size() * sizeof(unsigned),
+                        hipMemcpyDeviceToHost, h->stream));
+  KMC_CK(hipMemcpyAsync(qa.data(), h->qacnt, qa.size() * sizeof(unsigned),
+                        hipMemcpyDeviceToHost, h->stream));
   st = kmc_get_labels(h, labels);
   if (st != KMC_OK) return st;
   if (res) {
-    res->refined_two = q[0];
-    res->refined_full = q[1];
+    for (unsigned v : q2) res->refined_two += v;
+    for (unsigned v : qa) res->refined_full += v;
   }
   return KMC_OK;
 }

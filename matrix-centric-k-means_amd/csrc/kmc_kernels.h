@@ -7,6 +7,13 @@
 namespace kmc {
 
 enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_BF16 = 2 };
+enum { UPDATE_ORDERED = 0, UPDATE_TREE = 1 };
+constexpr int kTreeRows = 256;  // sorted rows per tree-update chunk
+constexpr int kMaxDP = 256;     // largest padded dimension of the Lloyd path
+// ||c||^2 of padding rows and excluded clusters: finite (so that index bits
+// can be packed into it) and above any real distance
+constexpr float kFar = 3.0e38f;
+constexpr float kFarTest = 1.0e38f;
 
 // Device-resident loop state (one per handle).
 struct DevState {
@@ -31,8 +38,8 @@ struct PrepParams {
   float* Cw;       // Kp x DP  (-2c, fp32 form)
   uint16_t* Chi;   // Kp x DP  (bf16 forms)
   uint16_t* Clo;
-  float* cn;       // Kp, ||c||^2 (+inf for padding rows)
-  unsigned* qcnt;  // refinement queue counters, cleared here
+  float* cn;       // Kp, ||c||^2 (kFar for padding rows and excluded clusters)
+  float* CT;       // DP x Kp transposed centroids (all-k refinement)
   const int* alive;  // K: 0 = cluster excluded from assignment (+inf)
   const int* done;
   int K, Kp, DP, prec;
@@ -49,24 +56,31 @@ struct AssignParams {
   const uint16_t* Clo;
   const float* cn;
   int K, Kp, kchunk;
+  uint32_t idx_mask;        // low mantissa bits that carry the centroid index
   float alpha, beta, gref;  // margin-band constants
   int* labels;
-  int* q2;        // (point, cand1, cand2) triples
-  int* qa;        // points needing all-k refinement
-  unsigned* qcnt; // [0] = q2 entries, [1] = qa entries
+  int* q2;          // (point, cand1, cand2) triples, one region per workgroup
+  int* qa;          // points needing all-k refinement, one region per workgroup
+  unsigned* q2cnt;  // entries per region
+  unsigned* qacnt;
+  long long qstride;  // entries per region
   const int* done;
 };
 
 struct RefineParams {
   const float* X;
   const float* C;
+  const float* CT;
   const int* q2;
   const int* qa;
-  const unsigned* qcnt;
+  const unsigned* q2cnt;
+  const unsigned* qacnt;
+  long long qstride;
+  int nregions;
   const int* alive;
   int* labels;
   const int* done;
-  int K, D, DP;
+  int K, Kp, D, DP;
 };
 
 struct SortParams {
@@ -77,24 +91,30 @@ struct SortParams {
   int* members;  // n point ids sorted by (label, id)
   const int* done;
   long long n;
-  int K, CH, nblk;
+  int K, CH, nblk, lbits;
 };
 
 struct UpdateParams {
   const float* X;
   const float* C;   // centroids the labels were assigned against
   float* Cnext;
+  const int* labels;
   const int* members;
   const int* start;
   const int* count;
-  double* ipart;    // K * ceil(D/64) partial inertias
+  double* ipart;    // partial inertias (ordered: K*S, tree: nchunks*S)
+  double* tpart;    // tree partials, (nchunks + K) x DP
   const int* done;
+  long long n;
   int K, D, DP;
 };
 
 struct FinishParams {
   float* C;
-  const float* Cnext;
+  float* Cnext;
+  const double* tpart;  // tree partials (tree update only)
+  const int* start;
+  int tree;
   int* conv;
   double* shift2;
   const int* count;  // cluster sizes of the assignment just used
@@ -109,7 +129,9 @@ struct StatsParams {
   int nparts;
   const double* shift2;
   const int* conv;
-  const unsigned* qcnt;
+  const unsigned* q2cnt;
+  const unsigned* qacnt;
+  int nregions;
   DevState* state;
   const int* done;
   float tol;
@@ -129,16 +151,20 @@ struct LoadParams {
 };
 
 hipError_t launch_prep(const PrepParams& P, hipStream_t s);
-hipError_t launch_assign(int DP, int prec, const AssignParams& P, int grid, size_t lds,
-                         hipStream_t s);
+hipError_t launch_assign(int DP, int prec, int variant, const AssignParams& P, int grid,
+                         size_t lds, hipStream_t s);
 size_t assign_lds(int DP, int prec, int kchunk);
-const void* assign_kernel_ptr(int DP, int prec);
-hipError_t launch_refine(const RefineParams& P, int grid2, int gridA, hipStream_t s);
+int assign_variants();
+int assign_threads(int variant);
+int assign_tile(int variant);
+const void* assign_kernel_ptr(int DP, int prec, int variant);
+hipError_t launch_refine(const RefineParams& P, hipStream_t s);
 hipError_t launch_sort(const SortParams& P, hipStream_t s);
-hipError_t launch_update(const UpdateParams& P, hipStream_t s);
+hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s);
 hipError_t launch_finish(const FinishParams& F, const StatsParams& S, hipStream_t s);
 hipError_t launch_load_points(const LoadParams& P, hipStream_t s);
-hipError_t launch_sum_f64(const double* v, long long n, double* out, hipStream_t s);
+hipError_t launch_sum_f64(const double* v, long long n, double* out, double* part256,
+                          hipStream_t s);
 hipError_t launch_fill_i32(int* p, long long n, int v, hipStream_t s);
 
 }  // namespace kmc

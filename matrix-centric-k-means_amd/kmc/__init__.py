@@ -22,6 +22,7 @@ OK, ERR_ARG, ERR_NO_DEVICE, ERR_OUT_OF_BOUNDS, ERR_INFILE, ERR_HIP, ERR_OOM, \
     ERR_STATE, ERR_UNSUPPORTED = range(9)
 
 PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16": 2}
+UPDATES = {"ordered": 0, "tree": 1}
 RUN_CHECK_CONVERGED = 0x1
 RUN_TIME_KERNELS = 0x2
 
@@ -50,6 +51,15 @@ class _Result(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+MAX_VARIANTS = 8
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [("nvariants", ctypes.c_int32), ("chosen", ctypes.c_int32),
+                ("ms", ctypes.c_float * MAX_VARIANTS), ("threads", ctypes.c_int32 * MAX_VARIANTS),
+                ("tile", ctypes.c_int32 * MAX_VARIANTS)]
+
+
 class _KKConfig(ctypes.Structure):
     _fields_ = [("n", ctypes.c_size_t), ("d", ctypes.c_uint32), ("k", ctypes.c_uint32),
                 ("tol", ctypes.c_float), ("kernel", ctypes.c_int32), ("store", ctypes.c_int32),
@@ -72,6 +82,7 @@ STORES = {"fp32": 0, "bf16": 1}
 RUN_CONVERGE_SCORE = 0x4
 RUN_EXCLUDE_EMPTY = 0x8
 RUN_STOP_STABLE = 0x10
+RUN_GRAPH = 0x20
 
 _lib = None
 
@@ -102,6 +113,8 @@ def load_library(path=LIB_PATH):
         "kmc_status_string": (ctypes.c_char_p, [ctypes.c_int]),
         "kmc_device_count": (ctypes.c_int, []),
         "kmc_memory_footprint": (ctypes.c_int, [ctypes.c_int, P(ctypes.c_size_t)]),
+        "kmc_autotune": (ctypes.c_int, [vp, P(_Tuning)]),
+        "kmc_set_assign_variant": (ctypes.c_int, [vp, ctypes.c_int]),
         "kmc_kk_create": (ctypes.c_int, [P(_KKConfig), P(vp)]),
         "kmc_kk_destroy": (ctypes.c_int, [vp]),
         "kmc_kk_set_points": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
@@ -141,11 +154,12 @@ def _f32(a, shape=None):
 class Engine:
     """One k-means problem resident on one GPU."""
 
-    def __init__(self, n, d, k, tol=1e-4, precision="fp32", device=0):
+    def __init__(self, n, d, k, tol=1e-4, precision="fp32", device=0, update="ordered"):
         self._lib = load_library()
         self.n, self.d, self.k = int(n), int(d), int(k)
         self.precision = precision
-        cfg = _Config(self.n, self.d, self.k, float(tol), PRECISIONS[precision], 0, int(device), 0)
+        cfg = _Config(self.n, self.d, self.k, float(tol), PRECISIONS[precision], UPDATES[update],
+                      int(device), 0)
         h = ctypes.c_void_p()
         st = self._lib.kmc_create(ctypes.byref(cfg), ctypes.byref(h))
         if st != OK:
@@ -191,12 +205,24 @@ class Engine:
         lab = np.ascontiguousarray(labels, dtype=np.int32)
         self._check(self._lib.kmc_set_labels(self._h, lab.ctypes.data), "set_labels")
 
-    def run(self, maxiter, check_converged=True, time_kernels=False):
+    def run(self, maxiter, check_converged=True, time_kernels=False, graph=False):
         flags = (RUN_CHECK_CONVERGED if check_converged else 0) | \
-                (RUN_TIME_KERNELS if time_kernels else 0)
+                (RUN_TIME_KERNELS if time_kernels else 0) | (RUN_GRAPH if graph else 0)
         r = _Result()
         self._check(self._lib.kmc_run(self._h, int(maxiter), flags, ctypes.byref(r)), "run")
         return r.as_dict()
+
+    def autotune(self):
+        """Time every launch variant of the distance kernel on the current
+        points and centroids and keep the fastest."""
+        t = _Tuning()
+        self._check(self._lib.kmc_autotune(self._h, ctypes.byref(t)), "autotune")
+        n = t.nvariants
+        return {"chosen": t.chosen, "ms": list(t.ms[:n]), "threads": list(t.threads[:n]),
+                "tile": list(t.tile[:n])}
+
+    def set_variant(self, v):
+        self._check(self._lib.kmc_set_assign_variant(self._h, int(v)), "set_assign_variant")
 
     def labels(self):
         out = np.empty(self.n, dtype=np.int32)
@@ -228,12 +254,12 @@ class Engine:
         return C_new, counts, r.as_dict()
 
 
-def fit(X, C0, maxiter, tol=1e-4, precision="fp32", check_converged=True):
+def fit(X, C0, maxiter, tol=1e-4, precision="fp32", check_converged=True, update="ordered"):
     """Run Lloyd from explicit centroids; returns (labels, centroids, result)."""
     X = _f32(X)
     n, d = X.shape
     C0 = _f32(C0)
-    with Engine(n, d, C0.shape[0], tol=tol, precision=precision) as e:
+    with Engine(n, d, C0.shape[0], tol=tol, precision=precision, update=update) as e:
         e.set_points(X)
         e.set_centroids(C0)
         res = e.run(maxiter, check_converged=check_converged)

@@ -71,3 +71,147 @@ def test_run_golden_small(kmc_mod, prec):
             assert res["converged_at"] == int(g["%d_converged" % i]), (i, res)
             assert np.array_equal(e.labels(), g["%d_labels" % i].astype(np.int32)), i
             assert np.array_equal(e.centroids(), g["%d_C" % i]), i
+
+
+def test_update_tree_matches_fp64_means(kmc_mod, oracle):
+    """Tree mode: FP64 sums rounded once -> within 1 ulp of the FP64 means."""
+    g = load_golden("ref_steps.npz")
+    for i in range(int(g["ncases"])):
+        X, C = g["%d_X" % i], g["%d_C" % i]
+        lab = g["%d_labels" % i].astype(np.int32)
+        k = C.shape[0]
+        _say("tree update case", i, X.shape, k)
+        with kmc_mod.Engine(X.shape[0], X.shape[1], k, update="tree") as e:
+            e.set_points(X)
+            Cn, cnt, res = e.update(lab, C)
+        C64, cnt64 = oracle.f64_centroids(X, lab.astype(np.uint32), k)
+        assert np.array_equal(cnt, cnt64)
+        assert np.allclose(Cn, C64, rtol=2.4e-7, atol=1e-30), np.max(np.abs(Cn - C64))
+        ref_in = oracle.f64_inertia(X, C, lab.astype(np.uint32))
+        assert abs(res["inertia"] - ref_in) <= 1e-9 * max(ref_in, 1e-30)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_run_tree_mode_golden(kmc_mod, prec):
+    g = load_golden("ref_c1_blobs.npz")
+    with kmc_mod.Engine(g["X"].shape[0], 64, 10, tol=float(g["tol"]), precision=prec,
+                        update="tree") as e:
+        e.set_points(g["X"])
+        e.set_centroids(g["C0"])
+        res = e.run(300, check_converged=True)
+        assert abs(int(res["converged_at"]) - int(g["converged"])) <= 1
+        assert np.array_equal(e.labels(), g["labels"].astype(np.int32))
+        assert np.allclose(e.centroids(), g["C"], rtol=1e-6, atol=1e-6)
+
+
+@pytest.fixture(scope="module")
+def c2_data():
+    from kmc import datagen
+    X, _, _ = datagen.large_blobs()
+    C0 = datagen.init_rows(X, 256, 7)
+    return X, C0
+
+
+def _reference_labels(oracle, X, C, margin_frac=1e-3):
+    """Reference FP32 first-min labels for every point: FP64 distances decide
+    wherever the FP64 top-2 gap is large; the rest are recomputed with the
+    oracle's float restatement of the reference distance."""
+    Xd, Cd = X.astype(np.float64), C.astype(np.float64)
+    D = (Xd * Xd).sum(1)[:, None] + (Cd * Cd).sum(1)[None, :] - 2.0 * (Xd @ Cd.T)
+    part = np.partition(D, 1, axis=1)
+    lab = np.argmin(D, axis=1)
+    gap = part[:, 1] - part[:, 0]
+    close = np.nonzero(gap <= margin_frac * np.maximum(part[:, 1], 1.0))[0]
+    if close.size:
+        Dc = oracle.distances(X[close], C)
+        lab[close], _ = oracle.argmin(Dc)
+    return lab.astype(np.int32), close.size
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_full_size_assign_exact(kmc_mod, oracle, c2_data, prec):
+    """N=1M, D=128, K=256: one assignment equals the reference's FP32 labels."""
+    X, C0 = c2_data
+    # centroids after a few iterations: blobs split between centroids give
+    # plenty of small margins
+    with kmc_mod.Engine(X.shape[0], 128, 256, precision="fp32") as e:
+        e.set_points(X)
+        e.set_centroids(C0)
+        e.run(3, check_converged=False)
+        C = e.centroids()
+    want, nclose = _reference_labels(oracle, X, C)
+    _say("full-size assign", prec, "close points", nclose)
+    with kmc_mod.Engine(X.shape[0], 128, 256, precision=prec) as e:
+        e.set_points(X)
+        lab, info = e.assign(C)
+    _say("refined", info["refined_two"], info["refined_full"])
+    assert np.array_equal(lab, want)
+
+
+def test_full_size_update_exact(kmc_mod, oracle, c2_data):
+    X, C0 = c2_data
+    rng = np.random.default_rng(3)
+    lab = rng.integers(0, 256, size=X.shape[0]).astype(np.int32)
+    lab[lab == 17] = 18  # one empty cluster -> zero vector
+    with kmc_mod.Engine(X.shape[0], 128, 256) as e:
+        e.set_points(X)
+        Cn, cnt, res = e.update(lab, C0)
+    Co, cnto = oracle.centroids(X, lab.astype(np.uint32), 256)
+    assert np.array_equal(cnt, cnto)
+    assert np.array_equal(Cn, Co)
+    assert not Cn[17].any()
+    ref_in = oracle.f64_inertia(X, C0, lab.astype(np.uint32))
+    assert abs(res["inertia"] - ref_in) <= 1e-9 * ref_in
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_every_launch_variant_gives_the_same_labels(kmc_mod, oracle, c2_data, prec):
+    """Every k_assign launch variant (and the autotuner's choice) reproduces
+    the reference's labels on a 200K-point slice."""
+    X, C0 = c2_data
+    Xs = np.ascontiguousarray(X[:200_000])
+    want, _ = _reference_labels(oracle, Xs, C0)
+    with kmc_mod.Engine(Xs.shape[0], 128, 256, precision=prec) as e:
+        e.set_points(Xs)
+        e.set_centroids(C0)
+        t = e.autotune()
+        _say("variants", prec, t)
+        for v in range(len(t["ms"])):
+            e.set_variant(v)
+            lab, _ = e.assign(C0)
+            assert np.array_equal(lab, want), v
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("update", ["ordered", "tree"])
+def test_graph_replays_match_eager(kmc_mod, prec, update):
+    """Captured iteration chunks replayed several times, with eager runs and
+    synchronisations in between, give the eager result every time."""
+    g = load_golden("ref_c1_blobs.npz")
+    X, C0 = g["X"], g["C0"]
+    with kmc_mod.Engine(X.shape[0], 64, 10, tol=float(g["tol"]), precision=prec,
+                        update=update) as e:
+        e.set_points(X)
+        e.set_centroids(C0)
+        r_eager = e.run(300, check_converged=True)
+        lab_e, C_e = e.labels(), e.centroids()
+        for rep in range(3):
+            _say("graph replay", prec, update, rep)
+            e.set_centroids(C0)
+            r_g = e.run(300, check_converged=True, graph=True)
+            assert r_g["converged_at"] == r_eager["converged_at"]
+            assert np.array_equal(e.labels(), lab_e)
+            assert np.array_equal(e.centroids(), C_e)
+            e.set_centroids(C0)
+            r2 = e.run(7, check_converged=False)  # eager work between replays
+            assert r2["iterations"] == 7
+        # fixed-length graph (one chunk) against eager
+        e.set_centroids(C0)
+        e.run(12, check_converged=False)
+        C12 = e.centroids()
+        for rep in range(2):
+            e.set_centroids(C0)
+            e.run(12, check_converged=False, graph=True)
+            assert np.array_equal(e.centroids(), C12)
+    if update == "ordered":
+        assert np.array_equal(lab_e, g["labels"].astype(np.int32))
