@@ -99,7 +99,7 @@ def cpu_baseline(X, C0, n_sample=100_000, iters=2):
 def traffic_from_profiles(prec):
     """HBM bytes per launch of the distance kernel from committed rocprofv3
     --pmc summaries (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
-    p = os.path.join(ROOT, "profiles", "assign_c2_traffic.json")
+    p = os.path.join(ROOT, "profiles", "assign_n1m_d128_k256_traffic.json")
     try:
         with open(p) as f:
             return json.load(f).get(prec, {}).get("hbm_bytes_per_launch")
