@@ -181,7 +181,7 @@ kmc_status alloc_all(kmc_handle* h) {
   size_t qtot = 0;
   for (int v = 0; v < kmc::assign_variants(); ++v)
     qtot = std::max(qtot, (size_t)h->vgrid[v] * (size_t)h->vqstride[v]);
-  KMC_CK(dalloc(&h->q2, 3 * qtot));
+  KMC_CK(dalloc(&h->q2, 4 * qtot));
   KMC_CK(dalloc(&h->qa, qtot));
   KMC_CK(dalloc(&h->q2cnt, (size_t)h->max_regions));
   KMC_CK(dalloc(&h->qacnt, (size_t)h->max_regions));

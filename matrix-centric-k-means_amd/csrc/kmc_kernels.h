@@ -59,7 +59,7 @@ struct AssignParams {
   uint32_t idx_mask;        // low mantissa bits that carry the centroid index
   float alpha, beta, gref;  // margin-band constants
   int* labels;
-  int* q2;          // (point, cand1, cand2) triples, one region per workgroup
+  int* q2;          // (point, cand1, cand2, cand3 or -1), one region per workgroup
   int* qa;          // points needing all-k refinement, one region per workgroup
   unsigned* q2cnt;  // entries per region
   unsigned* qacnt;

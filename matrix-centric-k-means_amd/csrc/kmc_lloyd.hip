@@ -223,21 +223,27 @@ __device__ __forceinline__ kmc_f32x16 mfma_block(const XFrag<DP, PREC>& x, const
   return acc;
 }
 
-// Top-3 update over one block's 16 accumulator values. Each value carries
+// Top-4 update over one block's 16 accumulator values. Each value carries
 // its centroid index in the low mantissa bits (idx[r] | (v & ~mask)), so the
-// running top-3 is three v_med3_f32 per value and the indices travel with
+// running top-4 is four v_med3_f32 per value and the indices travel with
 // the values. The values are full squared distances, so the perturbation is
 // relative to the (small) distances of the leading candidates; it is part of
 // the margin-band bound.
+// Insert v into the running top-4 b1 <= b2 <= b3 <= b4 (the k-th smallest of
+// {b1..b4, v} is med3(b_{k-1}, b_k, v)).
+__device__ __forceinline__ void top4_push(float v, float& b1, float& b2, float& b3, float& b4) {
+  b4 = __builtin_amdgcn_fmed3f(b3, b4, v);
+  b3 = __builtin_amdgcn_fmed3f(b2, b3, v);
+  b2 = __builtin_amdgcn_fmed3f(b1, b2, v);
+  b1 = __builtin_amdgcn_fmed3f(-__builtin_inff(), b1, v);
+}
+
 __device__ __forceinline__ void push_block(const kmc_f32x16& acc, const uint32_t* idx,
-                                           uint32_t keep, float& b1, float& b2, float& b3) {
+                                           uint32_t keep, float& b1, float& b2, float& b3,
+                                           float& b4) {
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const float v = __uint_as_float((__float_as_uint(acc[r]) & keep) | idx[r]);
-    b3 = __builtin_amdgcn_fmed3f(b2, b3, v);
-    b2 = __builtin_amdgcn_fmed3f(b1, b2, v);
-    b1 = __builtin_amdgcn_fmed3f(-__builtin_inff(), b1, v);
-  }
+  for (int r = 0; r < 16; ++r)
+    top4_push(__uint_as_float((__float_as_uint(acc[r]) & keep) | idx[r]), b1, b2, b3, b4);
 }
 
 // Work split: a workgroup of NW waves shares one LDS copy of the centroid
@@ -266,7 +272,7 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
     qac_s = 0u;
   }
   // each workgroup appends to its own queue region (no global atomics)
-  int* const q2 = P.q2 + 3 * (size_t)blockIdx.x * P.qstride;
+  int* const q2 = P.q2 + 4 * (size_t)blockIdx.x * P.qstride;
   int* const qa = P.qa + (size_t)blockIdx.x * P.qstride;
 
   // max ||c|| over the live centroids (for the error bound)
@@ -310,7 +316,7 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
       load_xfrag<DP, PREC>(P, nrow, nrow < P.n, h, nxt);
     }
 
-    float b1 = kFar, b2 = kFar, b3 = kFar;
+    float b1 = kFar, b2 = kFar, b3 = kFar, b4 = kFar;
     const float x2 = valid ? P.xn2[prow] : 0.0f;
     for (int c0 = 0; c0 < P.Kp; c0 += P.kchunk) {
       const int cnt = min(P.kchunk, P.Kp - c0);
@@ -333,8 +339,8 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
       for (int cb = 0; cb < cnt; cb += 64) {
         const kmc_f32x16 a0 = mfma_block<DP, PREC>(cur, cn_s, rows, cnt, cb, p, h, x2);
         const kmc_f32x16 a1 = mfma_block<DP, PREC>(cur, cn_s, rows, cnt, cb + 32, p, h, x2);
-        push_block(a0, idx0, keep, b1, b2, b3);
-        push_block(a1, idx1, keep, b1, b2, b3);
+        push_block(a0, idx0, keep, b1, b2, b3, b4);
+        push_block(a1, idx1, keep, b1, b2, b3, b4);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           idx0[r] += 64u;
@@ -348,38 +354,39 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
       const float o1 = __shfl_xor(b1, 32, 64);
       const float o2 = __shfl_xor(b2, 32, 64);
       const float o3 = __shfl_xor(b3, 32, 64);
-      b3 = __builtin_amdgcn_fmed3f(b2, b3, o1);
-      b2 = __builtin_amdgcn_fmed3f(b1, b2, o1);
-      b1 = __builtin_amdgcn_fmed3f(-__builtin_inff(), b1, o1);
-      b3 = __builtin_amdgcn_fmed3f(b2, b3, o2);
-      b2 = __builtin_amdgcn_fmed3f(b1, b2, o2);
-      b1 = __builtin_amdgcn_fmed3f(-__builtin_inff(), b1, o2);
-      b3 = __builtin_amdgcn_fmed3f(b2, b3, o3);
-      b2 = __builtin_amdgcn_fmed3f(b1, b2, o3);
-      b1 = __builtin_amdgcn_fmed3f(-__builtin_inff(), b1, o3);
+      const float o4 = __shfl_xor(b4, 32, 64);
+      top4_push(o1, b1, b2, b3, b4);
+      top4_push(o2, b1, b2, b3, b4);
+      top4_push(o3, b1, b2, b3, b4);
+      top4_push(o4, b1, b2, b3, b4);
     }
     const int i1 = (int)(__float_as_uint(b1) & P.idx_mask);
     const int i2 = (int)(__float_as_uint(b2) & P.idx_mask);
+    const int i3 = (int)(__float_as_uint(b3) & P.idx_mask);
 
     // ---- margin band test and output ----
     // b1 <= b2 <= b3 are full squared distances (plus the packed index bits);
     // E bounds the MFMA error of each, gref the reference's own float error
     // plus the packing perturbation, both relative to the distances.
-    bool flag2 = false, flagA = false;
+    bool flag2 = false, flagA = false, few3 = false;
     if (h == 0 && valid) {
       const float E = P.alpha * sqrtf(x2) * cmax + Ecm;
       const float twoE = 2.0f * E;
       const float b1p = fmaxf(b1 + E, 0.0f);
-      const bool uniq = (b2 >= kFarTest) ||
-                        (b2 - b1 > twoE + P.gref * (fmaxf(b2 + E, 0.0f) + b1p));
-      const bool two = (b3 >= kFarTest) ||
-                       (b3 - b1 > twoE + P.gref * (fmaxf(b3 + E, 0.0f) + b1p));
+      // clear(b): every centroid ranked at b or later loses to the first
+      auto clear = [&](float b) {
+        return (b >= kFarTest) || (b - b1 > twoE + P.gref * (fmaxf(b + E, 0.0f) + b1p));
+      };
+      const bool uniq = clear(b2);
+      const bool two = clear(b3);
+      const bool three = clear(b4);
       // a point without any finite distance goes to the all-k pass, which
       // applies the first-min rule
       const bool sane = (b1 < kFarTest) && (i1 < P.K);
       P.labels[prow] = sane ? i1 : 0;
-      flag2 = sane && !uniq && two;
-      flagA = !sane || (!uniq && !two);
+      flag2 = sane && !uniq && (two || three);
+      flagA = !sane || (!uniq && !two && !three);
+      few3 = !two;
     }
     const uint64_t m2 = __ballot(flag2);
     const uint64_t mA = __ballot(flagA);
@@ -390,9 +397,10 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
       base = __shfl(base, leader, 64);
       if (flag2) {
         const unsigned pos = base + mbcnt64(m2);
-        q2[3 * (size_t)pos + 0] = (int)prow;
-        q2[3 * (size_t)pos + 1] = i1;
-        q2[3 * (size_t)pos + 2] = i2;
+        q2[4 * (size_t)pos + 0] = (int)prow;
+        q2[4 * (size_t)pos + 1] = i1;
+        q2[4 * (size_t)pos + 2] = i2;
+        q2[4 * (size_t)pos + 3] = few3 ? i3 : -1;
       }
     }
     if (mA) {
@@ -419,20 +427,28 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
 // ---------------------------------------------------------------------------
 // Refinement: reference float arithmetic, first-min rule.
 // ---------------------------------------------------------------------------
-// Block r handles queue region r (written by assign workgroup r).
+// Band points with two or three candidates (the rest provably lose): each
+// candidate's distance in the reference's float arithmetic, first-min rule
+// (the smallest distance, the lowest index among equal ones). Block r
+// handles queue region r (written by assign workgroup r).
 __global__ __launch_bounds__(256) void k_refine2(RefineParams P) {
   if (*P.done) return;
   for (int r = blockIdx.x; r < P.nregions; r += gridDim.x) {
     const unsigned cnt = P.q2cnt[r];
-    const int* q2 = P.q2 + 3 * (size_t)r * P.qstride;
+    const int* q2 = P.q2 + 4 * (size_t)r * P.qstride;
     for (unsigned e = threadIdx.x; e < cnt; e += blockDim.x) {
-      const int i = q2[3 * (size_t)e + 0];
-      const int a = q2[3 * (size_t)e + 1];
-      const int b = q2[3 * (size_t)e + 2];
+      const int4 q = *reinterpret_cast<const int4*>(q2 + 4 * (size_t)e);
+      const int i = q.x;
       const float* x = P.X + (size_t)i * P.DP;
-      const float ra = direct_dist_f32(x, P.C + (size_t)a * P.DP, P.D);
-      const float rb = direct_dist_f32(x, P.C + (size_t)b * P.DP, P.D);
-      P.labels[i] = (rb < ra || (rb == ra && b < a)) ? b : a;
+      int best = q.y;
+      float rbest = direct_dist_f32(x, P.C + (size_t)q.y * P.DP, P.D);
+      const float rb = direct_dist_f32(x, P.C + (size_t)q.z * P.DP, P.D);
+      if (rb < rbest || (rb == rbest && q.z < best)) { rbest = rb; best = q.z; }
+      if (q.w >= 0) {
+        const float rc = direct_dist_f32(x, P.C + (size_t)q.w * P.DP, P.D);
+        if (rc < rbest || (rc == rbest && q.w < best)) { rbest = rc; best = q.w; }
+      }
+      P.labels[i] = best;
     }
   }
 }
