@@ -65,7 +65,7 @@ struct kmc_handle {
   int vgrid[8] = {0};
   long long vqstride[8] = {0};
   int max_regions = 0;
-  float alpha = 0, beta = 0, gref = 0;
+  float alpha = 0, beta = 0, gref = 0, xpk = 0;
   int idx_bits = 0;
 
   bool have_points = false, have_centroids = false;
@@ -139,7 +139,8 @@ double gam(double m, double u) { return m * u / (1.0 - m * u); }
 // and the packed index moves b_j by < 2^(idx_bits-23) b_j. All terms relative
 // to the distance go into gref; the band test evaluates them at the upper
 // bound b + E of each distance. Every constant carries a factor 2 of headroom.
-void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, float* gref) {
+void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, float* gref,
+                    float* xpk) {
   const double u = std::ldexp(1.0, -24);  // fp32 unit roundoff
   const double ub = std::ldexp(1.0, -9);  // bf16 unit roundoff
   double t_in, gA;
@@ -161,6 +162,9 @@ void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, fl
   *alpha = (float)(2.0 * a);
   *beta = (float)(2.0 * b);
   *gref = (float)(2.0 * g);
+  // single-product bf16 ranks ||c||^2 - 2 x.c and adds ||x||^2 afterwards:
+  // packing perturbation and the final add are then relative to d^2 + ||x||^2
+  *xpk = prec == kmc::PREC_BF16 ? (float)(2.0 * (pk + 2.0 * u)) : 0.0f;
 }
 
 kmc_status alloc_all(kmc_handle* h) {
@@ -286,6 +290,7 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1) {
   ap.alpha = h->alpha;
   ap.beta = h->beta;
   ap.gref = h->gref;
+  ap.xpk = h->xpk;
   ap.labels = h->labels;
   ap.q2 = h->q2;
   ap.qa = h->qa;
@@ -571,6 +576,7 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     bool ok = true;
     for (int v = 0; v < kmc::assign_variants() && ok; ++v) {
       const void* fn = kmc::assign_kernel_ptr(DP, h->prec, v);
+      if (!fn) continue;  // variant not built for this (DP, precision)
       if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)h->assign_lds) != hipSuccess) {
         st = fail(h, KMC_ERR_HIP, "hipFuncSetAttribute(assign)");
@@ -584,7 +590,8 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
         per_cu = 1;
       const int tile = kmc::assign_tile(v);
       const long long tiles = (long long)((h->n + tile - 1) / tile);
-      h->vgrid[v] = (int)std::min<long long>(tiles, (long long)per_cu * h->cus);
+      h->vgrid[v] = (int)std::min<long long>(
+          {tiles, (long long)per_cu * h->cus, (long long)kmc::kMaxRegions});
       // each assign workgroup owns a queue region large enough for all its points
       h->vqstride[v] = (tiles + h->vgrid[v] - 1) / h->vgrid[v] * tile;
       h->max_regions = std::max(h->max_regions, h->vgrid[v]);
@@ -595,7 +602,7 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     h->qstride = h->vqstride[0];
     h->idx_bits = 0;
     while ((1 << h->idx_bits) < h->Kp) ++h->idx_bits;
-    band_constants(h->prec, h->D, h->idx_bits, &h->alpha, &h->beta, &h->gref);
+    band_constants(h->prec, h->D, h->idx_bits, &h->alpha, &h->beta, &h->gref, &h->xpk);
     st = alloc_all(h);
   } while (0);
   if (st != KMC_OK) {
@@ -758,8 +765,8 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
 }
 
 kmc_status kmc_set_assign_variant(kmc_handle* h, int variant) {
-  if (!h || variant < 0 || variant >= kmc::assign_variants())
-    return fail(h, KMC_ERR_ARG, "kmc_set_assign_variant: no such variant");
+  if (!h || variant < 0 || variant >= kmc::assign_variants() || h->vgrid[variant] == 0)
+    return fail(h, KMC_ERR_ARG, "kmc_set_assign_variant: no such variant for this problem");
   h->variant = variant;
   h->assign_grid = h->vgrid[variant];
   h->qstride = h->vqstride[variant];
@@ -781,6 +788,10 @@ kmc_status kmc_autotune(kmc_handle* h, kmc_tuning* out) {
   kmc_tuning t{};
   t.nvariants = nv;
   for (int v = 0; v < nv; ++v) {
+    if (h->vgrid[v] == 0) {  // not built for this (DP, precision)
+      t.ms[v] = -1.0f;
+      continue;
+    }
     st = kmc_set_assign_variant(h, v);
     if (st != KMC_OK) break;
     // one untimed launch, then two timed ones

@@ -9,6 +9,8 @@ namespace kmc {
 enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_BF16 = 2 };
 enum { UPDATE_ORDERED = 0, UPDATE_TREE = 1 };
 constexpr int kTreeRows = 256;  // sorted rows per tree-update chunk
+constexpr int kMaxRegions = 4096;  // assign workgroups (refinement queue regions)
+constexpr int kRefineBlocks = 1024;  // refinement grid
 constexpr int kMaxDP = 256;     // largest padded dimension of the Lloyd path
 // ||c||^2 of padding rows and excluded clusters: finite (so that index bits
 // can be packed into it) and above any real distance
@@ -58,6 +60,7 @@ struct AssignParams {
   int K, Kp, kchunk;
   uint32_t idx_mask;        // low mantissa bits that carry the centroid index
   float alpha, beta, gref;  // margin-band constants
+  float xpk;  // packing bound relative to ||x||^2 (forms that accumulate without it)
   int* labels;
   int* q2;          // (point, cand1, cand2, cand3 or -1), one region per workgroup
   int* qa;          // points needing all-k refinement, one region per workgroup

@@ -19,6 +19,7 @@
 //
 // Every kernel returns at once when the device stop flag is set, so the host
 // may queue iterations ahead of the convergence decision.
+#include "kmc.h"
 #include "kmc_device.h"
 #include "kmc_kernels.h"
 
@@ -166,55 +167,85 @@ __device__ __forceinline__ void load_xfrag(const AssignParams& P, long long prow
   }
 }
 
-// One 32-centroid block on MFMA: all A fragments of the block are read from
-// LDS first, then the MFMA chain runs from ||c||^2.
+// Registers per lane of one point set's fragments.
 template <int DP, int PREC>
-__device__ __forceinline__ kmc_f32x16 mfma_block(const XFrag<DP, PREC>& x, const float* cn_s,
-                                                 const unsigned char* rows, int cnt, int cb,
-                                                 int p, int h, float x2) {
+constexpr int frag_regs() {
+  return PREC == PREC_FP32 ? DP / 2 : (PREC == PREC_BF16X3 ? DP / 2 : DP / 4);
+}
+
+// Register-resident A-operand fragments of one 32-centroid block (lane l:
+// centroid row cb + (l & 31), k-slice h = l >> 5, same permuted dimension
+// order as the point fragments).
+template <int DP, int PREC>
+struct AFrag {
+  static constexpr int NF = (PREC == PREC_FP32) ? DP / 8 : 1;
+  static constexpr int NH = (PREC == PREC_FP32) ? 1 : DP / 16;
+  static constexpr int NL = (PREC == PREC_BF16X3) ? DP / 16 : 1;
+  float4 f[NF];
+  uint4 hi[NH];
+  uint4 lo[NL];
+};
+
+template <int DP, int PREC>
+__device__ __forceinline__ void load_afrag(const unsigned char* rows, int cnt, int cb, int p,
+                                           int h, AFrag<DP, PREC>& a) {
   using T = AssignTraits<DP, PREC>;
-  kmc_f32x16 acc;
-#pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const float4 n4 = *reinterpret_cast<const float4*>(cn_s + cb + 8 * g + 4 * h);
-    acc[4 * g + 0] = n4.x + x2;
-    acc[4 * g + 1] = n4.y + x2;
-    acc[4 * g + 2] = n4.z + x2;
-    acc[4 * g + 3] = n4.w + x2;
-  }
   const unsigned char* arow = rows + (size_t)(cb + p) * T::kRowBytes;
   if (PREC == PREC_FP32) {
-    const float* a = reinterpret_cast<const float*>(arow);
-    float4 a4[DP / 8];
+    const float* af = reinterpret_cast<const float*>(arow);
 #pragma unroll
-    for (int q = 0; q < DP / 8; ++q) a4[q] = *reinterpret_cast<const float4*>(a + 8 * q + 4 * h);
-#pragma unroll
-    for (int q = 0; q < DP / 8; ++q) {
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[q].x, x.f[4 * q + 0], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[q].y, x.f[4 * q + 1], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[q].z, x.f[4 * q + 2], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[q].w, x.f[4 * q + 3], acc, 0, 0, 0);
-    }
+    for (int q = 0; q < DP / 8; ++q) a.f[q] = *reinterpret_cast<const float4*>(af + 8 * q + 4 * h);
   } else {
     const uint16_t* ah = reinterpret_cast<const uint16_t*>(arow);
     const uint16_t* al = reinterpret_cast<const uint16_t*>(
         rows + (size_t)cnt * T::kRowBytes + (size_t)(cb + p) * T::kRowBytes);
-    uint4 a_hi[DP / 16], a_lo[PREC == PREC_BF16X3 ? DP / 16 : 1];
 #pragma unroll
     for (int s = 0; s < DP / 16; ++s) {
-      a_hi[s] = *reinterpret_cast<const uint4*>(ah + 16 * s + 8 * h);
-      if (PREC == PREC_BF16X3) a_lo[s] = *reinterpret_cast<const uint4*>(al + 16 * s + 8 * h);
+      a.hi[s] = *reinterpret_cast<const uint4*>(ah + 16 * s + 8 * h);
+      if (PREC == PREC_BF16X3) a.lo[s] = *reinterpret_cast<const uint4*>(al + 16 * s + 8 * h);
     }
+  }
+}
+
+// Accumulator start for the block at cb: ||c||^2, plus ||x||^2 when kAddX2.
+template <bool kAddX2>
+__device__ __forceinline__ kmc_f32x16 init_acc(const float* cn_s, int cb, int h, float x2) {
+  kmc_f32x16 acc;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const float4 n4 = *reinterpret_cast<const float4*>(cn_s + cb + 8 * g + 4 * h);
+    acc[4 * g + 0] = kAddX2 ? n4.x + x2 : n4.x;
+    acc[4 * g + 1] = kAddX2 ? n4.y + x2 : n4.y;
+    acc[4 * g + 2] = kAddX2 ? n4.z + x2 : n4.z;
+    acc[4 * g + 3] = kAddX2 ? n4.w + x2 : n4.w;
+  }
+  return acc;
+}
+
+// The MFMA chain of one block against one point set: acc += (-2c) . x
+// (bf16x3: hi.hi + hi.lo + lo.hi).
+template <int DP, int PREC>
+__device__ __forceinline__ kmc_f32x16 mfma_chain(const AFrag<DP, PREC>& a,
+                                                 const XFrag<DP, PREC>& x, kmc_f32x16 acc) {
+  if (PREC == PREC_FP32) {
+#pragma unroll
+    for (int q = 0; q < DP / 8; ++q) {
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.f[q].x, x.f[4 * q + 0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.f[q].y, x.f[4 * q + 1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.f[q].z, x.f[4 * q + 2], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.f[q].w, x.f[4 * q + 3], acc, 0, 0, 0);
+    }
+  } else {
 #pragma unroll
     for (int s = 0; s < DP / 16; ++s) {
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a_hi[s]),
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a.hi[s]),
                                                     __builtin_bit_cast(kmc_bf16x8, x.hi[s]),
                                                     acc, 0, 0, 0);
       if (PREC == PREC_BF16X3) {
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a_hi[s]),
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a.hi[s]),
                                                       __builtin_bit_cast(kmc_bf16x8, x.lo[s]),
                                                       acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a_lo[s]),
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a.lo[s]),
                                                       __builtin_bit_cast(kmc_bf16x8, x.hi[s]),
                                                       acc, 0, 0, 0);
       }
@@ -224,49 +255,72 @@ __device__ __forceinline__ kmc_f32x16 mfma_block(const XFrag<DP, PREC>& x, const
 }
 
 // Top-4 update over one block's 16 accumulator values. Each value carries
-// its centroid index in the low mantissa bits (idx[r] | (v & ~mask)), so the
-// running top-4 is four v_med3_f32 per value and the indices travel with
-// the values. The values are full squared distances, so the perturbation is
-// relative to the (small) distances of the leading candidates; it is part of
-// the margin-band bound.
+// its centroid index in the low mantissa bits ((v & ~mask) | idx), so the
+// running top-4 is four v_med3_f32 per value and the indices travel with the
+// values. The packed index is the wave-uniform part base + 8*(r>>2) + (r&3)
+// (bit 2 always clear); the lane's k-half bit 4h is or-ed in once, when the
+// top-4 is read out. The perturbation of the low bits is part of the
+// margin-band bound.
 // Insert v into the running top-4 b1 <= b2 <= b3 <= b4 (the k-th smallest of
 // {b1..b4, v} is med3(b_{k-1}, b_k, v)).
-__device__ __forceinline__ void top4_push(float v, float& b1, float& b2, float& b3, float& b4) {
-  b4 = __builtin_amdgcn_fmed3f(b3, b4, v);
-  b3 = __builtin_amdgcn_fmed3f(b2, b3, v);
-  b2 = __builtin_amdgcn_fmed3f(b1, b2, v);
-  b1 = __builtin_amdgcn_fmed3f(-__builtin_inff(), b1, v);
+// ninf is -inf held in a vector register the compiler cannot see through,
+// so the first insertion stays one v_med3_f32 (a min would be preceded by a
+// canonicalisation of the bit-built value).
+__device__ __forceinline__ void top4_push(float v, float (&b)[4], float ninf) {
+  b[3] = __builtin_amdgcn_fmed3f(b[2], b[3], v);
+  b[2] = __builtin_amdgcn_fmed3f(b[1], b[2], v);
+  b[1] = __builtin_amdgcn_fmed3f(b[0], b[1], v);
+  b[0] = __builtin_amdgcn_fmed3f(ninf, b[0], v);
 }
 
-__device__ __forceinline__ void push_block(const kmc_f32x16& acc, const uint32_t* idx,
-                                           uint32_t keep, float& b1, float& b2, float& b3,
-                                           float& b4) {
+// keepv is the mask in a vector register (the wave-uniform index base is a
+// scalar; one v_and_or_b32 may read only one scalar operand).
+__device__ __forceinline__ void push_block(const kmc_f32x16& acc, uint32_t base, uint32_t keepv,
+                                           float (&b)[4], float ninf) {
 #pragma unroll
   for (int r = 0; r < 16; ++r)
-    top4_push(__uint_as_float((__float_as_uint(acc[r]) & keep) | idx[r]), b1, b2, b3, b4);
+    top4_push(__uint_as_float((__float_as_uint(acc[r]) & keepv) |
+                              (base + (uint32_t)(8 * (r >> 2) + (r & 3)))),
+              b, ninf);
+}
+
+// A copy of a wave-uniform value in a vector register, opaque to the
+// compiler's uniformity analysis.
+__device__ __forceinline__ uint32_t opaque_vgpr(uint32_t s) {
+  uint32_t v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+  return v;
 }
 
 // Work split: a workgroup of NW waves shares one LDS copy of the centroid
 // operands (all of them when they fit, else chunk by chunk); each wave owns
-// 32 points per tile, keeps their operand fragments in registers for the
-// whole sweep over the centroids and, with PF, prefetches the fragments of
-// its next tile while sweeping. The grid is persistent. (NW, PF) are chosen
-// per problem by the autotuner (kmc_autotune); the LDS copy limits a CU to
-// one workgroup when all centroids are resident, so NW sets the waves per
-// SIMD.
-template <int DP, int PREC, int NW, bool PF>
+// MS sets of 32 points per tile, keeps their operand fragments in registers
+// for the whole sweep over the centroids and, with PF, prefetches the
+// fragments of its next tile while sweeping. Every A fragment read from LDS
+// feeds MS MFMA chains (MS = 2 halves the LDS operand traffic), and the
+// chains of one block run while the previous block's epilogue executes.
+// The grid is persistent. (NW, MS, PF) are chosen per problem by the
+// autotuner (kmc_autotune); the LDS copy limits a CU to one workgroup when
+// all centroids are resident, so NW sets the waves per SIMD.
+//
+// Single-product bf16 accumulates ||c||^2 - 2 x.c without ||x||^2 (the
+// ranking does not depend on it) and adds ||x||^2 to the four candidates
+// after the sweep: one VALU op less per value. The packing perturbation is
+// then relative to | ||c||^2 - 2 x.c | <= d^2 + ||x||^2, which the band
+// covers through the xpk * ||x||^2 term.
+template <int DP, int PREC, int NW, int MS, bool PF>
 __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
   if (*P.done) return;
-  constexpr int kAssignWaves = NW;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  __shared__ float red_s[kAssignWaves];
+  __shared__ float red_s[NW];
   __shared__ unsigned q2c_s, qac_s;  // this workgroup's refinement queue fill
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int p = lane & 31;
   const int h = lane >> 5;
-  constexpr int kTile = 32 * kAssignWaves;  // points per workgroup tile
+  constexpr int kTile = 32 * MS * NW;  // points per workgroup tile
+  constexpr bool kAddX2 = PREC != PREC_BF16;
   if (threadIdx.x == 0) {
     q2c_s = 0u;
     qac_s = 0u;
@@ -286,10 +340,11 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
   __syncthreads();
   float cm_all = 0.0f;
 #pragma unroll
-  for (int w = 0; w < kAssignWaves; ++w) cm_all = fmaxf(cm_all, red_s[w]);
+  for (int w = 0; w < NW; ++w) cm_all = fmaxf(cm_all, red_s[w]);
   const float cmax = sqrtf(cm_all);
   const float Ecm = P.beta * cmax * cmax;
-  const uint32_t keep = ~P.idx_mask;
+  const uint32_t keepv = opaque_vgpr(~P.idx_mask);
+  const float ninf = __uint_as_float(opaque_vgpr(0xff800000u));
 
   const bool resident = P.kchunk >= P.Kp;
   if (resident) {
@@ -299,25 +354,35 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
 
   const long long ntiles = (P.n + kTile - 1) / kTile;
   // prefetch the next tile only while two fragment sets fit the registers
-  constexpr int kFragRegs = PREC == PREC_FP32 ? DP / 2 : (PREC == PREC_BF16X3 ? DP / 2 : DP / 4);
-  constexpr bool kPrefetch = PF && kFragRegs <= 64;
-  XFrag<DP, PREC> cur, nxt;
+  constexpr int kFragRegs = frag_regs<DP, PREC>();
+  constexpr bool kPrefetch = PF && MS == 1 && kFragRegs <= 64;
+  auto row_of = [&](long long t, int m) {
+    return t * kTile + (long long)(wave * MS + m) * 32 + p;
+  };
+  XFrag<DP, PREC> cur[MS], nxt;
   long long tile = blockIdx.x;
   if (tile < ntiles) {
-    const long long prow = tile * kTile + wave * 32 + p;
-    load_xfrag<DP, PREC>(P, prow, prow < P.n, h, cur);
+#pragma unroll
+    for (int m = 0; m < MS; ++m) {
+      const long long r = row_of(tile, m);
+      load_xfrag<DP, PREC>(P, r, r < P.n, h, cur[m]);
+    }
   }
   for (; tile < ntiles; tile += gridDim.x) {
-    const long long prow = tile * kTile + wave * 32 + p;
-    const bool valid = prow < P.n;
     const long long ntile = tile + gridDim.x;
     if (kPrefetch && ntile < ntiles) {
-      const long long nrow = ntile * kTile + wave * 32 + p;
-      load_xfrag<DP, PREC>(P, nrow, nrow < P.n, h, nxt);
+      const long long r = row_of(ntile, 0);
+      load_xfrag<DP, PREC>(P, r, r < P.n, h, nxt);
     }
 
-    float b1 = kFar, b2 = kFar, b3 = kFar, b4 = kFar;
-    const float x2 = valid ? P.xn2[prow] : 0.0f;
+    float b[MS][4], x2[MS];
+#pragma unroll
+    for (int m = 0; m < MS; ++m) {
+      const long long r = row_of(tile, m);
+      x2[m] = r < P.n ? P.xn2[r] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) b[m][k] = kFar;
+    }
     for (int c0 = 0; c0 < P.Kp; c0 += P.kchunk) {
       const int cnt = min(P.kchunk, P.Kp - c0);
       if (!resident) {
@@ -327,94 +392,123 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
       }
       const float* cn_s = reinterpret_cast<const float*>(smem);
       const unsigned char* rows = smem + (size_t)cnt * 4;
-      // centroid index of accumulator r of the first block of this chunk
-      uint32_t idx0[16], idx1[16];
+      // Software pipeline over the 32-centroid blocks of the chunk: the
+      // MFMA chains of block s are issued before the top-4 epilogue of block
+      // s-1 (independent registers), so one wave keeps both the matrix
+      // pipe and the VALU busy. The two accumulator sets alternate.
+      auto chains = [&](int blk, kmc_f32x16(&acc)[MS]) {
+        AFrag<DP, PREC> a;
+        load_afrag<DP, PREC>(rows, cnt, 32 * blk, p, h, a);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        idx0[r] = (uint32_t)(c0 + 4 * h + 8 * (r >> 2) + (r & 3));
-        idx1[r] = idx0[r] + 32u;
+        for (int m = 0; m < MS; ++m)
+          acc[m] = mfma_chain<DP, PREC>(a, cur[m], init_acc<kAddX2>(cn_s, 32 * blk, h, x2[m]));
+      };
+      auto epilogue = [&](int blk, const kmc_f32x16(&acc)[MS]) {
+#pragma unroll
+        for (int m = 0; m < MS; ++m)
+          push_block(acc[m], (uint32_t)(c0 + 32 * blk), keepv, b[m], ninf);
+      };
+      const int nblk = cnt / 32;  // even: chunks are multiples of 64
+      kmc_f32x16 accA[MS], accB[MS];
+      chains(0, accA);
+      int blk = 1;
+      for (; blk + 1 < nblk; blk += 2) {
+        chains(blk, accB);
+        epilogue(blk - 1, accA);
+        chains(blk + 1, accA);
+        epilogue(blk, accB);
       }
-      // blocks in pairs: the second block's MFMAs run while the first
-      // block's values go through the top-3
-      for (int cb = 0; cb < cnt; cb += 64) {
-        const kmc_f32x16 a0 = mfma_block<DP, PREC>(cur, cn_s, rows, cnt, cb, p, h, x2);
-        const kmc_f32x16 a1 = mfma_block<DP, PREC>(cur, cn_s, rows, cnt, cb + 32, p, h, x2);
-        push_block(a0, idx0, keep, b1, b2, b3, b4);
-        push_block(a1, idx1, keep, b1, b2, b3, b4);
+      chains(blk, accB);  // blk == nblk - 1
+      epilogue(blk - 1, accA);
+      epilogue(blk, accB);
+    }
+
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          idx0[r] += 64u;
-          idx1[r] += 64u;
+    for (int m = 0; m < MS; ++m) {
+      const long long prow = row_of(tile, m);
+      const bool valid = prow < P.n;
+      float b1, b2, b3, b4;
+      {
+        // this lane's k-half into bit 2 of the packed indices, then merge
+        // the two halves' top-4
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          v[k] = __uint_as_float(__float_as_uint(b[m][k]) | (uint32_t)(4 * h));
+        float o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = __shfl_xor(v[k], 32, 64);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) top4_push(o[k], v, ninf);
+        b1 = v[0];
+        b2 = v[1];
+        b3 = v[2];
+        b4 = v[3];
+      }
+      const int i1 = (int)(__float_as_uint(b1) & P.idx_mask);
+      const int i2 = (int)(__float_as_uint(b2) & P.idx_mask);
+      const int i3 = (int)(__float_as_uint(b3) & P.idx_mask);
+      if (!kAddX2) {
+        b1 += x2[m];
+        b2 += x2[m];
+        b3 += x2[m];
+        b4 += x2[m];
+      }
+
+      // ---- margin band test and output ----
+      // b1 <= b2 <= b3 <= b4 are full squared distances (plus the packed
+      // index bits); E bounds the MFMA error of each, gref the reference's
+      // own float error plus the packing perturbation, both relative to the
+      // distances.
+      bool flag2 = false, flagA = false, few3 = false;
+      if (h == 0 && valid) {
+        const float E = P.alpha * sqrtf(x2[m]) * cmax + Ecm + P.xpk * x2[m];
+        const float twoE = 2.0f * E;
+        const float b1p = fmaxf(b1 + E, 0.0f);
+        // clear(b): every centroid ranked at b or later loses to the first
+        auto clear = [&](float bb) {
+          return (bb >= kFarTest) || (bb - b1 > twoE + P.gref * (fmaxf(bb + E, 0.0f) + b1p));
+        };
+        const bool uniq = clear(b2);
+        const bool two = clear(b3);
+        const bool three = clear(b4);
+        // a point without any finite distance goes to the all-k pass, which
+        // applies the first-min rule
+        const bool sane = (b1 < kFarTest) && (i1 < P.K);
+        P.labels[prow] = sane ? i1 : 0;
+        flag2 = sane && !uniq && (two || three);
+        flagA = !sane || (!uniq && !two && !three);
+        few3 = !two;
+      }
+      const uint64_t m2 = __ballot(flag2);
+      const uint64_t mA = __ballot(flagA);
+      if (m2) {
+        const int leader = __ffsll((unsigned long long)m2) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&q2c_s, (unsigned)__popcll(m2));
+        base = __shfl(base, leader, 64);
+        if (flag2) {
+          const unsigned pos = base + mbcnt64(m2);
+          *reinterpret_cast<int4*>(q2 + 4 * (size_t)pos) =
+              make_int4((int)prow, i1, i2, few3 ? i3 : -1);
         }
       }
-    }
-
-    // ---- merge the two k-slices' halves of the centroid rows ----
-    {
-      const float o1 = __shfl_xor(b1, 32, 64);
-      const float o2 = __shfl_xor(b2, 32, 64);
-      const float o3 = __shfl_xor(b3, 32, 64);
-      const float o4 = __shfl_xor(b4, 32, 64);
-      top4_push(o1, b1, b2, b3, b4);
-      top4_push(o2, b1, b2, b3, b4);
-      top4_push(o3, b1, b2, b3, b4);
-      top4_push(o4, b1, b2, b3, b4);
-    }
-    const int i1 = (int)(__float_as_uint(b1) & P.idx_mask);
-    const int i2 = (int)(__float_as_uint(b2) & P.idx_mask);
-    const int i3 = (int)(__float_as_uint(b3) & P.idx_mask);
-
-    // ---- margin band test and output ----
-    // b1 <= b2 <= b3 are full squared distances (plus the packed index bits);
-    // E bounds the MFMA error of each, gref the reference's own float error
-    // plus the packing perturbation, both relative to the distances.
-    bool flag2 = false, flagA = false, few3 = false;
-    if (h == 0 && valid) {
-      const float E = P.alpha * sqrtf(x2) * cmax + Ecm;
-      const float twoE = 2.0f * E;
-      const float b1p = fmaxf(b1 + E, 0.0f);
-      // clear(b): every centroid ranked at b or later loses to the first
-      auto clear = [&](float b) {
-        return (b >= kFarTest) || (b - b1 > twoE + P.gref * (fmaxf(b + E, 0.0f) + b1p));
-      };
-      const bool uniq = clear(b2);
-      const bool two = clear(b3);
-      const bool three = clear(b4);
-      // a point without any finite distance goes to the all-k pass, which
-      // applies the first-min rule
-      const bool sane = (b1 < kFarTest) && (i1 < P.K);
-      P.labels[prow] = sane ? i1 : 0;
-      flag2 = sane && !uniq && (two || three);
-      flagA = !sane || (!uniq && !two && !three);
-      few3 = !two;
-    }
-    const uint64_t m2 = __ballot(flag2);
-    const uint64_t mA = __ballot(flagA);
-    if (m2) {
-      const int leader = __ffsll((unsigned long long)m2) - 1;
-      unsigned base = 0;
-      if (lane == leader) base = atomicAdd(&q2c_s, (unsigned)__popcll(m2));
-      base = __shfl(base, leader, 64);
-      if (flag2) {
-        const unsigned pos = base + mbcnt64(m2);
-        q2[4 * (size_t)pos + 0] = (int)prow;
-        q2[4 * (size_t)pos + 1] = i1;
-        q2[4 * (size_t)pos + 2] = i2;
-        q2[4 * (size_t)pos + 3] = few3 ? i3 : -1;
+      if (mA) {
+        const int leader = __ffsll((unsigned long long)mA) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&qac_s, (unsigned)__popcll(mA));
+        base = __shfl(base, leader, 64);
+        if (flagA) qa[base + mbcnt64(mA)] = (int)prow;
       }
     }
-    if (mA) {
-      const int leader = __ffsll((unsigned long long)mA) - 1;
-      unsigned base = 0;
-      if (lane == leader) base = atomicAdd(&qac_s, (unsigned)__popcll(mA));
-      base = __shfl(base, leader, 64);
-      if (flagA) qa[base + mbcnt64(mA)] = (int)prow;
-    }
     if (kPrefetch) {
-      cur = nxt;
+      cur[0] = nxt;
     } else if (ntile < ntiles) {
-      const long long nrow = ntile * kTile + wave * 32 + p;
-      load_xfrag<DP, PREC>(P, nrow, nrow < P.n, h, cur);
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        const long long r = row_of(ntile, m);
+        load_xfrag<DP, PREC>(P, r, r < P.n, h, cur[m]);
+      }
     }
   }
   __syncthreads();
@@ -426,102 +520,203 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
 
 // ---------------------------------------------------------------------------
 // Refinement: reference float arithmetic, first-min rule.
+//
+// The assign workgroups leave their band points in per-workgroup queue
+// regions. Each refinement block first forms the exclusive prefix of the
+// region counts in LDS; entry t of the flattened queue then lives in the
+// region r with pre[r] <= t < pre[r+1], so the work spreads evenly over the
+// grid however the band points cluster in memory.
 // ---------------------------------------------------------------------------
+__device__ unsigned region_prefix(const unsigned* cnt, int nregions, unsigned* pre_s) {
+  __shared__ unsigned part_s[256];
+  const int per = (nregions + 255) / 256;
+  const int lo = min(nregions, (int)threadIdx.x * per), hi = min(nregions, lo + per);
+  unsigned s = 0;
+  for (int r = lo; r < hi; ++r) s += cnt[r];
+  part_s[threadIdx.x] = s;
+  __syncthreads();
+  // inclusive scan of 256 slice sums (Hillis-Steele)
+  for (int o = 1; o < 256; o <<= 1) {
+    const unsigned v = threadIdx.x >= (unsigned)o ? part_s[threadIdx.x - o] : 0u;
+    __syncthreads();
+    part_s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned run = part_s[threadIdx.x] - s;
+  for (int r = lo; r < hi; ++r) {
+    pre_s[r] = run;
+    run += cnt[r];
+  }
+  if (threadIdx.x == 255) pre_s[nregions] = part_s[255];
+  __syncthreads();
+  return pre_s[nregions];
+}
+
+// region holding flattened entry t (pre is nondecreasing, pre[0] = 0)
+__device__ __forceinline__ int region_of(const unsigned* pre_s, int nregions, unsigned t) {
+  int lo = 0, hi = nregions - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre_s[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Band points with two or three candidates (the rest provably lose): each
 // candidate's distance in the reference's float arithmetic, first-min rule
-// (the smallest distance, the lowest index among equal ones). Block r
-// handles queue region r (written by assign workgroup r).
+// (the smallest distance, the lowest index among equal ones). One thread per
+// queue entry.
 __global__ __launch_bounds__(256) void k_refine2(RefineParams P) {
   if (*P.done) return;
-  for (int r = blockIdx.x; r < P.nregions; r += gridDim.x) {
-    const unsigned cnt = P.q2cnt[r];
-    const int* q2 = P.q2 + 4 * (size_t)r * P.qstride;
-    for (unsigned e = threadIdx.x; e < cnt; e += blockDim.x) {
-      const int4 q = *reinterpret_cast<const int4*>(q2 + 4 * (size_t)e);
-      const int i = q.x;
-      const float* x = P.X + (size_t)i * P.DP;
-      int best = q.y;
-      float rbest = direct_dist_f32(x, P.C + (size_t)q.y * P.DP, P.D);
-      const float rb = direct_dist_f32(x, P.C + (size_t)q.z * P.DP, P.D);
-      if (rb < rbest || (rb == rbest && q.z < best)) { rbest = rb; best = q.z; }
-      if (q.w >= 0) {
-        const float rc = direct_dist_f32(x, P.C + (size_t)q.w * P.DP, P.D);
-        if (rc < rbest || (rc == rbest && q.w < best)) { rbest = rc; best = q.w; }
-      }
-      P.labels[i] = best;
+  __shared__ unsigned pre_s[kMaxRegions + 1];
+  const unsigned total = region_prefix(P.q2cnt, P.nregions, pre_s);
+  for (unsigned t = blockIdx.x * 256u + threadIdx.x; t < total; t += gridDim.x * 256u) {
+    const int r = region_of(pre_s, P.nregions, t);
+    const int* q2 = P.q2 + 4 * ((size_t)r * P.qstride + (t - pre_s[r]));
+    const int4 q = *reinterpret_cast<const int4*>(q2);
+    const int i = q.x;
+    const float* x = P.X + (size_t)i * P.DP;
+    int best = q.y;
+    float rbest = direct_dist_f32(x, P.C + (size_t)q.y * P.DP, P.D);
+    const float rb = direct_dist_f32(x, P.C + (size_t)q.z * P.DP, P.D);
+    if (rb < rbest || (rb == rbest && q.z < best)) { rbest = rb; best = q.z; }
+    if (q.w >= 0) {
+      const float rc = direct_dist_f32(x, P.C + (size_t)q.w * P.DP, P.D);
+      if (rc < rbest || (rc == rbest && q.w < best)) { rbest = rc; best = q.w; }
     }
+    P.labels[i] = best;
   }
 }
 
-constexpr int kRefineSplit = 4;
-
-// One wave per queued point; lane c evaluates centroids c, c+64, c+128,
-// c+192 from the transposed centroids (coalesced across lanes). Dimensions go
-// in chunks of 16: the 64 centroid values and 16 point values of a chunk are
-// loaded first, then each of the four distances advances in the reference's
-// float order (padding dimensions are zero in X and C and add exact zeros).
-// First-min across lanes.
+// Band points whose four nearest MFMA candidates are all ambiguous: every
+// centroid in the reference's float arithmetic. A workgroup takes
+// kRefinePts consecutive queue entries and stages their rows in LDS; wave w
+// evaluates centroids w*64 + lane + 256*i from the transposed centroids
+// (coalesced across lanes, the next 16 dimensions prefetched while the
+// current ones are used), reading the point values as LDS broadcasts. Each
+// distance advances in the reference's float order (padding dimensions are
+// zero in X and C and add exact zeros). First-min across lanes, then across
+// the waves.
+constexpr int kRefinePts = 4;
 __global__ __launch_bounds__(256) void k_refine_all(RefineParams P) {
   if (*P.done) return;
-  const int lane = threadIdx.x & 63;
-  // kRefineSplit blocks share one region; wave w of block part q takes
-  // entries w*kRefineSplit + q, stepping by 4*kRefineSplit
-  const int part = blockIdx.x % kRefineSplit;
-  const int wave = (threadIdx.x >> 6) * kRefineSplit + part;
-  for (int r = blockIdx.x / kRefineSplit; r < P.nregions; r += gridDim.x / kRefineSplit) {
-    const unsigned cnt = P.qacnt[r];
-    const int* qa = P.qa + (size_t)r * P.qstride;
-    for (unsigned e = wave; e < cnt; e += 4 * kRefineSplit) {
-      const int i = qa[e];
-      const float* x = P.X + (size_t)i * P.DP;
-      float best = __builtin_inff();
-      int bi = 0x7fffffff;
-      for (int cg = 0; cg < P.K; cg += 256) {
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-        int cc[4];
+  __shared__ unsigned pre_s[kMaxRegions + 1];
+  __shared__ __attribute__((aligned(16))) float x_s[kRefinePts][kMaxDP];
+  __shared__ float bv_s[4][kRefinePts];
+  __shared__ int bi_s[4][kRefinePts];
+  __shared__ int pid_s[kRefinePts];
+  const unsigned total = region_prefix(P.qacnt, P.nregions, pre_s);
+  if (total == 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned ngroups = (total + kRefinePts - 1) / kRefinePts;
+  for (unsigned g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    __syncthreads();  // the previous group is done with the LDS
+    if (threadIdx.x < kRefinePts) {
+      const unsigned t = min(g * kRefinePts + threadIdx.x, total - 1);
+      const int r = region_of(pre_s, P.nregions, t);
+      pid_s[threadIdx.x] = P.qa[(size_t)r * P.qstride + (t - pre_s[r])];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kRefinePts * P.DP; e += 256) {
+      const int q = e / P.DP, j = e - q * P.DP;
+      x_s[q][j] = P.X[(size_t)pid_s[q] * P.DP + j];
+    }
+    __syncthreads();
+    float best[kRefinePts];
+    int bi[kRefinePts];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) cc[u] = min(cg + 64 * u + lane, P.Kp - 1);
-        for (int j0 = 0; j0 < P.DP; j0 += 16) {
-          float xv[16], cv[4][16];
+    for (int q = 0; q < kRefinePts; ++q) {
+      best[q] = __builtin_inff();
+      bi[q] = 0x7fffffff;
+    }
+    for (int c = wave * 64 + lane; c - lane < P.K; c += 256) {
+      const int cc = min(c, P.Kp - 1);
+      const float* ct = P.CT + cc;
+      float acc[kRefinePts];
 #pragma unroll
-          for (int jj = 0; jj < 16; ++jj) xv[jj] = x[j0 + jj];
+      for (int q = 0; q < kRefinePts; ++q) acc[q] = 0.0f;
+      float cv[16], cnx[16];
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+      for (int jj = 0; jj < 16; ++jj) cv[jj] = ct[(size_t)jj * P.Kp];
+      for (int j0 = 0; j0 < P.DP; j0 += 16) {
+        if (j0 + 16 < P.DP) {
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj)
-              cv[u][jj] = P.CT[(size_t)(j0 + jj) * P.Kp + cc[u]];
+          for (int jj = 0; jj < 16; ++jj) cnx[jj] = ct[(size_t)(j0 + 16 + jj) * P.Kp];
+        }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+        for (int q = 0; q < kRefinePts; ++q) {
+          float xv[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
+          for (int jj = 0; jj < 16; jj += 4) {
+            const float4 v = *reinterpret_cast<const float4*>(&x_s[q][j0 + jj]);
+            xv[jj] = v.x;
+            xv[jj + 1] = v.y;
+            xv[jj + 2] = v.z;
+            xv[jj + 3] = v.w;
+          }
+#pragma unroll
+          for (int jj = 0; jj < 16; ++jj) {
 #pragma clang fp contract(off)
-              const float t = xv[jj] - cv[u][jj];
-              const float q = t * t;
-              acc[u] = acc[u] + q;
-            }
+            const float t = xv[jj] - cv[jj];
+            const float sq = t * t;
+            acc[q] = acc[q] + sq;
           }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int c = cg + 64 * u + lane;
-          if (c < P.K && P.alive[c] && acc[u] < best) { best = acc[u]; bi = c; }
-        }
+        for (int jj = 0; jj < 16; ++jj) cv[jj] = cnx[jj];
       }
+      const bool ok = c < P.K && P.alive[cc];
+#pragma unroll
+      for (int q = 0; q < kRefinePts; ++q)
+        if (ok && acc[q] < best[q]) {
+          best[q] = acc[q];
+          bi[q] = c;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kRefinePts; ++q) {
+      float b = best[q];
+      int ix = bi[q];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
-        const float ov = __shfl_xor(best, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ov < best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        const float ov = __shfl_xor(b, o, 64);
+        const int oi = __shfl_xor(ix, o, 64);
+        if (ov < b || (ov == b && oi < ix)) {
+          b = ov;
+          ix = oi;
+        }
       }
-      if (lane == 0) P.labels[i] = (bi == 0x7fffffff) ? 0 : bi;
+      if (lane == 0) {
+        bv_s[wave][q] = b;
+        bi_s[wave][q] = ix;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < kRefinePts) {
+      const int q = threadIdx.x;
+      float b = bv_s[0][q];
+      int ix = bi_s[0][q];
+#pragma unroll
+      for (int w = 1; w < 4; ++w) {
+        const float ov = bv_s[w][q];
+        const int oi = bi_s[w][q];
+        if (ov < b || (ov == b && oi < ix)) {
+          b = ov;
+          ix = oi;
+        }
+      }
+      if (g * kRefinePts + q < total) P.labels[pid_s[q]] = (ix == 0x7fffffff) ? 0 : ix;
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Stable counting sort of point ids by label. One wave per block of CH points.
+// Stable counting sort of point ids by label. One wave per block of CH points
+// (CH a multiple of 1024); every lane loads its 16 labels of a 1024-point
+// round before any is used, so one memory latency covers the round.
 // bcount is cluster-major: bcount[c * nblk + b].
 // ---------------------------------------------------------------------------
+constexpr int kSortRound = 1024;
 __global__ __launch_bounds__(64) void k_hist(SortParams P) {
   if (*P.done) return;
   extern __shared__ int cnt_s[];
@@ -530,12 +725,23 @@ __global__ __launch_bounds__(64) void k_hist(SortParams P) {
   __syncthreads();
   const long long lo = (long long)b * P.CH;
   const long long hi = min(lo + P.CH, (long long)P.n);
-  for (long long i = lo + threadIdx.x; i < hi; i += 64) atomicAdd(&cnt_s[P.labels[i]], 1);
+  for (long long i0 = lo; i0 < hi; i0 += kSortRound) {
+    int lab[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const long long i = i0 + u * 64 + threadIdx.x;
+      lab[u] = i < hi ? P.labels[i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (lab[u] >= 0) atomicAdd(&cnt_s[lab[u]], 1);
+  }
   __syncthreads();
   for (int c = threadIdx.x; c < P.K; c += 64) P.bcount[(size_t)c * P.nblk + b] = cnt_s[c];
 }
 
-// Exclusive scan over blocks for each cluster (one wave per cluster).
+// Exclusive scan over blocks for each cluster (one wave per cluster); each
+// lane takes 8 consecutive block counts per round, all loaded first.
 __global__ __launch_bounds__(256) void k_scan_blocks(SortParams P) {
   if (*P.done) return;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -543,16 +749,32 @@ __global__ __launch_bounds__(256) void k_scan_blocks(SortParams P) {
   if (c >= P.K) return;
   int* a = P.bcount + (size_t)c * P.nblk;
   int carry = 0;
-  for (int base = 0; base < P.nblk; base += 64) {
-    const int idx = base + lane;
-    const int v = idx < P.nblk ? a[idx] : 0;
-    int incl = v;
+  for (int base = 0; base < P.nblk; base += 512) {
+    int v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = base + lane * 8 + u;
+      v[u] = idx < P.nblk ? a[idx] : 0;
+    }
+    int tot = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = v[u];
+      v[u] = tot;
+      tot += t;
+    }
+    int incl = tot;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const int t = __shfl_up(incl, o, 64);
       if (lane >= o) incl += t;
     }
-    if (idx < P.nblk) a[idx] = carry + incl - v;
+    const int off = carry + incl - tot;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = base + lane * 8 + u;
+      if (idx < P.nblk) a[idx] = off + v[u];
+    }
     carry += __shfl(incl, 63, 64);
   }
   if (lane == 0) P.count[c] = carry;
@@ -603,22 +825,31 @@ __global__ __launch_bounds__(64) void k_scatter(SortParams P) {
   const long long lo = (long long)b * P.CH;
   const long long hi = min(lo + P.CH, (long long)P.n);
   const uint64_t lt = lanemask_lt();
-  for (long long i0 = lo; i0 < hi; i0 += 64) {
-    const long long i = i0 + lane;
-    const bool valid = i < hi;
-    const int lab = valid ? P.labels[i] : 0;
-    uint64_t match = __ballot(valid);
-    for (int bit = 0; bit < P.lbits; ++bit) {
-      const bool on = (lab >> bit) & 1;
-      const uint64_t B = __ballot(on);
-      match &= on ? B : ~B;
+  for (long long r0 = lo; r0 < hi; r0 += kSortRound) {
+    int labs[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const long long i = r0 + u * 64 + lane;
+      labs[u] = i < hi ? P.labels[i] : -1;
     }
-    const int rank = __popcll(match & lt);
-    const int pos = valid ? cur_s[lab] + rank : 0;
-    __builtin_amdgcn_wave_barrier();
-    if (valid && rank == 0) cur_s[lab] = pos + __popcll(match);
-    if (valid) P.members[pos] = (int)i;
-    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const long long i = r0 + u * 64 + lane;
+      const bool valid = labs[u] >= 0;
+      const int lab = valid ? labs[u] : 0;
+      uint64_t match = __ballot(valid);
+      for (int bit = 0; bit < P.lbits; ++bit) {
+        const bool on = (lab >> bit) & 1;
+        const uint64_t B = __ballot(on);
+        match &= on ? B : ~B;
+      }
+      const int rank = __popcll(match & lt);
+      const int pos = valid ? cur_s[lab] + rank : 0;
+      __builtin_amdgcn_wave_barrier();
+      if (valid && rank == 0) cur_s[lab] = pos + __popcll(match);
+      if (valid) P.members[pos] = (int)i;
+      __builtin_amdgcn_wave_barrier();
+    }
   }
 }
 
@@ -787,21 +1018,26 @@ __global__ __launch_bounds__(256) void k_update_tree(UpdateParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// k_finish: one wave per centroid. With the tree update it first forms
-// c_new = (partials summed in chunk order) / count, rounded once (an empty
-// cluster becomes the zero vector). Then the float shift in the reference's
-// order on lane 0 (comparisons/CPU_Kmeans/src/kmeans.cpp:225-234), the FP64
-// shift, and C <- C_new, lane-parallel.
+// k_finish: one workgroup of kFinishWaves waves per centroid. With the tree
+// update the waves first add the cluster's chunk partials: wave w takes
+// partials q0+w, q0+w+kFinishWaves, ... in order, and the wave sums are added
+// in wave order, so the result depends only on the partials (a cluster of
+// any size costs the same few load rounds per wave). c_new = sum / count,
+// rounded once (an empty cluster becomes the zero vector). Then, on wave 0,
+// the float shift in the reference's order on lane 0
+// (comparisons/CPU_Kmeans/src/kmeans.cpp:225-234), the FP64 shift, and
+// C <- C_new, lane-parallel.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_finish(FinishParams P) {
+constexpr int kFinishWaves = 8;
+__global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
   if (*P.done) return;
-  __shared__ float new_s[4][kMaxDP];
+  __shared__ double part_s[kFinishWaves][kMaxDP];
+  __shared__ float ns[kMaxDP];
+  __shared__ __attribute__((aligned(16))) float sq_s[kMaxDP];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int c = blockIdx.x * 4 + wave;
-  if (c >= P.K) return;
+  const int c = blockIdx.x;
   float* cr = P.C + (size_t)c * P.DP;
   float* nrow = P.Cnext + (size_t)c * P.DP;
-  float* ns = new_s[wave];
   const int cnt = P.count[c];
   if (P.tree) {
     long long q0 = 0, q1 = -1;
@@ -809,40 +1045,63 @@ __global__ __launch_bounds__(256) void k_finish(FinishParams P) {
       q0 = (long long)P.start[c] / kTreeRows;
       q1 = ((long long)P.start[c] + cnt - 1) / kTreeRows;
     }
+    constexpr int S = kFinishWaves;
     for (int j = lane; j < P.D; j += 64) {
-      // partials summed in chunk order; 8 loads in flight per step
       const double* tp = P.tpart + (size_t)c * P.DP + j;
       double sum = 0.0;
-      long long q = q0;
-      for (; q + 8 <= q1 + 1; q += 8) {
-        double v[8];
+      long long q = q0 + wave;
+      for (; q + 3 * S <= q1; q += 4 * S) {
+        double v[4];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = tp[(size_t)(q + u) * P.DP];
+        for (int u = 0; u < 4; ++u) v[u] = tp[(size_t)(q + u * S) * P.DP];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) sum += v[u];
+        for (int u = 0; u < 4; ++u) sum += v[u];
       }
-      for (; q <= q1; ++q) sum += tp[(size_t)q * P.DP];
-      const float v = (float)(sum / (double)(cnt > 1 ? cnt : 1));
-      ns[j] = v;
-      nrow[j] = v;
+      for (; q <= q1; q += S) sum += tp[(size_t)q * P.DP];
+      part_s[wave][j] = sum;
     }
-  } else {
+    __syncthreads();
+    if (wave == 0) {
+      for (int j = lane; j < P.D; j += 64) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < S; ++w) sum += part_s[w][j];
+        const float v = (float)(sum / (double)(cnt > 1 ? cnt : 1));
+        ns[j] = v;
+        nrow[j] = v;
+      }
+    }
+  } else if (wave == 0) {
     for (int j = lane; j < P.D; j += 64) ns[j] = nrow[j];
   }
+  if (wave != 0) return;
   __builtin_amdgcn_wave_barrier();
+  // the squared terms of the float shift are independent: form them
+  // lane-parallel (each is one rounded difference and one rounded product,
+  // as in the reference), then add them in dimension order on lane 0
   double s64 = 0.0;
   for (int j = lane; j < P.D; j += 64) {
-    const double td = (double)ns[j] - (double)cr[j];
+    const float o = cr[j], nv = ns[j];
+    const double td = (double)nv - (double)o;
     s64 = fma(td, td, s64);
+    {
+#pragma clang fp contract(off)
+      const float t = nv - o;
+      sq_s[j] = t * t;
+    }
   }
+  for (int j = P.D + lane; j < ((P.D + 3) & ~3); j += 64) sq_s[j] = 0.0f;
   s64 = wave_sum_f64(s64);
+  __builtin_amdgcn_wave_barrier();
   if (lane == 0) {
     float s = 0.0f;
-    for (int j = 0; j < P.D; ++j) {
+    for (int j = 0; j < P.D; j += 4) {
 #pragma clang fp contract(off)
-      const float t = ns[j] - cr[j];
-      const float q = t * t;
-      s = s + q;
+      const float4 v = *reinterpret_cast<const float4*>(&sq_s[j]);
+      s = s + v.x;
+      if (j + 1 < P.D) s = s + v.y;
+      if (j + 2 < P.D) s = s + v.z;
+      if (j + 3 < P.D) s = s + v.w;
     }
     P.conv[c] = (__fsqrt_rn(s) > P.tol) ? 0 : 1;
     P.shift2[c] = s64;
@@ -1005,17 +1264,28 @@ size_t assign_lds(int DP, int prec, int kchunk) {
 #undef KMC_LDS_CASE
 }
 
-// Launch variants of k_assign: (waves per workgroup, prefetch).
+// Launch variants of k_assign: (waves per workgroup, point sets per wave,
+// prefetch). Two point sets per wave need both fragment sets in registers;
+// where they do not fit (kFragRegs > 64) the variant is not built.
 struct VariantShape {
   int nw;
+  int ms;
   bool pf;
 };
-constexpr VariantShape kVariants[] = {{8, true}, {8, false}, {12, false}, {16, false}, {4, true}};
+constexpr VariantShape kVariants[] = {{8, 1, true},   {8, 1, false}, {12, 1, false},
+                                      {16, 1, false}, {4, 1, true},  {4, 2, false},
+                                      {8, 2, false}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+static_assert(kNumVariants <= KMC_MAX_VARIANTS, "variant table exceeds kmc_tuning");
 
 template <int DP, int PREC, int V>
 static const void* kernel_of() {
-  return reinterpret_cast<const void*>(k_assign<DP, PREC, kVariants[V].nw, kVariants[V].pf>);
+  constexpr VariantShape S = kVariants[V];
+  if constexpr (S.ms > 1 && frag_regs<DP, PREC>() > 64) {
+    return nullptr;
+  } else {
+    return reinterpret_cast<const void*>(k_assign<DP, PREC, S.nw, S.ms, S.pf>);
+  }
 }
 
 template <int DP, int PREC>
@@ -1026,6 +1296,8 @@ static const void* kernel_of_v(int v) {
     case 2: return kernel_of<DP, PREC, 2>();
     case 3: return kernel_of<DP, PREC, 3>();
     case 4: return kernel_of<DP, PREC, 4>();
+    case 5: return kernel_of<DP, PREC, 5>();
+    case 6: return kernel_of<DP, PREC, 6>();
     default: return nullptr;
   }
 }
@@ -1053,7 +1325,7 @@ const void* assign_kernel_ptr(int DP, int prec, int v) {
 
 int assign_variants() { return kNumVariants; }
 int assign_threads(int v) { return 64 * kVariants[v].nw; }
-int assign_tile(int v) { return 32 * kVariants[v].nw; }
+int assign_tile(int v) { return 32 * kVariants[v].ms * kVariants[v].nw; }
 
 hipError_t launch_assign(int DP, int prec, int v, const AssignParams& P, int grid, size_t lds,
                          hipStream_t s) {
@@ -1069,8 +1341,8 @@ hipError_t launch_prep(const PrepParams& P, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_refine(const RefineParams& P, hipStream_t s) {
-  hipLaunchKernelGGL(k_refine2, dim3(P.nregions), dim3(256), 0, s, P);
-  hipLaunchKernelGGL(k_refine_all, dim3(P.nregions * kRefineSplit), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_refine2, dim3(kRefineBlocks), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_refine_all, dim3(kRefineBlocks), dim3(256), 0, s, P);
   return hipGetLastError();
 }
 hipError_t launch_sort(const SortParams& P, hipStream_t s) {
@@ -1095,7 +1367,7 @@ hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_finish(const FinishParams& F, const StatsParams& S, hipStream_t s) {
-  hipLaunchKernelGGL(k_finish, dim3((F.K + 3) / 4), dim3(256), 0, s, F);
+  hipLaunchKernelGGL(k_finish, dim3(F.K), dim3(64 * kFinishWaves), 0, s, F);
   hipLaunchKernelGGL(k_stats, dim3(1), dim3(256), 0, s, S);
   return hipGetLastError();
 }
