@@ -257,7 +257,7 @@ hipError_t record_event(hipStream_t s, hipEvent_t e) {
   return hipStreamUpdateCaptureDependencies(s, &node, 1, hipStreamSetCaptureDependencies);
 }
 
-kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1) {
+kmc::PrepParams prep_params(kmc_handle* h) {
   kmc::PrepParams pp{};
   pp.C = h->C;
   pp.Cw = h->Cw;
@@ -271,7 +271,13 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1) {
   pp.Kp = h->Kp;
   pp.DP = h->DP;
   pp.prec = h->prec;
-  KMC_CK(kmc::launch_prep(pp, h->stream));
+  return pp;
+}
+
+// prep: form the distance operands from C first (the first iteration of a
+// run; later iterations get them from the previous iteration's k_finish)
+kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep = true) {
+  if (prep) KMC_CK(kmc::launch_prep(prep_params(h), h->stream));
 
   kmc::AssignParams ap{};
   ap.X = h->X;
@@ -372,7 +378,9 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty) {
   fp.D = h->D;
   fp.DP = h->DP;
   fp.exclude_empty = exclude_empty;
-  kmc::StatsParams st{};
+  fp.do_prep = 1;
+  fp.prep = prep_params(h);
+  kmc::StatsParams& st = fp.stats;
   st.ipart = h->ipart;
   st.nparts = h->nparts;
   st.shift2 = h->shift2;
@@ -385,7 +393,7 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty) {
   st.tol = h->cfg.tol;
   st.K = h->K;
   st.mode = mode;
-  KMC_CK(kmc::launch_finish(fp, st, h->stream));
+  KMC_CK(kmc::launch_finish(fp, h->stream));
   return KMC_OK;
 }
 
@@ -397,7 +405,7 @@ kmc_status enqueue_chunk(kmc_handle* h, uint64_t first, uint64_t m, int mode, in
     const uint64_t gi = first + it;
     hipEvent_t e0 = gi < ntimed ? h->events[2 * gi] : nullptr;
     hipEvent_t e1 = gi < ntimed ? h->events[2 * gi + 1] : nullptr;
-    kmc_status st = enqueue_assign(h, e0, e1);
+    kmc_status st = enqueue_assign(h, e0, e1, it == 0);
     if (st == KMC_OK) st = enqueue_update(h, mode, exclude);
     if (st != KMC_OK) return st;
   }

@@ -112,20 +112,6 @@ struct UpdateParams {
   int K, D, DP;
 };
 
-struct FinishParams {
-  float* C;
-  float* Cnext;
-  const double* tpart;  // tree partials (tree update only)
-  const int* start;
-  int tree;
-  int* conv;
-  double* shift2;
-  const int* count;  // cluster sizes of the assignment just used
-  int* alive;        // updated when exclude_empty is set
-  const int* done;
-  float tol;
-  int K, D, DP, exclude_empty;
-};
 
 struct StatsParams {
   const double* ipart;
@@ -139,6 +125,24 @@ struct StatsParams {
   const int* done;
   float tol;
   int K, mode;  // mode: CONV_*
+};
+
+struct FinishParams {
+  float* C;
+  float* Cnext;
+  const double* tpart;  // tree partials (tree update only)
+  const int* start;
+  int tree;
+  int* conv;
+  double* shift2;
+  const int* count;  // cluster sizes of the assignment just used
+  int* alive;        // updated when exclude_empty is set
+  const int* done;
+  float tol;
+  int K, D, DP, exclude_empty;
+  int do_prep;        // write the next iteration's distance operands (prep)
+  PrepParams prep;
+  StatsParams stats;  // k_stats, launched after k_finish
 };
 
 struct LoadParams {
@@ -164,7 +168,7 @@ const void* assign_kernel_ptr(int DP, int prec, int variant);
 hipError_t launch_refine(const RefineParams& P, hipStream_t s);
 hipError_t launch_sort(const SortParams& P, hipStream_t s);
 hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s);
-hipError_t launch_finish(const FinishParams& F, const StatsParams& S, hipStream_t s);
+hipError_t launch_finish(const FinishParams& F, hipStream_t s);
 hipError_t launch_load_points(const LoadParams& P, hipStream_t s);
 hipError_t launch_sum_f64(const double* v, long long n, double* out, double* part256,
                           hipStream_t s);

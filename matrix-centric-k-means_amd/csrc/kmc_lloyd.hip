@@ -1,21 +1,22 @@
 // kmc_lloyd.hip -- gfx950 kernels of one Lloyd iteration (linear kernel).
 //
 // Per iteration, in stream order:
-//   k_prep       centroid operands (-2c in fp32 / bf16 hi+lo), ||c||^2;
-//                clears the refinement queue counters
+//   k_prep       centroid operands (-2c in fp32 / bf16 hi+lo), ||c||^2 (first
+//                iteration of a run; later ones get them from k_finish)
 //   k_assign     fused distance GEMM on MFMA (||c||^2 - 2 x.c accumulated
-//                from the norm), running top-3 per point, margin band test;
+//                from the norm), running top-4 per point, margin band test;
 //                writes labels and queues the points inside the band
-//   k_refine2    band points with two candidates: both distances in the
-//                reference's float arithmetic, first-min rule
-//   k_refine_all band points with three or more candidates: all k distances
+//   k_refine2    band points with two or three candidates: their distances
+//                in the reference's float arithmetic, first-min rule
+//   k_refine_all band points with four or more candidates: all k distances
 //   k_hist / k_scan_blocks / k_scan_clusters / k_scatter
 //                stable counting sort of point ids by label (the one-hot
 //                V matrix in CSR form: row offsets + column ids)
 //   k_update     C = V X with per-cluster float sums in point order, plus the
 //                FP64 inertia of the assignment
-//   k_finish     per-centroid shift test, C <- C_next
-//   k_stats      FP64 inertia / shift reductions, iteration counter, stop flag
+//   k_finish     per-centroid shift test, C <- C_next, the next iteration's
+//                operands; its last workgroup runs the FP64 inertia / shift
+//                reductions, the iteration counter and the stop flag
 //
 // Every kernel returns at once when the device stop flag is set, so the host
 // may queue iterations ahead of the convergence decision.
@@ -26,17 +27,18 @@
 namespace kmc {
 
 // ---------------------------------------------------------------------------
-// k_prep: one wave per (padded) centroid row.
+// Distance operands of centroid row c, computed by one wave: -2c (fp32, or
+// bf16 hi / hi+lo), the transposed copy for the all-k refinement and
+// ||c||^2 (kFar for padding rows and excluded clusters). get(j) yields the
+// row's value at dimension j (zero past D). Both k_prep and k_finish call
+// this, so the operands are the same bits whichever kernel forms them.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_prep(PrepParams P) {
-  if (*P.done) return;
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= P.Kp) return;
+template <class Get>
+__device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, Get get) {
   const size_t row = (size_t)c * P.DP;
   float s = 0.0f;
   for (int j = lane; j < P.DP; j += 64) {
-    const float v = (c < P.K) ? P.C[row + j] : 0.0f;
+    const float v = (c < P.K) ? get(j) : 0.0f;
     const float w = -2.0f * v;  // exact scaling
     s = fmaf(v, v, s);
     P.CT[(size_t)j * P.Kp + c] = v;
@@ -51,6 +53,16 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams P) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (lane == 0) P.cn[c] = (c < P.K && P.alive[c]) ? s : kFar;
+}
+
+// k_prep: one wave per (padded) centroid row.
+__global__ __launch_bounds__(256) void k_prep(PrepParams P) {
+  if (*P.done) return;
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= P.Kp) return;
+  const float* cr = P.C + (size_t)c * P.DP;
+  prep_row(P, c, lane, [&](int j) { return cr[j]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -564,27 +576,95 @@ __device__ __forceinline__ int region_of(const unsigned* pre_s, int nregions, un
 
 // Band points with two or three candidates (the rest provably lose): each
 // candidate's distance in the reference's float arithmetic, first-min rule
-// (the smallest distance, the lowest index among equal ones). One thread per
-// queue entry.
+// (the smallest distance, the lowest index among equal ones).
+//
+// A wave takes 64 queue entries (lane e owns entry e). Dimensions go in
+// chunks of 16: four lanes load one entry's 64-byte slice of the point row
+// and of each candidate row (16-byte loads, the next chunk's loads issued
+// before the current chunk is summed), form the squared terms t*t with
+// t = x - c (one rounding each, as in the reference), and leave them in LDS;
+// lane e then adds its entry's terms strictly in dimension order. Padding
+// dimensions are zero in X and C and add exact zeros. An entry with two
+// candidates repeats the second.
+constexpr int kR2Stride = 20;  // floats per LDS row of squared terms (b128 reads conflict-free)
 __global__ __launch_bounds__(256) void k_refine2(RefineParams P) {
   if (*P.done) return;
   __shared__ unsigned pre_s[kMaxRegions + 1];
+  __shared__ __attribute__((aligned(16))) float sq_s[4][3 * 64 * kR2Stride];
   const unsigned total = region_prefix(P.q2cnt, P.nregions, pre_s);
-  for (unsigned t = blockIdx.x * 256u + threadIdx.x; t < total; t += gridDim.x * 256u) {
+  if (total == 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float* const sq = sq_s[wave];
+  const int sub = lane & 3;  // which 16-byte piece of a 64-byte slice
+  const unsigned nbatch = (total + 63) / 64;
+  for (unsigned bt = blockIdx.x * 4u + wave; bt < nbatch; bt += gridDim.x * 4u) {
+    const unsigned t = min(bt * 64u + lane, total - 1);
     const int r = region_of(pre_s, P.nregions, t);
-    const int* q2 = P.q2 + 4 * ((size_t)r * P.qstride + (t - pre_s[r]));
-    const int4 q = *reinterpret_cast<const int4*>(q2);
-    const int i = q.x;
-    const float* x = P.X + (size_t)i * P.DP;
-    int best = q.y;
-    float rbest = direct_dist_f32(x, P.C + (size_t)q.y * P.DP, P.D);
-    const float rb = direct_dist_f32(x, P.C + (size_t)q.z * P.DP, P.D);
-    if (rb < rbest || (rb == rbest && q.z < best)) { rbest = rb; best = q.z; }
-    if (q.w >= 0) {
-      const float rc = direct_dist_f32(x, P.C + (size_t)q.w * P.DP, P.D);
-      if (rc < rbest || (rc == rbest && q.w < best)) { rbest = rc; best = q.w; }
+    const int4 q = *reinterpret_cast<const int4*>(P.q2 + 4 * ((size_t)r * P.qstride +
+                                                              (t - pre_s[r])));
+    const int cand0 = q.y, cand1 = q.z, cand2 = q.w >= 0 ? q.w : q.z;
+    // row pointers of the four entries this lane loads for (e = lane/4 + 16*rr)
+    const float* xp[4];
+    const float* cp[4][3];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int e = (lane >> 2) + 16 * rr;
+      xp[rr] = P.X + (size_t)__shfl(q.x, e, 64) * P.DP + 4 * sub;
+      cp[rr][0] = P.C + (size_t)__shfl(cand0, e, 64) * P.DP + 4 * sub;
+      cp[rr][1] = P.C + (size_t)__shfl(cand1, e, 64) * P.DP + 4 * sub;
+      cp[rr][2] = P.C + (size_t)__shfl(cand2, e, 64) * P.DP + 4 * sub;
     }
-    P.labels[i] = best;
+    float4 xv[4], cv[4][3];
+    auto load_chunk = [&](int j0) {
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        xv[rr] = *reinterpret_cast<const float4*>(xp[rr] + j0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) cv[rr][k] = *reinterpret_cast<const float4*>(cp[rr][k] + j0);
+      }
+    };
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    load_chunk(0);
+    for (int j0 = 0; j0 < P.DP; j0 += 16) {
+      // squared terms of this chunk into LDS
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int e = (lane >> 2) + 16 * rr;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+#pragma clang fp contract(off)
+          float4 sv;
+          float tt;
+          tt = xv[rr].x - cv[rr][k].x; sv.x = tt * tt;
+          tt = xv[rr].y - cv[rr][k].y; sv.y = tt * tt;
+          tt = xv[rr].z - cv[rr][k].z; sv.z = tt * tt;
+          tt = xv[rr].w - cv[rr][k].w; sv.w = tt * tt;
+          *reinterpret_cast<float4*>(sq + (k * 64 + e) * kR2Stride + 4 * sub) = sv;
+        }
+      }
+      if (j0 + 16 < P.DP) load_chunk(j0 + 16);
+      __builtin_amdgcn_wave_barrier();
+      // this lane's entry: its terms in dimension order
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float* row = sq + (k * 64 + lane) * kR2Stride;
+#pragma unroll
+        for (int v4 = 0; v4 < 4; ++v4) {
+#pragma clang fp contract(off)
+          const float4 sv = *reinterpret_cast<const float4*>(row + 4 * v4);
+          acc[k] = acc[k] + sv.x;
+          acc[k] = acc[k] + sv.y;
+          acc[k] = acc[k] + sv.z;
+          acc[k] = acc[k] + sv.w;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    int best = cand0;
+    float rbest = acc[0];
+    if (acc[1] < rbest || (acc[1] == rbest && cand1 < best)) { rbest = acc[1]; best = cand1; }
+    if (acc[2] < rbest || (acc[2] == rbest && cand2 < best)) { rbest = acc[2]; best = cand2; }
+    if (bt * 64u + lane < total) P.labels[q.x] = best;
   }
 }
 
@@ -1018,6 +1098,102 @@ __global__ __launch_bounds__(256) void k_update_tree(UpdateParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// Deterministic FP64 inertia / shift reductions, refinement counts, the
+// iteration counter and the stop flag; one workgroup of kStatsThreads.
+// ---------------------------------------------------------------------------
+constexpr int kStatsThreads = 512;
+__device__ void stats_reduce(const StatsParams& P) {
+  __shared__ double sd[kStatsThreads];
+  __shared__ double sm[kStatsThreads];
+  __shared__ int sc[kStatsThreads];
+  __shared__ unsigned long long sq2[kStatsThreads], sqa[kStatsThreads];
+  const int t = threadIdx.x;
+  double in = 0.0, mx = 0.0;
+  int all = 1;
+  unsigned long long r2 = 0, ra = 0;
+  // eight loads in flight per thread and round (fixed order: deterministic)
+  for (int r0 = 0; r0 < P.nregions; r0 += 8 * kStatsThreads) {
+    unsigned a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = r0 + u * kStatsThreads + t;
+      a[u] = r < P.nregions ? P.q2cnt[r] : 0u;
+      b[u] = r < P.nregions ? P.qacnt[r] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      r2 += a[u];
+      ra += b[u];
+    }
+  }
+  for (int i0 = 0; i0 < P.nparts; i0 += 8 * kStatsThreads) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * kStatsThreads + t;
+      v[u] = i < P.nparts ? P.ipart[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) in += v[u];
+  }
+  for (int c0 = 0; c0 < P.K; c0 += 8 * kStatsThreads) {
+    double v[8];
+    int cv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int c = c0 + u * kStatsThreads + t;
+      v[u] = c < P.K ? P.shift2[c] : 0.0;
+      cv[u] = c < P.K ? P.conv[c] : 1;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      mx = fmax(mx, v[u]);
+      all &= cv[u];
+    }
+  }
+  sd[t] = in;
+  sm[t] = mx;
+  sc[t] = all;
+  sq2[t] = r2;
+  sqa[t] = ra;
+  __syncthreads();
+  for (int o = kStatsThreads / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      sd[t] += sd[t + o];
+      sm[t] = fmax(sm[t], sm[t + o]);
+      sc[t] &= sc[t + o];
+      sq2[t] += sq2[t + o];
+      sqa[t] += sqa[t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    DevState* s = P.state;
+    const unsigned long long it = s->iter + 1;
+    s->iter = it;
+    s->inertia = sd[0];
+    s->max_shift2 = sm[0];
+    s->refined_two = sq2[0];
+    s->refined_all = sqa[0];
+    const double score = sd[0] - s->sum_x2;
+    if (P.mode == CONV_SHIFT && it >= 2 && sc[0]) {
+      // comparisons/CPU_Kmeans/src/kmeans.cpp:205-208: tested from iteration 2
+      s->converged_at = it;
+      s->done = 1;
+    }
+    if (P.mode == CONV_SCORE && it >= 2 && it < s->maxiter &&
+        fabsf((float)score - (float)s->last_score) < P.tol) {
+      // src/kmeans.cu:1064-1073: float scores, tested from iteration 2,
+      // never at the last iteration
+      s->converged_at = it;
+      s->done = 1;
+    }
+    s->last_score = score;
+    if (it >= s->maxiter) s->done = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_finish: one workgroup of kFinishWaves waves per centroid. With the tree
 // update the waves first add the cluster's chunk partials: wave w takes
 // partials q0+w, q0+w+kFinishWaves, ... in order, and the wave sums are added
@@ -1025,8 +1201,9 @@ __global__ __launch_bounds__(256) void k_update_tree(UpdateParams P) {
 // any size costs the same few load rounds per wave). c_new = sum / count,
 // rounded once (an empty cluster becomes the zero vector). Then, on wave 0,
 // the float shift in the reference's order on lane 0
-// (comparisons/CPU_Kmeans/src/kmeans.cpp:225-234), the FP64 shift, and
-// C <- C_new, lane-parallel.
+// (comparisons/CPU_Kmeans/src/kmeans.cpp:225-234), the FP64 shift,
+// C <- C_new lane-parallel, and the next iteration's distance operands for
+// the centroid (prep_row).
 // ---------------------------------------------------------------------------
 constexpr int kFinishWaves = 8;
 __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
@@ -1074,105 +1251,56 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
   } else if (wave == 0) {
     for (int j = lane; j < P.D; j += 64) ns[j] = nrow[j];
   }
-  if (wave != 0) return;
-  __builtin_amdgcn_wave_barrier();
-  // the squared terms of the float shift are independent: form them
-  // lane-parallel (each is one rounded difference and one rounded product,
-  // as in the reference), then add them in dimension order on lane 0
-  double s64 = 0.0;
-  for (int j = lane; j < P.D; j += 64) {
-    const float o = cr[j], nv = ns[j];
-    const double td = (double)nv - (double)o;
-    s64 = fma(td, td, s64);
-    {
+  if (wave == 0) {
+    __builtin_amdgcn_wave_barrier();
+    // the squared terms of the float shift are independent: form them
+    // lane-parallel (each is one rounded difference and one rounded product,
+    // as in the reference), then add them in dimension order on lane 0
+    double s64 = 0.0;
+    for (int j = lane; j < P.D; j += 64) {
+      const float o = cr[j], nv = ns[j];
+      const double td = (double)nv - (double)o;
+      s64 = fma(td, td, s64);
+      {
 #pragma clang fp contract(off)
-      const float t = nv - o;
-      sq_s[j] = t * t;
+        const float t = nv - o;
+        sq_s[j] = t * t;
+      }
+    }
+    for (int j = P.D + lane; j < ((P.D + 3) & ~3); j += 64) sq_s[j] = 0.0f;
+    s64 = wave_sum_f64(s64);
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      float s = 0.0f;
+      for (int j = 0; j < P.D; j += 4) {
+#pragma clang fp contract(off)
+        const float4 v = *reinterpret_cast<const float4*>(&sq_s[j]);
+        s = s + v.x;
+        if (j + 1 < P.D) s = s + v.y;
+        if (j + 2 < P.D) s = s + v.z;
+        if (j + 3 < P.D) s = s + v.w;
+      }
+      P.conv[c] = (__fsqrt_rn(s) > P.tol) ? 0 : 1;
+      P.shift2[c] = s64;
+      if (P.exclude_empty && cnt == 0) P.alive[c] = 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int j = lane; j < P.D; j += 64) cr[j] = ns[j];
+    // the next iteration's distance operands for this centroid (alive[c]
+    // is final for it now)
+    if (P.do_prep) {
+      __builtin_amdgcn_wave_barrier();
+      prep_row(P.prep, c, lane, [&](int j) { return j < P.D ? ns[j] : 0.0f; });
     }
   }
-  for (int j = P.D + lane; j < ((P.D + 3) & ~3); j += 64) sq_s[j] = 0.0f;
-  s64 = wave_sum_f64(s64);
-  __builtin_amdgcn_wave_barrier();
-  if (lane == 0) {
-    float s = 0.0f;
-    for (int j = 0; j < P.D; j += 4) {
-#pragma clang fp contract(off)
-      const float4 v = *reinterpret_cast<const float4*>(&sq_s[j]);
-      s = s + v.x;
-      if (j + 1 < P.D) s = s + v.y;
-      if (j + 2 < P.D) s = s + v.z;
-      if (j + 3 < P.D) s = s + v.w;
-    }
-    P.conv[c] = (__fsqrt_rn(s) > P.tol) ? 0 : 1;
-    P.shift2[c] = s64;
-    if (P.exclude_empty && cnt == 0) P.alive[c] = 0;
-  }
-  __builtin_amdgcn_wave_barrier();
-  for (int j = lane; j < P.D; j += 64) cr[j] = ns[j];
 }
 
-// ---------------------------------------------------------------------------
-// k_stats: one workgroup. Deterministic FP64 reductions and the stop flag.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_stats(StatsParams P) {
+// k_stats: one workgroup.
+__global__ __launch_bounds__(kStatsThreads) void k_stats(StatsParams P) {
   if (*P.done) return;
-  __shared__ double sd[256];
-  __shared__ double sm[256];
-  __shared__ int sc[256];
-  __shared__ unsigned long long sq2[256], sqa[256];
-  double in = 0.0, mx = 0.0;
-  int all = 1;
-  unsigned long long r2 = 0, ra = 0;
-  for (int r = threadIdx.x; r < P.nregions; r += 256) {
-    r2 += P.q2cnt[r];
-    ra += P.qacnt[r];
-  }
-  for (int i = threadIdx.x; i < P.nparts; i += 256) in += P.ipart[i];
-  for (int c = threadIdx.x; c < P.K; c += 256) {
-    mx = fmax(mx, P.shift2[c]);
-    all &= P.conv[c];
-  }
-  sd[threadIdx.x] = in;
-  sm[threadIdx.x] = mx;
-  sc[threadIdx.x] = all;
-  sq2[threadIdx.x] = r2;
-  sqa[threadIdx.x] = ra;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      sd[threadIdx.x] += sd[threadIdx.x + o];
-      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
-      sc[threadIdx.x] &= sc[threadIdx.x + o];
-      sq2[threadIdx.x] += sq2[threadIdx.x + o];
-      sqa[threadIdx.x] += sqa[threadIdx.x + o];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    DevState* s = P.state;
-    const unsigned long long it = s->iter + 1;
-    s->iter = it;
-    s->inertia = sd[0];
-    s->max_shift2 = sm[0];
-    s->refined_two = sq2[0];
-    s->refined_all = sqa[0];
-    const double score = sd[0] - s->sum_x2;
-    if (P.mode == CONV_SHIFT && it >= 2 && sc[0]) {
-      // comparisons/CPU_Kmeans/src/kmeans.cpp:205-208: tested from iteration 2
-      s->converged_at = it;
-      s->done = 1;
-    }
-    if (P.mode == CONV_SCORE && it >= 2 && it < s->maxiter &&
-        fabsf((float)score - (float)s->last_score) < P.tol) {
-      // src/kmeans.cu:1064-1073: float scores, tested from iteration 2,
-      // never at the last iteration
-      s->converged_at = it;
-      s->done = 1;
-    }
-    s->last_score = score;
-    if (it >= s->maxiter) s->done = 1;
-  }
+  stats_reduce(P);
 }
+
 
 // ---------------------------------------------------------------------------
 // Set-up kernels
@@ -1366,9 +1494,9 @@ hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
   }
   return hipGetLastError();
 }
-hipError_t launch_finish(const FinishParams& F, const StatsParams& S, hipStream_t s) {
+hipError_t launch_finish(const FinishParams& F, hipStream_t s) {
   hipLaunchKernelGGL(k_finish, dim3(F.K), dim3(64 * kFinishWaves), 0, s, F);
-  hipLaunchKernelGGL(k_stats, dim3(1), dim3(256), 0, s, S);
+  hipLaunchKernelGGL(k_stats, dim3(1), dim3(kStatsThreads), 0, s, F.stats);
   return hipGetLastError();
 }
 hipError_t launch_load_points(const LoadParams& P, hipStream_t s) {
