@@ -1,25 +1,31 @@
 """Lloyd-iteration throughput of the gfx950 k-means engine on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--prec bf16x3]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--prec bf16] [--update tree]
 
 Workload (the large plain-vs-BF16 configuration): N=1,000,000 points, D=128,
 K=256 Gaussian blobs (centres U[-10,10]^128, per-blob sigma U[0.5,2.0]),
 float32, seeded; 256 distinct seed-chosen rows as initial centroids; no
 convergence test (tol=0, every iteration runs). A step is one Lloyd
-iteration: centroid operands + fused MFMA distance/argmin + margin
-refinement + stable counting sort + ordered centroid update + FP64 inertia
-and shift. Points are resident in HBM before the timed region.
+iteration: fused MFMA distance/argmin + margin refinement + stable counting
+sort + centroid update + shift, next-iteration operands and FP64 inertia.
+Points are resident in HBM before the timed region.
 
-The headline form (``--prec``, default bf16x3: hi+lo split bf16 MFMA) and the
-plain FP32 form run from the same initial centroids; the line reports the
-headline's iterations/s as ``value`` and the FP32 form beside it, checks that
-both forms end with bit-identical centroids and labels, and measures the
-inertia error of the headline form against an FP64 oracle.
+The headline form (``--prec``, default bf16: single-product bf16 MFMA with
+exact recomputation of the margin-band points), the split form (bf16x3: hi+lo
+bf16 products) and the plain FP32 form run from the same initial centroids;
+the line reports the headline's iterations/s as ``value`` and the other forms
+beside it, checks that every form ends with bit-identical centroids and
+labels, and measures the inertia error of the headline form against an FP64
+oracle. ``--update tree`` (default) sums each cluster in FP64 and rounds once;
+``--update ordered`` reproduces the reference's float summation order.
 
-The roofline object covers the dominant kernel (the fused distance kernel):
-achieved = 2*N*K*D algorithmic flops per launch / mean launch time, the time
-taken live with HIP events recorded around every launch on the engine's
-stream during the timed region.
+The roofline object covers the dominant kernel of the iteration (the one
+with the larger mean launch time of the two large ones), the other is given
+beside it: the fused distance kernel against the MFMA peak (achieved =
+2*N*K*D algorithmic flops per launch / mean launch time) and the centroid
+update against HBM (achieved = N*D*4 + N*4 + K*D*4 algorithmic bytes / mean
+launch time). Both times are taken live with HIP events recorded around
+every launch on the engine's stream during the timed region.
 
 The CPU baseline is the reference CPU k-means (comparisons/CPU_Kmeans,
 compiled from its own sources into oracle/_ref) timed single-threaded on this
@@ -96,15 +102,20 @@ def cpu_baseline(X, C0, n_sample=100_000, iters=2):
                       (n_sample, N, D, K, iters, secs)}
 
 
-def traffic_from_profiles(prec):
-    """HBM bytes per launch of the distance kernel from committed rocprofv3
-    --pmc summaries (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
-    p = os.path.join(ROOT, "profiles", "assign_n1m_d128_k256_traffic.json")
+def traffic_from_profiles(role, form):
+    """HBM bytes per launch of a kernel ("assign" per form, "update_tree",
+    "update_ordered") from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factors)."""
+    p = os.path.join(ROOT, "profiles", "lloyd_n1m_d128_k256_hbm_traffic.json")
     try:
         with open(p) as f:
-            return json.load(f).get(prec, {}).get("hbm_bytes_per_launch")
+            t = json.load(f)
     except (OSError, ValueError):
         return None
+    e = t.get(role, {})
+    if role == "assign":
+        e = e.get(form, {})
+    return e.get("hbm_bytes_per_launch")
 
 
 def profile_copy(nbytes):
@@ -123,9 +134,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "bf16", "fp32"])
-    ap.add_argument("--update", default="ordered", choices=["ordered", "tree"],
-                    help="centroid update: reference summation order, or FP64 tree")
+    ap.add_argument("--prec", default="bf16", choices=["bf16x3", "bf16", "fp32"])
+    ap.add_argument("--update", default="tree", choices=["ordered", "tree"],
+                    help="centroid update: FP64 tree, or the reference's float summation order")
+    ap.add_argument("--no-x3", action="store_true", help="skip the bf16x3 comparison form")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison form")
@@ -161,6 +173,10 @@ def main():
     if not a.no_fp32 and a.prec != "fp32":
         log("timing fp32: warmup %d, steps %d" % (a.warmup, a.steps))
         fp32 = run_form(kmc, X, C0, "fp32", a.steps, a.warmup, sync, a.update, not a.eager)
+    x3 = None
+    if not a.no_x3 and a.prec != "bf16x3":
+        log("timing bf16x3: warmup %d, steps %d" % (a.warmup, a.steps))
+        x3 = run_form(kmc, X, C0, "bf16x3", a.steps, a.warmup, sync, a.update, not a.eager)
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
@@ -174,9 +190,29 @@ def main():
     ips = iters / head["wall_s"]
     launches = int(r["assign_launches"])
     t_assign = r["ms_assign"] / 1e3 / max(1, launches)
+    t_update = r["ms_update"] / 1e3 / max(1, int(r["update_launches"]))
     flops = 2.0 * N * K * D
     achieved = flops / t_assign
-    traffic = traffic_from_profiles(a.prec)
+    # the two largest kernels of an iteration, each against its own roof
+    roof_assign = {"bound": "mfma", "kernel": "k_assign<128,%s>" % a.prec,
+                   "achieved": round(achieved / 1e12, 3), "peak": PEAK[a.prec] / 1e12,
+                   "unit": "TFLOP/s", "frac": round(achieved / PEAK[a.prec], 4),
+                   "traffic": traffic_from_profiles("assign", a.prec),
+                   "algorithmic_flops_per_launch": flops,
+                   "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches}
+    ubytes = N * D * 4 + N * 4 + K * D * 4  # X rows once, member ids, centroids
+    uname = "k_update_tree<2>" if a.update == "tree" else "k_update"
+    roof_update = {"bound": "hbm", "kernel": uname,
+                   "achieved": round(ubytes / t_update / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                   "unit": "GB/s", "frac": round(ubytes / t_update / HBM_PEAK, 4),
+                   "traffic": traffic_from_profiles("update_" + a.update, a.prec),
+                   "algorithmic_bytes_per_launch": ubytes,
+                   "mean_launch_ms": round(t_update * 1e3, 5),
+                   "launches_timed": int(r["update_launches"])}
+    if t_update > t_assign:
+        roof, roof_other = roof_update, roof_assign
+    else:
+        roof, roof_other = roof_assign, roof_update
 
     out = {
         "metric": "Lloyd iter/s, N=1M D=128 K=256: BF16 MFMA vs FP32 at <=1e-5 inertia err",
@@ -197,12 +233,8 @@ def main():
                    "init": "256 distinct seed-chosen rows (seed %d)" % INIT_SEED,
                    "update": {"ordered": "ordered (reference summation order)",
                               "tree": "tree (FP64 chunk partials)"}[a.update]},
-        "roofline": {"bound": "mfma", "kernel": "k_assign<128,%s>" % a.prec,
-                     "achieved": round(achieved / 1e12, 3), "peak": PEAK[a.prec] / 1e12,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK[a.prec], 4),
-                     "traffic": traffic,
-                     "algorithmic_flops_per_launch": flops,
-                     "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches},
+        "roofline": roof,
+        "roofline_second_kernel": roof_other,
         "assign_variant": {"chosen": head["tuning"]["chosen"],
                            "threads": head["tuning"]["threads"][head["tuning"]["chosen"]],
                            "autotune_ms": [round(v, 4) for v in head["tuning"]["ms"]]},
@@ -223,6 +255,16 @@ def main():
         out["speedup_vs_fp32"] = round(ips / f_ips, 3)
         out["identical_to_fp32"] = bool(np.array_equal(fp32["C_end"], C_end) and
                                         np.array_equal(fp32["labels_end"], head["labels_end"]))
+    if x3 is not None:
+        f = x3["res"]
+        f_ips = int(f["iterations"]) / x3["wall_s"]
+        f_t = f["ms_assign"] / 1e3 / max(1, int(f["assign_launches"]))
+        out["bf16x3"] = {"value": round(f_ips, 3),
+                         "ms_per_step": round(x3["wall_s"] * 1e3 / max(1, f["iterations"]), 5),
+                         "assign_mean_launch_ms": round(f_t * 1e3, 5),
+                         "assign_frac_bf16_peak": round(3 * flops / f_t / PEAK["bf16"], 4),
+                         "identical": bool(np.array_equal(x3["C_end"], C_end) and
+                                           np.array_equal(x3["labels_end"], head["labels_end"]))}
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(X, C0)

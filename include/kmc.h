@@ -110,6 +110,8 @@ typedef struct {
   double ms_total;       /* wall time of the iteration loop, ms              */
   double ms_assign;      /* sum of distance-kernel times (KMC_RUN_TIME_KERNELS) */
   uint64_t assign_launches; /* distance-kernel launches timed                */
+  double ms_update;      /* sum of centroid-update kernel times (same flag)   */
+  uint64_t update_launches; /* update-kernel launches timed                   */
 } kmc_result;
 
 typedef struct kmc_handle kmc_handle;

@@ -330,7 +330,9 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
   return KMC_OK;
 }
 
-kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty) {
+// u0/u1: events around the centroid-update kernel (timed iterations)
+kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty, hipEvent_t u0 = nullptr,
+                          hipEvent_t u1 = nullptr) {
   kmc::SortParams sp{};
   sp.labels = h->labels;
   sp.bcount = h->bcount;
@@ -360,7 +362,9 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty) {
   up.K = h->K;
   up.D = h->D;
   up.DP = h->DP;
+  if (u0) KMC_CK(record_event(h->stream, u0));
   KMC_CK(kmc::launch_update(up, h->update_mode, h->stream));
+  if (u1) KMC_CK(record_event(h->stream, u1));
 
   kmc::FinishParams fp{};
   fp.C = h->C;
@@ -398,15 +402,16 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty) {
 }
 
 // m iterations starting at global iteration `first`; events around the
-// distance kernel for iterations below ntimed.
+// distance kernel and the update kernel for iterations below ntimed.
 kmc_status enqueue_chunk(kmc_handle* h, uint64_t first, uint64_t m, int mode, int exclude,
                          uint64_t ntimed) {
   for (uint64_t it = 0; it < m; ++it) {
     const uint64_t gi = first + it;
-    hipEvent_t e0 = gi < ntimed ? h->events[2 * gi] : nullptr;
-    hipEvent_t e1 = gi < ntimed ? h->events[2 * gi + 1] : nullptr;
-    kmc_status st = enqueue_assign(h, e0, e1, it == 0);
-    if (st == KMC_OK) st = enqueue_update(h, mode, exclude);
+    const bool tm = gi < ntimed;
+    hipEvent_t* ev = tm ? &h->events[4 * gi] : nullptr;
+    kmc_status st = enqueue_assign(h, tm ? ev[0] : nullptr, tm ? ev[1] : nullptr, it == 0);
+    if (st == KMC_OK)
+      st = enqueue_update(h, mode, exclude, tm ? ev[2] : nullptr, tm ? ev[3] : nullptr);
     if (st != KMC_OK) return st;
   }
   return KMC_OK;
@@ -712,7 +717,7 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
   if (!exclude) KMC_CK(kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream));
   const uint64_t kMaxTimed = 4096;
   const uint64_t ntimed = timing ? std::min<uint64_t>(maxiter, kMaxTimed) : 0;
-  while (h->events.size() < 2 * ntimed) {
+  while (h->events.size() < 4 * ntimed) {
     hipEvent_t e;
     KMC_CK(hipEventCreate(&e));
     h->events.push_back(e);
@@ -756,14 +761,18 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
       if (res) {
         res->ms_total = ms;
         const uint64_t ran = std::min<uint64_t>(s.iter, ntimed);
-        double acc = 0.0;
+        double acc = 0.0, accu = 0.0;
         for (uint64_t i = 0; i < ran; ++i) {
-          float a = 0.f;
-          if (hipEventElapsedTime(&a, h->events[2 * i], h->events[2 * i + 1]) == hipSuccess)
+          float a = 0.f, u = 0.f;
+          if (hipEventElapsedTime(&a, h->events[4 * i], h->events[4 * i + 1]) == hipSuccess)
             acc += a;
+          if (hipEventElapsedTime(&u, h->events[4 * i + 2], h->events[4 * i + 3]) == hipSuccess)
+            accu += u;
         }
         res->ms_assign = acc;
         res->assign_launches = ran;
+        res->ms_update = accu;
+        res->update_launches = ran;
       }
     }
   }

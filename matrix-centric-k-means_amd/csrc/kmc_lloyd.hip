@@ -266,6 +266,48 @@ __device__ __forceinline__ kmc_f32x16 mfma_chain(const AFrag<DP, PREC>& a,
   return acc;
 }
 
+// The same chain with each A fragment read from LDS just before its MFMA
+// (fewer live registers).
+template <int DP, int PREC>
+__device__ __forceinline__ kmc_f32x16 mfma_chain_lds(const unsigned char* rows, int cnt, int cb,
+                                                     int p, int h, const XFrag<DP, PREC>& x,
+                                                     kmc_f32x16 acc) {
+  using T = AssignTraits<DP, PREC>;
+  const unsigned char* arow = rows + (size_t)(cb + p) * T::kRowBytes;
+  if (PREC == PREC_FP32) {
+    const float* af = reinterpret_cast<const float*>(arow);
+#pragma unroll
+    for (int q = 0; q < DP / 8; ++q) {
+      const float4 a = *reinterpret_cast<const float4*>(af + 8 * q + 4 * h);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, x.f[4 * q + 0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, x.f[4 * q + 1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, x.f[4 * q + 2], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, x.f[4 * q + 3], acc, 0, 0, 0);
+    }
+  } else {
+    const uint16_t* ah = reinterpret_cast<const uint16_t*>(arow);
+    const uint16_t* al = reinterpret_cast<const uint16_t*>(
+        rows + (size_t)cnt * T::kRowBytes + (size_t)(cb + p) * T::kRowBytes);
+#pragma unroll
+    for (int s = 0; s < DP / 16; ++s) {
+      const uint4 a_hi = *reinterpret_cast<const uint4*>(ah + 16 * s + 8 * h);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a_hi),
+                                                    __builtin_bit_cast(kmc_bf16x8, x.hi[s]),
+                                                    acc, 0, 0, 0);
+      if (PREC == PREC_BF16X3) {
+        const uint4 a_lo = *reinterpret_cast<const uint4*>(al + 16 * s + 8 * h);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a_hi),
+                                                      __builtin_bit_cast(kmc_bf16x8, x.lo[s]),
+                                                      acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a_lo),
+                                                      __builtin_bit_cast(kmc_bf16x8, x.hi[s]),
+                                                      acc, 0, 0, 0);
+      }
+    }
+  }
+  return acc;
+}
+
 // Top-4 update over one block's 16 accumulator values. Each value carries
 // its centroid index in the low mantissa bits ((v & ~mask) | idx), so the
 // running top-4 is four v_med3_f32 per value and the indices travel with the
@@ -320,8 +362,9 @@ __device__ __forceinline__ uint32_t opaque_vgpr(uint32_t s) {
 // after the sweep: one VALU op less per value. The packing perturbation is
 // then relative to | ||c||^2 - 2 x.c | <= d^2 + ||x||^2, which the band
 // covers through the xpk * ||x||^2 term.
-template <int DP, int PREC, int NW, int MS, bool PF>
-__global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
+template <int DP, int PREC, int NW, int MS, bool PF, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_assign(AssignParams P) {
   if (*P.done) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ float red_s[NW];
@@ -333,6 +376,7 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
   const int h = lane >> 5;
   constexpr int kTile = 32 * MS * NW;  // points per workgroup tile
   constexpr bool kAddX2 = PREC != PREC_BF16;
+  constexpr bool kLean = WPE > 4;  // occupancy above four waves per SIMD
   if (threadIdx.x == 0) {
     q2c_s = 0u;
     qac_s = 0u;
@@ -421,18 +465,32 @@ __global__ __launch_bounds__(64 * NW) void k_assign(AssignParams P) {
           push_block(acc[m], (uint32_t)(c0 + 32 * blk), keepv, b[m], ninf);
       };
       const int nblk = cnt / 32;  // even: chunks are multiples of 64
-      kmc_f32x16 accA[MS], accB[MS];
-      chains(0, accA);
-      int blk = 1;
-      for (; blk + 1 < nblk; blk += 2) {
-        chains(blk, accB);
+      if constexpr (kLean) {
+        // register-lean form for high occupancy: one accumulator set, the
+        // A fragments read from LDS step by step
+#pragma unroll 1
+        for (int blk = 0; blk < nblk; ++blk) {
+          kmc_f32x16 acc[MS];
+#pragma unroll
+          for (int m = 0; m < MS; ++m)
+            acc[m] = mfma_chain_lds<DP, PREC>(rows, cnt, 32 * blk, p, h, cur[m],
+                                              init_acc<kAddX2>(cn_s, 32 * blk, h, x2[m]));
+          epilogue(blk, acc);
+        }
+      } else {
+        kmc_f32x16 accA[MS], accB[MS];
+        chains(0, accA);
+        int blk = 1;
+        for (; blk + 1 < nblk; blk += 2) {
+          chains(blk, accB);
+          epilogue(blk - 1, accA);
+          chains(blk + 1, accA);
+          epilogue(blk, accB);
+        }
+        chains(blk, accB);  // blk == nblk - 1
         epilogue(blk - 1, accA);
-        chains(blk + 1, accA);
         epilogue(blk, accB);
       }
-      chains(blk, accB);  // blk == nblk - 1
-      epilogue(blk - 1, accA);
-      epilogue(blk, accB);
     }
 
 #pragma unroll
@@ -1393,26 +1451,32 @@ size_t assign_lds(int DP, int prec, int kchunk) {
 }
 
 // Launch variants of k_assign: (waves per workgroup, point sets per wave,
-// prefetch). Two point sets per wave need both fragment sets in registers;
-// where they do not fit (kFragRegs > 64) the variant is not built.
+// prefetch, waves per SIMD the register allocation targets; 0 = none).
+// Two point sets per wave need both fragment sets in registers; where they
+// do not fit (kFragRegs > 64) the variant is not built. Targets above four
+// waves per SIMD select the register-lean loop and let two workgroups share
+// a CU when their centroid copies fit the LDS together.
 struct VariantShape {
   int nw;
   int ms;
   bool pf;
+  int wpe;
 };
-constexpr VariantShape kVariants[] = {{8, 1, true},   {8, 1, false}, {12, 1, false},
-                                      {16, 1, false}, {4, 1, true},  {4, 2, false},
-                                      {8, 2, false}};
+constexpr VariantShape kVariants[] = {{8, 1, true, 0},   {8, 1, false, 0}, {12, 1, false, 0},
+                                      {16, 1, false, 0}, {12, 1, false, 6}, {4, 2, false, 0},
+                                      {8, 2, false, 0},  {8, 1, false, 8}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static_assert(kNumVariants <= KMC_MAX_VARIANTS, "variant table exceeds kmc_tuning");
 
 template <int DP, int PREC, int V>
 static const void* kernel_of() {
   constexpr VariantShape S = kVariants[V];
-  if constexpr (S.ms > 1 && frag_regs<DP, PREC>() > 64) {
+  // a register target leaves room for the point fragments plus ~40 others
+  constexpr bool kFits = S.wpe == 0 || frag_regs<DP, PREC>() * S.ms + 40 <= 512 / S.wpe;
+  if constexpr ((S.ms > 1 && frag_regs<DP, PREC>() > 64) || !kFits) {
     return nullptr;
   } else {
-    return reinterpret_cast<const void*>(k_assign<DP, PREC, S.nw, S.ms, S.pf>);
+    return reinterpret_cast<const void*>(k_assign<DP, PREC, S.nw, S.ms, S.pf, S.wpe>);
   }
 }
 
@@ -1426,6 +1490,7 @@ static const void* kernel_of_v(int v) {
     case 4: return kernel_of<DP, PREC, 4>();
     case 5: return kernel_of<DP, PREC, 5>();
     case 6: return kernel_of<DP, PREC, 6>();
+    case 7: return kernel_of<DP, PREC, 7>();
     default: return nullptr;
   }
 }

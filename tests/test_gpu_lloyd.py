@@ -166,8 +166,9 @@ def test_full_size_update_exact(kmc_mod, oracle, c2_data):
 
 @pytest.mark.parametrize("prec", PRECS)
 def test_every_launch_variant_gives_the_same_labels(kmc_mod, oracle, c2_data, prec):
-    """Every k_assign launch variant (and the autotuner's choice) reproduces
-    the reference's labels on a 200K-point slice."""
+    """Every k_assign launch variant built for the problem (and the
+    autotuner's choice) reproduces the reference's labels on a 200K-point
+    slice."""
     X, C0 = c2_data
     Xs = np.ascontiguousarray(X[:200_000])
     want, _ = _reference_labels(oracle, Xs, C0)
@@ -177,6 +178,8 @@ def test_every_launch_variant_gives_the_same_labels(kmc_mod, oracle, c2_data, pr
         t = e.autotune()
         _say("variants", prec, t)
         for v in range(len(t["ms"])):
+            if t["ms"][v] < 0:  # variant not built for this (D, precision)
+                continue
             e.set_variant(v)
             lab, _ = e.assign(C0)
             assert np.array_equal(lab, want), v
