@@ -9,7 +9,7 @@
 //   k_refine2    band points with two or three candidates: their distances
 //                in the reference's float arithmetic, first-min rule
 //   k_refine_all band points with four or more candidates: all k distances
-//   k_hist / k_scan_blocks / k_scan_clusters / k_scatter
+//   k_hist / k_scan_blocks / k_scatter
 //                stable counting sort of point ids by label (the one-hot
 //                V matrix in CSR form: row offsets + column ids)
 //   k_update     C = V X with per-cluster float sums in point order, plus the
@@ -918,35 +918,6 @@ __global__ __launch_bounds__(256) void k_scan_blocks(SortParams P) {
   if (lane == 0) P.count[c] = carry;
 }
 
-// Exclusive scan of the cluster sizes (one workgroup).
-__global__ __launch_bounds__(1024) void k_scan_clusters(SortParams P) {
-  if (*P.done) return;
-  __shared__ int wsum[16];
-  __shared__ int carry_s;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) carry_s = 0;
-  __syncthreads();
-  for (int base = 0; base < P.K; base += 1024) {
-    const int idx = base + threadIdx.x;
-    const int v = idx < P.K ? P.count[idx] : 0;
-    int incl = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int t = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += t;
-    }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    int woff = 0;
-    for (int w = 0; w < wave; ++w) woff += wsum[w];
-    const int carry = carry_s;
-    if (idx < P.K) P.start[idx] = carry + woff + incl - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry_s = carry + woff + incl;
-    __syncthreads();
-  }
-}
-
 // Scatter point ids to their sorted slot; within a block the points are
 // visited in index order, 64 at a time. The lanes holding the same label are
 // found with one ballot per label bit (a lane keeps the lanes that agree with
@@ -957,8 +928,27 @@ __global__ __launch_bounds__(64) void k_scatter(SortParams P) {
   extern __shared__ int cur_s[];
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
-  for (int c = lane; c < P.K; c += 64)
-    cur_s[c] = P.start[c] + P.bcount[(size_t)c * P.nblk + b];
+  // start[] = exclusive scan of the cluster sizes: every block forms it
+  // (lane l scans a contiguous run of clusters), block 0 publishes it
+  {
+    const int S = (P.K + 63) / 64;
+    const int c0 = min(P.K, lane * S), c1 = min(P.K, c0 + S);
+    int tot = 0;
+    for (int c = c0; c < c1; ++c) tot += P.count[c];
+    int incl = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    int run = incl - tot;
+    for (int c = c0; c < c1; ++c) {
+      const int n_c = P.count[c];
+      if (b == 0) P.start[c] = run;
+      cur_s[c] = run + P.bcount[(size_t)c * P.nblk + b];
+      run += n_c;
+    }
+  }
   __syncthreads();
   const long long lo = (long long)b * P.CH;
   const long long hi = min(lo + P.CH, (long long)P.n);
@@ -1107,13 +1097,16 @@ __global__ __launch_bounds__(256) void k_update_tree(UpdateParams P) {
       cold[v] = act ? (double)P.C[(size_t)c * P.DP + j0 + v] : 0.0;
       sum[v] = 0.0;
     }
-    long long t = pos;
+    // batches of U rows, all loads of a batch issued before any is used; the
+    // last, partial batch of a segment loads a valid row in place of the rows
+    // past the end and masks them out (the select keeps the sums as they are)
     constexpr int U = 16;
-    for (; t + U <= seg_end; t += U) {
+    auto batch = [&](long long t, bool partial) {
       float xv[U][VW];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const float* r = X + (size_t)P.members[t + u] * P.DP;
+        const long long tt = partial ? min(t + u, seg_end - 1) : t + u;
+        const float* r = X + (size_t)P.members[tt] * P.DP;
         if (VW == 4) {
           const float4 f = *reinterpret_cast<const float4*>(r);
           xv[u][0] = f.x; xv[u][1 % VW] = f.y; xv[u][2 % VW] = f.z; xv[u][3 % VW] = f.w;
@@ -1125,25 +1118,20 @@ __global__ __launch_bounds__(256) void k_update_tree(UpdateParams P) {
         }
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u)
+      for (int u = 0; u < U; ++u) {
+        const bool ok = !partial || t + u < seg_end;  // wave-uniform
 #pragma unroll
         for (int v = 0; v < VW; ++v) {
           const double xd = (double)xv[u][v];
-          sum[v] += xd;
           const double df = xd - cold[v];
-          in = fma(df, df, in);
+          sum[v] = ok ? sum[v] + xd : sum[v];
+          in = ok ? fma(df, df, in) : in;
         }
-    }
-    for (; t < seg_end; ++t) {
-      const float* r = X + (size_t)P.members[t] * P.DP;
-#pragma unroll
-      for (int v = 0; v < VW; ++v) {
-        const double xd = (double)r[v];
-        sum[v] += xd;
-        const double df = xd - cold[v];
-        in = fma(df, df, in);
       }
-    }
+    };
+    long long t = pos;
+    for (; t + U <= seg_end; t += U) batch(t, false);
+    if (t < seg_end) batch(t, true);
     if (act) {
 #pragma unroll
       for (int v = 0; v < VW; ++v) P.tpart[(size_t)(c + q) * P.DP + j0 + v] = sum[v];
@@ -1542,7 +1530,6 @@ hipError_t launch_sort(const SortParams& P, hipStream_t s) {
   const size_t lds = (size_t)P.K * sizeof(int);
   hipLaunchKernelGGL(k_hist, dim3(P.nblk), dim3(64), lds, s, P);
   hipLaunchKernelGGL(k_scan_blocks, dim3((P.K + 3) / 4), dim3(256), 0, s, P);
-  hipLaunchKernelGGL(k_scan_clusters, dim3(1), dim3(1024), 0, s, P);
   hipLaunchKernelGGL(k_scatter, dim3(P.nblk), dim3(64), lds, s, P);
   return hipGetLastError();
 }
