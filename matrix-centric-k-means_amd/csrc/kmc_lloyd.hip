@@ -645,17 +645,23 @@ __device__ __forceinline__ int region_of(const unsigned* pre_s, int nregions, un
 // dimensions are zero in X and C and add exact zeros. An entry with two
 // candidates repeats the second.
 constexpr int kR2Stride = 20;  // floats per LDS row of squared terms (b128 reads conflict-free)
-__global__ __launch_bounds__(256) void k_refine2(RefineParams P) {
-  if (*P.done) return;
-  __shared__ unsigned pre_s[kMaxRegions + 1];
-  __shared__ __attribute__((aligned(16))) float sq_s[4][3 * 64 * kR2Stride];
+constexpr int kRefinePts = 4;  // points per all-k group
+constexpr size_t kRefinePreBytes = ((size_t)(kMaxRegions + 1) * 4 + 15) / 16 * 16;
+constexpr size_t kRefineLds =
+    kRefinePreBytes + (size_t)4 * 3 * 64 * kR2Stride * 4;  // two/three-candidate pass
+static_assert(kRefinePts * kMaxDP * 4 + 48 * kRefinePts <= 4 * 3 * 64 * kR2Stride * 4,
+              "all-k pass needs less LDS than the two/three-candidate pass");
+__device__ void refine2_body(const RefineParams& P, unsigned char* smem, unsigned bid,
+                             unsigned nblk) {
+  unsigned* const pre_s = reinterpret_cast<unsigned*>(smem);
+  float* const sq_all = reinterpret_cast<float*>(smem + kRefinePreBytes);
   const unsigned total = region_prefix(P.q2cnt, P.nregions, pre_s);
   if (total == 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float* const sq = sq_s[wave];
+  float* const sq = sq_all + wave * (3 * 64 * kR2Stride);
   const int sub = lane & 3;  // which 16-byte piece of a 64-byte slice
   const unsigned nbatch = (total + 63) / 64;
-  for (unsigned bt = blockIdx.x * 4u + wave; bt < nbatch; bt += gridDim.x * 4u) {
+  for (unsigned bt = bid * 4u + wave; bt < nbatch; bt += nblk * 4u) {
     const unsigned t = min(bt * 64u + lane, total - 1);
     const int r = region_of(pre_s, P.nregions, t);
     const int4 q = *reinterpret_cast<const int4*>(P.q2 + 4 * ((size_t)r * P.qstride +
@@ -735,19 +741,21 @@ __global__ __launch_bounds__(256) void k_refine2(RefineParams P) {
 // distance advances in the reference's float order (padding dimensions are
 // zero in X and C and add exact zeros). First-min across lanes, then across
 // the waves.
-constexpr int kRefinePts = 4;
-__global__ __launch_bounds__(256) void k_refine_all(RefineParams P) {
-  if (*P.done) return;
-  __shared__ unsigned pre_s[kMaxRegions + 1];
-  __shared__ __attribute__((aligned(16))) float x_s[kRefinePts][kMaxDP];
-  __shared__ float bv_s[4][kRefinePts];
-  __shared__ int bi_s[4][kRefinePts];
-  __shared__ int pid_s[kRefinePts];
+__device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsigned bid,
+                                unsigned nblk) {
+  unsigned* const pre_s = reinterpret_cast<unsigned*>(smem);
+  float(*const x_s)[kMaxDP] = reinterpret_cast<float(*)[kMaxDP]>(smem + kRefinePreBytes);
+  float(*const bv_s)[kRefinePts] =
+      reinterpret_cast<float(*)[kRefinePts]>(smem + kRefinePreBytes + kRefinePts * kMaxDP * 4);
+  int(*const bi_s)[kRefinePts] = reinterpret_cast<int(*)[kRefinePts]>(
+      smem + kRefinePreBytes + kRefinePts * kMaxDP * 4 + 16 * kRefinePts);
+  int* const pid_s = reinterpret_cast<int*>(smem + kRefinePreBytes + kRefinePts * kMaxDP * 4 +
+                                            32 * kRefinePts);
   const unsigned total = region_prefix(P.qacnt, P.nregions, pre_s);
   if (total == 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned ngroups = (total + kRefinePts - 1) / kRefinePts;
-  for (unsigned g = blockIdx.x; g < ngroups; g += gridDim.x) {
+  for (unsigned g = bid; g < ngroups; g += nblk) {
     __syncthreads();  // the previous group is done with the LDS
     if (threadIdx.x < kRefinePts) {
       const unsigned t = min(g * kRefinePts + threadIdx.x, total - 1);
@@ -848,6 +856,20 @@ __global__ __launch_bounds__(256) void k_refine_all(RefineParams P) {
   }
 }
 
+// The two refinement passes (they touch disjoint points). LDS: one dynamic
+// buffer each (the region prefix, then the pass's own arrays).
+constexpr size_t kRefineAllLds = kRefinePreBytes + kRefinePts * kMaxDP * 4 + 48 * kRefinePts;
+__global__ __launch_bounds__(256) void k_refine2(RefineParams P) {
+  if (*P.done) return;
+  extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
+  refine2_body(P, rsm, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(256) void k_refine_all(RefineParams P) {
+  if (*P.done) return;
+  extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
+  refine_all_body(P, rsm, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Stable counting sort of point ids by label. One wave per block of CH points
 // (CH a multiple of 1024); every lane loads its 16 labels of a 1024-point
@@ -934,7 +956,13 @@ __global__ __launch_bounds__(64) void k_scatter(SortParams P) {
     const int S = (P.K + 63) / 64;
     const int c0 = min(P.K, lane * S), c1 = min(P.K, c0 + S);
     int tot = 0;
-    for (int c = c0; c < c1; ++c) tot += P.count[c];
+    for (int c = c0; c < c1; c += 4) {
+      int v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = c + u < c1 ? P.count[c + u] : 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) tot += v[u];
+    }
     int incl = tot;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -942,11 +970,21 @@ __global__ __launch_bounds__(64) void k_scatter(SortParams P) {
       if (lane >= o) incl += t;
     }
     int run = incl - tot;
-    for (int c = c0; c < c1; ++c) {
-      const int n_c = P.count[c];
-      if (b == 0) P.start[c] = run;
-      cur_s[c] = run + P.bcount[(size_t)c * P.nblk + b];
-      run += n_c;
+    for (int c = c0; c < c1; c += 4) {
+      int n_c[4], bc[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        n_c[u] = c + u < c1 ? P.count[c + u] : 0;
+        bc[u] = c + u < c1 ? P.bcount[(size_t)(c + u) * P.nblk + b] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (c + u < c1) {
+          if (b == 0) P.start[c + u] = run;
+          cur_s[c + u] = run + bc[u];
+        }
+        run += n_c[u];
+      }
     }
   }
   __syncthreads();
@@ -1522,8 +1560,16 @@ hipError_t launch_prep(const PrepParams& P, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_refine(const RefineParams& P, hipStream_t s) {
-  hipLaunchKernelGGL(k_refine2, dim3(kRefineBlocks), dim3(256), 0, s, P);
-  hipLaunchKernelGGL(k_refine_all, dim3(kRefineBlocks), dim3(256), 0, s, P);
+  static bool lds_set = false;  // opt in to the large dynamic LDS once per process
+  if (!lds_set) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_refine2),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)kRefineLds);
+    if (e != hipSuccess) return e;
+    lds_set = true;
+  }
+  hipLaunchKernelGGL(k_refine2, dim3(kRefineBlocks), dim3(256), kRefineLds, s, P);
+  hipLaunchKernelGGL(k_refine_all, dim3(kRefineBlocks), dim3(256), kRefineAllLds, s, P);
   return hipGetLastError();
 }
 hipError_t launch_sort(const SortParams& P, hipStream_t s) {
