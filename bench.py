@@ -43,6 +43,18 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "matrix-centric-k-means_amd"))
 
+# workloads: c2 (default, the metric's configuration), c5 (large K), c3
+# (kernel k-means, its own line)
+WORKLOADS = {
+    "c2": dict(n=1_000_000, d=128, k=256, seed=20261203, gen="large_blobs", steps=50,
+               cpu_sample=100_000, cpu_iters=2,
+               metric="Lloyd iter/s, N=1M D=128 K=256: BF16 MFMA vs FP32 at <=1e-5 inertia err",
+               text="Gaussian blobs N=1000000 D=128 K=256"),
+    "c5": dict(n=4_000_000, d=256, k=4096, seed=20261204, gen="huge_blobs", steps=20,
+               cpu_sample=10_000, cpu_iters=1,
+               metric="Lloyd iter/s, N=4M D=256 K=4096: BF16 MFMA vs FP32",
+               text="Gaussian blobs N=4000000 D=256 K=4096"),
+}
 N, D, K = 1_000_000, 128, 256
 DATA_SEED, INIT_SEED = 20261203, 7
 PEAK = {"fp32": 157.3e12, "bf16x3": 2.5e15, "bf16": 2.5e15}
@@ -83,6 +95,7 @@ def run_form(kmc, X, C0, prec, steps, warmup, sync, update, graph):
 
 
 def cpu_baseline(X, C0, n_sample=100_000, iters=2):
+    n_sample = min(n_sample, X.shape[0])
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle, Reference
     Xs = np.ascontiguousarray(X[:n_sample])
@@ -102,10 +115,12 @@ def cpu_baseline(X, C0, n_sample=100_000, iters=2):
                       (n_sample, N, D, K, iters, secs)}
 
 
-def traffic_from_profiles(role, form):
+def traffic_from_profiles(role, form, workload="c2"):
     """HBM bytes per launch of a kernel ("assign" per form, "update_tree",
-    "update_ordered") from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factors)."""
+    "update_ordered") from the committed rocprofv3 --pmc summary of the
+    workload (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factors)."""
+    if workload != "c2":
+        return None
     p = os.path.join(ROOT, "profiles", "lloyd_n1m_d128_k256_hbm_traffic.json")
     try:
         with open(p) as f:
@@ -129,10 +144,75 @@ def profile_copy(nbytes):
     print(json.dumps({"copy_bytes": nbytes, "launches": 3}))
 
 
+def bench_kernel_kmeans(a):
+    """Kernel k-means (the reference's Popcorn path) at N=65536, D=784, K=10:
+    U[0,1] data with 70% zeros and 10-blob structure; polynomial (x.y+1)^2
+    and RBF gamma=1/784 kernels, K stored fp32 (17.2 GB) and bf16 (8.6 GB).
+    A step is one iteration: the S = K.H product streaming all of K, the self
+    terms, the distances and the argmin. The kernel matrix is built once,
+    outside the timed region (the reference builds it in the constructor)."""
+    import torch
+    import kmc
+    from kmc import datagen
+    n, d, k = 65536, 784, 10
+    steps = a.steps if a.steps is not None else 100
+    X = datagen.sparse_unit_blobs(n, d, k)[0]
+    rng = np.random.default_rng(7)
+    lab0 = rng.integers(0, k, size=n).astype(np.int32)
+    forms = {}
+    for kernel in ("polynomial", "rbf"):
+        for store in ("fp32", "bf16"):
+            with kmc.KernelEngine(n, d, k, kernel=kernel, store=store,
+                                  gamma=1.0 / d if kernel == "rbf" else 0.0) as e:
+                t0 = time.perf_counter()
+                e.set_points(X)
+                torch.cuda.synchronize()
+                build = time.perf_counter() - t0
+                e.set_labels(lab0)
+                e.run(a.warmup, check_converged=False)
+                e.set_labels(lab0)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r = e.run(steps, check_converged=False, time_kernels=True)
+                torch.cuda.synchronize()
+                wall = time.perf_counter() - t0
+                lab = e.labels()
+            its = int(r["iterations"])
+            t_spmm = r["ms_spmm"] / 1e3 / max(1, int(r["spmm_launches"]))
+            kbytes = n * n * (4 if store == "fp32" else 2)
+            forms["%s_%s" % (kernel, store)] = {
+                "value": round(its / wall, 3), "ms_per_step": round(wall * 1e3 / max(1, its), 4),
+                "build_s": round(build, 3), "objective": r["objective"],
+                "spmm_mean_launch_ms": round(t_spmm * 1e3, 4),
+                "spmm_hbm_frac": round(kbytes / t_spmm / HBM_PEAK, 4),
+                "cluster_sizes": np.bincount(lab, minlength=k).tolist()}
+            log("kernel k-means", kernel, store, forms["%s_%s" % (kernel, store)])
+    head = forms["polynomial_bf16"]
+    t_spmm = head["spmm_mean_launch_ms"] / 1e3
+    kbytes = n * n * 2
+    out = {"metric": "kernel k-means iter/s, N=65536 D=784 K=10 (polynomial, K in bf16)",
+           "value": head["value"], "unit": "iter/s", "n_gpus": 1, "steps": steps,
+           "warmup": a.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+           "config": {"workload": "kernel k-means, sparse U[0,1] blobs N=65536 D=784 K=10, "
+                                  "%d iterations" % steps},
+           "roofline": {"bound": "hbm", "kernel": "k_kk_spmm (K bf16)",
+                        "achieved": round(kbytes / t_spmm / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": round(kbytes / t_spmm / HBM_PEAK, 4),
+                        "traffic": None, "algorithmic_bytes_per_launch": kbytes,
+                        "mean_launch_ms": round(t_spmm * 1e3, 4)},
+           "forms": forms}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c5", "c3"],
+                    help="c2: N=1M D=128 K=256 (default); c5: N=4M D=256 K=4096; "
+                         "c3: kernel k-means N=65536 D=784 K=10")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed iterations (default: 50 for c2, 20 for c5, 100 for c3)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--prec", default="bf16", choices=["bf16x3", "bf16", "fp32"])
     ap.add_argument("--update", default="tree", choices=["ordered", "tree"],
@@ -152,6 +232,14 @@ def main():
     if a.profile_copy:
         profile_copy(a.profile_copy)
         return
+    if a.workload == "c3":
+        bench_kernel_kmeans(a)
+        return
+    global N, D, K, DATA_SEED
+    W = WORKLOADS[a.workload]
+    N, D, K, DATA_SEED = W["n"], W["d"], W["k"], W["seed"]
+    if a.steps is None:
+        a.steps = W["steps"]
 
     import torch
     import kmc
@@ -164,7 +252,7 @@ def main():
         sys.exit("bench.py: no GPU visible")
 
     log("generating N=%d D=%d K=%d blobs" % (N, D, K))
-    X, _, _ = datagen.large_blobs(N, D, K, DATA_SEED)
+    X, _, _ = getattr(datagen, W["gen"])(N, D, K, DATA_SEED)
     C0 = datagen.init_rows(X, K, INIT_SEED)
 
     log("timing %s: warmup %d, steps %d" % (a.prec, a.warmup, a.steps))
@@ -197,7 +285,7 @@ def main():
     roof_assign = {"bound": "mfma", "kernel": "k_assign<128,%s>" % a.prec,
                    "achieved": round(achieved / 1e12, 3), "peak": PEAK[a.prec] / 1e12,
                    "unit": "TFLOP/s", "frac": round(achieved / PEAK[a.prec], 4),
-                   "traffic": traffic_from_profiles("assign", a.prec),
+                   "traffic": traffic_from_profiles("assign", a.prec, a.workload),
                    "algorithmic_flops_per_launch": flops,
                    "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches}
     ubytes = N * D * 4 + N * 4 + K * D * 4  # X rows once, member ids, centroids
@@ -205,7 +293,7 @@ def main():
     roof_update = {"bound": "hbm", "kernel": uname,
                    "achieved": round(ubytes / t_update / 1e9, 1), "peak": HBM_PEAK / 1e9,
                    "unit": "GB/s", "frac": round(ubytes / t_update / HBM_PEAK, 4),
-                   "traffic": traffic_from_profiles("update_" + a.update, a.prec),
+                   "traffic": traffic_from_profiles("update_" + a.update, a.prec, a.workload),
                    "algorithmic_bytes_per_launch": ubytes,
                    "mean_launch_ms": round(t_update * 1e3, 5),
                    "launches_timed": int(r["update_launches"])}
@@ -215,7 +303,7 @@ def main():
         roof, roof_other = roof_assign, roof_update
 
     out = {
-        "metric": "Lloyd iter/s, N=1M D=128 K=256: BF16 MFMA vs FP32 at <=1e-5 inertia err",
+        "metric": W["metric"],
         "value": round(ips, 3),
         "unit": "iter/s",
         "n_gpus": 1,
@@ -227,10 +315,9 @@ def main():
         "vs_baseline": None,
         "dtype": {"bf16x3": "bf16", "bf16": "bf16", "fp32": "fp32"}[a.prec],
         "data": "synthetic",
-        "config": {"workload": "Lloyd k-means, Gaussian blobs N=1000000 D=128 K=256, "
-                               "maxiter=50 tol=0",
+        "config": {"workload": "Lloyd k-means, %s, maxiter=%d tol=0" % (W["text"], a.steps),
                    "n": N, "d": D, "k": K, "form": a.prec, "seed": DATA_SEED,
-                   "init": "256 distinct seed-chosen rows (seed %d)" % INIT_SEED,
+                   "init": "%d distinct seed-chosen rows (seed %d)" % (K, INIT_SEED),
                    "update": {"ordered": "ordered (reference summation order)",
                               "tree": "tree (FP64 chunk partials)"}[a.update]},
         "roofline": roof,
@@ -267,7 +354,7 @@ def main():
                                            np.array_equal(x3["labels_end"], head["labels_end"]))}
     if not a.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(X, C0)
+        out["cpu_baseline"] = cpu_baseline(X, C0, W["cpu_sample"], W["cpu_iters"])
     print(json.dumps(out))
 
 

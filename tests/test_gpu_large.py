@@ -1,0 +1,103 @@
+"""GPU parity at the large-K shape (D=256, K=4096: the centroid operands no
+longer fit the LDS, so the distance kernel stages them chunk by chunk).
+
+Bar: labels bit-identical to the reference CPU k-means' float first-min
+labels for every precision form and every launch variant built for the
+shape; Lloyd iterations identical across the precision forms. The reference
+labels come from FP64 distances wherever the FP64 top-2 gap exceeds 1e-3 of
+the distance (the reference's float error is below 2e-5 relative there), and
+from the oracle's float restatement of the reference distance elsewhere.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ("fp32", "bf16x3", "bf16")
+N, D, K = 100_000, 256, 4096
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def c5_slice():
+    from kmc import datagen
+    X, _, _ = datagen.huge_blobs(n=N)
+    C0 = datagen.init_rows(X, K, 11)
+    return X, C0
+
+
+def _reference_labels(oracle, X, C, rows=4096, margin_frac=1e-3):
+    Cd = C.astype(np.float64)
+    cn = (Cd * Cd).sum(1)
+    out = np.empty(X.shape[0], np.int32)
+    nclose = 0
+    for s in range(0, X.shape[0], rows):
+        Xs = X[s:s + rows]
+        Xd = Xs.astype(np.float64)
+        Dd = (Xd * Xd).sum(1)[:, None] + cn[None, :] - 2.0 * (Xd @ Cd.T)
+        part = np.partition(Dd, 1, axis=1)
+        lab = np.argmin(Dd, axis=1)
+        gap = part[:, 1] - part[:, 0]
+        close = np.nonzero(gap <= margin_frac * np.maximum(part[:, 1], 1.0))[0]
+        if close.size:
+            Dc = oracle.distances(np.ascontiguousarray(Xs[close]), C)
+            lab[close], _ = oracle.argmin(Dc)
+            nclose += close.size
+        out[s:s + rows] = lab
+    return out, nclose
+
+
+@pytest.fixture(scope="module")
+def c5_centroids(kmc_mod, c5_slice):
+    """Centroids after two FP32 iterations (split blobs give small margins)."""
+    X, C0 = c5_slice
+    with kmc_mod.Engine(N, D, K, precision="fp32", update="tree") as e:
+        e.set_points(X)
+        e.set_centroids(C0)
+        e.run(2, check_converged=False)
+        return e.centroids()
+
+
+@pytest.fixture(scope="module")
+def c5_want(oracle, c5_slice, c5_centroids):
+    X, _ = c5_slice
+    return _reference_labels(oracle, X, c5_centroids)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_large_k_assign_every_variant(kmc_mod, c5_slice, c5_centroids, c5_want, prec):
+    X, _ = c5_slice
+    want, nclose = c5_want
+    with kmc_mod.Engine(N, D, K, precision=prec) as e:
+        e.set_points(X)
+        e.set_centroids(c5_centroids)
+        t = e.autotune()
+        _say("large-K", prec, "close", nclose, "tuning", t)
+        for v in range(len(t["ms"])):
+            if t["ms"][v] < 0:
+                continue
+            e.set_variant(v)
+            lab, info = e.assign(c5_centroids)
+            _say("variant", v, "refined", info["refined_two"], info["refined_full"])
+            assert np.array_equal(lab, want), (v, int(np.sum(lab != want)))
+
+
+def test_large_k_lloyd_forms_agree(kmc_mod, c5_slice):
+    X, C0 = c5_slice
+    res = {}
+    for prec in PRECS:
+        with kmc_mod.Engine(N, D, K, precision=prec, update="tree") as e:
+            e.set_points(X)
+            e.set_centroids(C0)
+            r = e.run(3, check_converged=False, graph=True)
+            res[prec] = (e.labels(), e.centroids(), r)
+            _say("large-K run", prec, r)
+    for prec in ("bf16x3", "bf16"):
+        assert np.array_equal(res[prec][0], res["fp32"][0]), prec
+        assert np.array_equal(res[prec][1], res["fp32"][1]), prec
