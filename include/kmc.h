@@ -266,6 +266,55 @@ const char* kmc_kk_last_error(const kmc_kk_handle* h);
 
 kmc_status kmc_kk_destroy(kmc_kk_handle* h);
 
+/* ---- Batches of independent Lloyd problems -------------------------------
+ * Replaces the reference's `-r` multi-run loop (src/main.cu:89-105: one
+ * Kmeans object and one run per repetition) for many independent
+ * clusterings of uneven n, d and k: every stage of an iteration is one launch
+ * for the whole batch, the distance stage a persistent kernel over a packed
+ * (problem, tile) queue. Each problem follows the reference CPU k-means
+ * (comparisons/CPU_Kmeans/src/kmeans.cpp:86-200): exact first-min labels,
+ * its own shift test from iteration 2 (a converged problem is frozen while
+ * the others go on); centroids are FP64 sums rounded once, as
+ * KMC_UPDATE_TREE. Limits: d <= 256, k <= 256, and the centroid operands of
+ * the largest problem must fit the LDS (k x d <= 256 x 128 in fp32 or
+ * bf16x3). */
+typedef struct kmc_problem {
+  uint64_t n;        /* points */
+  uint32_t d, k;     /* dimensions, clusters */
+  const float* X;    /* host, n x d row-major, leading dimension ld */
+  size_t ld;
+  const float* C0;   /* host, k x d initial centroids */
+} kmc_problem;
+
+typedef struct kmc_batch_config {
+  float tol;          /* shift tolerance (KMC_RUN_CHECK_CONVERGED) */
+  int32_t precision;  /* kmc_precision, the same for every problem */
+  int32_t device;
+  int32_t reserved;
+} kmc_batch_config;
+
+typedef struct kmc_batch_result {
+  uint64_t iterations;   /* iterations run by this problem */
+  uint64_t converged_at; /* iteration whose shift test passed, or maxiter (the
+                            value Kmeans::run returns, src/kmeans.cu:1122) */
+  double inertia;        /* FP64 sum of ||x - c_label||^2, final state */
+} kmc_batch_result;
+
+typedef struct kmc_batch kmc_batch;
+
+/* Packs the problems into one device layout and uploads them. */
+kmc_status kmc_batch_create(const kmc_problem* probs, uint32_t count,
+                            const kmc_batch_config* cfg, kmc_batch** out);
+/* Runs every problem from its initial centroids for at most maxiter
+ * iterations. flags: KMC_RUN_CHECK_CONVERGED. res: count entries (may be
+ * NULL). ms_total: device time of the run (may be NULL). */
+kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags,
+                         kmc_batch_result* res, double* ms_total);
+kmc_status kmc_batch_get_labels(kmc_batch* b, uint32_t problem, int32_t* labels);
+kmc_status kmc_batch_get_centroids(kmc_batch* b, uint32_t problem, float* C);
+const char* kmc_batch_last_error(const kmc_batch* b);
+kmc_status kmc_batch_destroy(kmc_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,8 @@ INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-HIP_SOURCES = ["kmc_lloyd.hip", "kmc_api.hip", "kmc_kkmeans.hip", "kmc_kk_api.hip"]
+HIP_SOURCES = ["kmc_lloyd.hip", "kmc_api.hip", "kmc_kkmeans.hip", "kmc_kk_api.hip",
+               "kmc_batch.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -80,44 +80,7 @@ struct kmc_handle {
   std::vector<GraphEntry> graphs;
 };
 
-namespace {
-
-const char* kStatusText[] = {"ok",
-                             "invalid or missing argument",
-                             "no HIP device available",
-                             "index out of bounds",
-                             "invalid input file",
-                             "HIP runtime error",
-                             "out of device memory",
-                             "call out of order",
-                             "unsupported configuration"};
-
-kmc_status fail(kmc_handle* h, kmc_status s, const std::string& msg) {
-  if (h) h->err = msg;
-  return s;
-}
-
-#define KMC_CK(expr)                                                                \
-  do {                                                                              \
-    hipError_t e_ = (expr);                                                         \
-    if (e_ != hipSuccess)                                                           \
-      return fail(h, e_ == hipErrorOutOfMemory ? KMC_ERR_OOM : KMC_ERR_HIP,         \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));               \
-  } while (0)
-
-template <typename T>
-hipError_t dalloc(T** p, size_t count) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
-}
-
-int pick_dp(int d) {
-  const int opts[] = {16, 32, 64, 128, 256};
-  for (int o : opts)
-    if (d <= o) return o;
-  return 0;
-}
+namespace kmc {
 
 // gamma_m = m u / (1 - m u), the classical bound for m rounded operations.
 double gam(double m, double u) { return m * u / (1.0 - m * u); }
@@ -166,6 +129,48 @@ void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, fl
   // packing perturbation and the final add are then relative to d^2 + ||x||^2
   *xpk = prec == kmc::PREC_BF16 ? (float)(2.0 * (pk + 2.0 * u)) : 0.0f;
 }
+
+}  // namespace kmc
+
+namespace {
+
+const char* kStatusText[] = {"ok",
+                             "invalid or missing argument",
+                             "no HIP device available",
+                             "index out of bounds",
+                             "invalid input file",
+                             "HIP runtime error",
+                             "out of device memory",
+                             "call out of order",
+                             "unsupported configuration"};
+
+kmc_status fail(kmc_handle* h, kmc_status s, const std::string& msg) {
+  if (h) h->err = msg;
+  return s;
+}
+
+#define KMC_CK(expr)                                                                \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(h, e_ == hipErrorOutOfMemory ? KMC_ERR_OOM : KMC_ERR_HIP,         \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));               \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+int pick_dp(int d) {
+  const int opts[] = {16, 32, 64, 128, 256};
+  for (int o : opts)
+    if (d <= o) return o;
+  return 0;
+}
+
 
 kmc_status alloc_all(kmc_handle* h) {
   const size_t n = h->n;
@@ -615,7 +620,8 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     h->qstride = h->vqstride[0];
     h->idx_bits = 0;
     while ((1 << h->idx_bits) < h->Kp) ++h->idx_bits;
-    band_constants(h->prec, h->D, h->idx_bits, &h->alpha, &h->beta, &h->gref, &h->xpk);
+    kmc::band_constants(h->prec, h->D, h->idx_bits, &h->alpha, &h->beta, &h->gref,
+                        &h->xpk);
     st = alloc_all(h);
   } while (0);
   if (st != KMC_OK) {

@@ -47,6 +47,21 @@ struct PrepParams {
   int K, Kp, DP, prec;
 };
 
+// One problem of a batch (kmc_batch_*): points [poff, poff + n) of the packed
+// point space (poff a multiple of kBatchAlign; the rows up to the next
+// problem are zero and belong to the problem's extra "dead" cluster k),
+// master clusters [coff, coff + k] (k real + the dead one), distance-operand
+// rows [cpoff, cpoff + kpad) (kpad = k rounded up to 64, the padding rows
+// carry kFar), distance-kernel tiles [tile0, tile0 + ceil(n / tile)).
+struct BatchProblem {
+  long long poff;
+  int n, k, kpad, coff, cpoff, tile0, nblk;  // nblk: kBatchAlign blocks of the problem
+  long long bcoff;  // offset of the problem's (k + 1) x nblk block histogram
+};
+constexpr int kBatchAlign = 1024;  // point-space alignment of a batch problem
+constexpr int kBatchAssignThreads = 512;  // batch distance kernel: 8 waves,
+constexpr int kBatchTile = 256;           // 256 points per tile
+
 struct AssignParams {
   const float* X;       // n x DP
   const uint16_t* Xhi;  // n x DP
@@ -68,6 +83,11 @@ struct AssignParams {
   unsigned* qacnt;
   long long qstride;  // entries per region
   const int* done;
+  // batch mode (k_assign<..., BATCH=true>): the packed (problem, tile) queue
+  const BatchProblem* bp;
+  const int* item_prob;  // problem of each tile
+  const int* pdone;      // per-problem stop flags (converged problems are frozen)
+  int nitems;
 };
 
 struct RefineParams {
@@ -84,6 +104,9 @@ struct RefineParams {
   int* labels;
   const int* done;
   int K, Kp, D, DP;
+  // batch mode: the all-k pass is restricted to the point's own problem
+  const BatchProblem* bp;  // null for a single problem
+  const int* blk_prob;     // problem of each kBatchAlign-point block
 };
 
 struct SortParams {
@@ -157,7 +180,14 @@ struct LoadParams {
   double* xn2_64;
 };
 
+// margin-band constants of a precision form (host; kmc_api.hip)
+void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, float* gref,
+                    float* xpk);
+
 hipError_t launch_prep(const PrepParams& P, hipStream_t s);
+const void* assign_batch_kernel_ptr(int DP, int prec);
+hipError_t launch_assign_batch(int DP, int prec, const AssignParams& P, int grid, size_t lds,
+                               hipStream_t s);
 hipError_t launch_assign(int DP, int prec, int variant, const AssignParams& P, int grid,
                          size_t lds, hipStream_t s);
 size_t assign_lds(int DP, int prec, int kchunk);

@@ -362,7 +362,7 @@ __device__ __forceinline__ uint32_t opaque_vgpr(uint32_t s) {
 // after the sweep: one VALU op less per value. The packing perturbation is
 // then relative to | ||c||^2 - 2 x.c | <= d^2 + ||x||^2, which the band
 // covers through the xpk * ||x||^2 term.
-template <int DP, int PREC, int NW, int MS, bool PF, int WPE>
+template <int DP, int PREC, int NW, int MS, bool PF, int WPE, bool BATCH = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_assign(AssignParams P) {
   if (*P.done) return;
@@ -385,48 +385,96 @@ void k_assign(AssignParams P) {
   int* const q2 = P.q2 + 4 * (size_t)blockIdx.x * P.qstride;
   int* const qa = P.qa + (size_t)blockIdx.x * P.qstride;
 
-  // max ||c|| over the live centroids (for the error bound)
-  float cm = 0.0f;
-  for (int c = threadIdx.x; c < P.K; c += blockDim.x) {
-    const float v = P.cn[c];
-    if (v < kFarTest) cm = fmaxf(cm, v);  // excluded clusters carry kFar
-  }
-  cm = wave_max_f32(cm);
-  if (lane == 0) red_s[wave] = cm;
-  __syncthreads();
-  float cm_all = 0.0f;
+  // max ||c|| over the live operand rows [lo, hi) (for the error bound)
+  auto cnorm_max = [&](int lo, int hi) {
+    float cm = 0.0f;
+    for (int c = lo + threadIdx.x; c < hi; c += blockDim.x) {
+      const float v = P.cn[c];
+      if (v < kFarTest) cm = fmaxf(cm, v);  // excluded clusters carry kFar
+    }
+    cm = wave_max_f32(cm);
+    __syncthreads();  // red_s free
+    if (lane == 0) red_s[wave] = cm;
+    __syncthreads();
+    float cm_all = 0.0f;
 #pragma unroll
-  for (int w = 0; w < NW; ++w) cm_all = fmaxf(cm_all, red_s[w]);
-  const float cmax = sqrtf(cm_all);
-  const float Ecm = P.beta * cmax * cmax;
+    for (int w = 0; w < NW; ++w) cm_all = fmaxf(cm_all, red_s[w]);
+    return sqrtf(cm_all);
+  };
+  float cmax = 0.0f, Ecm = 0.0f;
+  if constexpr (!BATCH) {
+    cmax = cnorm_max(0, P.K);
+    Ecm = P.beta * cmax * cmax;
+  }
   const uint32_t keepv = opaque_vgpr(~P.idx_mask);
   const float ninf = __uint_as_float(opaque_vgpr(0xff800000u));
 
-  const bool resident = P.kchunk >= P.Kp;
-  if (resident) {
+  const bool resident = BATCH || P.kchunk >= P.Kp;
+  if (!BATCH && resident) {
     stage_centroids<DP, PREC>(P, smem, 0, P.Kp);
     __syncthreads();
   }
 
-  const long long ntiles = (P.n + kTile - 1) / kTile;
+  // tiles: a single problem strides over its tiles; a batch walks a
+  // contiguous run of the packed (problem, tile) queue, so consecutive tiles
+  // mostly share a problem and its staged centroids
+  long long t_begin, t_end, t_step;
+  if constexpr (BATCH) {
+    const long long per = ((long long)P.nitems + gridDim.x - 1) / gridDim.x;
+    t_begin = (long long)blockIdx.x * per;
+    t_end = min((long long)P.nitems, t_begin + per);
+    t_step = 1;
+  } else {
+    t_begin = blockIdx.x;
+    t_end = (P.n + kTile - 1) / kTile;
+    t_step = gridDim.x;
+  }
   // prefetch the next tile only while two fragment sets fit the registers
   constexpr int kFragRegs = frag_regs<DP, PREC>();
-  constexpr bool kPrefetch = PF && MS == 1 && kFragRegs <= 64;
+  constexpr bool kPrefetch = PF && MS == 1 && kFragRegs <= 64 && !BATCH;
+  // point row of set m of tile t and its bound (single problem)
   auto row_of = [&](long long t, int m) {
     return t * kTile + (long long)(wave * MS + m) * 32 + p;
   };
   XFrag<DP, PREC> cur[MS], nxt;
-  long long tile = blockIdx.x;
-  if (tile < ntiles) {
+  if (!BATCH && t_begin < t_end) {
 #pragma unroll
     for (int m = 0; m < MS; ++m) {
-      const long long r = row_of(tile, m);
+      const long long r = row_of(t_begin, m);
       load_xfrag<DP, PREC>(P, r, r < P.n, h, cur[m]);
     }
   }
-  for (; tile < ntiles; tile += gridDim.x) {
-    const long long ntile = tile + gridDim.x;
-    if (kPrefetch && ntile < ntiles) {
+  int staged = -1;  // batch: problem whose centroids are in LDS
+  for (long long tile = t_begin; tile < t_end; tile += t_step) {
+    const long long ntile = tile + t_step;
+    // this tile's rows: [rbase, rbase + rlim) of the packed point space
+    long long rbase = 0, rlim = P.n;
+    int Kp = P.Kp, Kv = P.K, cbase = 0;
+    long long lt = tile;
+    if constexpr (BATCH) {
+      const int pb = P.item_prob[tile];
+      if (P.pdone[pb]) continue;  // converged problem: frozen
+      const BatchProblem bpr = P.bp[pb];
+      lt = tile - bpr.tile0;
+      rbase = bpr.poff;
+      rlim = bpr.n;
+      Kp = bpr.kpad;
+      Kv = bpr.k;
+      cbase = bpr.coff;
+      if (pb != staged) {
+        __syncthreads();  // every wave is done with the previous problem
+        stage_centroids<DP, PREC>(P, smem, bpr.cpoff, Kp);
+        cmax = cnorm_max(bpr.cpoff, bpr.cpoff + Kp);  // synchronises
+        Ecm = P.beta * cmax * cmax;
+        staged = pb;
+      }
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        const long long r = row_of(lt, m);
+        load_xfrag<DP, PREC>(P, rbase + r, r < rlim, h, cur[m]);
+      }
+    }
+    if (kPrefetch && ntile < t_end) {
       const long long r = row_of(ntile, 0);
       load_xfrag<DP, PREC>(P, r, r < P.n, h, nxt);
     }
@@ -434,13 +482,14 @@ void k_assign(AssignParams P) {
     float b[MS][4], x2[MS];
 #pragma unroll
     for (int m = 0; m < MS; ++m) {
-      const long long r = row_of(tile, m);
-      x2[m] = r < P.n ? P.xn2[r] : 0.0f;
+      const long long r = row_of(lt, m);
+      x2[m] = r < rlim ? P.xn2[rbase + r] : 0.0f;
 #pragma unroll
       for (int k = 0; k < 4; ++k) b[m][k] = kFar;
     }
-    for (int c0 = 0; c0 < P.Kp; c0 += P.kchunk) {
-      const int cnt = min(P.kchunk, P.Kp - c0);
+    const int kchunk = BATCH ? Kp : P.kchunk;
+    for (int c0 = 0; c0 < Kp; c0 += kchunk) {
+      const int cnt = min(kchunk, Kp - c0);
       if (!resident) {
         __syncthreads();
         stage_centroids<DP, PREC>(P, smem, c0, cnt);
@@ -495,8 +544,9 @@ void k_assign(AssignParams P) {
 
 #pragma unroll
     for (int m = 0; m < MS; ++m) {
-      const long long prow = row_of(tile, m);
-      const bool valid = prow < P.n;
+      const long long lrow = row_of(lt, m);
+      const bool valid = lrow < rlim;
+      const long long prow = rbase + lrow;
       float b1, b2, b3, b4;
       {
         // this lane's k-half into bit 2 of the packed indices, then merge
@@ -544,8 +594,8 @@ void k_assign(AssignParams P) {
         const bool three = clear(b4);
         // a point without any finite distance goes to the all-k pass, which
         // applies the first-min rule
-        const bool sane = (b1 < kFarTest) && (i1 < P.K);
-        P.labels[prow] = sane ? i1 : 0;
+        const bool sane = (b1 < kFarTest) && (i1 < Kv);
+        P.labels[prow] = cbase + (sane ? i1 : 0);
         flag2 = sane && !uniq && (two || three);
         flagA = !sane || (!uniq && !two && !three);
         few3 = !two;
@@ -560,7 +610,7 @@ void k_assign(AssignParams P) {
         if (flag2) {
           const unsigned pos = base + mbcnt64(m2);
           *reinterpret_cast<int4*>(q2 + 4 * (size_t)pos) =
-              make_int4((int)prow, i1, i2, few3 ? i3 : -1);
+              make_int4((int)prow, cbase + i1, cbase + i2, few3 ? cbase + i3 : -1);
         }
       }
       if (mA) {
@@ -571,13 +621,15 @@ void k_assign(AssignParams P) {
         if (flagA) qa[base + mbcnt64(mA)] = (int)prow;
       }
     }
-    if (kPrefetch) {
-      cur[0] = nxt;
-    } else if (ntile < ntiles) {
+    if constexpr (!BATCH) {
+      if (kPrefetch) {
+        cur[0] = nxt;
+      } else if (ntile < t_end) {
 #pragma unroll
-      for (int m = 0; m < MS; ++m) {
-        const long long r = row_of(ntile, m);
-        load_xfrag<DP, PREC>(P, r, r < P.n, h, cur[m]);
+        for (int m = 0; m < MS; ++m) {
+          const long long r = row_of(ntile, m);
+          load_xfrag<DP, PREC>(P, r, r < P.n, h, cur[m]);
+        }
       }
     }
   }
@@ -770,12 +822,24 @@ __device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsi
     __syncthreads();
     float best[kRefinePts];
     int bi[kRefinePts];
+    // candidate clusters of each point: all K, or its own problem's in a batch
+    int clo[kRefinePts], chi[kRefinePts];
+    int cfrom = 0x7fffffff, cto = 0;
 #pragma unroll
     for (int q = 0; q < kRefinePts; ++q) {
       best[q] = __builtin_inff();
       bi[q] = 0x7fffffff;
+      clo[q] = 0;
+      chi[q] = P.K;
+      if (P.bp) {
+        const BatchProblem bpr = P.bp[P.blk_prob[pid_s[q] / kBatchAlign]];
+        clo[q] = bpr.coff;
+        chi[q] = bpr.coff + bpr.k;
+      }
+      cfrom = min(cfrom, clo[q]);
+      cto = max(cto, chi[q]);
     }
-    for (int c = wave * 64 + lane; c - lane < P.K; c += 256) {
+    for (int c = cfrom + wave * 64 + lane; c - lane < cto; c += 256) {
       const int cc = min(c, P.Kp - 1);
       const float* ct = P.CT + cc;
       float acc[kRefinePts];
@@ -811,10 +875,10 @@ __device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsi
 #pragma unroll
         for (int jj = 0; jj < 16; ++jj) cv[jj] = cnx[jj];
       }
-      const bool ok = c < P.K && P.alive[cc];
+      const bool ok = c < P.K && P.alive[min(c, P.K - 1)];
 #pragma unroll
       for (int q = 0; q < kRefinePts; ++q)
-        if (ok && acc[q] < best[q]) {
+        if (ok && c >= clo[q] && c < chi[q] && acc[q] < best[q]) {
           best[q] = acc[q];
           bi[q] = c;
         }
@@ -851,7 +915,8 @@ __device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsi
           ix = oi;
         }
       }
-      if (g * kRefinePts + q < total) P.labels[pid_s[q]] = (ix == 0x7fffffff) ? 0 : ix;
+      // no finite distance (NaN data): the first cluster of the point's range
+      if (g * kRefinePts + q < total) P.labels[pid_s[q]] = (ix == 0x7fffffff) ? clo[q] : ix;
     }
   }
 }
@@ -1416,7 +1481,7 @@ __global__ __launch_bounds__(256) void k_load_points(LoadParams P) {
   }
   if (lane == 0) {
     P.xn2[i] = s;
-    P.xn2_64[i] = s64;
+    if (P.xn2_64) P.xn2_64[i] = s64;
   }
 }
 
@@ -1555,6 +1620,35 @@ hipError_t launch_assign(int DP, int prec, int v, const AssignParams& P, int gri
   return hipLaunchKernel(fn, dim3(grid), dim3(assign_threads(v)), args, lds, s);
 }
 
+// Batch mode of the distance kernel: one shape (8 waves, one point set, 256
+// points per tile) for every (DP, precision).
+template <int PREC>
+static const void* batch_kernel_of(int DP) {
+  switch (DP) {
+    case 16: return reinterpret_cast<const void*>(k_assign<16, PREC, 8, 1, false, 0, true>);
+    case 32: return reinterpret_cast<const void*>(k_assign<32, PREC, 8, 1, false, 0, true>);
+    case 64: return reinterpret_cast<const void*>(k_assign<64, PREC, 8, 1, false, 0, true>);
+    case 128: return reinterpret_cast<const void*>(k_assign<128, PREC, 8, 1, false, 0, true>);
+    case 256: return reinterpret_cast<const void*>(k_assign<256, PREC, 8, 1, false, 0, true>);
+    default: return nullptr;
+  }
+}
+const void* assign_batch_kernel_ptr(int DP, int prec) {
+  switch (prec) {
+    case PREC_FP32: return batch_kernel_of<PREC_FP32>(DP);
+    case PREC_BF16X3: return batch_kernel_of<PREC_BF16X3>(DP);
+    case PREC_BF16: return batch_kernel_of<PREC_BF16>(DP);
+    default: return nullptr;
+  }
+}
+hipError_t launch_assign_batch(int DP, int prec, const AssignParams& P, int grid, size_t lds,
+                               hipStream_t s) {
+  const void* fn = assign_batch_kernel_ptr(DP, prec);
+  if (!fn) return hipErrorInvalidValue;
+  void* args[] = {const_cast<AssignParams*>(&P)};
+  return hipLaunchKernel(fn, dim3(grid), dim3(kBatchAssignThreads), args, lds, s);
+}
+
 hipError_t launch_prep(const PrepParams& P, hipStream_t s) {
   hipLaunchKernelGGL(k_prep, dim3((P.Kp + 3) / 4), dim3(256), 0, s, P);
   return hipGetLastError();
@@ -1594,7 +1688,7 @@ hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
 }
 hipError_t launch_finish(const FinishParams& F, hipStream_t s) {
   hipLaunchKernelGGL(k_finish, dim3(F.K), dim3(64 * kFinishWaves), 0, s, F);
-  hipLaunchKernelGGL(k_stats, dim3(1), dim3(kStatsThreads), 0, s, F.stats);
+  if (F.stats.state) hipLaunchKernelGGL(k_stats, dim3(1), dim3(kStatsThreads), 0, s, F.stats);
   return hipGetLastError();
 }
 hipError_t launch_load_points(const LoadParams& P, hipStream_t s) {

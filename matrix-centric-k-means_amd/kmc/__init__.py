@@ -78,6 +78,21 @@ class _KKResult(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class _Problem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("d", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("X", ctypes.c_void_p), ("ld", ctypes.c_size_t), ("C0", ctypes.c_void_p)]
+
+
+class _BatchConfig(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_float), ("precision", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class _BatchResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("converged_at", ctypes.c_uint64),
+                ("inertia", ctypes.c_double)]
+
+
 KERNELS = {"linear": 0, "polynomial": 1, "sigmoid": 2, "rbf": 3}
 STORES = {"fp32": 0, "bf16": 1}
 RUN_CONVERGE_SCORE = 0x4
@@ -125,6 +140,14 @@ def load_library(path=LIB_PATH):
         "kmc_kk_distances": (ctypes.c_int, [vp, vp, vp]),
         "kmc_kk_get_kernel_rows": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.c_size_t, vp]),
         "kmc_kk_last_error": (ctypes.c_char_p, [vp]),
+        "kmc_batch_create": (ctypes.c_int, [P(_Problem), ctypes.c_uint32, P(_BatchConfig),
+                                            P(vp)]),
+        "kmc_batch_run": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32, P(_BatchResult),
+                                         P(ctypes.c_double)]),
+        "kmc_batch_get_labels": (ctypes.c_int, [vp, ctypes.c_uint32, vp]),
+        "kmc_batch_get_centroids": (ctypes.c_int, [vp, ctypes.c_uint32, vp]),
+        "kmc_batch_last_error": (ctypes.c_char_p, [vp]),
+        "kmc_batch_destroy": (ctypes.c_int, [vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -337,4 +360,78 @@ class KernelEngine:
         out = np.empty((nrows, self.n), dtype=np.float32)
         self._check(self._lib.kmc_kk_get_kernel_rows(self._h, row0, nrows, out.ctypes.data),
                     "kk_get_kernel_rows")
+        return out
+
+
+class Batch:
+    """Many independent Lloyd problems of uneven n, d and k run together
+    (kmc_batch_*): every stage of an iteration is one launch for the whole
+    batch. problems: sequence of (X n x d, C0 k x d) float32 arrays. Each
+    problem follows the reference CPU k-means with its own shift test; the
+    centroids are FP64 sums rounded once (as update="tree")."""
+
+    def __init__(self, problems, tol=1e-4, precision="fp32", device=0):
+        self._lib = load_library()
+        self._keep = []
+        arr = (_Problem * len(problems))()
+        self.shapes = []
+        for i, (X, C0) in enumerate(problems):
+            X = np.ascontiguousarray(X, dtype=np.float32)
+            C0 = np.ascontiguousarray(C0, dtype=np.float32)
+            if X.ndim != 2 or C0.ndim != 2 or C0.shape[1] != X.shape[1]:
+                raise ValueError("problem %d: X n x d and C0 k x d expected" % i)
+            self._keep += [X, C0]
+            n, d = X.shape
+            k = C0.shape[0]
+            arr[i] = _Problem(n, d, k, X.ctypes.data, d, C0.ctypes.data)
+            self.shapes.append((n, d, k))
+        cfg = _BatchConfig(float(tol), PRECISIONS[precision], int(device), 0)
+        h = ctypes.c_void_p()
+        st = self._lib.kmc_batch_create(arr, len(problems), ctypes.byref(cfg), ctypes.byref(h))
+        self._keep = []
+        if st != OK:
+            raise KmcError(st, "kmc_batch_create(%d problems)" % len(problems))
+        self._h = h
+
+    def _check(self, st, what):
+        if st != OK:
+            msg = self._lib.kmc_batch_last_error(self._h)
+            raise KmcError(st, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.kmc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, maxiter, check_converged=True):
+        res = (_BatchResult * len(self.shapes))()
+        ms = ctypes.c_double(0.0)
+        self._check(self._lib.kmc_batch_run(self._h, int(maxiter),
+                                            RUN_CHECK_CONVERGED if check_converged else 0,
+                                            res, ctypes.byref(ms)), "batch_run")
+        return [{"iterations": r.iterations, "converged_at": r.converged_at,
+                 "inertia": r.inertia} for r in res], ms.value
+
+    def labels(self, p):
+        out = np.empty(self.shapes[p][0], dtype=np.int32)
+        self._check(self._lib.kmc_batch_get_labels(self._h, p, out.ctypes.data), "batch_labels")
+        return out
+
+    def centroids(self, p):
+        n, d, k = self.shapes[p]
+        out = np.empty((k, d), dtype=np.float32)
+        self._check(self._lib.kmc_batch_get_centroids(self._h, p, out.ctypes.data),
+                    "batch_centroids")
         return out

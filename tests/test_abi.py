@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -56,4 +57,42 @@ def test_no_device_is_reported(kmc_mod):
         pytest.skip("a HIP device is present")
     with pytest.raises(kmc_mod.KmcError) as ei:
         kmc_mod.Engine(100, 8, 4)
+    assert ei.value.status == kmc_mod.ERR_NO_DEVICE
+
+
+def test_batch_create_rejects_bad_problems(kmc_mod):
+    lib = kmc_mod.load_library()
+    X = np.zeros((100, 8), np.float32)
+    C = np.zeros((4, 8), np.float32)
+    P = kmc_mod._Problem
+    cfg = kmc_mod._BatchConfig(1e-4, 0, 0, 0)
+    h = ctypes.c_void_p()
+    empty = (P * 1)()
+    assert lib.kmc_batch_create(empty, 0, ctypes.byref(cfg), ctypes.byref(h)) == kmc_mod.ERR_ARG
+    cases = [
+        P(0, 8, 4, X.ctypes.data, 8, C.ctypes.data),      # n = 0
+        P(100, 8, 0, X.ctypes.data, 8, C.ctypes.data),    # k = 0
+        P(100, 8, 101, X.ctypes.data, 8, C.ctypes.data),  # k > n
+        P(100, 8, 4, None, 8, C.ctypes.data),             # no points
+        P(100, 8, 4, X.ctypes.data, 4, C.ctypes.data),    # ld < d
+    ]
+    for c in cases:
+        arr = (P * 1)(c)
+        assert lib.kmc_batch_create(arr, 1, ctypes.byref(cfg), ctypes.byref(h)) == kmc_mod.ERR_ARG
+    big = (P * 1)(P(100, 300, 4, np.zeros((100, 300), np.float32).ctypes.data, 300,
+                    np.zeros((4, 300), np.float32).ctypes.data))
+    assert lib.kmc_batch_create(big, 1, ctypes.byref(cfg), ctypes.byref(h)) in (
+        kmc_mod.ERR_UNSUPPORTED,)
+    bad_prec = kmc_mod._BatchConfig(1e-4, 7, 0, 0)
+    ok = (P * 1)(P(100, 8, 4, X.ctypes.data, 8, C.ctypes.data))
+    assert lib.kmc_batch_create(ok, 1, ctypes.byref(bad_prec), ctypes.byref(h)) == kmc_mod.ERR_ARG
+
+
+def test_batch_without_device(kmc_mod):
+    if kmc_mod.device_count() > 0:
+        pytest.skip("a device is visible")
+    X = np.zeros((100, 8), np.float32)
+    C = np.zeros((4, 8), np.float32)
+    with pytest.raises(kmc_mod.KmcError) as ei:
+        kmc_mod.Batch([(X, C)])
     assert ei.value.status == kmc_mod.ERR_NO_DEVICE
