@@ -205,12 +205,105 @@ def bench_kernel_kmeans(a):
     print(json.dumps(out))
 
 
+def c4_problems(count=512, seed=20261205):
+    """512 independent problems: N log-uniform in [1e3, 2e5], D in
+    {16, 32, 64, 128}, K uniform in [4, 128], Gaussian blobs with per-instance
+    seeds, K seed-chosen rows as initial centroids."""
+    from kmc import datagen
+    rng = np.random.default_rng(seed)
+    probs = []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))))
+        d = int(rng.choice([16, 32, 64, 128]))
+        k = int(rng.integers(4, 129))
+        X, _, _ = datagen.blobs(n, d, k, seed + 1 + i, -10.0, 10.0, 0.5, 2.0)
+        probs.append((X, datagen.init_rows(X, k, seed + 10_000 + i)))
+    return probs
+
+
+def bench_batch(a):
+    """Batched Lloyd (512 problems of uneven n, d, k, 30 iterations each, no
+    convergence test): every stage of an iteration is one launch for the
+    whole batch. value = problem-iterations per second of the headline form;
+    FP32 beside it; a sample of problems run one by one gives the
+    unbatched rate for comparison."""
+    import torch
+    import kmc
+    steps = a.steps if a.steps is not None else 30
+    log("generating 512 problems")
+    probs = c4_problems()
+    nprob = len(probs)
+    npts = sum(X.shape[0] for X, _ in probs)
+    flops = sum(2.0 * X.shape[0] * C.shape[0] * X.shape[1] for X, C in probs)
+    forms = {}
+    for prec in (a.prec, "fp32") if a.prec != "fp32" else ("fp32",):
+        with kmc.Batch(probs, tol=0.0, precision=prec) as b:
+            b.run(min(a.warmup, steps), check_converged=False)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res, ms = b.run(steps, check_converged=False)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            its = sum(int(r["iterations"]) for r in res)
+            forms[prec] = {"problem_iters_per_s": round(its / wall, 1),
+                           "batch_iters_per_s": round(steps / wall, 3),
+                           "ms_per_batch_iteration": round(wall * 1e3 / steps, 4),
+                           "device_ms": round(ms, 3),
+                           "distance_tflops": round(flops * steps / wall / 1e12, 2)}
+            log("batch", prec, forms[prec])
+    # unbatched comparison: the first 16 problems, one engine each
+    sample = probs[:16]
+    t_lone = 0.0
+    for X, C0 in sample:
+        with kmc.Engine(X.shape[0], X.shape[1], C0.shape[0], tol=0.0, precision=a.prec,
+                        update="tree") as e:
+            e.set_points(X)
+            e.set_centroids(C0)
+            e.run(1, check_converged=False)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e.run(steps, check_converged=False)
+            torch.cuda.synchronize()
+            t_lone += time.perf_counter() - t0
+    lone_rate = len(sample) * steps / t_lone
+    head = forms[a.prec]
+    wall_head = head["ms_per_batch_iteration"] * steps / 1e3
+    ubytes = sum(X.shape[0] * (X.shape[1] * 4 + 4) for X, _ in probs)
+    out = {"metric": "batched Lloyd problem-iterations/s, 512 problems N~logU[1e3,2e5] "
+                     "D in {16,32,64,128} K~U[4,128]",
+           "value": head["problem_iters_per_s"], "unit": "problem-iter/s", "n_gpus": 1,
+           "steps": steps, "warmup": a.warmup, "ms_per_step": head["ms_per_batch_iteration"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": {"bf16x3": "bf16", "bf16": "bf16", "fp32": "fp32"}[a.prec],
+           "data": "synthetic",
+           "config": {"workload": "512 batched Lloyd problems, %d points in total, %d "
+                                  "iterations each, tol=0" % (npts, steps),
+                      "problems": nprob, "points": npts, "form": a.prec},
+           "roofline": {"bound": "hbm", "kernel": "whole batch iteration",
+                        "achieved": round(ubytes * steps / wall_head / 1e9, 1),
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": round(ubytes * steps / wall_head / HBM_PEAK, 4),
+                        "traffic": None, "algorithmic_bytes_per_launch": ubytes,
+                        "note": "algorithmic bytes of one batch iteration (one pass over "
+                                "every problem's X, n*d*4 + n*4) / wall time of an "
+                                "iteration; the packed layout pads every problem to the "
+                                "batch's largest D (128), so the update streams "
+                                "sum(ceil(n/1024)*1024)*128*4 bytes"},
+           "distance_flops_per_iteration": flops,
+           "forms": forms,
+           "unbatched_sample": {"problems": len(sample),
+                                "problem_iters_per_s": round(lone_rate, 1),
+                                "batched_speedup": round(head["problem_iters_per_s"] /
+                                                         lone_rate, 2)}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c5", "c3"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c5", "c3", "c4"],
                     help="c2: N=1M D=128 K=256 (default); c5: N=4M D=256 K=4096; "
-                         "c3: kernel k-means N=65536 D=784 K=10")
+                         "c3: kernel k-means N=65536 D=784 K=10; c4: 512 batched problems")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed iterations (default: 50 for c2, 20 for c5, 100 for c3)")
     ap.add_argument("--warmup", type=int, default=5)
@@ -234,6 +327,9 @@ def main():
         return
     if a.workload == "c3":
         bench_kernel_kmeans(a)
+        return
+    if a.workload == "c4":
+        bench_batch(a)
         return
     global N, D, K, DATA_SEED
     W = WORKLOADS[a.workload]
