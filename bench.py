@@ -308,8 +308,10 @@ def main():
                     help="timed iterations (default: 50 for c2, 20 for c5, 100 for c3)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--prec", default="bf16", choices=["bf16x3", "bf16", "fp32"])
-    ap.add_argument("--update", default="tree", choices=["ordered", "tree"],
-                    help="centroid update: FP64 tree, or the reference's float summation order")
+    ap.add_argument("--update", default="tree", choices=["ordered", "tree", "fixed"],
+                    help="centroid update: FP64 tree over the sorted members, exact "
+                         "fixed-point streaming (no sort), or the reference's float "
+                         "summation order")
     ap.add_argument("--no-x3", action="store_true", help="skip the bf16x3 comparison form")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying a HIP graph")
@@ -385,7 +387,8 @@ def main():
                    "algorithmic_flops_per_launch": flops,
                    "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches}
     ubytes = N * D * 4 + N * 4 + K * D * 4  # X rows once, member ids, centroids
-    uname = "k_update_tree<2>" if a.update == "tree" else "k_update"
+    uname = {"tree": "k_update_tree<2>", "ordered": "k_update",
+             "fixed": "k_update_fx<64>"}[a.update]
     roof_update = {"bound": "hbm", "kernel": uname,
                    "achieved": round(ubytes / t_update / 1e9, 1), "peak": HBM_PEAK / 1e9,
                    "unit": "GB/s", "frac": round(ubytes / t_update / HBM_PEAK, 4),
@@ -415,7 +418,8 @@ def main():
                    "n": N, "d": D, "k": K, "form": a.prec, "seed": DATA_SEED,
                    "init": "%d distinct seed-chosen rows (seed %d)" % (K, INIT_SEED),
                    "update": {"ordered": "ordered (reference summation order)",
-                              "tree": "tree (FP64 chunk partials)"}[a.update]},
+                              "tree": "tree (FP64 chunk partials)",
+                              "fixed": "fixed (exact fixed-point streaming sums)"}[a.update]},
         "roofline": roof,
         "roofline_second_kernel": roof_other,
         "assign_variant": {"chosen": head["tuning"]["chosen"],

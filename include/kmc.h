@@ -61,7 +61,13 @@ typedef enum {
   /* per cluster, FP64 partial sums over fixed chunks of the sorted member
    * list, combined in chunk order and rounded once: deterministic, at least
    * as accurate, every CU busy; not bit-identical to the float sums */
-  KMC_UPDATE_TREE = 1
+  KMC_UPDATE_TREE = 1,
+  /* one streaming pass over the points in index order (no sort): exact 64-bit
+   * fixed-point sums (x scaled by a power of two chosen from max|x| and n;
+   * exact for |x| >= 2^-14 relative to that scale), means rounded once;
+   * deterministic. Falls back to KMC_UPDATE_TREE when the per-workgroup
+   * accumulators (K x 16 x 8 bytes at least) exceed the LDS */
+  KMC_UPDATE_FIXED = 2
 } kmc_update_mode;
 
 /* kmc_run flags */

@@ -52,6 +52,14 @@ struct kmc_handle {
   double* tpart = nullptr;
   int nparts = 0;
   int update_mode = 0;
+  // fixed-point streaming update
+  double* xn64 = nullptr;
+  long long* fpart = nullptr;
+  long long* qpart = nullptr;
+  int* cpart = nullptr;
+  int fx_ds = 0, fx_nslice = 0, fx_wgps = 0;
+  float fx_scale = 1.0f;
+  double fx_qscale = 1.0;
   int* conv = nullptr;
   double* shift2 = nullptr;
   int* alive = nullptr;
@@ -201,6 +209,13 @@ kmc_status alloc_all(kmc_handle* h) {
   KMC_CK(dalloc(&h->ipart, (size_t)h->nparts));
   if (h->update_mode == kmc::UPDATE_TREE)
     KMC_CK(dalloc(&h->tpart, ((n + kmc::kTreeRows - 1) / kmc::kTreeRows + K) * DP));
+  if (h->update_mode == kmc::UPDATE_FIXED) {
+    const size_t Kp = (size_t)h->Kp, ws = (size_t)h->fx_wgps;
+    KMC_CK(dalloc(&h->xn64, n));
+    KMC_CK(dalloc(&h->fpart, (size_t)h->fx_nslice * ws * Kp * (size_t)h->fx_ds));
+    KMC_CK(dalloc(&h->qpart, ws * Kp));
+    KMC_CK(dalloc(&h->cpart, ws * Kp));
+  }
   KMC_CK(dalloc(&h->conv, K));
   KMC_CK(dalloc(&h->shift2, K));
   KMC_CK(dalloc(&h->alive, K));
@@ -220,7 +235,7 @@ void free_all(kmc_handle* h) {
                   h->Clo, h->cn, h->CT, h->labels, h->q2, h->qa, h->q2cnt, h->qacnt,
                   h->bcount,
                   h->count, h->start, h->members, h->ipart, h->tpart, h->conv,
-                  h->shift2, h->alive, h->state};
+                  h->shift2, h->alive, h->state, h->xn64, h->fpart, h->qpart, h->cpart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -350,7 +365,8 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty, hipEvent_t
   sp.CH = h->CH;
   sp.nblk = h->nblk;
   sp.lbits = h->lbits;
-  KMC_CK(kmc::launch_sort(sp, h->stream));
+  const bool fixed = h->update_mode == kmc::UPDATE_FIXED;
+  if (!fixed) KMC_CK(kmc::launch_sort(sp, h->stream));
 
   kmc::UpdateParams up{};
   up.X = h->X;
@@ -367,6 +383,16 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty, hipEvent_t
   up.K = h->K;
   up.D = h->D;
   up.DP = h->DP;
+  up.xn64 = h->xn64;
+  up.fpart = h->fpart;
+  up.qpart = h->qpart;
+  up.cpart = h->cpart;
+  up.fscale = h->fx_scale;
+  up.qscale = h->fx_qscale;
+  up.Kp = h->Kp;
+  up.nslice = h->fx_nslice;
+  up.wgps = h->fx_wgps;
+  up.ds = h->fx_ds;
   if (u0) KMC_CK(record_event(h->stream, u0));
   KMC_CK(kmc::launch_update(up, h->update_mode, h->stream));
   if (u1) KMC_CK(record_event(h->stream, u1));
@@ -389,6 +415,18 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty, hipEvent_t
   fp.exclude_empty = exclude_empty;
   fp.do_prep = 1;
   fp.prep = prep_params(h);
+  fp.fixed = fixed ? 1 : 0;
+  fp.fpart = h->fpart;
+  fp.qpart = h->qpart;
+  fp.cpart = h->cpart;
+  fp.count_out = h->count;
+  fp.ipart_out = h->ipart;
+  fp.Kp = h->Kp;
+  fp.nslice = h->fx_nslice;
+  fp.wgps = h->fx_wgps;
+  fp.ds = h->fx_ds;
+  fp.inv_fscale = 1.0 / (double)h->fx_scale;
+  fp.inv_qscale = 1.0 / h->fx_qscale;
   kmc::StatsParams& st = fp.stats;
   st.ipart = h->ipart;
   st.nparts = h->nparts;
@@ -476,8 +514,9 @@ void fill_result(const kmc_handle* h, const DevState& s, kmc_result* r) {
 }
 
 kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
-  double* xn64 = nullptr;
-  KMC_CK(dalloc(&xn64, h->n + 256));
+  const bool keep64 = h->update_mode == kmc::UPDATE_FIXED;
+  double* xn64 = keep64 ? h->xn64 : nullptr;
+  if (!keep64) KMC_CK(dalloc(&xn64, h->n + 256));
   kmc::LoadParams lp{};
   lp.src = dsrc;
   lp.ld = ld;
@@ -490,10 +529,41 @@ kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
   lp.xn2 = h->xn2;
   lp.xn2_64 = xn64;
   hipError_t e = kmc::launch_load_points(lp, h->stream);
+  double* part = nullptr;
+  if (e == hipSuccess) e = dalloc(&part, 257 + 2);
   if (e == hipSuccess)
-    e = kmc::launch_sum_f64(xn64, (long long)h->n, &h->state->sum_x2, xn64 + h->n, h->stream);
+    e = kmc::launch_sum_f64(xn64, (long long)h->n, &h->state->sum_x2, part, h->stream);
+  if (e == hipSuccess && keep64) {
+    // fixed-point scales: powers of two that keep n * max|x| * scale and
+    // n * max||x||^2 * qscale below 2^61
+    unsigned* amax = reinterpret_cast<unsigned*>(part + 257);
+    unsigned long long* qmax = reinterpret_cast<unsigned long long*>(part + 258);
+    e = hipMemsetAsync(part + 257, 0, 2 * sizeof(double), h->stream);
+    if (e == hipSuccess)
+      e = kmc::launch_absmax(h->X, (long long)h->n * h->DP, xn64, (long long)h->n, amax, qmax,
+                             h->stream);
+    unsigned long long hv[2] = {0, 0};
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hv, part + 257, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) {
+      float am;
+      const unsigned ab = (unsigned)hv[0];
+      std::memcpy(&am, &ab, sizeof(am));
+      double qm;
+      std::memcpy(&qm, &hv[1], sizeof(qm));
+      const double nn = (double)std::max<size_t>(h->n, 1);
+      const int s1 = std::max(-120, std::min(120, (int)std::floor(
+                                                      61.0 - std::log2(nn * std::max<double>(am, 1e-30)))));
+      const int s2 = std::max(-900, std::min(900, (int)std::floor(
+                                                      61.0 - std::log2(nn * std::max(qm, 1e-300)))));
+      h->fx_scale = std::ldexp(1.0f, s1);
+      h->fx_qscale = std::ldexp(1.0, s2);
+    }
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  (void)hipFree(xn64);
+  if (part) (void)hipFree(part);
+  if (!keep64) (void)hipFree(xn64);
   KMC_CK(e);
   h->have_points = true;
   return KMC_OK;
@@ -538,7 +608,9 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
   if (cfg->n > (size_t)0x7fffffff) return KMC_ERR_UNSUPPORTED;
   if (cfg->k > 16384u) return KMC_ERR_UNSUPPORTED;
   if (cfg->precision < 0 || cfg->precision > 2) return KMC_ERR_ARG;
-  if (cfg->update != KMC_UPDATE_ORDERED && cfg->update != KMC_UPDATE_TREE) return KMC_ERR_ARG;
+  if (cfg->update != KMC_UPDATE_ORDERED && cfg->update != KMC_UPDATE_TREE &&
+      cfg->update != KMC_UPDATE_FIXED)
+    return KMC_ERR_ARG;
   if (!(cfg->tol >= 0.0f)) return KMC_ERR_ARG;
   const int DP = pick_dp((int)cfg->d);
   if (DP == 0) return KMC_ERR_UNSUPPORTED;
@@ -556,7 +628,9 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
   h->DP = DP;
   h->Kp = (h->K + 63) / 64 * 64;  // centroid blocks are processed in pairs
   h->prec = cfg->precision;
-  h->update_mode = cfg->update == KMC_UPDATE_TREE ? kmc::UPDATE_TREE : kmc::UPDATE_ORDERED;
+  h->update_mode = cfg->update == KMC_UPDATE_TREE    ? kmc::UPDATE_TREE
+                   : cfg->update == KMC_UPDATE_FIXED ? kmc::UPDATE_FIXED
+                                                     : kmc::UPDATE_ORDERED;
   kmc_status st = KMC_OK;
   do {
     if (hipSetDevice(h->dev) != hipSuccess) { st = fail(h, KMC_ERR_HIP, "hipSetDevice"); break; }
@@ -576,10 +650,27 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     h->nblk = (int)((h->n + h->CH - 1) / h->CH);
     h->lbits = 0;
     while ((1 << h->lbits) < h->K) ++h->lbits;
+    if (h->update_mode == kmc::UPDATE_FIXED) {
+      // slice width: the widest whose per-workgroup accumulators fit the
+      // LDS; none (very large K): the tree update instead
+      const size_t kFxLdsMax = 150 * 1024;
+      h->fx_ds = 0;
+      for (int ds : {64, 32, 16})
+        if (ds <= DP && kmc::update_fx_lds(h->Kp, ds) <= kFxLdsMax) {
+          h->fx_ds = ds;
+          break;
+        }
+      if (h->fx_ds == 0) {
+        h->update_mode = kmc::UPDATE_TREE;
+      } else {
+        h->fx_nslice = DP / h->fx_ds;
+        h->fx_wgps = std::max(1, h->cus / h->fx_nslice);
+      }
+    }
     const long long S = (h->D + 63) / 64;
     h->nparts = h->update_mode == kmc::UPDATE_TREE
                     ? (int)(((long long)h->n + kmc::kTreeRows - 1) / kmc::kTreeRows)
-                    : (int)(h->K * S);
+                    : h->update_mode == kmc::UPDATE_FIXED ? h->K : (int)(h->K * S);
 
     // assign launch shape: all centroids resident in LDS when they fit
     const size_t kResidentMax = 144 * 1024, kChunkMax = 64 * 1024;

@@ -7,7 +7,7 @@
 namespace kmc {
 
 enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_BF16 = 2 };
-enum { UPDATE_ORDERED = 0, UPDATE_TREE = 1 };
+enum { UPDATE_ORDERED = 0, UPDATE_TREE = 1, UPDATE_FIXED = 2 };
 constexpr int kTreeRows = 256;  // sorted rows per tree-update chunk
 constexpr int kMaxRegions = 4096;  // assign workgroups (refinement queue regions)
 constexpr int kRefineBlocks = 1024;  // refinement grid
@@ -133,6 +133,16 @@ struct UpdateParams {
   const int* done;
   long long n;
   int K, D, DP;
+  // fixed-point streaming update (UPDATE_FIXED): workgroup w of dimension
+  // slice s (ds dims) writes exact integer sums fpart[s][w][Kp][ds], and
+  // (slice 0) the fixed-point sum of ||x||^2 and the member count per cluster
+  const double* xn64;  // FP64 ||x||^2 per point
+  long long* fpart;
+  long long* qpart;    // [w][Kp]
+  int* cpart;          // [w][Kp]
+  float fscale;        // x -> round(x * fscale), a power of two
+  double qscale;       // ||x||^2 -> round(||x||^2 * qscale), a power of two
+  int Kp, nslice, wgps, ds;
 };
 
 
@@ -164,6 +174,15 @@ struct FinishParams {
   float tol;
   int K, D, DP, exclude_empty;
   int do_prep;        // write the next iteration's distance operands (prep)
+  // fixed-point update: the partials to add; count and inertia are written
+  int fixed;
+  const long long* fpart;
+  const long long* qpart;
+  const int* cpart;
+  int* count_out;      // cluster sizes
+  double* ipart_out;   // FP64 inertia of the assignment, per cluster
+  int Kp, nslice, wgps, ds;
+  double inv_fscale, inv_qscale;
   PrepParams prep;
   StatsParams stats;  // k_stats, launched after k_finish
 };
@@ -199,6 +218,11 @@ hipError_t launch_refine(const RefineParams& P, hipStream_t s);
 hipError_t launch_sort(const SortParams& P, hipStream_t s);
 hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s);
 hipError_t launch_finish(const FinishParams& F, hipStream_t s);
+// fixed-point streaming update: LDS bytes of one workgroup, 0 when it does
+// not fit (then the tree update is used)
+size_t update_fx_lds(int Kp, int ds);
+hipError_t launch_absmax(const float* X, long long count, const double* xn64, long long n,
+                         unsigned* amax, unsigned long long* qmax, hipStream_t s);
 hipError_t launch_load_points(const LoadParams& P, hipStream_t s);
 hipError_t launch_sum_f64(const double* v, long long n, double* out, double* part256,
                           hipStream_t s);

@@ -1247,6 +1247,135 @@ __global__ __launch_bounds__(256) void k_update_tree(UpdateParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// k_update_fx (UPDATE_FIXED): one streaming pass over X in point order, no
+// sort. Workgroup w of dimension slice s owns the points [w*per, (w+1)*per)
+// and the DS dimensions [s*DS, (s+1)*DS); each value becomes the integer
+// round(x * fscale) (fscale a power of two chosen from max|x| and n, so every
+// |x| >= 2^-14 converts exactly) and is added to the cluster's LDS
+// accumulator with a 64-bit integer atomic. Integer sums do not depend on
+// the order of the additions, so the result is deterministic however the
+// waves interleave. Slice-0 workgroups also count the members and add the
+// fixed-point ||x||^2 (for the FP64 inertia). k_finish adds the workgroup
+// partials exactly and rounds the means once.
+// ---------------------------------------------------------------------------
+constexpr int kFxThreads = 512;
+// LDS row stride of a cluster's accumulators: one extra word so that lanes
+// of different points (different clusters) fall in different banks
+__host__ __device__ constexpr int fx_stride(int ds) { return ds + 1; }
+size_t update_fx_lds(int Kp, int ds) {
+  return (size_t)Kp * fx_stride(ds) * 8 + (size_t)Kp * 8 + (size_t)Kp * 4;
+}
+
+template <int DS>
+__global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
+  if (*P.done) return;
+  constexpr int ST = fx_stride(DS);
+  extern __shared__ __attribute__((aligned(16))) unsigned char fsm[];
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(fsm);
+  unsigned long long* qacc = acc + (size_t)P.Kp * ST;
+  unsigned* cacc = reinterpret_cast<unsigned*>(qacc + P.Kp);
+  const int slice = blockIdx.x % P.nslice, wg = blockIdx.x / P.nslice;
+  for (int t = threadIdx.x; t < P.Kp * ST + P.Kp; t += kFxThreads) acc[t] = 0ull;
+  for (int t = threadIdx.x; t < P.Kp; t += kFxThreads) cacc[t] = 0u;
+  __syncthreads();
+  const long long per = (P.n + P.wgps - 1) / P.wgps;
+  const long long lo = (long long)wg * per, hi = min(P.n, lo + per);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // lane = (point of the step, 4-dimension group): a float4 load per lane
+  constexpr int LPP = DS / 4;    // lanes per point
+  constexpr int PPW = 64 / LPP;  // points per wave step
+  constexpr int U = 8;           // steps whose loads are issued together
+  constexpr int NW = kFxThreads / 64;
+  const int pp = lane / LPP, dg = lane % LPP;
+  const int col = slice * DS + 4 * dg;
+  const float fs = P.fscale;
+  // software pipeline: the loads of step i+1 are in flight while the
+  // atomics of step i run
+  const long long stride = (long long)NW * PPW * U;
+  float4 v[U], vn[U];
+  int lab[U], labn[U];
+  auto load_step = [&](long long i0, float4(&vv)[U], int(&ll)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = i0 + u * PPW + pp;
+      const bool ok = i < hi;
+      const long long ic = ok ? i : lo;  // a valid row; its values are masked
+      ll[u] = ok ? P.labels[ic] : 0;
+      vv[u] = *reinterpret_cast<const float4*>(P.X + (size_t)ic * P.DP + col);
+      if (!ok) vv[u] = make_float4(0.f, 0.f, 0.f, 0.f);  // adds exact zeros
+    }
+  };
+  long long i0 = lo + (long long)wave * PPW * U;
+  if (i0 < hi) load_step(i0, v, lab);
+  for (; i0 < hi; i0 += stride) {
+    const bool more = i0 + stride < hi;
+    if (more) load_step(i0 + stride, vn, labn);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      unsigned long long* a = acc + (size_t)lab[u] * ST + 4 * dg;
+      atomicAdd(a + 0, (unsigned long long)__float2ll_rn(v[u].x * fs));
+      atomicAdd(a + 1, (unsigned long long)__float2ll_rn(v[u].y * fs));
+      atomicAdd(a + 2, (unsigned long long)__float2ll_rn(v[u].z * fs));
+      atomicAdd(a + 3, (unsigned long long)__float2ll_rn(v[u].w * fs));
+    }
+    if (slice == 0 && dg == 0) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long long i = i0 + u * PPW + pp;
+        const bool ok = i < hi;
+        const double q = ok ? P.xn64[i] : 0.0;
+        atomicAdd(&qacc[lab[u]], (unsigned long long)__double2ll_rn(q * P.qscale));
+        atomicAdd(&cacc[lab[u]], ok ? 1u : 0u);
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        v[u] = vn[u];
+        lab[u] = labn[u];
+      }
+    }
+  }
+  __syncthreads();
+  long long* out = P.fpart + ((size_t)slice * P.wgps + wg) * (size_t)P.Kp * DS;
+  for (int t = threadIdx.x; t < P.Kp * DS; t += kFxThreads) {
+    const int c = t / DS, d = t - c * DS;
+    out[t] = (long long)acc[(size_t)c * ST + d];
+  }
+  if (slice == 0) {
+    for (int t = threadIdx.x; t < P.Kp; t += kFxThreads) {
+      P.qpart[(size_t)wg * P.Kp + t] = (long long)qacc[t];
+      P.cpart[(size_t)wg * P.Kp + t] = (int)cacc[t];
+    }
+  }
+}
+
+// max |x| (as bits of a non-negative float) and max ||x||^2 (bits of a
+// non-negative double) for the fixed-point scales; bit patterns of
+// non-negative numbers order like the numbers
+__global__ __launch_bounds__(256) void k_absmax(const float* X, long long count,
+                                                const double* xn64, long long n, unsigned* amax,
+                                                unsigned long long* qmax) {
+  unsigned a = 0;
+  unsigned long long q = 0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count;
+       i += (long long)gridDim.x * 256)
+    a = max(a, __float_as_uint(fabsf(X[i])));
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    q = max(q, (unsigned long long)__double_as_longlong(xn64[i]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a = max(a, (unsigned)__shfl_xor((int)a, o, 64));
+    q = max(q, (unsigned long long)__shfl_xor((long long)q, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(amax, a);
+    atomicMax(qmax, q);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic FP64 inertia / shift reductions, refinement counts, the
 // iteration counter and the stop flag; one workgroup of kStatsThreads.
 // ---------------------------------------------------------------------------
@@ -1364,8 +1493,78 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
   const int c = blockIdx.x;
   float* cr = P.C + (size_t)c * P.DP;
   float* nrow = P.Cnext + (size_t)c * P.DP;
-  const int cnt = P.count[c];
-  if (P.tree) {
+  __shared__ int cnt_s;
+  int cnt = P.fixed ? 0 : P.count[c];
+  if (P.fixed) {
+    // exact integer sums of the workgroup partials (any order), then the
+    // means rounded once and the FP64 inertia of the assignment:
+    //   sum ||x - c||^2 = Q - 2 c.S + n ||c||^2
+    long long(*fs_s)[kMaxDP] = reinterpret_cast<long long(*)[kMaxDP]>(part_s);
+    __shared__ long long q_s[kFinishWaves];
+    __shared__ int n_s[kFinishWaves];
+    long long qsum = 0;
+    int nsum = 0;
+    // wave w adds the partials of workgroups w, w+8, ...; four loads in
+    // flight per dimension (integer sums: the order does not matter)
+    for (int j = lane; j < P.D; j += 64) {
+      const int sl = j / P.ds, dj = j - sl * P.ds;
+      const long long* fp = P.fpart + ((size_t)sl * P.wgps * P.Kp + c) * P.ds + dj;
+      const size_t wst = (size_t)P.Kp * P.ds;  // one workgroup further
+      long long acc = 0;
+      int w = wave;
+      for (; w + 3 * kFinishWaves < P.wgps; w += 4 * kFinishWaves) {
+        long long t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[u] = fp[(size_t)(w + u * kFinishWaves) * wst];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += t[u];
+      }
+      for (; w < P.wgps; w += kFinishWaves) acc += fp[(size_t)w * wst];
+      fs_s[wave][j] = acc;
+    }
+    if (lane == 0) {
+      for (int w = wave; w < P.wgps; w += kFinishWaves) {
+        qsum += P.qpart[(size_t)w * P.Kp + c];
+        nsum += P.cpart[(size_t)w * P.Kp + c];
+      }
+    }
+    if (lane == 0) {
+      q_s[wave] = qsum;
+      n_s[wave] = nsum;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      long long qt = 0;
+      int nt = 0;
+#pragma unroll
+      for (int w = 0; w < kFinishWaves; ++w) {
+        qt += q_s[w];
+        nt += n_s[w];
+      }
+      double cs = 0.0, cc = 0.0;
+      for (int j = lane; j < P.D; j += 64) {
+        long long sj = 0;
+#pragma unroll
+        for (int w = 0; w < kFinishWaves; ++w) sj += fs_s[w][j];
+        const double S = (double)sj * P.inv_fscale;
+        const float v = (float)(S / (double)(nt > 1 ? nt : 1));
+        ns[j] = v;
+        nrow[j] = v;
+        const double co = (double)cr[j];
+        cs = fma(co, S, cs);
+        cc = fma(co, co, cc);
+      }
+      cs = wave_sum_f64(cs);
+      cc = wave_sum_f64(cc);
+      if (lane == 0) {
+        cnt_s = nt;
+        P.count_out[c] = nt;
+        P.ipart_out[c] = (double)qt * P.inv_qscale - 2.0 * cs + (double)nt * cc;
+      }
+    }
+    __syncthreads();
+    cnt = cnt_s;
+  } else if (P.tree) {
     long long q0 = 0, q1 = -1;
     if (cnt > 0) {
       q0 = (long long)P.start[c] / kTreeRows;
@@ -1675,6 +1874,15 @@ hipError_t launch_sort(const SortParams& P, hipStream_t s) {
 }
 hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
   const int S = (P.D + 63) / 64;
+  if (mode == UPDATE_FIXED) {
+    const size_t lds = update_fx_lds(P.Kp, P.ds);
+    const void* fn = P.ds == 64   ? reinterpret_cast<const void*>(k_update_fx<64>)
+                     : P.ds == 32 ? reinterpret_cast<const void*>(k_update_fx<32>)
+                                  : reinterpret_cast<const void*>(k_update_fx<16>);
+    if (P.ds != 64 && P.ds != 32 && P.ds != 16) return hipErrorInvalidValue;
+    void* args[] = {const_cast<UpdateParams*>(&P)};
+    return hipLaunchKernel(fn, dim3(P.nslice * P.wgps), dim3(kFxThreads), args, lds, s);
+  }
   if (mode == UPDATE_TREE) {
     const long long waves = (P.n + kTreeRows - 1) / kTreeRows;
     const dim3 g((unsigned)((waves + 3) / 4));
@@ -1689,6 +1897,11 @@ hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
 hipError_t launch_finish(const FinishParams& F, hipStream_t s) {
   hipLaunchKernelGGL(k_finish, dim3(F.K), dim3(64 * kFinishWaves), 0, s, F);
   if (F.stats.state) hipLaunchKernelGGL(k_stats, dim3(1), dim3(kStatsThreads), 0, s, F.stats);
+  return hipGetLastError();
+}
+hipError_t launch_absmax(const float* X, long long count, const double* xn64, long long n,
+                         unsigned* amax, unsigned long long* qmax, hipStream_t s) {
+  hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, X, count, xn64, n, amax, qmax);
   return hipGetLastError();
 }
 hipError_t launch_load_points(const LoadParams& P, hipStream_t s) {

@@ -22,7 +22,7 @@ OK, ERR_ARG, ERR_NO_DEVICE, ERR_OUT_OF_BOUNDS, ERR_INFILE, ERR_HIP, ERR_OOM, \
     ERR_STATE, ERR_UNSUPPORTED = range(9)
 
 PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16": 2}
-UPDATES = {"ordered": 0, "tree": 1}
+UPDATES = {"ordered": 0, "tree": 1, "fixed": 2}
 RUN_CHECK_CONVERGED = 0x1
 RUN_TIME_KERNELS = 0x2
 

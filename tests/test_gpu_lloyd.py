@@ -73,15 +73,17 @@ def test_run_golden_small(kmc_mod, prec):
             assert np.array_equal(e.centroids(), g["%d_C" % i]), i
 
 
-def test_update_tree_matches_fp64_means(kmc_mod, oracle):
-    """Tree mode: FP64 sums rounded once -> within 1 ulp of the FP64 means."""
+@pytest.mark.parametrize("update", ["tree", "fixed"])
+def test_update_tree_matches_fp64_means(kmc_mod, oracle, update):
+    """Tree mode (FP64 sums rounded once) and fixed mode (exact fixed-point
+    sums rounded once): within 1 ulp of the FP64 means; FP64 inertia."""
     g = load_golden("ref_steps.npz")
     for i in range(int(g["ncases"])):
         X, C = g["%d_X" % i], g["%d_C" % i]
         lab = g["%d_labels" % i].astype(np.int32)
         k = C.shape[0]
-        _say("tree update case", i, X.shape, k)
-        with kmc_mod.Engine(X.shape[0], X.shape[1], k, update="tree") as e:
+        _say(update, "update case", i, X.shape, k)
+        with kmc_mod.Engine(X.shape[0], X.shape[1], k, update=update) as e:
             e.set_points(X)
             Cn, cnt, res = e.update(lab, C)
         C64, cnt64 = oracle.f64_centroids(X, lab.astype(np.uint32), k)
@@ -91,11 +93,12 @@ def test_update_tree_matches_fp64_means(kmc_mod, oracle):
         assert abs(res["inertia"] - ref_in) <= 1e-9 * max(ref_in, 1e-30)
 
 
+@pytest.mark.parametrize("update", ["tree", "fixed"])
 @pytest.mark.parametrize("prec", PRECS)
-def test_run_tree_mode_golden(kmc_mod, prec):
+def test_run_tree_mode_golden(kmc_mod, prec, update):
     g = load_golden("ref_c1_blobs.npz")
     with kmc_mod.Engine(g["X"].shape[0], 64, 10, tol=float(g["tol"]), precision=prec,
-                        update="tree") as e:
+                        update=update) as e:
         e.set_points(g["X"])
         e.set_centroids(g["C0"])
         res = e.run(300, check_converged=True)
@@ -186,7 +189,7 @@ def test_every_launch_variant_gives_the_same_labels(kmc_mod, oracle, c2_data, pr
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
-@pytest.mark.parametrize("update", ["ordered", "tree"])
+@pytest.mark.parametrize("update", ["ordered", "tree", "fixed"])
 def test_graph_replays_match_eager(kmc_mod, prec, update):
     """Captured iteration chunks replayed several times, with eager runs and
     synchronisations in between, give the eager result every time."""
@@ -218,3 +221,21 @@ def test_graph_replays_match_eager(kmc_mod, prec, update):
             assert np.array_equal(e.centroids(), C12)
     if update == "ordered":
         assert np.array_equal(lab_e, g["labels"].astype(np.int32))
+
+
+def test_full_size_fixed_update_matches_fp64(kmc_mod, oracle, c2_data):
+    """Fixed-point update at N=1M: counts exact, means within 1 ulp of the
+    FP64 means, FP64 inertia within 1e-9."""
+    X, C0 = c2_data
+    rng = np.random.default_rng(4)
+    lab = rng.integers(0, 256, size=X.shape[0]).astype(np.int32)
+    lab[lab == 17] = 18  # one empty cluster -> zero vector
+    with kmc_mod.Engine(X.shape[0], 128, 256, update="fixed") as e:
+        e.set_points(X)
+        Cn, cnt, res = e.update(lab, C0)
+    C64, cnt64 = oracle.f64_centroids(X, lab.astype(np.uint32), 256)
+    assert np.array_equal(cnt, cnt64)
+    assert np.allclose(Cn, C64, rtol=2.4e-7, atol=1e-30), np.max(np.abs(Cn - C64))
+    assert not Cn[17].any()
+    ref_in = oracle.f64_inertia(X, C0, lab.astype(np.uint32))
+    assert abs(res["inertia"] - ref_in) <= 1e-9 * ref_in
