@@ -688,14 +688,14 @@ __device__ __forceinline__ int region_of(const unsigned* pre_s, int nregions, un
 // candidate's distance in the reference's float arithmetic, first-min rule
 // (the smallest distance, the lowest index among equal ones).
 //
-// A wave takes 64 queue entries (lane e owns entry e). Dimensions go in
-// chunks of 16: four lanes load one entry's 64-byte slice of the point row
-// and of each candidate row (16-byte loads, the next chunk's loads issued
-// before the current chunk is summed), form the squared terms t*t with
-// t = x - c (one rounding each, as in the reference), and leave them in LDS;
-// lane e then adds its entry's terms strictly in dimension order. Padding
-// dimensions are zero in X and C and add exact zeros. An entry with two
-// candidates repeats the second.
+// A wave takes EPW queue entries (lane e < EPW owns entry e). Dimensions go
+// in chunks of CH (32, or 16 when D is padded to 16): CH/4 lanes load one
+// entry's slice of the point row and of each candidate row (16-byte loads,
+// the next chunk's loads issued before the current chunk is summed), form
+// the squared terms t*t with t = x - c (one rounding each, as in the
+// reference), and leave them in LDS; lane e then adds its entry's terms
+// strictly in dimension order. Padding dimensions are zero in X and C and add
+// exact zeros. An entry with two candidates repeats the second.
 constexpr int kR2Stride = 20;  // floats per LDS row of squared terms (b128 reads conflict-free)
 constexpr int kRefinePts = 4;  // points per all-k group
 constexpr size_t kRefinePreBytes = ((size_t)(kMaxRegions + 1) * 4 + 15) / 16 * 16;
@@ -703,28 +703,34 @@ constexpr size_t kRefineLds =
     kRefinePreBytes + (size_t)4 * 3 * 64 * kR2Stride * 4;  // two/three-candidate pass
 static_assert(kRefinePts * kMaxDP * 4 + 48 * kRefinePts <= 4 * 3 * 64 * kR2Stride * 4,
               "all-k pass needs less LDS than the two/three-candidate pass");
+template <int CH>
 __device__ void refine2_body(const RefineParams& P, unsigned char* smem, unsigned bid,
                              unsigned nblk) {
+  constexpr int LPE = CH / 4;     // lanes per entry in the load phase (16 B each)
+  constexpr int EPW = 256 / LPE;  // entries per wave (each lane loads 4 rows)
+  constexpr int RS = CH + 4;      // LDS row stride in floats (b128 reads conflict-free)
+  static_assert(3 * EPW * RS <= 3 * 64 * kR2Stride, "LDS slice of a wave");
   unsigned* const pre_s = reinterpret_cast<unsigned*>(smem);
   float* const sq_all = reinterpret_cast<float*>(smem + kRefinePreBytes);
   const unsigned total = region_prefix(P.q2cnt, P.nregions, pre_s);
   if (total == 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float* const sq = sq_all + wave * (3 * 64 * kR2Stride);
-  const int sub = lane & 3;  // which 16-byte piece of a 64-byte slice
-  const unsigned nbatch = (total + 63) / 64;
+  const int sub = lane % LPE;  // which 16-byte piece of a CH-dimension slice
+  const unsigned nbatch = (total + EPW - 1) / EPW;
   for (unsigned bt = bid * 4u + wave; bt < nbatch; bt += nblk * 4u) {
-    const unsigned t = min(bt * 64u + lane, total - 1);
+    const unsigned t = min(bt * EPW + min(lane, EPW - 1), total - 1);
     const int r = region_of(pre_s, P.nregions, t);
     const int4 q = *reinterpret_cast<const int4*>(P.q2 + 4 * ((size_t)r * P.qstride +
                                                               (t - pre_s[r])));
     const int cand0 = q.y, cand1 = q.z, cand2 = q.w >= 0 ? q.w : q.z;
-    // row pointers of the four entries this lane loads for (e = lane/4 + 16*rr)
+    // row pointers of the four entries this lane loads for
+    // (e = lane/LPE + (64/LPE)*rr)
     const float* xp[4];
     const float* cp[4][3];
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
-      const int e = (lane >> 2) + 16 * rr;
+      const int e = lane / LPE + (64 / LPE) * rr;
       xp[rr] = P.X + (size_t)__shfl(q.x, e, 64) * P.DP + 4 * sub;
       cp[rr][0] = P.C + (size_t)__shfl(cand0, e, 64) * P.DP + 4 * sub;
       cp[rr][1] = P.C + (size_t)__shfl(cand1, e, 64) * P.DP + 4 * sub;
@@ -741,11 +747,11 @@ __device__ void refine2_body(const RefineParams& P, unsigned char* smem, unsigne
     };
     float acc[3] = {0.0f, 0.0f, 0.0f};
     load_chunk(0);
-    for (int j0 = 0; j0 < P.DP; j0 += 16) {
+    for (int j0 = 0; j0 < P.DP; j0 += CH) {
       // squared terms of this chunk into LDS
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int e = (lane >> 2) + 16 * rr;
+        const int e = lane / LPE + (64 / LPE) * rr;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
 #pragma clang fp contract(off)
@@ -755,23 +761,25 @@ __device__ void refine2_body(const RefineParams& P, unsigned char* smem, unsigne
           tt = xv[rr].y - cv[rr][k].y; sv.y = tt * tt;
           tt = xv[rr].z - cv[rr][k].z; sv.z = tt * tt;
           tt = xv[rr].w - cv[rr][k].w; sv.w = tt * tt;
-          *reinterpret_cast<float4*>(sq + (k * 64 + e) * kR2Stride + 4 * sub) = sv;
+          *reinterpret_cast<float4*>(sq + (k * EPW + e) * RS + 4 * sub) = sv;
         }
       }
-      if (j0 + 16 < P.DP) load_chunk(j0 + 16);
+      if (j0 + CH < P.DP) load_chunk(j0 + CH);
       __builtin_amdgcn_wave_barrier();
       // this lane's entry: its terms in dimension order
+      if (lane < EPW) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float* row = sq + (k * 64 + lane) * kR2Stride;
+        for (int k = 0; k < 3; ++k) {
+          const float* row = sq + (k * EPW + lane) * RS;
 #pragma unroll
-        for (int v4 = 0; v4 < 4; ++v4) {
+          for (int v4 = 0; v4 < CH / 4; ++v4) {
 #pragma clang fp contract(off)
-          const float4 sv = *reinterpret_cast<const float4*>(row + 4 * v4);
-          acc[k] = acc[k] + sv.x;
-          acc[k] = acc[k] + sv.y;
-          acc[k] = acc[k] + sv.z;
-          acc[k] = acc[k] + sv.w;
+            const float4 sv = *reinterpret_cast<const float4*>(row + 4 * v4);
+            acc[k] = acc[k] + sv.x;
+            acc[k] = acc[k] + sv.y;
+            acc[k] = acc[k] + sv.z;
+            acc[k] = acc[k] + sv.w;
+          }
         }
       }
       __builtin_amdgcn_wave_barrier();
@@ -780,7 +788,7 @@ __device__ void refine2_body(const RefineParams& P, unsigned char* smem, unsigne
     float rbest = acc[0];
     if (acc[1] < rbest || (acc[1] == rbest && cand1 < best)) { rbest = acc[1]; best = cand1; }
     if (acc[2] < rbest || (acc[2] == rbest && cand2 < best)) { rbest = acc[2]; best = cand2; }
-    if (bt * 64u + lane < total) P.labels[q.x] = best;
+    if (lane < EPW && bt * EPW + lane < total) P.labels[q.x] = best;
   }
 }
 
@@ -788,11 +796,12 @@ __device__ void refine2_body(const RefineParams& P, unsigned char* smem, unsigne
 // centroid in the reference's float arithmetic. A workgroup takes
 // kRefinePts consecutive queue entries and stages their rows in LDS; wave w
 // evaluates centroids w*64 + lane + 256*i from the transposed centroids
-// (coalesced across lanes, the next 16 dimensions prefetched while the
+// (coalesced across lanes, the next RC dimensions prefetched while the
 // current ones are used), reading the point values as LDS broadcasts. Each
 // distance advances in the reference's float order (padding dimensions are
 // zero in X and C and add exact zeros). First-min across lanes, then across
 // the waves.
+template <int RC>
 __device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsigned bid,
                                 unsigned nblk) {
   unsigned* const pre_s = reinterpret_cast<unsigned*>(smem);
@@ -845,19 +854,19 @@ __device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsi
       float acc[kRefinePts];
 #pragma unroll
       for (int q = 0; q < kRefinePts; ++q) acc[q] = 0.0f;
-      float cv[16], cnx[16];
+      float cv[RC], cnx[RC];
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj) cv[jj] = ct[(size_t)jj * P.Kp];
-      for (int j0 = 0; j0 < P.DP; j0 += 16) {
-        if (j0 + 16 < P.DP) {
+      for (int jj = 0; jj < RC; ++jj) cv[jj] = ct[(size_t)jj * P.Kp];
+      for (int j0 = 0; j0 < P.DP; j0 += RC) {
+        if (j0 + RC < P.DP) {
 #pragma unroll
-          for (int jj = 0; jj < 16; ++jj) cnx[jj] = ct[(size_t)(j0 + 16 + jj) * P.Kp];
+          for (int jj = 0; jj < RC; ++jj) cnx[jj] = ct[(size_t)(j0 + RC + jj) * P.Kp];
         }
 #pragma unroll
         for (int q = 0; q < kRefinePts; ++q) {
-          float xv[16];
+          float xv[RC];
 #pragma unroll
-          for (int jj = 0; jj < 16; jj += 4) {
+          for (int jj = 0; jj < RC; jj += 4) {
             const float4 v = *reinterpret_cast<const float4*>(&x_s[q][j0 + jj]);
             xv[jj] = v.x;
             xv[jj + 1] = v.y;
@@ -865,7 +874,7 @@ __device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsi
             xv[jj + 3] = v.w;
           }
 #pragma unroll
-          for (int jj = 0; jj < 16; ++jj) {
+          for (int jj = 0; jj < RC; ++jj) {
 #pragma clang fp contract(off)
             const float t = xv[jj] - cv[jj];
             const float sq = t * t;
@@ -873,7 +882,7 @@ __device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsi
           }
         }
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj) cv[jj] = cnx[jj];
+        for (int jj = 0; jj < RC; ++jj) cv[jj] = cnx[jj];
       }
       const bool ok = c < P.K && P.alive[min(c, P.K - 1)];
 #pragma unroll
@@ -927,12 +936,15 @@ constexpr size_t kRefineAllLds = kRefinePreBytes + kRefinePts * kMaxDP * 4 + 48 
 __global__ __launch_bounds__(256) void k_refine2(RefineParams P) {
   if (*P.done) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
-  refine2_body(P, rsm, blockIdx.x, gridDim.x);
+  if (P.DP % 32 == 0)
+    refine2_body<32>(P, rsm, blockIdx.x, gridDim.x);
+  else
+    refine2_body<16>(P, rsm, blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(256) void k_refine_all(RefineParams P) {
   if (*P.done) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
-  refine_all_body(P, rsm, blockIdx.x, gridDim.x);
+  refine_all_body<16>(P, rsm, blockIdx.x, gridDim.x);  // 32-dimension steps measured slower
 }
 
 // ---------------------------------------------------------------------------
