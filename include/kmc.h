@@ -139,6 +139,16 @@ kmc_status kmc_set_points_device(kmc_handle* h, const float* dX, size_t ld);
  * kmeans.cpp:84-119). */
 kmc_status kmc_set_centroids(kmc_handle* h, const float* C0);
 
+/* k-means++ seeding (D^2 sampling) of the K initial centroids from the
+ * loaded points. Replaces init_centroids_plus_plus (src/kmeans.cu:447-769),
+ * which the reference compiles out and which reads freed buffers when
+ * enabled; this one is new: round 0 takes point floor(u0 * n), round r the
+ * first point (in index order) whose running sum of FP64 squared distances
+ * to the chosen set exceeds u_r times their total, with u_r the r-th
+ * splitmix64(seed) uniform. Deterministic for a seed. chosen: host K point
+ * indices (may be NULL). */
+kmc_status kmc_init_plusplus(kmc_handle* h, uint64_t seed, int64_t* chosen);
+
 /* Initial labels (host, n values in [0,k)); the initial centroids become the
  * label means. Replaces init_centroids_rand (src/kmeans.cu:365-444), which
  * uses labels i % k. */

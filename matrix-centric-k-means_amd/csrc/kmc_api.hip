@@ -771,6 +771,48 @@ kmc_status kmc_set_centroids(kmc_handle* h, const float* C0) {
   return KMC_OK;
 }
 
+// splitmix64 (Steele, Lea, Flood 2014; Vigna's constants): the uniforms of
+// the k-means++ draws, u = top 53 bits / 2^53
+static double splitmix_uniform(uint64_t& state) {
+  uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+kmc_status kmc_init_plusplus(kmc_handle* h, uint64_t seed, int64_t* chosen) {
+  if (!h) return KMC_ERR_ARG;
+  if (!h->have_points) return fail(h, KMC_ERR_STATE, "kmc_init_plusplus: points not set");
+  KMC_CK(hipSetDevice(h->dev));
+  std::vector<double> uni(h->K);
+  uint64_t st = seed;
+  for (int r = 0; r < h->K; ++r) uni[r] = splitmix_uniform(st);
+  const long long n = (long long)h->n;
+  const size_t nb = (size_t)((n + 1023) / 1024);
+  double *du = nullptr, *D2 = nullptr, *bsum = nullptr;
+  long long* dch = nullptr;
+  hipError_t e = dalloc(&du, (size_t)h->K);
+  if (e == hipSuccess) e = dalloc(&D2, (size_t)n);
+  if (e == hipSuccess) e = dalloc(&bsum, nb);
+  if (e == hipSuccess) e = dalloc(&dch, (size_t)h->K);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(du, uni.data(), h->K * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = kmc::launch_plusplus(h->X, n, h->DP, h->C, h->K, du, D2, bsum, dch, h->stream);
+  if (e == hipSuccess && chosen)
+    e = hipMemcpyAsync(chosen, dch, h->K * sizeof(long long), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(du);
+  (void)hipFree(D2);
+  (void)hipFree(bsum);
+  (void)hipFree(dch);
+  KMC_CK(e);
+  h->have_centroids = true;
+  return KMC_OK;
+}
+
 kmc_status kmc_set_labels(kmc_handle* h, const int32_t* labels0) {
   if (!h || !labels0) return fail(h, KMC_ERR_ARG, "kmc_set_labels: bad argument");
   if (!h->have_points) return fail(h, KMC_ERR_STATE, "kmc_set_labels: points not set");

@@ -1388,6 +1388,115 @@ __global__ __launch_bounds__(256) void k_absmax(const float* X, long long count,
 }
 
 // ---------------------------------------------------------------------------
+// k-means++ seeding (D^2 sampling). k_pp_dist folds the newest centroid into
+// every point's FP64 squared distance to the chosen set (sequential over the
+// dimensions) and forms per-block sums in a fixed order: a block of 256
+// threads owns kPPBlock consecutive points, thread t adds points t, t+256,
+// t+512, t+768 in that order, then a pairwise tree over the threads.
+// k_pp_pick (one workgroup) draws the next centroid: with u the round's
+// uniform in [0,1) and T the sum of the block sums in block order, the first
+// point whose running sum (block prefix, then the block's points in order)
+// exceeds u*T. Round 0 takes point floor(u*n).
+// ---------------------------------------------------------------------------
+constexpr int kPPBlock = 1024;
+__global__ __launch_bounds__(256) void k_pp_dist(const float* X, long long n, int DP,
+                                                 const float* crow, int first, double* D2,
+                                                 double* bsum) {
+  __shared__ float c_s[kMaxDP];
+  __shared__ double r_s[256];
+  for (int j = threadIdx.x; j < DP; j += 256) c_s[j] = crow[j];
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kPPBlock;
+  double part = 0.0;
+#pragma unroll
+  for (int u = 0; u < kPPBlock / 256; ++u) {
+    const long long i = base + threadIdx.x + 256 * u;
+    if (i < n) {
+      const float* x = X + (size_t)i * DP;
+      double acc = 0.0;
+      for (int j = 0; j < DP; j += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(x + j);
+        double t;
+        t = (double)v.x - (double)c_s[j];
+        acc = fma(t, t, acc);
+        t = (double)v.y - (double)c_s[j + 1];
+        acc = fma(t, t, acc);
+        t = (double)v.z - (double)c_s[j + 2];
+        acc = fma(t, t, acc);
+        t = (double)v.w - (double)c_s[j + 3];
+        acc = fma(t, t, acc);
+      }
+      const double dv = first ? acc : fmin(D2[i], acc);
+      D2[i] = dv;
+      part += dv;
+    }
+  }
+  r_s[threadIdx.x] = part;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) r_s[threadIdx.x] += r_s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = r_s[0];
+}
+
+__global__ __launch_bounds__(256) void k_pp_pick(const double* bsum, int nb, const double* D2,
+                                                 long long n, const float* X, int DP, float* C,
+                                                 int r, const double* uni, long long* chosen) {
+  __shared__ long long idx_s;
+  if (threadIdx.x == 0) {
+    const double u = uni[r];
+    long long idx = min((long long)(u * (double)n), n - 1);
+    if (r > 0) {
+      double total = 0.0;
+      for (int b = 0; b < nb; ++b) total += bsum[b];
+      if (total > 0.0) {
+        const double target = u * total;
+        double run = 0.0;
+        int b = 0;
+        for (; b < nb - 1; ++b) {
+          if (run + bsum[b] > target) break;
+          run += bsum[b];
+        }
+        const long long lo = (long long)b * kPPBlock, hi = min(n, lo + kPPBlock);
+        long long pick = -1, last_pos = -1;
+        for (long long i = lo; i < hi; ++i) {
+          const double dv = D2[i];
+          if (dv > 0.0) last_pos = i;
+          run += dv;
+          if (run > target && dv > 0.0) {
+            pick = i;
+            break;
+          }
+        }
+        if (pick < 0) pick = last_pos;  // rounding left the target unreached
+        if (pick >= 0) idx = pick;
+      }
+    }
+    chosen[r] = idx;
+    idx_s = idx;
+  }
+  __syncthreads();
+  const long long idx = idx_s;
+  for (int j = threadIdx.x; j < DP; j += 256) C[(size_t)r * DP + j] = X[(size_t)idx * DP + j];
+}
+
+hipError_t launch_plusplus(const float* X, long long n, int DP, float* C, int K,
+                           const double* uni, double* D2, double* bsum, long long* chosen,
+                           hipStream_t s) {
+  const int nb = (int)((n + kPPBlock - 1) / kPPBlock);
+  hipLaunchKernelGGL(k_pp_pick, dim3(1), dim3(256), 0, s, bsum, nb, D2, n, X, DP, C, 0, uni,
+                     chosen);
+  for (int r = 1; r < K; ++r) {
+    hipLaunchKernelGGL(k_pp_dist, dim3(nb), dim3(256), 0, s, X, n, DP,
+                       C + (size_t)(r - 1) * DP, r == 1 ? 1 : 0, D2, bsum);
+    hipLaunchKernelGGL(k_pp_pick, dim3(1), dim3(256), 0, s, bsum, nb, D2, n, X, DP, C, r, uni,
+                       chosen);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic FP64 inertia / shift reductions, refinement counts, the
 // iteration counter and the stop flag; one workgroup of kStatsThreads.
 // ---------------------------------------------------------------------------

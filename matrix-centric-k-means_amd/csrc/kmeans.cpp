@@ -56,8 +56,6 @@ void Kmeans::init(const int* seed) {
   if (kernel != Kernel::linear)
     throw kmc::Error(KMC_ERR_UNSUPPORTED,
                      "Kmeans: kernel k-means runs through kmc_kernel_* (see kmc.h)");
-  if (initMethod == InitMethod::plus_plus)
-    throw kmc::Error(KMC_ERR_UNSUPPORTED, "Kmeans: k-means++ initialisation is not built");
 
   kmc_config cfg{};
   cfg.n = n;
@@ -77,7 +75,18 @@ void Kmeans::init(const int* seed) {
     for (uint32_t j = 0; j < d; ++j) X[i * d + j] = points[i]->get(j);
   check(h, kmc_set_points(h, X.data(), d), "kmc_set_points");
 
-  if (opts.reference_gpu) {
+  if (initMethod == InitMethod::plus_plus) {
+    // k-means++ (D^2) seeding on the device; the reference's
+    // init_centroids_plus_plus (src/kmeans.cu:447-769) is compiled out
+    uint64_t s = 0;
+    if (seed) {
+      s = (uint64_t)(uint32_t)*seed;
+    } else {
+      std::random_device rd;
+      s = ((uint64_t)rd() << 32) ^ rd();
+    }
+    check(h, kmc_init_plusplus(h, s, nullptr), "kmc_init_plusplus");
+  } else if (opts.reference_gpu) {
     // init_centroids_rand (src/kmeans.cu:365-444): labels i % k, the
     // initial centroids are the label means.
     std::vector<int32_t> lab(n);

@@ -120,6 +120,7 @@ def load_library(path=LIB_PATH):
         "kmc_set_points_device": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
         "kmc_set_centroids": (ctypes.c_int, [vp, vp]),
         "kmc_set_labels": (ctypes.c_int, [vp, vp]),
+        "kmc_init_plusplus": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
         "kmc_run": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32, P(_Result)]),
         "kmc_get_labels": (ctypes.c_int, [vp, vp]),
         "kmc_get_centroids": (ctypes.c_int, [vp, vp]),
@@ -228,6 +229,14 @@ class Engine:
     def set_labels(self, labels):
         lab = np.ascontiguousarray(labels, dtype=np.int32)
         self._check(self._lib.kmc_set_labels(self._h, lab.ctypes.data), "set_labels")
+
+    def init_plusplus(self, seed):
+        """k-means++ seeding of the initial centroids; returns the chosen
+        point indices."""
+        out = np.empty(self.k, dtype=np.int64)
+        self._check(self._lib.kmc_init_plusplus(self._h, int(seed) & ((1 << 64) - 1),
+                                                out.ctypes.data), "init_plusplus")
+        return out
 
     def run(self, maxiter, check_converged=True, time_kernels=False, graph=False):
         flags = (RUN_CHECK_CONVERGED if check_converged else 0) | \

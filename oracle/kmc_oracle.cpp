@@ -12,6 +12,10 @@
 //   3. orc_kk_*   : kernel k-means in the Popcorn formulation of the GPU
 //                   reference (src/kmeans.cu, src/kernels/distances.cu,
 //                   src/kernels/kernels.cuh), computed in FP64.
+//   4. orc_plusplus: k-means++ (D^2) seeding with the engine's fixed
+//                   summation order (the reference's own, src/kmeans.cu:
+//                   447-769, is compiled out and broken, so the order is the
+//                   engine's; the mathematics is Arthur & Vassilvitskii 2007).
 //
 // How each function is pinned (checks live in tests/test_oracle.py):
 //   orc_cpu_init_centroids, orc_cpu_distances, orc_cpu_argmin,
@@ -23,6 +27,9 @@
 //                 tiny inputs), the identity inertia = sum ||x||^2 - ..., and
 //                 Lloyd monotonicity of inertia.
 //   orc_kk_kernel_matrix -> numpy FP64 GEMM X X^T and the kernel closed forms.
+//   orc_plusplus -> a plain cumulative-sum D^2 sampler on integer-valued data
+//                 (every summation order is exact there), first-round index
+//                 floor(u*n), distinct picks.
 //   orc_kk_distances     -> the reference unit test's O(n^3) host formula
 //                 (tests/unit_kernels.cu:365-395) restated independently in
 //                 the test, and the linear-kernel identity
@@ -337,6 +344,82 @@ uint64_t orc_kk_lloyd(const double* K, size_t n, uint32_t k, uint64_t maxiter,
     if (!changed) break;
   }
   return it;
+}
+
+
+// k-means++ seeding: K rounds of D^2 sampling with uniforms u[0..K).
+// Per point, the FP64 squared distance to the newest centroid is summed over
+// the dimensions in order (t = x - c in double, s = fma(t, t, s)) and kept as
+// the minimum over the chosen set. Blocks of 1024 points: thread t of 256
+// adds points t, t+256, t+512, t+768 in that order, then a pairwise tree
+// (o = 128, 64, ..., 1: s[t] += s[t + o]); the total is the block sums added
+// in block order. Round r > 0 picks the first block whose running prefix
+// exceeds u_r * total (else the last block), then the first point of it with
+// D^2 > 0 at which the running sum exceeds the target (else the block's last
+// point with D^2 > 0); a zero total falls back to floor(u_r * n).
+void orc_plusplus(const float* X, size_t n, uint32_t d, uint32_t k, const double* u,
+                  int64_t* chosen, float* C) {
+  const size_t nb = (n + 1023) / 1024;
+  std::vector<double> D2(n, 0.0), bsum(nb, 0.0);
+  auto pick_row = [&](uint32_t r, int64_t idx) {
+    chosen[r] = idx;
+    for (uint32_t j = 0; j < d; ++j) C[(size_t)r * d + j] = X[(size_t)idx * d + j];
+  };
+  int64_t first = (int64_t)(u[0] * (double)n);
+  if (first > (int64_t)n - 1) first = (int64_t)n - 1;
+  pick_row(0, first);
+  for (uint32_t r = 1; r < k; ++r) {
+    const float* c = C + (size_t)(r - 1) * d;
+    for (size_t b = 0; b < nb; ++b) {
+      double part[256];
+      for (int t = 0; t < 256; ++t) {
+        double ps = 0.0;
+        for (int q = 0; q < 4; ++q) {
+          const size_t i = b * 1024 + (size_t)t + 256 * (size_t)q;
+          if (i >= n) continue;
+          double acc = 0.0;
+          for (uint32_t j = 0; j < d; ++j) {
+            const double tt = (double)X[i * d + j] - (double)c[j];
+            acc = std::fma(tt, tt, acc);
+          }
+          const double dv = (r == 1) ? acc : std::fmin(D2[i], acc);
+          D2[i] = dv;
+          ps += dv;
+        }
+        part[t] = ps;
+      }
+      for (int o = 128; o > 0; o >>= 1)
+        for (int t = 0; t < o; ++t) part[t] += part[t + o];
+      bsum[b] = part[0];
+    }
+    int64_t idx = (int64_t)(u[r] * (double)n);
+    if (idx > (int64_t)n - 1) idx = (int64_t)n - 1;
+    double total = 0.0;
+    for (size_t b = 0; b < nb; ++b) total += bsum[b];
+    if (total > 0.0) {
+      const double target = u[r] * total;
+      double run = 0.0;
+      size_t b = 0;
+      for (; b + 1 < nb; ++b) {
+        if (run + bsum[b] > target) break;
+        run += bsum[b];
+      }
+      const size_t lo = b * 1024, hi = std::min(n, lo + 1024);
+      int64_t pick = -1, last_pos = -1;
+      for (size_t i = lo; i < hi; ++i) {
+        const double dv = D2[i];
+        if (dv > 0.0) last_pos = (int64_t)i;
+        run += dv;
+        if (run > target && dv > 0.0) {
+          pick = (int64_t)i;
+          break;
+        }
+      }
+      if (pick < 0) pick = last_pos;
+      if (pick >= 0) idx = pick;
+    }
+    pick_row(r, idx);
+  }
 }
 
 }  // extern "C"

@@ -44,6 +44,7 @@ class Oracle:
         L.orc_cpu_lloyd.restype = _u64
         L.orc_cpu_lloyd.argtypes = [_P, _sz, _u32, _u32, ctypes.c_float, _u64, _P, _P, _P, _P]
         L.orc_f64_row_norms.argtypes = [_P, _sz, _u32, _P]
+        L.orc_plusplus.argtypes = [_P, _sz, _u32, _u32, _P, _P, _P]
         L.orc_f64_distances.argtypes = [_P, _sz, _u32, _P, _u32, _P]
         L.orc_f64_inertia.restype = ctypes.c_double
         L.orc_f64_inertia.argtypes = [_P, _sz, _u32, _P, _P]
@@ -111,6 +112,15 @@ class Oracle:
         D = np.empty((X.shape[0], C.shape[0]), np.float64)
         self.L.orc_f64_distances(_ptr(X), X.shape[0], X.shape[1], _ptr(C), C.shape[0], _ptr(D))
         return D
+
+    def plusplus(self, X, k, uniforms):
+        """k-means++ seeding with the given uniforms: (chosen indices, C)."""
+        X = np.ascontiguousarray(X, np.float32)
+        u = np.ascontiguousarray(uniforms, np.float64)
+        chosen = np.empty(k, np.int64)
+        C = np.empty((k, X.shape[1]), np.float32)
+        self.L.orc_plusplus(_ptr(X), X.shape[0], X.shape[1], k, _ptr(u), _ptr(chosen), _ptr(C))
+        return chosen, C
 
     def f64_inertia(self, X, C, labels):
         X = np.ascontiguousarray(X, np.float32)
@@ -224,3 +234,19 @@ class Reference:
         Clast = np.ascontiguousarray(Clast, np.float32)
         return bool(self.L.kref_converged(_ptr(X), X.shape[0], X.shape[1], C.shape[0],
                                           ctypes.c_float(tol), _ptr(C), _ptr(Clast)))
+
+
+def splitmix_uniforms(seed, k):
+    """The k uniforms in [0,1) of the engine's k-means++ draws: splitmix64
+    (Steele, Lea, Flood 2014), top 53 bits / 2^53."""
+    mask = (1 << 64) - 1
+    st = seed & mask
+    out = []
+    for _ in range(k):
+        st = (st + 0x9E3779B97F4A7C15) & mask
+        z = st
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        z ^= z >> 31
+        out.append((z >> 11) * 2.0 ** -53)
+    return np.array(out, dtype=np.float64)

@@ -1,0 +1,53 @@
+"""The gpukmeans command line on the GPU (the drop-in for src/main.cu):
+with --cpu-semantics its labels and convergence iteration are the reference
+CPU k-means' from the same seeded initial rows (random) or from the engine's
+k-means++ seeding (plusplus), checked against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "matrix-centric-k-means_amd", "bin", "gpukmeans")
+
+
+def _run_cli(tmp_path, X, k, init, seed, tol, maxiter):
+    n, d = X.shape
+    inp = tmp_path / "in.csv"
+    out = tmp_path / "out.csv"
+    with open(inp, "w") as f:
+        f.write("k," + ",".join("d%d" % j for j in range(d)) + "\n")
+        for i in range(n):
+            f.write("0," + ",".join("%.9g" % v for v in X[i]) + "\n")
+    cmd = [EXE, "-d", str(d), "-n", str(n), "-k", str(k), "-m", str(maxiter), "-t", repr(tol),
+           "-c", "1", "-b", init, "-f", "linear", "-l", "2", "-s", str(seed), "-r", "2",
+           "-i", str(inp), "-o", str(out), "--label-col", "first", "--cpu-semantics"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+    lab = np.loadtxt(out, delimiter=",", skiprows=1, usecols=0, dtype=np.int64)
+    # the points as the command read them: decimal text -> double -> float
+    Xr = np.loadtxt(inp, delimiter=",", skiprows=1, dtype=np.float64)[:, 1:].astype(np.float32)
+    return lab, Xr, r.stdout
+
+
+@pytest.mark.parametrize("init", ["random", "plusplus"])
+def test_cli_cpu_semantics_matches_reference(tmp_path, oracle, init):
+    from oracle import splitmix_uniforms
+    g = load_golden("ref_c1_blobs.npz")
+    X = g["X"]
+    k, seed, tol, maxiter = 10, 42, 1e-4, 300
+    lab, X, text = _run_cli(tmp_path, X, k, init, seed, tol, maxiter)
+    if init == "random":
+        C0 = oracle.init_centroids(X, k, seed)
+    else:
+        chosen, C0 = oracle.plusplus(X, k, splitmix_uniforms(seed, k))
+    conv, want, C, cnt, ran = oracle.lloyd(X, C0, tol, maxiter)
+    assert np.array_equal(lab, want.astype(np.int64))
+    assert ("K-means converged at iteration %d" % conv) in text, text
+    assert "GPU_Kmeans:" in text and "Score:" in text

@@ -239,3 +239,22 @@ def test_full_size_fixed_update_matches_fp64(kmc_mod, oracle, c2_data):
     assert not Cn[17].any()
     ref_in = oracle.f64_inertia(X, C0, lab.astype(np.uint32))
     assert abs(res["inertia"] - ref_in) <= 1e-9 * ref_in
+
+
+@pytest.mark.parametrize("n,d,k,seed", [(5000, 20, 16, 3), (70000, 64, 40, 11),
+                                        (1797, 64, 10, 2026)])
+def test_plusplus_matches_oracle(kmc_mod, oracle, n, d, k, seed):
+    """k-means++ seeding on the GPU picks exactly the oracle's points (same
+    uniforms, same FP64 summation order)."""
+    from kmc import datagen
+    from oracle import splitmix_uniforms
+    X, _, _ = datagen.blobs(n, d, max(2, k // 2), seed, -8.0, 8.0, 0.5, 2.0)
+    _say("k-means++", n, d, k, seed)
+    with kmc_mod.Engine(n, d, k) as e:
+        e.set_points(X)
+        chosen = e.init_plusplus(seed)
+        C = e.centroids()
+    want, Cw = oracle.plusplus(X, k, splitmix_uniforms(seed, k))
+    assert np.array_equal(chosen, want), (chosen, want)
+    assert np.array_equal(C, X[want])
+    assert len(set(chosen.tolist())) == k

@@ -185,3 +185,51 @@ def test_kernel_kmeans_oracle(oracle, kind):
         mu = np.stack([Xd[labels == c].mean(0) for c in range(k)])
         direct = ((Xd[:, None, :] - mu[None]) ** 2).sum(-1)
         assert np.allclose(D, direct, rtol=1e-9, atol=1e-9)
+
+
+def _plain_plusplus(X, k, u):
+    """Textbook D^2 sampling (Arthur & Vassilvitskii 2007) with a plain
+    cumulative sum: the first point whose running sum of squared distances
+    exceeds u_r times the total."""
+    Xd = X.astype(np.float64)
+    n = X.shape[0]
+    chosen = [min(int(u[0] * n), n - 1)]
+    D2 = None
+    for r in range(1, k):
+        d = ((Xd - Xd[chosen[-1]]) ** 2).sum(1)
+        D2 = d if D2 is None else np.minimum(D2, d)
+        total = D2.sum()
+        cs = np.cumsum(D2)
+        hit = np.nonzero((cs > u[r] * total) & (D2 > 0))[0]
+        chosen.append(int(hit[0]))
+    return np.array(chosen)
+
+
+def test_plusplus_matches_plain_sampler_on_exact_data(oracle):
+    """On integer-valued data every summation order is exact, so the oracle's
+    blocked sums must pick exactly what a plain cumulative sum picks."""
+    from oracle import splitmix_uniforms
+    rng = np.random.default_rng(3)
+    for n, d, k, seed in [(700, 5, 12, 1), (5000, 8, 20, 99), (3100, 3, 7, 2026)]:
+        X = rng.integers(-50, 51, size=(n, d)).astype(np.float32)
+        u = splitmix_uniforms(seed, k)
+        got, C = oracle.plusplus(X, k, u)
+        want = _plain_plusplus(X, k, u)
+        assert np.array_equal(got, want), (n, d, k, got, want)
+        assert np.array_equal(C, X[got])
+        assert len(set(got.tolist())) == k
+
+
+def test_plusplus_first_pick_and_distinct(oracle):
+    from oracle import splitmix_uniforms
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((2000, 16)).astype(np.float32)
+    u = splitmix_uniforms(7, 30)
+    got, _ = oracle.plusplus(X, 30, u)
+    assert got[0] == int(u[0] * 2000)
+    assert len(set(got.tolist())) == 30
+    # splitmix64 reference values (seed 0): first outputs of the published
+    # generator, 0xe220a8397b1dcdaf and 0x6e789e6aa1b965f4
+    v = splitmix_uniforms(0, 2)
+    assert v[0] == (0xE220A8397B1DCDAF >> 11) * 2.0 ** -53
+    assert v[1] == (0x6E789E6AA1B965F4 >> 11) * 2.0 ** -53
