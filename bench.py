@@ -286,9 +286,9 @@ def bench_batch(a):
                         "traffic": None, "algorithmic_bytes_per_launch": ubytes,
                         "note": "algorithmic bytes of one batch iteration (one pass over "
                                 "every problem's X, n*d*4 + n*4) / wall time of an "
-                                "iteration; the packed layout pads every problem to the "
-                                "batch's largest D (128), so the update streams "
-                                "sum(ceil(n/1024)*1024)*128*4 bytes"},
+                                "iteration; problems are packed per padded width "
+                                "(16/32/64/128) and each to a multiple of 1024 points, "
+                                "so the update streams sum(ceil(n/1024)*1024)*DP*4 bytes"},
            "distance_flops_per_iteration": flops,
            "forms": forms,
            "unbatched_sample": {"problems": len(sample),

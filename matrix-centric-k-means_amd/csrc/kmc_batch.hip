@@ -1,7 +1,11 @@
 // kmc_batch.hip -- many independent Lloyd problems in one device layout
 // (kmc_batch_* of include/kmc.h).
 //
-// Layout: the problems' points are packed into one point space, problem p at
+// Problems are grouped by padded width DP (d rounded up to 16, 32, 64, 128 or
+// 256); each group is packed and launched on its own, the groups of one
+// iteration queued one after another on the batch's stream.
+//
+// Layout of a group: its problems' points are packed into one point space, problem p at
 // rows [poff_p, poff_p + n_p) with poff_p a multiple of kBatchAlign; the rows
 // up to the next problem are zero and carry the problem's extra "dead"
 // cluster (local index k_p), so every stage can run over the whole packed
@@ -10,7 +14,7 @@
 // are [coff_p, coff_p + k_p]; the distance operands of its k_p real clusters
 // are rows [cpoff_p, cpoff_p + kpad_p) (padding rows carry kFar).
 //
-// One iteration, one launch per stage for the whole batch:
+// One iteration, one launch per stage for the whole group:
 //   k_bprep      distance operands + transposed centroids of every problem
 //   k_assign<..., BATCH>  persistent distance kernel over the packed
 //                (problem, tile) queue; converged problems are skipped
@@ -271,30 +275,50 @@ __global__ __launch_bounds__(256) void k_binit_labels(int* labels, const BatchPr
   labels[i] = bpr.coff + bpr.k;
 }
 
-// FP64 inertia of each problem's final state, one workgroup per problem,
-// fixed summation order.
-__global__ __launch_bounds__(256) void k_binertia(const float* X, const float* C,
-                                                  const int* labels, const BatchProblem* bp,
-                                                  int DP, double* out) {
-  __shared__ double red_s[256];
-  const BatchProblem bpr = bp[blockIdx.x];
+// FP64 inertia of each problem's final state in a fixed order: every
+// kBatchAlign-point block sums its points (lanes over dimensions, waves over
+// points, a fixed tree over the waves), then one wave per problem adds its
+// block sums.
+constexpr int kBInertiaThreads = 256;
+__global__ __launch_bounds__(kBInertiaThreads) void k_binertia_blocks(
+    const float* X, const float* C, const int* labels, const BatchProblem* bp,
+    const int* blk_prob, int DP, double* bpart) {
+  __shared__ double red_s[kBInertiaThreads / 64];
+  const int b = blockIdx.x;
+  const BatchProblem bpr = bp[blk_prob[b]];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long lo = (long long)b * kBatchAlign;
+  const long long hi = min(lo + (long long)kBatchAlign, bpr.poff + (long long)bpr.n);
   double s = 0.0;
-  for (long long i = threadIdx.x; i < bpr.n; i += 256) {
-    const long long r = bpr.poff + i;
+  for (long long r = lo + wave; r < hi; r += kBInertiaThreads / 64) {
     const float* x = X + (size_t)r * DP;
     const float* c = C + (size_t)labels[r] * DP;
-    for (int j = 0; j < DP; ++j) {
+    for (int j = lane; j < DP; j += 64) {
       const double t = (double)x[j] - (double)c[j];
       s = fma(t, t, s);
     }
   }
-  red_s[threadIdx.x] = s;
+  s = wave_sum_f64(s);
+  if (lane == 0) red_s[wave] = s;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red_s[threadIdx.x] += red_s[threadIdx.x + o];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBInertiaThreads / 64; ++w) t += red_s[w];
+    bpart[b] = t;
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = red_s[0];
+}
+
+__global__ __launch_bounds__(256) void k_binertia_sum(const double* bpart, const BatchProblem* bp,
+                                                      int nprob, double* out) {
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= nprob) return;
+  const BatchProblem bpr = bp[p];
+  const double* a = bpart + bpr.poff / kBatchAlign;
+  double s = 0.0;
+  for (int i = lane; i < bpr.nblk; i += 64) s += a[i];
+  s = wave_sum_f64(s);
+  if (lane == 0) out[p] = s;
 }
 
 }  // namespace kmc
@@ -302,12 +326,12 @@ __global__ __launch_bounds__(256) void k_binertia(const float* X, const float* C
 using kmc::BatchProblem;
 using kmc::DevState;
 
-struct kmc_batch {
-  kmc_batch_config cfg{};
-  int dev = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  uint32_t nprob = 0;
+// One width class of a batch: the problems whose d rounds up to the same
+// padded width DP share one packed point space and one launch per stage, so
+// a problem of d=16 streams 64-byte rows, not the widest problem's.
+struct BatchGroup {
+  uint32_t nprob = 0;           // problems in this group
+  std::vector<uint32_t> ids;    // their indices in the batch
   int DP = 0, prec = 0, idx_bits = 0, lbits = 0, cus = 0;
   float alpha = 0, beta = 0, gref = 0, xpk = 0;
   long long npad = 0;
@@ -324,12 +348,23 @@ struct kmc_batch {
   unsigned *q2cnt = nullptr, *qacnt = nullptr;
   int *bcount = nullptr, *count = nullptr, *start = nullptr, *members = nullptr;
   double *tpart = nullptr, *ipart = nullptr, *shift2 = nullptr, *inertia = nullptr;
+  double* bpart = nullptr;
   int *conv = nullptr, *alive = nullptr;
   DevState* state = nullptr;
   int* pdone = nullptr;
   unsigned *piter = nullptr, *pconv = nullptr;
   BatchProblem* bp = nullptr;
   int *item_prob = nullptr, *blk_prob = nullptr, *clus_prob = nullptr, *row_prob = nullptr;
+};
+
+struct kmc_batch {
+  kmc_batch_config cfg{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  uint32_t nprob = 0;
+  std::vector<BatchGroup*> groups;
+  std::vector<int> grp_of, loc_of;  // group of each problem, index inside it
 };
 
 namespace {
@@ -362,15 +397,23 @@ hipError_t upload(T** p, const std::vector<T>& v, hipStream_t s) {
   return e;
 }
 
-void bfree_all(kmc_batch* b) {
-  void* ptrs[] = {b->X,     b->xn2,   b->C,       b->C0,      b->Cnext,   b->Cw,     b->cn,
-                  b->CT,    b->Xhi,   b->Xlo,     b->Chi,     b->Clo,     b->labels, b->q2,
-                  b->qa,    b->q2cnt, b->qacnt,   b->bcount,  b->count,   b->start,  b->members,
-                  b->tpart, b->ipart, b->shift2,  b->inertia, b->conv,    b->alive,  b->state,
-                  b->pdone, b->piter, b->pconv,   b->bp,      b->item_prob, b->blk_prob,
-                  b->clus_prob, b->row_prob};
+void gfree_all(BatchGroup* g) {
+  void* ptrs[] = {g->X,     g->xn2,   g->C,       g->C0,      g->Cnext,   g->Cw,     g->cn,
+                  g->CT,    g->Xhi,   g->Xlo,     g->Chi,     g->Clo,     g->labels, g->q2,
+                  g->qa,    g->q2cnt, g->qacnt,   g->bcount,  g->count,   g->start,  g->members,
+                  g->tpart, g->ipart, g->shift2,  g->inertia, g->conv,    g->alive,  g->state,
+                  g->pdone, g->piter, g->pconv,   g->bp,      g->item_prob, g->blk_prob,
+                  g->clus_prob, g->row_prob, g->bpart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+}
+
+void bfree_all(kmc_batch* b) {
+  for (BatchGroup* g : b->groups) {
+    gfree_all(g);
+    delete g;
+  }
+  b->groups.clear();
 }
 
 int dp_of(uint32_t d) {
@@ -380,146 +423,303 @@ int dp_of(uint32_t d) {
   return 0;
 }
 
-kmc_status enqueue_iteration(kmc_batch* b, uint64_t maxiter, int check) {
-  const int* done = &b->state->done;
+kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int check) {
+  const int* done = &g->state->done;
   kmc::BPrepParams pp{};
-  pp.C = b->C;
-  pp.Cw = b->Cw;
-  pp.Chi = b->Chi;
-  pp.Clo = b->Clo;
-  pp.cn = b->cn;
-  pp.CT = b->CT;
-  pp.bp = b->bp;
-  pp.row_prob = b->row_prob;
+  pp.C = g->C;
+  pp.Cw = g->Cw;
+  pp.Chi = g->Chi;
+  pp.Clo = g->Clo;
+  pp.cn = g->cn;
+  pp.CT = g->CT;
+  pp.bp = g->bp;
+  pp.row_prob = g->row_prob;
   pp.done = done;
-  pp.nrows = b->nops;
-  pp.KTp = b->KTp;
-  pp.DP = b->DP;
-  pp.prec = b->prec;
-  hipLaunchKernelGGL(kmc::k_bprep, dim3((b->nops + 3) / 4), dim3(256), 0, b->stream, pp);
+  pp.nrows = g->nops;
+  pp.KTp = g->KTp;
+  pp.DP = g->DP;
+  pp.prec = g->prec;
+  hipLaunchKernelGGL(kmc::k_bprep, dim3((g->nops + 3) / 4), dim3(256), 0, b->stream, pp);
   KMC_BCK(hipGetLastError());
 
   kmc::AssignParams ap{};
-  ap.X = b->X;
-  ap.Xhi = b->Xhi;
-  ap.Xlo = b->Xlo;
-  ap.xn2 = b->xn2;
-  ap.n = b->npad;
-  ap.Cw = b->Cw;
-  ap.Chi = b->Chi;
-  ap.Clo = b->Clo;
-  ap.cn = b->cn;
-  ap.K = b->KT;
-  ap.Kp = b->nops;
-  ap.kchunk = b->nops;
-  ap.idx_mask = (1u << b->idx_bits) - 1u;
-  ap.alpha = b->alpha;
-  ap.beta = b->beta;
-  ap.gref = b->gref;
-  ap.xpk = b->xpk;
-  ap.labels = b->labels;
-  ap.q2 = b->q2;
-  ap.qa = b->qa;
-  ap.q2cnt = b->q2cnt;
-  ap.qacnt = b->qacnt;
-  ap.qstride = b->qstride;
+  ap.X = g->X;
+  ap.Xhi = g->Xhi;
+  ap.Xlo = g->Xlo;
+  ap.xn2 = g->xn2;
+  ap.n = g->npad;
+  ap.Cw = g->Cw;
+  ap.Chi = g->Chi;
+  ap.Clo = g->Clo;
+  ap.cn = g->cn;
+  ap.K = g->KT;
+  ap.Kp = g->nops;
+  ap.kchunk = g->nops;
+  ap.idx_mask = (1u << g->idx_bits) - 1u;
+  ap.alpha = g->alpha;
+  ap.beta = g->beta;
+  ap.gref = g->gref;
+  ap.xpk = g->xpk;
+  ap.labels = g->labels;
+  ap.q2 = g->q2;
+  ap.qa = g->qa;
+  ap.q2cnt = g->q2cnt;
+  ap.qacnt = g->qacnt;
+  ap.qstride = g->qstride;
   ap.done = done;
-  ap.bp = b->bp;
-  ap.item_prob = b->item_prob;
-  ap.pdone = b->pdone;
-  ap.nitems = b->nitems;
-  KMC_BCK(kmc::launch_assign_batch(b->DP, b->prec, ap, b->grid, b->lds, b->stream));
+  ap.bp = g->bp;
+  ap.item_prob = g->item_prob;
+  ap.pdone = g->pdone;
+  ap.nitems = g->nitems;
+  KMC_BCK(kmc::launch_assign_batch(g->DP, g->prec, ap, g->grid, g->lds, b->stream));
 
   kmc::RefineParams rp{};
-  rp.X = b->X;
-  rp.C = b->C;
-  rp.CT = b->CT;
-  rp.q2 = b->q2;
-  rp.qa = b->qa;
-  rp.q2cnt = b->q2cnt;
-  rp.qacnt = b->qacnt;
-  rp.qstride = b->qstride;
-  rp.nregions = b->grid;
-  rp.alive = b->alive;
-  rp.labels = b->labels;
+  rp.X = g->X;
+  rp.C = g->C;
+  rp.CT = g->CT;
+  rp.q2 = g->q2;
+  rp.qa = g->qa;
+  rp.q2cnt = g->q2cnt;
+  rp.qacnt = g->qacnt;
+  rp.qstride = g->qstride;
+  rp.nregions = g->grid;
+  rp.alive = g->alive;
+  rp.labels = g->labels;
   rp.done = done;
-  rp.K = b->KT;
-  rp.Kp = b->KTp;
-  rp.D = b->DP;
-  rp.DP = b->DP;
-  rp.bp = b->bp;
-  rp.blk_prob = b->blk_prob;
+  rp.K = g->KT;
+  rp.Kp = g->KTp;
+  rp.D = g->DP;
+  rp.DP = g->DP;
+  rp.bp = g->bp;
+  rp.blk_prob = g->blk_prob;
   KMC_BCK(kmc::launch_refine(rp, b->stream));
 
   kmc::BSortParams sp{};
-  sp.labels = b->labels;
-  sp.bcount = b->bcount;
-  sp.count = b->count;
-  sp.start = b->start;
-  sp.members = b->members;
-  sp.bp = b->bp;
-  sp.blk_prob = b->blk_prob;
-  sp.clus_prob = b->clus_prob;
+  sp.labels = g->labels;
+  sp.bcount = g->bcount;
+  sp.count = g->count;
+  sp.start = g->start;
+  sp.members = g->members;
+  sp.bp = g->bp;
+  sp.blk_prob = g->blk_prob;
+  sp.clus_prob = g->clus_prob;
   sp.done = done;
-  sp.nblk = b->nblk;
-  sp.KT = b->KT;
-  sp.lbits = b->lbits;
-  const size_t hl = (size_t)(1 << b->lbits) * sizeof(int);
-  hipLaunchKernelGGL(kmc::k_bhist, dim3(b->nblk), dim3(64), hl, b->stream, sp);
-  hipLaunchKernelGGL(kmc::k_bscan_blocks, dim3((b->KT + 3) / 4), dim3(256), 0, b->stream, sp);
-  hipLaunchKernelGGL(kmc::k_bscatter, dim3(b->nblk), dim3(64), hl, b->stream, sp);
+  sp.nblk = g->nblk;
+  sp.KT = g->KT;
+  sp.lbits = g->lbits;
+  const size_t hl = (size_t)(1 << g->lbits) * sizeof(int);
+  hipLaunchKernelGGL(kmc::k_bhist, dim3(g->nblk), dim3(64), hl, b->stream, sp);
+  hipLaunchKernelGGL(kmc::k_bscan_blocks, dim3((g->KT + 3) / 4), dim3(256), 0, b->stream, sp);
+  hipLaunchKernelGGL(kmc::k_bscatter, dim3(g->nblk), dim3(64), hl, b->stream, sp);
   KMC_BCK(hipGetLastError());
 
   kmc::UpdateParams up{};
-  up.X = b->X;
-  up.C = b->C;
-  up.Cnext = b->Cnext;
-  up.labels = b->labels;
-  up.members = b->members;
-  up.start = b->start;
-  up.count = b->count;
-  up.ipart = b->ipart;
-  up.tpart = b->tpart;
+  up.X = g->X;
+  up.C = g->C;
+  up.Cnext = g->Cnext;
+  up.labels = g->labels;
+  up.members = g->members;
+  up.start = g->start;
+  up.count = g->count;
+  up.ipart = g->ipart;
+  up.tpart = g->tpart;
   up.done = done;
-  up.n = b->npad;
-  up.K = b->KT;
-  up.D = b->DP;
-  up.DP = b->DP;
+  up.n = g->npad;
+  up.K = g->KT;
+  up.D = g->DP;
+  up.DP = g->DP;
   KMC_BCK(kmc::launch_update(up, kmc::UPDATE_TREE, b->stream));
 
   kmc::FinishParams fp{};
-  fp.C = b->C;
-  fp.Cnext = b->Cnext;
-  fp.tpart = b->tpart;
-  fp.start = b->start;
+  fp.C = g->C;
+  fp.Cnext = g->Cnext;
+  fp.tpart = g->tpart;
+  fp.start = g->start;
   fp.tree = 1;
-  fp.conv = b->conv;
-  fp.shift2 = b->shift2;
-  fp.count = b->count;
-  fp.alive = b->alive;
+  fp.conv = g->conv;
+  fp.shift2 = g->shift2;
+  fp.count = g->count;
+  fp.alive = g->alive;
   fp.done = done;
   fp.tol = b->cfg.tol;
-  fp.K = b->KT;
-  fp.D = b->DP;
-  fp.DP = b->DP;
+  fp.K = g->KT;
+  fp.D = g->DP;
+  fp.DP = g->DP;
   fp.exclude_empty = 0;
   fp.do_prep = 0;
   fp.stats.state = nullptr;  // the batch's own stop test follows
   KMC_BCK(kmc::launch_finish(fp, b->stream));
 
   kmc::BStatsParams st{};
-  st.conv = b->conv;
-  st.bp = b->bp;
-  st.pdone = b->pdone;
-  st.piter = b->piter;
-  st.pconv = b->pconv;
-  st.state = b->state;
+  st.conv = g->conv;
+  st.bp = g->bp;
+  st.pdone = g->pdone;
+  st.piter = g->piter;
+  st.pconv = g->pconv;
+  st.state = g->state;
   st.done = done;
-  st.nprob = (int)b->nprob;
+  st.nprob = (int)g->nprob;
   st.check = check;
   st.maxiter = maxiter;
   hipLaunchKernelGGL(kmc::k_bstats, dim3(1), dim3(kmc::kBStatsThreads), 0, b->stream, st);
   KMC_BCK(hipGetLastError());
+  return KMC_OK;
+}
+
+
+// Packs one width class: tables, launch shape, buffers, points and initial
+// centroids, norms and bf16 planes.
+kmc_status create_group(kmc_batch* b, BatchGroup* g, const std::vector<kmc_problem>& probs) {
+  const uint32_t count = (uint32_t)probs.size();
+  uint32_t dmax = 0, kmax = 0;
+  for (const kmc_problem& q : probs) {
+    dmax = std::max(dmax, q.d);
+    kmax = std::max(kmax, q.k);
+  }
+  g->nprob = count;
+  g->prec = b->cfg.precision;
+  g->DP = dp_of(dmax);
+  hipDeviceProp_t prop;
+  KMC_BCK(hipGetDeviceProperties(&prop, b->dev));
+  g->cus = prop.multiProcessorCount;
+  // operand rows of the largest problem must fit the LDS with one copy
+  const int kpad_max = (int)((kmax + 63) / 64 * 64);
+  g->lds = kmc::assign_lds(g->DP, g->prec, kpad_max);
+  if (g->lds > 160 * 1024)
+    return bfail(b, KMC_ERR_UNSUPPORTED, "kmc_batch_create: centroid operands of the "
+                                         "largest problem exceed the LDS");
+  g->idx_bits = 0;
+  while ((1 << g->idx_bits) < kpad_max) ++g->idx_bits;
+  g->lbits = 0;
+  while ((1 << g->lbits) < (int)kmax + 1) ++g->lbits;
+  kmc::band_constants(g->prec, (int)dmax, g->idx_bits, &g->alpha, &g->beta, &g->gref, &g->xpk);
+
+  // tables
+  g->hp.resize(count);
+  g->d.resize(count);
+  long long poff = 0, bcoff = 0;
+  int coff = 0, cpoff = 0, tile0 = 0;
+  for (uint32_t p = 0; p < count; ++p) {
+    const kmc_problem& q = probs[p];
+    BatchProblem& e = g->hp[p];
+    const long long npad = ((long long)q.n + kmc::kBatchAlign - 1) / kmc::kBatchAlign *
+                           kmc::kBatchAlign;
+    e.poff = poff;
+    e.n = (int)q.n;
+    e.k = (int)q.k;
+    e.kpad = (int)((q.k + 63) / 64 * 64);
+    e.coff = coff;
+    e.cpoff = cpoff;
+    e.tile0 = tile0;
+    e.nblk = (int)(npad / kmc::kBatchAlign);
+    e.bcoff = bcoff;
+    g->d[p] = q.d;
+    poff += npad;
+    coff += (int)q.k + 1;
+    cpoff += e.kpad;
+    tile0 += (int)((q.n + kmc::kBatchTile - 1) / kmc::kBatchTile);
+    bcoff += (long long)(q.k + 1) * e.nblk;
+  }
+  g->npad = poff;
+  g->KT = coff;
+  g->KTp = (coff + 63) / 64 * 64;
+  g->nops = cpoff;
+  g->nitems = tile0;
+  g->nblk = (int)(poff / kmc::kBatchAlign);
+  g->nbcount = bcoff;
+  std::vector<int> item_prob(g->nitems), blk_prob(g->nblk), clus_prob(g->KT), row_prob(g->nops);
+  for (uint32_t p = 0; p < count; ++p) {
+    const BatchProblem& e = g->hp[p];
+    const int nt = (e.n + kmc::kBatchTile - 1) / kmc::kBatchTile;
+    for (int t = 0; t < nt; ++t) item_prob[e.tile0 + t] = (int)p;
+    for (int k = 0; k < e.nblk; ++k) blk_prob[e.poff / kmc::kBatchAlign + k] = (int)p;
+    for (int c = 0; c <= e.k; ++c) clus_prob[e.coff + c] = (int)p;
+    for (int r = 0; r < e.kpad; ++r) row_prob[e.cpoff + r] = (int)p;
+  }
+
+  // launch shape of the distance kernel
+  const void* fn = kmc::assign_batch_kernel_ptr(g->DP, g->prec);
+  if (!fn) return bfail(b, KMC_ERR_UNSUPPORTED, "kmc_batch_create: no kernel for this shape");
+  KMC_BCK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g->lds));
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kmc::kBatchAssignThreads,
+                                                   g->lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  g->grid = (int)std::min<long long>(
+      {(long long)g->nitems, (long long)per_cu * g->cus, (long long)kmc::kMaxRegions});
+  g->qstride = ((long long)g->nitems + g->grid - 1) / g->grid * kmc::kBatchTile;
+
+  // buffers
+  const size_t DP = (size_t)g->DP, np = (size_t)g->npad;
+  KMC_BCK(balloc(&g->X, np * DP));
+  if (g->prec != kmc::PREC_FP32) KMC_BCK(balloc(&g->Xhi, np * DP));
+  if (g->prec == kmc::PREC_BF16X3) KMC_BCK(balloc(&g->Xlo, np * DP));
+  KMC_BCK(balloc(&g->xn2, np));
+  KMC_BCK(balloc(&g->C, (size_t)g->KT * DP));
+  KMC_BCK(balloc(&g->C0, (size_t)g->KT * DP));
+  KMC_BCK(balloc(&g->Cnext, (size_t)g->KT * DP));
+  if (g->prec == kmc::PREC_FP32) KMC_BCK(balloc(&g->Cw, (size_t)g->nops * DP));
+  else KMC_BCK(balloc(&g->Chi, (size_t)g->nops * DP));
+  if (g->prec == kmc::PREC_BF16X3) KMC_BCK(balloc(&g->Clo, (size_t)g->nops * DP));
+  KMC_BCK(balloc(&g->cn, (size_t)g->nops));
+  KMC_BCK(balloc(&g->CT, (size_t)g->KTp * DP));
+  KMC_BCK(balloc(&g->labels, np));
+  KMC_BCK(balloc(&g->q2, 4 * (size_t)g->grid * (size_t)g->qstride));
+  KMC_BCK(balloc(&g->qa, (size_t)g->grid * (size_t)g->qstride));
+  KMC_BCK(balloc(&g->q2cnt, (size_t)g->grid));
+  KMC_BCK(balloc(&g->qacnt, (size_t)g->grid));
+  KMC_BCK(balloc(&g->bcount, (size_t)g->nbcount));
+  KMC_BCK(balloc(&g->count, (size_t)g->KT));
+  KMC_BCK(balloc(&g->start, (size_t)g->KT));
+  KMC_BCK(balloc(&g->members, np));
+  KMC_BCK(balloc(&g->tpart, (np / kmc::kTreeRows + (size_t)g->KT) * DP));
+  KMC_BCK(balloc(&g->ipart, np / kmc::kTreeRows + 1));
+  KMC_BCK(balloc(&g->shift2, (size_t)g->KT));
+  KMC_BCK(balloc(&g->conv, (size_t)g->KT));
+  KMC_BCK(balloc(&g->alive, (size_t)g->KT));
+  KMC_BCK(balloc(&g->inertia, (size_t)count));
+  KMC_BCK(balloc(&g->bpart, (size_t)g->nblk));
+  KMC_BCK(balloc(&g->state, 1));
+  KMC_BCK(balloc(&g->pdone, (size_t)count));
+  KMC_BCK(balloc(&g->piter, (size_t)count));
+  KMC_BCK(balloc(&g->pconv, (size_t)count));
+  KMC_BCK(upload(&g->bp, g->hp, b->stream));
+  KMC_BCK(upload(&g->item_prob, item_prob, b->stream));
+  KMC_BCK(upload(&g->blk_prob, blk_prob, b->stream));
+  KMC_BCK(upload(&g->clus_prob, clus_prob, b->stream));
+  KMC_BCK(upload(&g->row_prob, row_prob, b->stream));
+  KMC_BCK(kmc::launch_fill_i32(g->alive, (long long)g->KT, 1, b->stream));
+  KMC_BCK(hipMemsetAsync(g->X, 0, np * DP * sizeof(float), b->stream));
+  KMC_BCK(hipMemsetAsync(g->C0, 0, (size_t)g->KT * DP * sizeof(float), b->stream));
+  KMC_BCK(hipMemsetAsync(g->CT, 0, (size_t)g->KTp * DP * sizeof(float), b->stream));
+  KMC_BCK(hipMemsetAsync(g->q2cnt, 0, (size_t)g->grid * sizeof(unsigned), b->stream));
+  KMC_BCK(hipMemsetAsync(g->qacnt, 0, (size_t)g->grid * sizeof(unsigned), b->stream));
+  // points and initial centroids of every problem
+  for (uint32_t p = 0; p < count; ++p) {
+    const kmc_problem& q = probs[p];
+    const BatchProblem& e = g->hp[p];
+    KMC_BCK(hipMemcpy2DAsync(g->X + (size_t)e.poff * DP, DP * sizeof(float), q.X,
+                             q.ld * sizeof(float), q.d * sizeof(float), q.n,
+                             hipMemcpyHostToDevice, b->stream));
+    KMC_BCK(hipMemcpy2DAsync(g->C0 + (size_t)e.coff * DP, DP * sizeof(float), q.C0,
+                             q.d * sizeof(float), q.d * sizeof(float), q.k,
+                             hipMemcpyHostToDevice, b->stream));
+  }
+  // norms and bf16 planes, in place over the packed rows
+  kmc::LoadParams lp{};
+  lp.src = g->X;
+  lp.ld = DP;
+  lp.n = g->npad;
+  lp.D = g->DP;
+  lp.DP = g->DP;
+  lp.X = g->X;
+  lp.Xhi = g->Xhi;
+  lp.Xlo = g->Xlo;
+  lp.xn2 = g->xn2;
+  lp.xn2_64 = nullptr;
+  KMC_BCK(kmc::launch_load_points(lp, b->stream));
+  KMC_BCK(hipStreamSynchronize(b->stream));
   return KMC_OK;
 }
 
@@ -532,15 +732,12 @@ kmc_status kmc_batch_create(const kmc_problem* probs, uint32_t count,
   if (!probs || !cfg || !out || count == 0) return KMC_ERR_ARG;
   *out = nullptr;
   if (cfg->precision < kmc::PREC_FP32 || cfg->precision > kmc::PREC_BF16) return KMC_ERR_ARG;
-  uint32_t dmax = 0, kmax = 0;
   for (uint32_t p = 0; p < count; ++p) {
     const kmc_problem& q = probs[p];
     if (q.n == 0 || q.d == 0 || q.k == 0 || q.k > q.n || !q.X || !q.C0 || q.ld < q.d ||
         q.n > (uint64_t)0x7fffffff)
       return KMC_ERR_ARG;
     if (q.d > 256 || q.k > 256) return KMC_ERR_UNSUPPORTED;
-    dmax = std::max(dmax, q.d);
-    kmax = std::max(kmax, q.k);
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return KMC_ERR_NO_DEVICE;
@@ -551,157 +748,36 @@ kmc_status kmc_batch_create(const kmc_problem* probs, uint32_t count,
   b->cfg = *cfg;
   b->dev = cfg->device;
   b->nprob = count;
-  b->prec = cfg->precision;
-  b->DP = dp_of(dmax);
-  kmc_status st = KMC_OK;
+  b->grp_of.assign(count, -1);
+  b->loc_of.assign(count, -1);
   auto body = [&]() -> kmc_status {
     KMC_BCK(hipSetDevice(b->dev));
-    hipDeviceProp_t prop;
-    KMC_BCK(hipGetDeviceProperties(&prop, b->dev));
-    b->cus = prop.multiProcessorCount;
     KMC_BCK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    // operand rows of the largest problem must fit the LDS with one copy
-    const int kpad_max = (int)((kmax + 63) / 64 * 64);
-    b->lds = kmc::assign_lds(b->DP, b->prec, kpad_max);
-    if (b->lds > 160 * 1024)
-      return bfail(b, KMC_ERR_UNSUPPORTED, "kmc_batch_create: centroid operands of the "
-                                           "largest problem exceed the LDS");
-    b->idx_bits = 0;
-    while ((1 << b->idx_bits) < kpad_max) ++b->idx_bits;
-    b->lbits = 0;
-    while ((1 << b->lbits) < (int)kmax + 1) ++b->lbits;
-    kmc::band_constants(b->prec, (int)dmax, b->idx_bits, &b->alpha, &b->beta, &b->gref,
-                        &b->xpk);
-
-    // tables
-    b->hp.resize(count);
-    b->d.resize(count);
-    long long poff = 0, bcoff = 0;
-    int coff = 0, cpoff = 0, tile0 = 0;
-    for (uint32_t p = 0; p < count; ++p) {
-      const kmc_problem& q = probs[p];
-      BatchProblem& e = b->hp[p];
-      const long long npad = ((long long)q.n + kmc::kBatchAlign - 1) / kmc::kBatchAlign *
-                             kmc::kBatchAlign;
-      e.poff = poff;
-      e.n = (int)q.n;
-      e.k = (int)q.k;
-      e.kpad = (int)((q.k + 63) / 64 * 64);
-      e.coff = coff;
-      e.cpoff = cpoff;
-      e.tile0 = tile0;
-      e.nblk = (int)(npad / kmc::kBatchAlign);
-      e.bcoff = bcoff;
-      b->d[p] = q.d;
-      poff += npad;
-      coff += (int)q.k + 1;
-      cpoff += e.kpad;
-      tile0 += (int)((q.n + kmc::kBatchTile - 1) / kmc::kBatchTile);
-      bcoff += (long long)(q.k + 1) * e.nblk;
+    // one group per padded width, problems in batch order inside a group
+    const uint32_t classes[] = {16, 32, 64, 128, 256};
+    for (uint32_t dp : classes) {
+      std::vector<kmc_problem> sub;
+      std::vector<uint32_t> ids;
+      for (uint32_t p = 0; p < count; ++p)
+        if (dp_of(probs[p].d) == (int)dp) {
+          sub.push_back(probs[p]);
+          ids.push_back(p);
+        }
+      if (sub.empty()) continue;
+      BatchGroup* g = new (std::nothrow) BatchGroup();
+      if (!g) return bfail(b, KMC_ERR_OOM, "kmc_batch_create: out of host memory");
+      b->groups.push_back(g);
+      g->ids = ids;
+      for (size_t i = 0; i < ids.size(); ++i) {
+        b->grp_of[ids[i]] = (int)b->groups.size() - 1;
+        b->loc_of[ids[i]] = (int)i;
+      }
+      const kmc_status s = create_group(b, g, sub);
+      if (s != KMC_OK) return s;
     }
-    b->npad = poff;
-    b->KT = coff;
-    b->KTp = (coff + 63) / 64 * 64;
-    b->nops = cpoff;
-    b->nitems = tile0;
-    b->nblk = (int)(poff / kmc::kBatchAlign);
-    b->nbcount = bcoff;
-    std::vector<int> item_prob(b->nitems), blk_prob(b->nblk), clus_prob(b->KT),
-        row_prob(b->nops);
-    for (uint32_t p = 0; p < count; ++p) {
-      const BatchProblem& e = b->hp[p];
-      const int nt = (e.n + kmc::kBatchTile - 1) / kmc::kBatchTile;
-      for (int t = 0; t < nt; ++t) item_prob[e.tile0 + t] = (int)p;
-      for (int k = 0; k < e.nblk; ++k) blk_prob[e.poff / kmc::kBatchAlign + k] = (int)p;
-      for (int c = 0; c <= e.k; ++c) clus_prob[e.coff + c] = (int)p;
-      for (int r = 0; r < e.kpad; ++r) row_prob[e.cpoff + r] = (int)p;
-    }
-
-    // launch shape of the distance kernel
-    const void* fn = kmc::assign_batch_kernel_ptr(b->DP, b->prec);
-    if (!fn) return bfail(b, KMC_ERR_UNSUPPORTED, "kmc_batch_create: no kernel for this shape");
-    KMC_BCK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds));
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kmc::kBatchAssignThreads,
-                                                     b->lds) != hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    b->grid = (int)std::min<long long>(
-        {(long long)b->nitems, (long long)per_cu * b->cus, (long long)kmc::kMaxRegions});
-    b->qstride = ((long long)b->nitems + b->grid - 1) / b->grid * kmc::kBatchTile;
-
-    // buffers
-    const size_t DP = (size_t)b->DP, np = (size_t)b->npad;
-    KMC_BCK(balloc(&b->X, np * DP));
-    if (b->prec != kmc::PREC_FP32) KMC_BCK(balloc(&b->Xhi, np * DP));
-    if (b->prec == kmc::PREC_BF16X3) KMC_BCK(balloc(&b->Xlo, np * DP));
-    KMC_BCK(balloc(&b->xn2, np));
-    KMC_BCK(balloc(&b->C, (size_t)b->KT * DP));
-    KMC_BCK(balloc(&b->C0, (size_t)b->KT * DP));
-    KMC_BCK(balloc(&b->Cnext, (size_t)b->KT * DP));
-    if (b->prec == kmc::PREC_FP32) KMC_BCK(balloc(&b->Cw, (size_t)b->nops * DP));
-    else KMC_BCK(balloc(&b->Chi, (size_t)b->nops * DP));
-    if (b->prec == kmc::PREC_BF16X3) KMC_BCK(balloc(&b->Clo, (size_t)b->nops * DP));
-    KMC_BCK(balloc(&b->cn, (size_t)b->nops));
-    KMC_BCK(balloc(&b->CT, (size_t)b->KTp * DP));
-    KMC_BCK(balloc(&b->labels, np));
-    KMC_BCK(balloc(&b->q2, 4 * (size_t)b->grid * (size_t)b->qstride));
-    KMC_BCK(balloc(&b->qa, (size_t)b->grid * (size_t)b->qstride));
-    KMC_BCK(balloc(&b->q2cnt, (size_t)b->grid));
-    KMC_BCK(balloc(&b->qacnt, (size_t)b->grid));
-    KMC_BCK(balloc(&b->bcount, (size_t)b->nbcount));
-    KMC_BCK(balloc(&b->count, (size_t)b->KT));
-    KMC_BCK(balloc(&b->start, (size_t)b->KT));
-    KMC_BCK(balloc(&b->members, np));
-    KMC_BCK(balloc(&b->tpart, (np / kmc::kTreeRows + (size_t)b->KT) * DP));
-    KMC_BCK(balloc(&b->ipart, np / kmc::kTreeRows + 1));
-    KMC_BCK(balloc(&b->shift2, (size_t)b->KT));
-    KMC_BCK(balloc(&b->conv, (size_t)b->KT));
-    KMC_BCK(balloc(&b->alive, (size_t)b->KT));
-    KMC_BCK(balloc(&b->inertia, (size_t)count));
-    KMC_BCK(balloc(&b->state, 1));
-    KMC_BCK(balloc(&b->pdone, (size_t)count));
-    KMC_BCK(balloc(&b->piter, (size_t)count));
-    KMC_BCK(balloc(&b->pconv, (size_t)count));
-    KMC_BCK(upload(&b->bp, b->hp, b->stream));
-    KMC_BCK(upload(&b->item_prob, item_prob, b->stream));
-    KMC_BCK(upload(&b->blk_prob, blk_prob, b->stream));
-    KMC_BCK(upload(&b->clus_prob, clus_prob, b->stream));
-    KMC_BCK(upload(&b->row_prob, row_prob, b->stream));
-    KMC_BCK(kmc::launch_fill_i32(b->alive, (long long)b->KT, 1, b->stream));
-    KMC_BCK(hipMemsetAsync(b->X, 0, np * DP * sizeof(float), b->stream));
-    KMC_BCK(hipMemsetAsync(b->C0, 0, (size_t)b->KT * DP * sizeof(float), b->stream));
-    KMC_BCK(hipMemsetAsync(b->CT, 0, (size_t)b->KTp * DP * sizeof(float), b->stream));
-    KMC_BCK(hipMemsetAsync(b->q2cnt, 0, (size_t)b->grid * sizeof(unsigned), b->stream));
-    KMC_BCK(hipMemsetAsync(b->qacnt, 0, (size_t)b->grid * sizeof(unsigned), b->stream));
-    // points and initial centroids of every problem
-    for (uint32_t p = 0; p < count; ++p) {
-      const kmc_problem& q = probs[p];
-      const BatchProblem& e = b->hp[p];
-      KMC_BCK(hipMemcpy2DAsync(b->X + (size_t)e.poff * DP, DP * sizeof(float), q.X,
-                               q.ld * sizeof(float), q.d * sizeof(float), q.n,
-                               hipMemcpyHostToDevice, b->stream));
-      KMC_BCK(hipMemcpy2DAsync(b->C0 + (size_t)e.coff * DP, DP * sizeof(float), q.C0,
-                               q.d * sizeof(float), q.d * sizeof(float), q.k,
-                               hipMemcpyHostToDevice, b->stream));
-    }
-    // norms and bf16 planes, in place over the packed rows
-    kmc::LoadParams lp{};
-    lp.src = b->X;
-    lp.ld = DP;
-    lp.n = b->npad;
-    lp.D = b->DP;
-    lp.DP = b->DP;
-    lp.X = b->X;
-    lp.Xhi = b->Xhi;
-    lp.Xlo = b->Xlo;
-    lp.xn2 = b->xn2;
-    lp.xn2_64 = nullptr;
-    KMC_BCK(kmc::launch_load_points(lp, b->stream));
-    KMC_BCK(hipStreamSynchronize(b->stream));
     return KMC_OK;
   };
-  st = body();
+  const kmc_status st = body();
   if (st != KMC_OK) {
     fprintf(stderr, "kmc_batch_create: %s\n", b->err.c_str());
     bfree_all(b);
@@ -718,58 +794,82 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
   if (!b) return KMC_ERR_ARG;
   KMC_BCK(hipSetDevice(b->dev));
   const int check = (flags & KMC_RUN_CHECK_CONVERGED) ? 1 : 0;
-  const size_t DP = (size_t)b->DP;
   // every run starts from the initial centroids
-  KMC_BCK(hipMemcpyAsync(b->C, b->C0, (size_t)b->KT * DP * sizeof(float),
-                         hipMemcpyDeviceToDevice, b->stream));
-  KMC_BCK(hipMemcpyAsync(b->Cnext, b->C0, (size_t)b->KT * DP * sizeof(float),
-                         hipMemcpyDeviceToDevice, b->stream));
-  hipLaunchKernelGGL(kmc::k_binit_labels, dim3((unsigned)((b->npad + 255) / 256)), dim3(256), 0,
-                     b->stream, b->labels, b->bp, b->blk_prob, b->npad);
-  KMC_BCK(hipGetLastError());
-  KMC_BCK(hipMemsetAsync(b->pdone, 0, b->nprob * sizeof(int), b->stream));
-  KMC_BCK(hipMemsetAsync(b->piter, 0, b->nprob * sizeof(unsigned), b->stream));
-  KMC_BCK(hipMemsetAsync(b->pconv, 0, b->nprob * sizeof(unsigned), b->stream));
-  KMC_BCK(hipMemsetAsync(b->state, 0, sizeof(DevState), b->stream));
+  for (BatchGroup* g : b->groups) {
+    const size_t DP = (size_t)g->DP;
+    KMC_BCK(hipMemcpyAsync(g->C, g->C0, (size_t)g->KT * DP * sizeof(float),
+                           hipMemcpyDeviceToDevice, b->stream));
+    KMC_BCK(hipMemcpyAsync(g->Cnext, g->C0, (size_t)g->KT * DP * sizeof(float),
+                           hipMemcpyDeviceToDevice, b->stream));
+    hipLaunchKernelGGL(kmc::k_binit_labels, dim3((unsigned)((g->npad + 255) / 256)), dim3(256),
+                       0, b->stream, g->labels, g->bp, g->blk_prob, g->npad);
+    KMC_BCK(hipGetLastError());
+    KMC_BCK(hipMemsetAsync(g->pdone, 0, g->nprob * sizeof(int), b->stream));
+    KMC_BCK(hipMemsetAsync(g->piter, 0, g->nprob * sizeof(unsigned), b->stream));
+    KMC_BCK(hipMemsetAsync(g->pconv, 0, g->nprob * sizeof(unsigned), b->stream));
+    KMC_BCK(hipMemsetAsync(g->state, 0, sizeof(DevState), b->stream));
+  }
   hipEvent_t t0, t1;
   KMC_BCK(hipEventCreate(&t0));
   KMC_BCK(hipEventCreate(&t1));
   KMC_BCK(hipEventRecord(t0, b->stream));
   kmc_status st = KMC_OK;
+  const size_t ng = b->groups.size();
   if (maxiter > 0) {
-    // chunks of iterations between host looks at the stop flag (the device
-    // flag makes the queued iterations after the stop no-ops)
+    // chunks of iterations between host looks at the stop flags (a group's
+    // device flag makes its queued iterations after the stop no-ops)
+    std::vector<DevState> hs(ng);
+    std::vector<char> gdone(ng, 0);
     uint64_t launched = 0;
     while (launched < maxiter && st == KMC_OK) {
       const uint64_t m = std::min<uint64_t>(16, maxiter - launched);
-      for (uint64_t i = 0; i < m && st == KMC_OK; ++i) st = enqueue_iteration(b, maxiter, check);
+      for (uint64_t i = 0; i < m && st == KMC_OK; ++i)
+        for (size_t gi = 0; gi < ng && st == KMC_OK; ++gi)
+          if (!gdone[gi]) st = enqueue_iteration(b, b->groups[gi], maxiter, check);
       launched += m;
       if (st != KMC_OK) break;
-      DevState s;
-      KMC_BCK(hipMemcpyAsync(&s, b->state, sizeof(DevState), hipMemcpyDeviceToHost, b->stream));
+      for (size_t gi = 0; gi < ng; ++gi)
+        if (!gdone[gi])
+          KMC_BCK(hipMemcpyAsync(&hs[gi], b->groups[gi]->state, sizeof(DevState),
+                                 hipMemcpyDeviceToHost, b->stream));
       KMC_BCK(hipStreamSynchronize(b->stream));
-      if (s.done) break;
+      bool all = true;
+      for (size_t gi = 0; gi < ng; ++gi) {
+        if (!gdone[gi] && hs[gi].done) gdone[gi] = 1;
+        all = all && gdone[gi];
+      }
+      if (all) break;
     }
   }
   if (st == KMC_OK) {
-    hipLaunchKernelGGL(kmc::k_binertia, dim3(b->nprob), dim3(256), 0, b->stream, b->X, b->C,
-                       b->labels, b->bp, b->DP, b->inertia);
-    KMC_BCK(hipGetLastError());
+    for (BatchGroup* g : b->groups) {
+      hipLaunchKernelGGL(kmc::k_binertia_blocks, dim3(g->nblk), dim3(kmc::kBInertiaThreads), 0,
+                         b->stream, g->X, g->C, g->labels, g->bp, g->blk_prob, g->DP, g->bpart);
+      hipLaunchKernelGGL(kmc::k_binertia_sum, dim3((g->nprob + 3) / 4), dim3(256), 0,
+                         b->stream, g->bpart, g->bp, (int)g->nprob, g->inertia);
+      KMC_BCK(hipGetLastError());
+    }
     KMC_BCK(hipEventRecord(t1, b->stream));
     KMC_BCK(hipEventSynchronize(t1));
     float ms = 0.f;
     KMC_BCK(hipEventElapsedTime(&ms, t0, t1));
     if (ms_total) *ms_total = ms;
     if (res) {
-      std::vector<unsigned> it(b->nprob), cv(b->nprob);
-      std::vector<double> in(b->nprob);
-      KMC_BCK(hipMemcpy(it.data(), b->piter, b->nprob * sizeof(unsigned), hipMemcpyDeviceToHost));
-      KMC_BCK(hipMemcpy(cv.data(), b->pconv, b->nprob * sizeof(unsigned), hipMemcpyDeviceToHost));
-      KMC_BCK(hipMemcpy(in.data(), b->inertia, b->nprob * sizeof(double), hipMemcpyDeviceToHost));
-      for (uint32_t p = 0; p < b->nprob; ++p) {
-        res[p].iterations = it[p];
-        res[p].converged_at = cv[p];
-        res[p].inertia = in[p];
+      for (BatchGroup* g : b->groups) {
+        std::vector<unsigned> it(g->nprob), cv(g->nprob);
+        std::vector<double> in(g->nprob);
+        KMC_BCK(hipMemcpy(it.data(), g->piter, g->nprob * sizeof(unsigned),
+                          hipMemcpyDeviceToHost));
+        KMC_BCK(hipMemcpy(cv.data(), g->pconv, g->nprob * sizeof(unsigned),
+                          hipMemcpyDeviceToHost));
+        KMC_BCK(hipMemcpy(in.data(), g->inertia, g->nprob * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        for (uint32_t p = 0; p < g->nprob; ++p) {
+          kmc_batch_result& r = res[g->ids[p]];
+          r.iterations = it[p];
+          r.converged_at = cv[p];
+          r.inertia = in[p];
+        }
       }
     }
   }
@@ -781,8 +881,9 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
 kmc_status kmc_batch_get_labels(kmc_batch* b, uint32_t problem, int32_t* labels) {
   if (!b || !labels || problem >= b->nprob) return bfail(b, KMC_ERR_ARG, "kmc_batch_get_labels");
   KMC_BCK(hipSetDevice(b->dev));
-  const BatchProblem& e = b->hp[problem];
-  KMC_BCK(hipMemcpy(labels, b->labels + e.poff, (size_t)e.n * sizeof(int32_t),
+  const BatchGroup* g = b->groups[b->grp_of[problem]];
+  const BatchProblem& e = g->hp[b->loc_of[problem]];
+  KMC_BCK(hipMemcpy(labels, g->labels + e.poff, (size_t)e.n * sizeof(int32_t),
                     hipMemcpyDeviceToHost));
   for (int i = 0; i < e.n; ++i) labels[i] -= e.coff;
   return KMC_OK;
@@ -791,10 +892,12 @@ kmc_status kmc_batch_get_labels(kmc_batch* b, uint32_t problem, int32_t* labels)
 kmc_status kmc_batch_get_centroids(kmc_batch* b, uint32_t problem, float* C) {
   if (!b || !C || problem >= b->nprob) return bfail(b, KMC_ERR_ARG, "kmc_batch_get_centroids");
   KMC_BCK(hipSetDevice(b->dev));
-  const BatchProblem& e = b->hp[problem];
-  const size_t d = b->d[problem];
-  KMC_BCK(hipMemcpy2D(C, d * sizeof(float), b->C + (size_t)e.coff * b->DP,
-                      (size_t)b->DP * sizeof(float), d * sizeof(float), (size_t)e.k,
+  const BatchGroup* g = b->groups[b->grp_of[problem]];
+  const int loc = b->loc_of[problem];
+  const BatchProblem& e = g->hp[loc];
+  const size_t d = g->d[loc];
+  KMC_BCK(hipMemcpy2D(C, d * sizeof(float), g->C + (size_t)e.coff * g->DP,
+                      (size_t)g->DP * sizeof(float), d * sizeof(float), (size_t)e.k,
                       hipMemcpyDeviceToHost));
   return KMC_OK;
 }
