@@ -61,8 +61,11 @@ PEAK = {"fp32": 157.3e12, "bf16x3": 2.5e15, "bf16": 2.5e15}
 HBM_PEAK = 8.0e12
 
 
+_T0 = time.perf_counter()
+
+
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print("[%7.1fs]" % (time.perf_counter() - _T0), *a, file=sys.stderr, flush=True)
 
 
 def run_form(kmc, X, C0, prec, steps, warmup, sync, update, graph):

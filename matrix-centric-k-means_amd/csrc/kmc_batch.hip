@@ -276,26 +276,45 @@ __global__ __launch_bounds__(256) void k_binit_labels(int* labels, const BatchPr
 }
 
 // FP64 inertia of each problem's final state in a fixed order: every
-// kBatchAlign-point block sums its points (lanes over dimensions, waves over
-// points, a fixed tree over the waves), then one wave per problem adds its
-// block sums.
+// kBatchAlign-point block sums its points (a lane takes four dimensions of a
+// point, a wave DP/4-lane groups of points, four steps' loads issued
+// together), then one wave per problem adds its block sums.
 constexpr int kBInertiaThreads = 256;
+template <int DP>
 __global__ __launch_bounds__(kBInertiaThreads) void k_binertia_blocks(
     const float* X, const float* C, const int* labels, const BatchProblem* bp,
-    const int* blk_prob, int DP, double* bpart) {
-  __shared__ double red_s[kBInertiaThreads / 64];
+    const int* blk_prob, double* bpart) {
+  constexpr int LPP = DP / 4;    // lanes per point
+  constexpr int PPS = 64 / LPP;  // points per wave step
+  constexpr int U = 4;
+  constexpr int NW = kBInertiaThreads / 64;
+  constexpr int PW = kBatchAlign / NW;  // points per wave
+  __shared__ double red_s[NW];
   const int b = blockIdx.x;
   const BatchProblem bpr = bp[blk_prob[b]];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long lo = (long long)b * kBatchAlign;
-  const long long hi = min(lo + (long long)kBatchAlign, bpr.poff + (long long)bpr.n);
+  const int sub = lane / LPP, j = 4 * (lane % LPP);
+  const long long lo = (long long)b * kBatchAlign + (long long)wave * PW;
+  const long long hi = min(lo + (long long)PW, bpr.poff + (long long)bpr.n);
   double s = 0.0;
-  for (long long r = lo + wave; r < hi; r += kBInertiaThreads / 64) {
-    const float* x = X + (size_t)r * DP;
-    const float* c = C + (size_t)labels[r] * DP;
-    for (int j = lane; j < DP; j += 64) {
-      const double t = (double)x[j] - (double)c[j];
-      s = fma(t, t, s);
+  for (long long r0 = lo; r0 < hi; r0 += U * PPS) {
+    float4 xv[U], cv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long r = r0 + u * PPS + sub;
+      const long long rr = r < hi ? r : lo;
+      xv[u] = *reinterpret_cast<const float4*>(X + (size_t)rr * DP + j);
+      cv[u] = *reinterpret_cast<const float4*>(C + (size_t)labels[rr] * DP + j);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = r0 + u * PPS + sub < hi;
+      const double t0 = (double)xv[u].x - (double)cv[u].x;
+      const double t1 = (double)xv[u].y - (double)cv[u].y;
+      const double t2 = (double)xv[u].z - (double)cv[u].z;
+      const double t3 = (double)xv[u].w - (double)cv[u].w;
+      const double q = fma(t3, t3, fma(t2, t2, fma(t1, t1, t0 * t0)));
+      s = ok ? s + q : s;
     }
   }
   s = wave_sum_f64(s);
@@ -303,7 +322,7 @@ __global__ __launch_bounds__(kBInertiaThreads) void k_binertia_blocks(
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
-    for (int w = 0; w < kBInertiaThreads / 64; ++w) t += red_s[w];
+    for (int w = 0; w < NW; ++w) t += red_s[w];
     bpart[b] = t;
   }
 }
@@ -691,35 +710,57 @@ kmc_status create_group(kmc_batch* b, BatchGroup* g, const std::vector<kmc_probl
   KMC_BCK(upload(&g->row_prob, row_prob, b->stream));
   KMC_BCK(kmc::launch_fill_i32(g->alive, (long long)g->KT, 1, b->stream));
   KMC_BCK(hipMemsetAsync(g->X, 0, np * DP * sizeof(float), b->stream));
-  KMC_BCK(hipMemsetAsync(g->C0, 0, (size_t)g->KT * DP * sizeof(float), b->stream));
+  if (g->Xhi) KMC_BCK(hipMemsetAsync(g->Xhi, 0, np * DP * sizeof(uint16_t), b->stream));
+  if (g->Xlo) KMC_BCK(hipMemsetAsync(g->Xlo, 0, np * DP * sizeof(uint16_t), b->stream));
+  KMC_BCK(hipMemsetAsync(g->xn2, 0, np * sizeof(float), b->stream));
   KMC_BCK(hipMemsetAsync(g->CT, 0, (size_t)g->KTp * DP * sizeof(float), b->stream));
   KMC_BCK(hipMemsetAsync(g->q2cnt, 0, (size_t)g->grid * sizeof(unsigned), b->stream));
   KMC_BCK(hipMemsetAsync(g->qacnt, 0, (size_t)g->grid * sizeof(unsigned), b->stream));
-  // points and initial centroids of every problem
+  // initial centroids, packed on the host and copied at once
+  std::vector<float> c0((size_t)g->KT * DP, 0.0f);
   for (uint32_t p = 0; p < count; ++p) {
     const kmc_problem& q = probs[p];
     const BatchProblem& e = g->hp[p];
-    KMC_BCK(hipMemcpy2DAsync(g->X + (size_t)e.poff * DP, DP * sizeof(float), q.X,
-                             q.ld * sizeof(float), q.d * sizeof(float), q.n,
-                             hipMemcpyHostToDevice, b->stream));
-    KMC_BCK(hipMemcpy2DAsync(g->C0 + (size_t)e.coff * DP, DP * sizeof(float), q.C0,
-                             q.d * sizeof(float), q.d * sizeof(float), q.k,
-                             hipMemcpyHostToDevice, b->stream));
+    for (uint32_t c = 0; c < q.k; ++c)
+      std::memcpy(&c0[((size_t)e.coff + c) * DP], q.C0 + (size_t)c * q.d, q.d * sizeof(float));
   }
-  // norms and bf16 planes, in place over the packed rows
-  kmc::LoadParams lp{};
-  lp.src = g->X;
-  lp.ld = DP;
-  lp.n = g->npad;
-  lp.D = g->DP;
-  lp.DP = g->DP;
-  lp.X = g->X;
-  lp.Xhi = g->Xhi;
-  lp.Xlo = g->Xlo;
-  lp.xn2 = g->xn2;
-  lp.xn2_64 = nullptr;
-  KMC_BCK(kmc::launch_load_points(lp, b->stream));
-  KMC_BCK(hipStreamSynchronize(b->stream));
+  KMC_BCK(hipMemcpyAsync(g->C0, c0.data(), c0.size() * sizeof(float), hipMemcpyHostToDevice,
+                         b->stream));
+  // points: each problem's rows as one contiguous copy into a staging buffer
+  // (a pitched copy from pageable memory goes row by row), then the norms,
+  // padded rows and bf16 planes at the problem's offset
+  size_t smax = 0;
+  for (const kmc_problem& q : probs) smax = std::max(smax, (size_t)((q.n - 1) * q.ld + q.d));
+  float* stage = nullptr;
+  KMC_BCK(balloc(&stage, smax));
+  kmc_status st = KMC_OK;
+  for (uint32_t p = 0; p < count && st == KMC_OK; ++p) {
+    const kmc_problem& q = probs[p];
+    const BatchProblem& e = g->hp[p];
+    hipError_t er = hipMemcpyAsync(stage, q.X, ((q.n - 1) * q.ld + q.d) * sizeof(float),
+                                   hipMemcpyHostToDevice, b->stream);
+    if (er == hipSuccess) {
+      kmc::LoadParams lp{};
+      lp.src = stage;
+      lp.ld = q.ld;
+      lp.n = (long long)q.n;
+      lp.D = (int)q.d;
+      lp.DP = g->DP;
+      lp.X = g->X + (size_t)e.poff * DP;
+      lp.Xhi = g->Xhi ? g->Xhi + (size_t)e.poff * DP : nullptr;
+      lp.Xlo = g->Xlo ? g->Xlo + (size_t)e.poff * DP : nullptr;
+      lp.xn2 = g->xn2 + e.poff;
+      lp.xn2_64 = nullptr;
+      er = kmc::launch_load_points(lp, b->stream);
+    }
+    if (er != hipSuccess)
+      st = bfail(b, KMC_ERR_HIP, std::string("kmc_batch_create: upload: ") +
+                                     hipGetErrorString(er));
+  }
+  const hipError_t es = hipStreamSynchronize(b->stream);
+  (void)hipFree(stage);
+  if (st != KMC_OK) return st;
+  KMC_BCK(es);
   return KMC_OK;
 }
 
@@ -843,8 +884,19 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
   }
   if (st == KMC_OK) {
     for (BatchGroup* g : b->groups) {
-      hipLaunchKernelGGL(kmc::k_binertia_blocks, dim3(g->nblk), dim3(kmc::kBInertiaThreads), 0,
-                         b->stream, g->X, g->C, g->labels, g->bp, g->blk_prob, g->DP, g->bpart);
+      const dim3 grid(g->nblk), block(kmc::kBInertiaThreads);
+      switch (g->DP) {
+        case 16: hipLaunchKernelGGL(kmc::k_binertia_blocks<16>, grid, block, 0, b->stream,
+                                    g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
+        case 32: hipLaunchKernelGGL(kmc::k_binertia_blocks<32>, grid, block, 0, b->stream,
+                                    g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
+        case 64: hipLaunchKernelGGL(kmc::k_binertia_blocks<64>, grid, block, 0, b->stream,
+                                    g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
+        case 128: hipLaunchKernelGGL(kmc::k_binertia_blocks<128>, grid, block, 0, b->stream,
+                                     g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
+        default: hipLaunchKernelGGL(kmc::k_binertia_blocks<256>, grid, block, 0, b->stream,
+                                    g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
+      }
       hipLaunchKernelGGL(kmc::k_binertia_sum, dim3((g->nprob + 3) / 4), dim3(256), 0,
                          b->stream, g->bpart, g->bp, (int)g->nprob, g->inertia);
       KMC_BCK(hipGetLastError());

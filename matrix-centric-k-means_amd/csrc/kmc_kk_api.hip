@@ -150,8 +150,12 @@ kmc_status kmc_kk_set_points(kmc_kk_handle* h, const float* X, size_t ld) {
   if (!h || !X || ld < (size_t)h->P.d) return kfail(h, KMC_ERR_ARG, "kmc_kk_set_points: bad argument");
   kmc::KKParams& P = h->P;
   KK_CK(hipSetDevice(h->dev));
-  KK_CK(hipMemcpy2DAsync(h->dX, P.d * sizeof(float), X, ld * sizeof(float), P.d * sizeof(float),
-                         (size_t)P.n, hipMemcpyHostToDevice, h->stream));
+  if (ld == (size_t)P.d)  // one contiguous copy (a pitched copy from pageable memory goes row by row)
+    KK_CK(hipMemcpyAsync(h->dX, X, (size_t)P.n * P.d * sizeof(float), hipMemcpyHostToDevice,
+                         h->stream));
+  else
+    KK_CK(hipMemcpy2DAsync(h->dX, P.d * sizeof(float), X, ld * sizeof(float),
+                           P.d * sizeof(float), (size_t)P.n, hipMemcpyHostToDevice, h->stream));
   hipEvent_t e0, e1;
   KK_CK(hipEventCreate(&e0));
   KK_CK(hipEventCreate(&e1));

@@ -1179,62 +1179,101 @@ __global__ __launch_bounds__(256) void k_update(UpdateParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// Tree update: the sorted member list is cut into chunks of kTreeRows rows;
-// one wave per (chunk, 64-dim slab) sums its rows segment by segment (a
-// segment = the part of one cluster inside the chunk) in FP64 and writes one
-// partial per segment at index cluster + chunk (injective, because clusters
-// occupy increasing chunk ranges). k_update_combine adds a cluster's partials
-// in chunk order. Deterministic; the FP64 sums are rounded once, so centroids
-// are at least as accurate as the reference's float sums (not bit-identical).
+// Tree update: the sorted member list is cut into chunks of kTreeRows rows,
+// one wave per chunk. The wave first stages the chunk's member ids and their
+// labels in LDS (coalesced loads, two memory latencies per chunk), then walks
+// the chunk segment by segment (a segment = the part of one cluster inside
+// the chunk, found by a ballot over the staged labels) and sums each
+// segment's rows in FP64, with every row load of a batch issued before any
+// is used. One partial per segment goes to index cluster + chunk (injective,
+// because clusters occupy increasing chunk ranges); k_finish adds a
+// cluster's partials in chunk order. Deterministic; the FP64 sums are
+// rounded once, so centroids are at least as accurate as the reference's
+// float sums (not bit-identical).
 // ---------------------------------------------------------------------------
+constexpr int kTreeWaves = 4;
+
+// stage member ids and labels of local rows [0, L) of a chunk
+__device__ __forceinline__ void tree_stage(const UpdateParams& P, long long p0, int L,
+                                           int* mem_s, int* lab_s, int lane) {
+  int id[kTreeRows / 64], lab[kTreeRows / 64];
+#pragma unroll
+  for (int m = 0; m < kTreeRows / 64; ++m) {
+    const int i = m * 64 + lane;
+    id[m] = i < L ? P.members[p0 + i] : 0;
+  }
+#pragma unroll
+  for (int m = 0; m < kTreeRows / 64; ++m) {
+    const int i = m * 64 + lane;
+    lab[m] = i < L ? P.labels[id[m]] : -1;
+  }
+#pragma unroll
+  for (int m = 0; m < kTreeRows / 64; ++m) {
+    mem_s[m * 64 + lane] = id[m];
+    lab_s[m * 64 + lane] = lab[m];
+  }
+}
+
+// first local index in (r, L) whose label differs from c, or L
+__device__ __forceinline__ int tree_seg_end(const int* lab_s, int r, int L, int c, int lane) {
+  for (int w = r + 1; w < L; w += 64) {
+    const int i = w + lane;
+    const uint64_t m = __ballot(i >= L || lab_s[i < L ? i : 0] != c);
+    if (m) return w + __ffsll((unsigned long long)m) - 1;
+  }
+  return L;
+}
+
+// Wide rows (DP = 128 with VW = 2, DP = 256 with VW = 4): lane owns dimensions
+// [VW*lane, VW*lane + VW) of every row, one contiguous 256*VW-byte read per
+// row, batches of 16 rows.
 template <int VW>
-__global__ __launch_bounds__(256) void k_update_tree(UpdateParams P) {
+__global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree(UpdateParams P) {
   if (*P.done) return;
-  const int lane = threadIdx.x & 63;
-  const long long q = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ int mem_s[kTreeWaves][kTreeRows], lab_s[kTreeWaves][kTreeRows];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long q = (long long)blockIdx.x * kTreeWaves + wv;
   const long long nchunks = (P.n + kTreeRows - 1) / kTreeRows;
-  if (q >= nchunks) return;
-  // lane owns dimensions [VW*lane, VW*lane + VW) of every row (whole rows per
-  // wave: one contiguous 4*VW*64-byte read per row)
+  const long long p0 = q * kTreeRows;
+  const int L = q < nchunks ? (int)min((long long)kTreeRows, P.n - p0) : 0;
+  tree_stage(P, p0, L, mem_s[wv], lab_s[wv], lane);
+  __syncthreads();
+  if (L == 0) return;
+  const int* ms = mem_s[wv];
   const int j0 = VW * lane;
   const bool act = j0 < P.DP;
-  const long long p0 = q * kTreeRows;
-  const long long p1 = min(p0 + (long long)kTreeRows, P.n);
   const float* X = P.X + (act ? j0 : 0);
   double in = 0.0;
-  long long pos = p0;
-  while (pos < p1) {
-    const int c = P.labels[P.members[pos]];
-    const long long seg_end = min(p1, (long long)P.start[c] + P.count[c]);
+  int r = 0;
+  while (r < L) {
+    const int c = lab_s[wv][r];
+    const int e = tree_seg_end(lab_s[wv], r, L, c, lane);
     double cold[VW], sum[VW];
 #pragma unroll
     for (int v = 0; v < VW; ++v) {
       cold[v] = act ? (double)P.C[(size_t)c * P.DP + j0 + v] : 0.0;
       sum[v] = 0.0;
     }
-    // batches of U rows, all loads of a batch issued before any is used; the
-    // last, partial batch of a segment loads a valid row in place of the rows
-    // past the end and masks them out (the select keeps the sums as they are)
+    // batches of U rows; the last, partial batch of a segment loads a valid
+    // row in place of the rows past the end and masks them out
     constexpr int U = 16;
-    auto batch = [&](long long t, bool partial) {
+    auto batch = [&](int t, bool partial) {
       float xv[U][VW];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const long long tt = partial ? min(t + u, seg_end - 1) : t + u;
-        const float* r = X + (size_t)P.members[tt] * P.DP;
+        const int tt = partial ? min(t + u, e - 1) : t + u;
+        const float* row = X + (size_t)ms[tt] * P.DP;
         if (VW == 4) {
-          const float4 f = *reinterpret_cast<const float4*>(r);
+          const float4 f = *reinterpret_cast<const float4*>(row);
           xv[u][0] = f.x; xv[u][1 % VW] = f.y; xv[u][2 % VW] = f.z; xv[u][3 % VW] = f.w;
-        } else if (VW == 2) {
-          const float2 f = *reinterpret_cast<const float2*>(r);
-          xv[u][0] = f.x; xv[u][1 % VW] = f.y;
         } else {
-          xv[u][0] = r[0];
+          const float2 f = *reinterpret_cast<const float2*>(row);
+          xv[u][0] = f.x; xv[u][1 % VW] = f.y;
         }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const bool ok = !partial || t + u < seg_end;  // wave-uniform
+        const bool ok = !partial || t + u < e;  // wave-uniform
 #pragma unroll
         for (int v = 0; v < VW; ++v) {
           const double xd = (double)xv[u][v];
@@ -1244,16 +1283,86 @@ __global__ __launch_bounds__(256) void k_update_tree(UpdateParams P) {
         }
       }
     };
-    long long t = pos;
-    for (; t + U <= seg_end; t += U) batch(t, false);
-    if (t < seg_end) batch(t, true);
+    int t = r;
+    for (; t + U <= e; t += U) batch(t, false);
+    if (t < e) batch(t, true);
     if (act) {
 #pragma unroll
       for (int v = 0; v < VW; ++v) P.tpart[(size_t)(c + q) * P.DP + j0 + v] = sum[v];
     }
-    pos = seg_end;
+    r = e;
   }
   if (!act) in = 0.0;
+  in = wave_sum_f64(in);
+  if (lane == 0) P.ipart[q] = in;
+}
+
+// Narrow rows (DP = 16, 32, 64): a lane takes four dimensions (one 16-byte
+// load), so one wave load covers G = 256 / DP rows and a batch of U loads 64
+// rows. Lane group g sums the rows t + u*G + g of a segment; at the segment's
+// end the G group sums are added by a fixed butterfly, so the result is
+// deterministic.
+template <int DP>
+__global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree_rows(UpdateParams P) {
+  if (*P.done) return;
+  constexpr int LPR = DP / 4;    // lanes per row
+  constexpr int G = 64 / LPR;    // rows per wave load
+  constexpr int U = 64 / G;      // loads per batch
+  __shared__ int mem_s[kTreeWaves][kTreeRows], lab_s[kTreeWaves][kTreeRows];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long q = (long long)blockIdx.x * kTreeWaves + wv;
+  const long long nchunks = (P.n + kTreeRows - 1) / kTreeRows;
+  const long long p0 = q * kTreeRows;
+  const int L = q < nchunks ? (int)min((long long)kTreeRows, P.n - p0) : 0;
+  tree_stage(P, p0, L, mem_s[wv], lab_s[wv], lane);
+  __syncthreads();
+  if (L == 0) return;
+  const int* ms = mem_s[wv];
+  const int g = lane / LPR, j0 = 4 * (lane % LPR);
+  const float* X = P.X + j0;
+  double in = 0.0;
+  int r = 0;
+  while (r < L) {
+    const int c = lab_s[wv][r];
+    const int e = tree_seg_end(lab_s[wv], r, L, c, lane);
+    const float4 cf = *reinterpret_cast<const float4*>(P.C + (size_t)c * DP + j0);
+    const double cold[4] = {(double)cf.x, (double)cf.y, (double)cf.z, (double)cf.w};
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    auto batch = [&](int t, bool partial) {
+      float4 xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int rr = t + u * G + g;
+        const int tt = partial ? min(rr, e - 1) : rr;
+        xv[u] = *reinterpret_cast<const float4*>(X + (size_t)ms[tt] * DP);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = !partial || t + u * G + g < e;
+        const double xd[4] = {(double)xv[u].x, (double)xv[u].y, (double)xv[u].z,
+                              (double)xv[u].w};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double df = xd[v] - cold[v];
+          sum[v] = ok ? sum[v] + xd[v] : sum[v];
+          in = ok ? fma(df, df, in) : in;
+        }
+      }
+    };
+    int t = r;
+    for (; t + U * G <= e; t += U * G) batch(t, false);
+    if (t < e) batch(t, true);
+#pragma unroll
+    for (int o = LPR; o < 64; o <<= 1) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sum[v] += __shfl_xor(sum[v], o, 64);
+    }
+    if (g == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) P.tpart[(size_t)(c + q) * DP + j0 + v] = sum[v];
+    }
+    r = e;
+  }
   in = wave_sum_f64(in);
   if (lane == 0) P.ipart[q] = in;
 }
@@ -2006,10 +2115,14 @@ hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
   }
   if (mode == UPDATE_TREE) {
     const long long waves = (P.n + kTreeRows - 1) / kTreeRows;
-    const dim3 g((unsigned)((waves + 3) / 4));
-    if (P.DP >= 256) hipLaunchKernelGGL(k_update_tree<4>, g, dim3(256), 0, s, P);
-    else if (P.DP >= 128) hipLaunchKernelGGL(k_update_tree<2>, g, dim3(256), 0, s, P);
-    else hipLaunchKernelGGL(k_update_tree<1>, g, dim3(256), 0, s, P);
+    const dim3 g((unsigned)((waves + kTreeWaves - 1) / kTreeWaves));
+    const dim3 b(64 * kTreeWaves);
+    if (P.DP >= 256) hipLaunchKernelGGL(k_update_tree<4>, g, b, 0, s, P);
+    else if (P.DP >= 128) hipLaunchKernelGGL(k_update_tree<2>, g, b, 0, s, P);
+    else if (P.DP == 64) hipLaunchKernelGGL(k_update_tree_rows<64>, g, b, 0, s, P);
+    else if (P.DP == 32) hipLaunchKernelGGL(k_update_tree_rows<32>, g, b, 0, s, P);
+    else if (P.DP == 16) hipLaunchKernelGGL(k_update_tree_rows<16>, g, b, 0, s, P);
+    else return hipErrorInvalidValue;
   } else {
     hipLaunchKernelGGL(k_update, dim3((P.K * S + 3) / 4), dim3(256), 0, s, P);
   }
