@@ -12,7 +12,9 @@ Points are resident in HBM before the timed region.
 
 The headline form (``--prec``, default bf16: single-product bf16 MFMA with
 exact recomputation of the margin-band points), the split form (bf16x3: hi+lo
-bf16 products) and the plain FP32 form run from the same initial centroids;
+bf16 products), the fp8 form (e4m3 with per-row power-of-two scales applied
+by the matrix core) and the plain FP32 form run from the same initial
+centroids;
 the line reports the headline's iterations/s as ``value`` and the other forms
 beside it, checks that every form ends with bit-identical centroids and
 labels, and measures the inertia error of the headline form against an FP64
@@ -57,7 +59,8 @@ WORKLOADS = {
 }
 N, D, K = 1_000_000, 128, 256
 DATA_SEED, INIT_SEED = 20261203, 7
-PEAK = {"fp32": 157.3e12, "bf16x3": 2.5e15, "bf16": 2.5e15}
+PEAK = {"fp32": 157.3e12, "bf16x3": 2.5e15, "bf16": 2.5e15, "fp8": 5.0e15}
+DTYPE = {"bf16x3": "bf16", "bf16": "bf16", "fp32": "fp32", "fp8": "fp8"}
 HBM_PEAK = 8.0e12
 
 
@@ -277,7 +280,7 @@ def bench_batch(a):
            "value": head["problem_iters_per_s"], "unit": "problem-iter/s", "n_gpus": 1,
            "steps": steps, "warmup": a.warmup, "ms_per_step": head["ms_per_batch_iteration"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": {"bf16x3": "bf16", "bf16": "bf16", "fp32": "fp32"}[a.prec],
+           "dtype": DTYPE[a.prec],
            "data": "synthetic",
            "config": {"workload": "512 batched Lloyd problems, %d points in total, %d "
                                   "iterations each, tol=0" % (npts, steps),
@@ -310,12 +313,13 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed iterations (default: 50 for c2, 20 for c5, 100 for c3)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--prec", default="bf16", choices=["bf16x3", "bf16", "fp32"])
+    ap.add_argument("--prec", default="bf16", choices=["bf16x3", "bf16", "fp32", "fp8"])
     ap.add_argument("--update", default="tree", choices=["ordered", "tree", "fixed"],
                     help="centroid update: FP64 tree over the sorted members, exact "
                          "fixed-point streaming (no sort), or the reference's float "
                          "summation order")
     ap.add_argument("--no-x3", action="store_true", help="skip the bf16x3 comparison form")
+    ap.add_argument("--no-fp8", action="store_true", help="skip the fp8 comparison form")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison form")
@@ -366,6 +370,10 @@ def main():
     if not a.no_x3 and a.prec != "bf16x3":
         log("timing bf16x3: warmup %d, steps %d" % (a.warmup, a.steps))
         x3 = run_form(kmc, X, C0, "bf16x3", a.steps, a.warmup, sync, a.update, not a.eager)
+    f8 = None
+    if not a.no_fp8 and a.prec != "fp8":
+        log("timing fp8: warmup %d, steps %d" % (a.warmup, a.steps))
+        f8 = run_form(kmc, X, C0, "fp8", a.steps, a.warmup, sync, a.update, not a.eager)
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
@@ -383,14 +391,15 @@ def main():
     flops = 2.0 * N * K * D
     achieved = flops / t_assign
     # the two largest kernels of an iteration, each against its own roof
-    roof_assign = {"bound": "mfma", "kernel": "k_assign<128,%s>" % a.prec,
+    dp = 128 if D <= 128 else 256
+    roof_assign = {"bound": "mfma", "kernel": "k_assign<%d,%s>" % (dp, a.prec),
                    "achieved": round(achieved / 1e12, 3), "peak": PEAK[a.prec] / 1e12,
                    "unit": "TFLOP/s", "frac": round(achieved / PEAK[a.prec], 4),
                    "traffic": traffic_from_profiles("assign", a.prec, a.workload),
                    "algorithmic_flops_per_launch": flops,
                    "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches}
     ubytes = N * D * 4 + N * 4 + K * D * 4  # X rows once, member ids, centroids
-    uname = {"tree": "k_update_tree<2>", "ordered": "k_update",
+    uname = {"tree": "k_update_tree<%d>" % (dp // 64), "ordered": "k_update",
              "fixed": "k_update_fx<64>"}[a.update]
     roof_update = {"bound": "hbm", "kernel": uname,
                    "achieved": round(ubytes / t_update / 1e9, 1), "peak": HBM_PEAK / 1e9,
@@ -415,7 +424,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"bf16x3": "bf16", "bf16": "bf16", "fp32": "fp32"}[a.prec],
+        "dtype": DTYPE[a.prec],
         "data": "synthetic",
         "config": {"workload": "Lloyd k-means, %s, maxiter=%d tol=0" % (W["text"], a.steps),
                    "n": N, "d": D, "k": K, "form": a.prec, "seed": DATA_SEED,
@@ -455,6 +464,18 @@ def main():
                          "assign_frac_bf16_peak": round(3 * flops / f_t / PEAK["bf16"], 4),
                          "identical": bool(np.array_equal(x3["C_end"], C_end) and
                                            np.array_equal(x3["labels_end"], head["labels_end"]))}
+    if f8 is not None:
+        f = f8["res"]
+        f_ips = int(f["iterations"]) / f8["wall_s"]
+        f_t = f["ms_assign"] / 1e3 / max(1, int(f["assign_launches"]))
+        out["fp8"] = {"value": round(f_ips, 3),
+                      "ms_per_step": round(f8["wall_s"] * 1e3 / max(1, f["iterations"]), 5),
+                      "assign_mean_launch_ms": round(f_t * 1e3, 5),
+                      "assign_frac_fp8_peak": round(flops / f_t / PEAK["fp8"], 4),
+                      "refined_points_last_iter": {"two_candidates": int(f["refined_two"]),
+                                                   "all_k": int(f["refined_full"])},
+                      "identical": bool(np.array_equal(f8["C_end"], C_end) and
+                                        np.array_equal(f8["labels_end"], head["labels_end"]))}
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(X, C0, W["cpu_sample"], W["cpu_iters"])

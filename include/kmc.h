@@ -50,7 +50,12 @@ typedef enum {
 typedef enum {
   KMC_PREC_FP32 = 0,   /* v_mfma_f32_32x32x2_f32                          */
   KMC_PREC_BF16X3 = 1, /* hi+lo split bf16, 3 x v_mfma_f32_32x32x16_bf16  */
-  KMC_PREC_BF16 = 2    /* single bf16 product, wide refinement band       */
+  KMC_PREC_BF16 = 2,   /* single bf16 product, wide refinement band       */
+  KMC_PREC_FP8 = 3     /* e4m3 operands with one power-of-two scale per
+                          point and per centroid, applied by the matrix core
+                          (v_mfma_scale_f32_32x32x64_f8f6f4); widest band.
+                          Rows are padded to at least 64 dimensions. Not
+                          available for batches. */
 } kmc_precision;
 
 /* Centroid update summation order. */

@@ -53,7 +53,7 @@ class Kmeans {
  public:
   enum class InitMethod { random, plus_plus };
   enum class Kernel { linear, polynomial, sigmoid, rbf };
-  enum class Precision { fp32, bf16x3, bf16 };
+  enum class Precision { fp32, bf16x3, bf16, fp8 };
 
   struct Options {
     // The reference GPU engine's semantics (src/kmeans.cu:772-1122): initial

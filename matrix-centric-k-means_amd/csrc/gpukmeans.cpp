@@ -5,7 +5,7 @@
 //   gpukmeans -d D -n N -k K -m MAXITER -b random -f linear -l 2
 //             [-i in.csv|in.svm] [-o out.csv] [-r RUNS] [-s SEED] [-t TOL] [-c 0|1]
 // Extensions (not in the reference):
-//   -p/--precision fp32|bf16x3|bf16   distance GEMM form (default fp32)
+//   -p/--precision fp32|bf16x3|bf16|fp8   distance GEMM form (default fp32)
 //   --label-col first|last|none       CSV class-label column (default first, as
 //                                     the reference reader; its shipped data
 //                                     files put the label last)
@@ -66,7 +66,7 @@ void usage() {
          "  -b, --init arg           Method to use to initialize centroids\n"
          "  -f, --kernel arg         kernel function\n"
          "  -l, --level arg          Optimization level\n"
-         "  -p, --precision arg      fp32 | bf16x3 | bf16 (default: fp32)\n"
+         "  -p, --precision arg      fp32 | bf16x3 | bf16 | fp8 (default: fp32)\n"
          "      --label-col arg      first | last | none (default: first)\n"
          "      --cpu-semantics      reference CPU k-means semantics\n";
 }
@@ -280,6 +280,7 @@ int main(int argc, char** argv) {
   opts.reference_gpu = !args.count("cpu-semantics");
   if (prec == "fp32") opts.precision = Kmeans::Precision::fp32;
   else if (prec == "bf16x3") opts.precision = Kmeans::Precision::bf16x3;
+  else if (prec == "fp8") opts.precision = Kmeans::Precision::fp8;
   else if (prec == "bf16") opts.precision = Kmeans::Precision::bf16;
   else missing("precision");
 

@@ -26,6 +26,8 @@ struct kmc_handle {
   float* X = nullptr;
   uint16_t* Xhi = nullptr;
   uint16_t* Xlo = nullptr;
+  uint8_t* Xq = nullptr;  // fp8 form: e4m3 plane and per-row E8M0 scales
+  int* xsc = nullptr;
   float* xn2 = nullptr;
   // centroids
   float* C = nullptr;
@@ -33,6 +35,7 @@ struct kmc_handle {
   float* Cw = nullptr;
   uint16_t* Chi = nullptr;
   uint16_t* Clo = nullptr;
+  uint8_t* Cq = nullptr;
   float* cn = nullptr;
   float* CT = nullptr;
   // assignment
@@ -102,25 +105,41 @@ double gam(double m, double u) { return m * u / (1.0 - m * u); }
 //   |eta_j| <= (u + gA)(||c||^2 + ||x||^2 + 2||x||c) + gD c^2
 //              + 2 (t_in + gA (1 + t_in)) ||x|| c          (operands, products)
 // where c = cmax, u = 2^-24, gA = accumulation bound, gD = gamma_D, t_in =
-// operand rounding of the bf16 forms. With ||x||^2 <= 2 d_j + 2 c^2:
+// operand rounding of the bf16 and fp8 forms. With ||x||^2 <= 2 d_j + 2 c^2:
 //   |eta_j| <= alpha ||x|| c + beta c^2 + g1 d_j
 //   alpha = 2(u + gA) + 2(t_in + gA(1 + t_in)),  beta = 3(u + gA) + gD,
 //   g1 = 2(u + gA).
 // The reference's direct float form r_j satisfies |r_j - d_j| <= g_{D+3} d_j,
 // and the packed index moves b_j by < 2^(idx_bits-23) b_j. All terms relative
 // to the distance go into gref; the band test evaluates them at the upper
-// bound b + E of each distance. Every constant carries a factor 2 of headroom.
+// bound b + E of each distance. Every constant carries a factor 1.25 of
+// headroom.
 void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, float* gref,
                     float* xpk) {
-  const double u = std::ldexp(1.0, -24);  // fp32 unit roundoff
-  const double ub = std::ldexp(1.0, -9);  // bf16 unit roundoff
+  const double u = std::ldexp(1.0, -24);   // fp32 unit roundoff
+  const double ub = std::ldexp(1.0, -8);   // bf16 unit roundoff (8-bit significand)
+  const double u8 = std::ldexp(1.0, -4);   // e4m3 unit roundoff (4-bit significand)
   double t_in, gA;
   if (prec == kmc::PREC_FP32) {
     t_in = 0.0;             // operands exact
     gA = gam(D + 2, u);     // product rounding + D-term accumulation
   } else if (prec == kmc::PREC_BF16X3) {
-    t_in = 3.2 * ub * ub;   // hi+lo split: dropped lo*lo and residuals
+    // x = hi + lo + r with |r| <= ub^2 |x| (likewise -2c); the dropped
+    // lo*lo and residual products sum to at most 3.2 ub^2 |x||c|
+    t_in = 3.2 * ub * ub;
     gA = gam(3 * D + 1, u); // products exact in fp32, 3D-term accumulation
+  } else if (prec == kmc::PREC_FP8) {
+    // each operand row is scaled by 2^-e so that its largest entry lands in
+    // [224, 448]: normal entries round with relative error u8, subnormal
+    // ones with absolute error 2^-10 * 2^e <= eta * max|row| <= eta * ||row||
+    const double eta = std::ldexp(1.0, -10) / 224.0;
+    t_in = 2.0 * u8 + u8 * u8 + 2.0 * (1.0 + u8) * std::sqrt((double)D) * eta +
+           (double)D * eta * eta;
+    // the scaled fp8 MFMA does not accumulate like sequential fp32: its error
+    // measured on random e4m3 operands (wide exponent spread, 64-term dot
+    // products) stays below 2^-10.7 of the largest product; 2^-8 of the
+    // absolute sum is taken
+    gA = std::ldexp(1.0, -8);
   } else {
     t_in = 2.0 * ub + ub * ub;
     gA = gam(D + 1, u);
@@ -130,12 +149,14 @@ void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, fl
   const double a = 2.0 * (u + gA) + 2.0 * (t_in + gA * (1.0 + t_in));
   const double b = 3.0 * (u + gA) + gD;
   const double g = gam(D + 3, u) + pk + 2.0 * (u + gA);
-  *alpha = (float)(2.0 * a);
-  *beta = (float)(2.0 * b);
-  *gref = (float)(2.0 * g);
-  // single-product bf16 ranks ||c||^2 - 2 x.c and adds ||x||^2 afterwards:
-  // packing perturbation and the final add are then relative to d^2 + ||x||^2
-  *xpk = prec == kmc::PREC_BF16 ? (float)(2.0 * (pk + 2.0 * u)) : 0.0f;
+  const double H = 1.25;  // headroom over the bound
+  *alpha = (float)(H * a);
+  *beta = (float)(H * b);
+  *gref = (float)(H * g);
+  // single-product bf16 and fp8 rank ||c||^2 - 2 x.c and add ||x||^2
+  // afterwards: packing perturbation and the final add are then relative to
+  // d^2 + ||x||^2
+  *xpk = (prec == kmc::PREC_BF16 || prec == kmc::PREC_FP8) ? (float)(H * (pk + 2.0 * u)) : 0.0f;
 }
 
 }  // namespace kmc
@@ -184,12 +205,18 @@ kmc_status alloc_all(kmc_handle* h) {
   const size_t n = h->n;
   const size_t DP = h->DP, K = h->K, Kp = h->Kp;
   KMC_CK(dalloc(&h->X, n * DP));
-  if (h->prec != kmc::PREC_FP32) KMC_CK(dalloc(&h->Xhi, n * DP));
+  if (h->prec == kmc::PREC_BF16 || h->prec == kmc::PREC_BF16X3)
+    KMC_CK(dalloc(&h->Xhi, n * DP));
   if (h->prec == kmc::PREC_BF16X3) KMC_CK(dalloc(&h->Xlo, n * DP));
+  if (h->prec == kmc::PREC_FP8) {
+    KMC_CK(dalloc(&h->Xq, n * DP));
+    KMC_CK(dalloc(&h->xsc, n));
+  }
   KMC_CK(dalloc(&h->xn2, n));
   KMC_CK(dalloc(&h->C, K * DP));
   KMC_CK(dalloc(&h->Cnext, K * DP));
   if (h->prec == kmc::PREC_FP32) KMC_CK(dalloc(&h->Cw, Kp * DP));
+  else if (h->prec == kmc::PREC_FP8) KMC_CK(dalloc(&h->Cq, Kp * (DP + kmc::kFp8RowPad)));
   else KMC_CK(dalloc(&h->Chi, Kp * DP));
   if (h->prec == kmc::PREC_BF16X3) KMC_CK(dalloc(&h->Clo, Kp * DP));
   KMC_CK(dalloc(&h->cn, Kp));
@@ -235,7 +262,8 @@ void free_all(kmc_handle* h) {
                   h->Clo, h->cn, h->CT, h->labels, h->q2, h->qa, h->q2cnt, h->qacnt,
                   h->bcount,
                   h->count, h->start, h->members, h->ipart, h->tpart, h->conv,
-                  h->shift2, h->alive, h->state, h->xn64, h->fpart, h->qpart, h->cpart};
+                  h->shift2, h->alive, h->state, h->xn64, h->fpart, h->qpart, h->cpart,
+                  h->Xq, h->xsc, h->Cq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -283,6 +311,7 @@ kmc::PrepParams prep_params(kmc_handle* h) {
   pp.Cw = h->Cw;
   pp.Chi = h->Chi;
   pp.Clo = h->Clo;
+  pp.Cq = h->Cq;
   pp.cn = h->cn;
   pp.CT = h->CT;
   pp.alive = h->alive;
@@ -302,6 +331,9 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
   kmc::AssignParams ap{};
   ap.X = h->X;
   ap.Xhi = h->Xhi;
+  ap.Xq = h->Xq;
+  ap.xsc = h->xsc;
+  ap.Cq = h->Cq;
   ap.Xlo = h->Xlo;
   ap.xn2 = h->xn2;
   ap.n = (long long)h->n;
@@ -525,6 +557,8 @@ kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
   lp.DP = h->DP;
   lp.X = h->X;
   lp.Xhi = h->Xhi;
+  lp.Xq = h->Xq;
+  lp.xsc = h->xsc;
   lp.Xlo = h->Xlo;
   lp.xn2 = h->xn2;
   lp.xn2_64 = xn64;
@@ -607,12 +641,15 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
   if (cfg->k > cfg->n) return KMC_ERR_ARG;
   if (cfg->n > (size_t)0x7fffffff) return KMC_ERR_UNSUPPORTED;
   if (cfg->k > 16384u) return KMC_ERR_UNSUPPORTED;
-  if (cfg->precision < 0 || cfg->precision > 2) return KMC_ERR_ARG;
+  if (cfg->precision < 0 || cfg->precision > 3) return KMC_ERR_ARG;
   if (cfg->update != KMC_UPDATE_ORDERED && cfg->update != KMC_UPDATE_TREE &&
       cfg->update != KMC_UPDATE_FIXED)
     return KMC_ERR_ARG;
   if (!(cfg->tol >= 0.0f)) return KMC_ERR_ARG;
-  const int DP = pick_dp((int)cfg->d);
+  // one fp8 MFMA step covers 64 dimensions
+  const int DP = cfg->precision == KMC_PREC_FP8 && pick_dp((int)cfg->d) > 0
+                     ? std::max(64, pick_dp((int)cfg->d))
+                     : pick_dp((int)cfg->d);
   if (DP == 0) return KMC_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return KMC_ERR_NO_DEVICE;

@@ -772,6 +772,7 @@ kmc_status kmc_batch_create(const kmc_problem* probs, uint32_t count,
                             const kmc_batch_config* cfg, kmc_batch** out) {
   if (!probs || !cfg || !out || count == 0) return KMC_ERR_ARG;
   *out = nullptr;
+  if (cfg->precision == kmc::PREC_FP8) return KMC_ERR_UNSUPPORTED;
   if (cfg->precision < kmc::PREC_FP32 || cfg->precision > kmc::PREC_BF16) return KMC_ERR_ARG;
   for (uint32_t p = 0; p < count; ++p) {
     const kmc_problem& q = probs[p];

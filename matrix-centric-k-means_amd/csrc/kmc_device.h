@@ -6,6 +6,7 @@
 typedef __attribute__((ext_vector_type(8))) __bf16 kmc_bf16x8;
 typedef __attribute__((ext_vector_type(16))) float kmc_f32x16;
 typedef __attribute__((ext_vector_type(4))) float kmc_f32x4;
+typedef __attribute__((ext_vector_type(8))) int kmc_i32x8;
 
 namespace kmc {
 
@@ -22,6 +23,34 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t b) {
   return __uint_as_float(((uint32_t)b) << 16);
+}
+
+// fp8 (OCP e4m3: 4-bit significand, exponent bias 7, largest finite 448 =
+// 1.75 * 2^8, subnormal spacing 2^-9) with one power-of-two scale per row.
+// The scale exponent e is the smallest with amax * 2^-e <= 448, so the row's
+// largest entry lands in the top binade and no entry saturates; a zero row
+// takes e = 0.
+__device__ __forceinline__ int fp8_scale_exp(float amax) {
+  if (!(amax > 0.0f)) return 0;
+  int ex;
+  const float f = frexpf(amax, &ex);  // amax = f * 2^ex, f in [0.5, 1)
+  const int e = f <= 0.875f ? ex - 9 : ex - 8;
+  return e < -127 ? -127 : (e > 127 ? 127 : e);
+}
+
+// Round to nearest even onto the e4m3 grid, for |v| <= 448: with E the
+// binade of |v| (at least -6, the subnormal range), the grid spacing is
+// 2^(E-3), m = rint(|v| / 2^(E-3)) lies in [0, 16] and the code is
+// ((E + 7) << 3) + m - 8, which also covers the subnormals (E = -6, m < 8)
+// and the carry into the next binade (m = 16).
+__device__ __forceinline__ uint32_t fp8_e4m3_rne(float v) {
+  const float a = fabsf(v);
+  int ex;
+  (void)frexpf(a, &ex);
+  const int E = a > 0.0f ? max(ex - 1, -6) : -6;
+  const float m = __builtin_rintf(ldexpf(a, 3 - E));
+  const uint32_t code = (uint32_t)(((E + 7) << 3) + (int)m - 8);
+  return code | (v < 0.0f ? 0x80u : 0u);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {

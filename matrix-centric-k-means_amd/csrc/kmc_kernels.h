@@ -6,7 +6,10 @@
 
 namespace kmc {
 
-enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_BF16 = 2 };
+enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_BF16 = 2, PREC_FP8 = 3 };
+// fp8 centroid operand row: DP e4m3 bytes, then the row's E8M0 scale as an
+// int at byte DP, padded to DP + kFp8RowPad bytes
+constexpr int kFp8RowPad = 16;
 enum { UPDATE_ORDERED = 0, UPDATE_TREE = 1, UPDATE_FIXED = 2 };
 constexpr int kTreeRows = 256;  // sorted rows per tree-update chunk
 constexpr int kMaxRegions = 4096;  // assign workgroups (refinement queue regions)
@@ -40,6 +43,7 @@ struct PrepParams {
   float* Cw;       // Kp x DP  (-2c, fp32 form)
   uint16_t* Chi;   // Kp x DP  (bf16 forms)
   uint16_t* Clo;
+  uint8_t* Cq;     // Kp x (DP + kFp8RowPad): e4m3 of -2c * 2^-e, scale e + 127 (fp8 form)
   float* cn;       // Kp, ||c||^2 (kFar for padding rows and excluded clusters)
   float* CT;       // DP x Kp transposed centroids (all-k refinement)
   const int* alive;  // K: 0 = cluster excluded from assignment (+inf)
@@ -66,11 +70,14 @@ struct AssignParams {
   const float* X;       // n x DP
   const uint16_t* Xhi;  // n x DP
   const uint16_t* Xlo;
+  const uint8_t* Xq;    // n x DP e4m3 of x * 2^-e (fp8 form)
+  const int* xsc;       // n, e + 127 (the E8M0 scale of the row)
   const float* xn2;     // n, ||x||^2
   long long n;
   const float* Cw;
   const uint16_t* Chi;
   const uint16_t* Clo;
+  const uint8_t* Cq;
   const float* cn;
   int K, Kp, kchunk;
   uint32_t idx_mask;        // low mantissa bits that carry the centroid index
@@ -195,6 +202,8 @@ struct LoadParams {
   float* X;
   uint16_t* Xhi;
   uint16_t* Xlo;
+  uint8_t* Xq;  // fp8 form: e4m3 plane and per-row scales
+  int* xsc;
   float* xn2;
   double* xn2_64;
 };

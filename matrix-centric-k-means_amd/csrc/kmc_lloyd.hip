@@ -36,6 +36,36 @@ namespace kmc {
 template <class Get>
 __device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, Get get) {
   const size_t row = (size_t)c * P.DP;
+  if (P.prec == PREC_FP8) {
+    // -2c quantised with the row's own power-of-two scale (lane owns
+    // dimensions lane + 64 t)
+    float w[kMaxDP / 64];
+    float s = 0.0f, amax = 0.0f;
+#pragma unroll
+    for (int t = 0; t < kMaxDP / 64; ++t) {
+      const int j = lane + 64 * t;
+      const float v = (j < P.DP && c < P.K) ? get(j) : 0.0f;
+      w[t] = -2.0f * v;  // exact scaling
+      s = fmaf(v, v, s);
+      amax = fmaxf(amax, fabsf(w[t]));
+      if (j < P.DP) P.CT[(size_t)j * P.Kp + c] = v;
+    }
+    amax = wave_max_f32(amax);
+    const int e = fp8_scale_exp(amax);
+    uint8_t* q = P.Cq + (size_t)c * (P.DP + kFp8RowPad);
+#pragma unroll
+    for (int t = 0; t < kMaxDP / 64; ++t) {
+      const int j = lane + 64 * t;
+      if (j < P.DP) q[j] = (uint8_t)fp8_e4m3_rne(ldexpf(w[t], -e));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+      *reinterpret_cast<int*>(q + P.DP) = e + 127;
+      P.cn[c] = (c < P.K && P.alive[c]) ? s : kFar;
+    }
+    return;
+  }
   float s = 0.0f;
   for (int j = lane; j < P.DP; j += 64) {
     const float v = (c < P.K) ? get(j) : 0.0f;
@@ -80,6 +110,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams P) {
 // 16-byte pieces of the row:
 //   fp32  step s : dim = 8*(s>>2) + 4*h + (s&3)          (32x32x2 f32)
 //   bf16  step s : dims 16*s + 8*h + [0,8)               (32x32x16 bf16)
+//   fp8   step s : dims 64*s + 32*h + [0,32)             (32x32x64 e4m3, scaled)
 // Accumulator register r of lane l holds centroid 8*(r>>2) + 4*h + (r&3) of
 // the block against point (l & 31). The accumulator starts at ||c||^2 and the
 // A operand is -2c, so the MFMA result is ||c||^2 - 2 x.c directly.
@@ -106,6 +137,13 @@ struct AssignTraits<DP, PREC_BF16> {
   static constexpr int kPlanes = 1;
 };
 
+template <int DP>
+struct AssignTraits<DP, PREC_FP8> {
+  static constexpr int kRowElems = DP + kFp8RowPad;  // bytes: e4m3 row, then its scale
+  static constexpr int kRowBytes = kRowElems;
+  static constexpr int kPlanes = 1;
+};
+
 template <int DP, int PREC>
 size_t assign_lds_bytes(int kchunk) {
   using T = AssignTraits<DP, PREC>;
@@ -121,12 +159,16 @@ __device__ __forceinline__ void stage_centroids(const AssignParams& P,
   float* cn_s = reinterpret_cast<float*>(smem);
   unsigned char* rows = smem + (size_t)cnt * 4;
   for (int i = threadIdx.x; i < cnt; i += blockDim.x) cn_s[i] = P.cn[c0 + i];
-  constexpr int kVecPerRow = (PREC == PREC_FP32) ? DP / 4 : DP / 8;  // 16-B units
+  constexpr int kVecPerRow = (PREC == PREC_FP32)  ? DP / 4
+                             : (PREC == PREC_FP8) ? (DP + kFp8RowPad) / 16
+                                                  : DP / 8;  // 16-B units
   const int total = cnt * kVecPerRow;
   for (int plane = 0; plane < T::kPlanes; ++plane) {
     const uint4* src;
     if (PREC == PREC_FP32)
       src = reinterpret_cast<const uint4*>(P.Cw + (size_t)c0 * DP);
+    else if (PREC == PREC_FP8)
+      src = reinterpret_cast<const uint4*>(P.Cq + (size_t)c0 * (DP + kFp8RowPad));
     else
       src = reinterpret_cast<const uint4*>((plane == 0 ? P.Chi : P.Clo) + (size_t)c0 * DP);
     unsigned char* dst = rows + (size_t)plane * cnt * T::kRowBytes;
@@ -138,21 +180,36 @@ __device__ __forceinline__ void stage_centroids(const AssignParams& P,
 }
 
 // Register-resident B-operand fragments of one point (this lane's k-slice).
+// fp8: 32 bytes (two uint4) per 64-dimension step, plus the row's scale.
 template <int DP, int PREC>
 struct XFrag {
   static constexpr int NF = (PREC == PREC_FP32) ? DP / 2 : 1;
-  static constexpr int NH = (PREC == PREC_FP32) ? 1 : DP / 16;
+  static constexpr int NH = (PREC == PREC_BF16 || PREC == PREC_BF16X3) ? DP / 16 : 1;
   static constexpr int NL = (PREC == PREC_BF16X3) ? DP / 16 : 1;
+  static constexpr int NQ = (PREC == PREC_FP8) ? DP / 32 : 1;
   float f[NF];
   uint4 hi[NH];
   uint4 lo[NL];
+  uint4 q[NQ];
+  int sc;
 };
 
 template <int DP, int PREC>
 __device__ __forceinline__ void load_xfrag(const AssignParams& P, long long prow, bool valid,
                                            int h, XFrag<DP, PREC>& x) {
   const size_t row = (size_t)(valid ? prow : 0) * DP;
-  if (PREC == PREC_FP32) {
+  if constexpr (PREC == PREC_FP8) {
+    // step s: bytes 64 s + 32 h + [0, 32) of the row
+    const uint8_t* xr = P.Xq + row;
+#pragma unroll
+    for (int s = 0; s < DP / 64; ++s) {
+      x.q[2 * s] = valid ? *reinterpret_cast<const uint4*>(xr + 64 * s + 32 * h)
+                         : make_uint4(0u, 0u, 0u, 0u);
+      x.q[2 * s + 1] = valid ? *reinterpret_cast<const uint4*>(xr + 64 * s + 32 * h + 16)
+                             : make_uint4(0u, 0u, 0u, 0u);
+    }
+    x.sc = valid ? P.xsc[prow] : 127;
+  } else if (PREC == PREC_FP32) {
     const float* xr = P.X + row;
 #pragma unroll
     for (int q = 0; q < DP / 8; ++q) {
@@ -182,7 +239,10 @@ __device__ __forceinline__ void load_xfrag(const AssignParams& P, long long prow
 // Registers per lane of one point set's fragments.
 template <int DP, int PREC>
 constexpr int frag_regs() {
-  return PREC == PREC_FP32 ? DP / 2 : (PREC == PREC_BF16X3 ? DP / 2 : DP / 4);
+  return PREC == PREC_FP32     ? DP / 2
+         : PREC == PREC_BF16X3 ? DP / 2
+         : PREC == PREC_FP8    ? DP / 8 + 1
+                               : DP / 4;
 }
 
 // Register-resident A-operand fragments of one 32-centroid block (lane l:
@@ -191,11 +251,14 @@ constexpr int frag_regs() {
 template <int DP, int PREC>
 struct AFrag {
   static constexpr int NF = (PREC == PREC_FP32) ? DP / 8 : 1;
-  static constexpr int NH = (PREC == PREC_FP32) ? 1 : DP / 16;
+  static constexpr int NH = (PREC == PREC_BF16 || PREC == PREC_BF16X3) ? DP / 16 : 1;
   static constexpr int NL = (PREC == PREC_BF16X3) ? DP / 16 : 1;
+  static constexpr int NQ = (PREC == PREC_FP8) ? DP / 32 : 1;
   float4 f[NF];
   uint4 hi[NH];
   uint4 lo[NL];
+  uint4 q[NQ];
+  int sc;
 };
 
 template <int DP, int PREC>
@@ -203,7 +266,14 @@ __device__ __forceinline__ void load_afrag(const unsigned char* rows, int cnt, i
                                            int h, AFrag<DP, PREC>& a) {
   using T = AssignTraits<DP, PREC>;
   const unsigned char* arow = rows + (size_t)(cb + p) * T::kRowBytes;
-  if (PREC == PREC_FP32) {
+  if constexpr (PREC == PREC_FP8) {
+#pragma unroll
+    for (int s = 0; s < DP / 64; ++s) {
+      a.q[2 * s] = *reinterpret_cast<const uint4*>(arow + 64 * s + 32 * h);
+      a.q[2 * s + 1] = *reinterpret_cast<const uint4*>(arow + 64 * s + 32 * h + 16);
+    }
+    a.sc = *reinterpret_cast<const int*>(arow + DP);
+  } else if (PREC == PREC_FP32) {
     const float* af = reinterpret_cast<const float*>(arow);
 #pragma unroll
     for (int q = 0; q < DP / 8; ++q) a.f[q] = *reinterpret_cast<const float4*>(af + 8 * q + 4 * h);
@@ -236,10 +306,27 @@ __device__ __forceinline__ kmc_f32x16 init_acc(const float* cn_s, int cb, int h,
 
 // The MFMA chain of one block against one point set: acc += (-2c) . x
 // (bf16x3: hi.hi + hi.lo + lo.hi).
+// One scaled fp8 step: 32x32x64 e4m3 (format code 0 for both operands), the
+// per-lane E8M0 scales of the centroid row and of the point applied by the
+// matrix core, so the product is (2^ea * a) . (2^eb * b).
+__device__ __forceinline__ kmc_f32x16 mfma_fp8_step(uint4 a0, uint4 a1, int sa, uint4 b0,
+                                                    uint4 b1, int sb, kmc_f32x16 acc) {
+  const kmc_i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
+                        (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+  const kmc_i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
+                        (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, sa, 0, sb);
+}
+
 template <int DP, int PREC>
 __device__ __forceinline__ kmc_f32x16 mfma_chain(const AFrag<DP, PREC>& a,
                                                  const XFrag<DP, PREC>& x, kmc_f32x16 acc) {
-  if (PREC == PREC_FP32) {
+  if constexpr (PREC == PREC_FP8) {
+#pragma unroll
+    for (int s = 0; s < DP / 64; ++s)
+      acc = mfma_fp8_step(a.q[2 * s], a.q[2 * s + 1], a.sc, x.q[2 * s], x.q[2 * s + 1], x.sc,
+                          acc);
+  } else if (PREC == PREC_FP32) {
 #pragma unroll
     for (int q = 0; q < DP / 8; ++q) {
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.f[q].x, x.f[4 * q + 0], acc, 0, 0, 0);
@@ -274,7 +361,15 @@ __device__ __forceinline__ kmc_f32x16 mfma_chain_lds(const unsigned char* rows, 
                                                      kmc_f32x16 acc) {
   using T = AssignTraits<DP, PREC>;
   const unsigned char* arow = rows + (size_t)(cb + p) * T::kRowBytes;
-  if (PREC == PREC_FP32) {
+  if constexpr (PREC == PREC_FP8) {
+    const int sa = *reinterpret_cast<const int*>(arow + DP);
+#pragma unroll
+    for (int s = 0; s < DP / 64; ++s) {
+      const uint4 a0 = *reinterpret_cast<const uint4*>(arow + 64 * s + 32 * h);
+      const uint4 a1 = *reinterpret_cast<const uint4*>(arow + 64 * s + 32 * h + 16);
+      acc = mfma_fp8_step(a0, a1, sa, x.q[2 * s], x.q[2 * s + 1], x.sc, acc);
+    }
+  } else if (PREC == PREC_FP32) {
     const float* af = reinterpret_cast<const float*>(arow);
 #pragma unroll
     for (int q = 0; q < DP / 8; ++q) {
@@ -357,8 +452,8 @@ __device__ __forceinline__ uint32_t opaque_vgpr(uint32_t s) {
 // autotuner (kmc_autotune); the LDS copy limits a CU to one workgroup when
 // all centroids are resident, so NW sets the waves per SIMD.
 //
-// Single-product bf16 accumulates ||c||^2 - 2 x.c without ||x||^2 (the
-// ranking does not depend on it) and adds ||x||^2 to the four candidates
+// Single-product bf16 and fp8 accumulate ||c||^2 - 2 x.c without ||x||^2
+// (the ranking does not depend on it) and add ||x||^2 to the four candidates
 // after the sweep: one VALU op less per value. The packing perturbation is
 // then relative to | ||c||^2 - 2 x.c | <= d^2 + ||x||^2, which the band
 // covers through the xpk * ||x||^2 term.
@@ -375,7 +470,7 @@ void k_assign(AssignParams P) {
   const int p = lane & 31;
   const int h = lane >> 5;
   constexpr int kTile = 32 * MS * NW;  // points per workgroup tile
-  constexpr bool kAddX2 = PREC != PREC_BF16;
+  constexpr bool kAddX2 = PREC != PREC_BF16 && PREC != PREC_FP8;
   constexpr bool kLean = WPE > 4;  // occupancy above four waves per SIMD
   if (threadIdx.x == 0) {
     q2c_s = 0u;
@@ -1890,6 +1985,18 @@ __global__ __launch_bounds__(256) void k_load_points(LoadParams P) {
   const long long i = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= P.n) return;
   const float* src = P.src + (size_t)i * P.ld;
+  if (P.Xq) {
+    // fp8 plane: the row quantised with its own power-of-two scale
+    float amax = 0.0f;
+    for (int j = lane; j < P.D; j += 64) amax = fmaxf(amax, fabsf(src[j]));
+    amax = wave_max_f32(amax);
+    const int e = fp8_scale_exp(amax);
+    for (int j = lane; j < P.DP; j += 64) {
+      const float v = j < P.D ? src[j] : 0.0f;
+      P.Xq[(size_t)i * P.DP + j] = (uint8_t)fp8_e4m3_rne(ldexpf(v, -e));
+    }
+    if (lane == 0) P.xsc[i] = e + 127;
+  }
   float s = 0.0f;
   double s64 = 0.0;
   for (int j = lane; j < P.DP; j += 64) {
@@ -1958,6 +2065,7 @@ size_t assign_lds(int DP, int prec, int kchunk) {
   case dp:                                                                           \
     return prec == PREC_FP32     ? assign_lds_bytes<dp, PREC_FP32>(kchunk)           \
            : prec == PREC_BF16X3 ? assign_lds_bytes<dp, PREC_BF16X3>(kchunk)         \
+           : prec == PREC_FP8    ? assign_lds_bytes<dp, PREC_FP8>(kchunk)            \
                                  : assign_lds_bytes<dp, PREC_BF16>(kchunk);
   switch (DP) {
     KMC_LDS_CASE(16)
@@ -2017,13 +2125,22 @@ static const void* kernel_of_v(int v) {
 
 template <int PREC>
 static const void* kernel_of_dv(int DP, int v) {
-  switch (DP) {
-    case 16: return kernel_of_v<16, PREC>(v);
-    case 32: return kernel_of_v<32, PREC>(v);
-    case 64: return kernel_of_v<64, PREC>(v);
-    case 128: return kernel_of_v<128, PREC>(v);
-    case 256: return kernel_of_v<256, PREC>(v);
-    default: return nullptr;
+  if constexpr (PREC == PREC_FP8) {  // one fp8 step covers 64 dimensions
+    switch (DP) {
+      case 64: return kernel_of_v<64, PREC>(v);
+      case 128: return kernel_of_v<128, PREC>(v);
+      case 256: return kernel_of_v<256, PREC>(v);
+      default: return nullptr;
+    }
+  } else {
+    switch (DP) {
+      case 16: return kernel_of_v<16, PREC>(v);
+      case 32: return kernel_of_v<32, PREC>(v);
+      case 64: return kernel_of_v<64, PREC>(v);
+      case 128: return kernel_of_v<128, PREC>(v);
+      case 256: return kernel_of_v<256, PREC>(v);
+      default: return nullptr;
+    }
   }
 }
 
@@ -2032,6 +2149,7 @@ const void* assign_kernel_ptr(int DP, int prec, int v) {
     case PREC_FP32: return kernel_of_dv<PREC_FP32>(DP, v);
     case PREC_BF16X3: return kernel_of_dv<PREC_BF16X3>(DP, v);
     case PREC_BF16: return kernel_of_dv<PREC_BF16>(DP, v);
+    case PREC_FP8: return kernel_of_dv<PREC_FP8>(DP, v);
     default: return nullptr;
   }
 }

@@ -64,6 +64,7 @@ void Kmeans::init(const int* seed) {
   cfg.tol = tol;
   cfg.precision = opts.precision == Precision::fp32     ? KMC_PREC_FP32
                   : opts.precision == Precision::bf16x3 ? KMC_PREC_BF16X3
+                  : opts.precision == Precision::fp8    ? KMC_PREC_FP8
                                                         : KMC_PREC_BF16;
   cfg.update = KMC_UPDATE_ORDERED;
   cfg.device = deviceProps ? deviceProps->ordinal : 0;

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libkmc.so")
 OK, ERR_ARG, ERR_NO_DEVICE, ERR_OUT_OF_BOUNDS, ERR_INFILE, ERR_HIP, ERR_OOM, \
     ERR_STATE, ERR_UNSUPPORTED = range(9)
 
-PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16": 2}
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16": 2, "fp8": 3}
 UPDATES = {"ordered": 0, "tree": 1, "fixed": 2}
 RUN_CHECK_CONVERGED = 0x1
 RUN_TIME_KERNELS = 0x2

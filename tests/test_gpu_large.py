@@ -15,7 +15,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-PRECS = ("fp32", "bf16x3", "bf16")
+PRECS = ("fp32", "bf16x3", "bf16", "fp8")
 N, D, K = 100_000, 256, 4096
 
 
@@ -98,6 +98,6 @@ def test_large_k_lloyd_forms_agree(kmc_mod, c5_slice):
             r = e.run(3, check_converged=False, graph=True)
             res[prec] = (e.labels(), e.centroids(), r)
             _say("large-K run", prec, r)
-    for prec in ("bf16x3", "bf16"):
+    for prec in ("bf16x3", "bf16", "fp8"):
         assert np.array_equal(res[prec][0], res["fp32"][0]), prec
         assert np.array_equal(res[prec][1], res["fp32"][1]), prec

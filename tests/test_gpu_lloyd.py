@@ -14,7 +14,7 @@ from conftest import load_golden
 
 pytestmark = pytest.mark.gpu
 
-PRECS = ("fp32", "bf16x3", "bf16")
+PRECS = ("fp32", "bf16x3", "bf16", "fp8")
 
 
 def _say(*a):
