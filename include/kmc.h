@@ -95,6 +95,18 @@ typedef enum {
                                         default it becomes the zero vector and
                                         stays selectable, as in the reference
                                         CPU k-means                           */
+/* The explicit-Lloyd rules of the reference's RAFT benchmark
+ * (experiments/raft-bench/src/kmeans.cu:28-265):                            */
+#define KMC_RUN_CONVERGE_SHIFT_SUM 0x40u /* with KMC_RUN_CHECK_CONVERGED: stop
+                                        when sum_c ||c_new - c_old||^2 < tol,
+                                        tested every iteration (kmeans.cu:
+                                        168-212)                              */
+#define KMC_RUN_COST_RATIO 0x80u     /* stop when cost_t / cost_{t-1} > 1 - tol
+                                        from iteration 2, cost = FP64 inertia of
+                                        the iteration's assignment (its
+                                        inertia_check, kmeans.cu:186-204)    */
+#define KMC_RUN_KEEP_EMPTY 0x100u    /* a cluster left empty keeps its
+                                        centroid (RAFT's update_centroids)    */
 
 typedef struct {
   size_t n;          /* points                                   */

@@ -5,6 +5,7 @@
 Outputs (in-tree, git-ignored, shipped with the tree):
     matrix-centric-k-means_amd/lib/libkmc.so     C ABI of include/kmc.h
     matrix-centric-k-means_amd/bin/gpukmeans     CLI with the reference's flags
+    matrix-centric-k-means_amd/bin/raftbench     the explicit-Lloyd benchmark program
 
 Every HIP source is compiled for gfx950 only, with packed-FP32 instruction
 selection disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
@@ -68,6 +69,13 @@ def build(jobs=4, verbose=False):
         exe = os.path.join(BIN, "gpukmeans")
         _run([CXX, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + INCLUDE] + host_srcs +
              ["-L" + LIB, "-lkmc", "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"])
+        os.replace(exe + ".tmp", exe)
+    # the explicit-Lloyd benchmark program of the reference's harness
+    rb = os.path.join(CSRC, "raftbench.cpp")
+    if os.path.exists(rb):
+        exe = os.path.join(BIN, "raftbench")
+        _run([CXX, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + INCLUDE, rb,
+              "-L" + LIB, "-lkmc", "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"])
         os.replace(exe + ".tmp", exe)
     return so
 

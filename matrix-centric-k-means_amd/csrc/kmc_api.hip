@@ -85,7 +85,7 @@ struct kmc_handle {
   // captured iteration chunks (KMC_RUN_GRAPH), reused by later runs
   struct GraphEntry {
     uint64_t iters, ev_off;
-    int mode, exclude, variant, timed;
+    int mode, rules, variant, timed;
     hipGraphExec_t exec;
   };
   std::vector<GraphEntry> graphs;
@@ -383,7 +383,7 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
 }
 
 // u0/u1: events around the centroid-update kernel (timed iterations)
-kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty, hipEvent_t u0 = nullptr,
+kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nullptr,
                           hipEvent_t u1 = nullptr) {
   kmc::SortParams sp{};
   sp.labels = h->labels;
@@ -444,7 +444,8 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty, hipEvent_t
   fp.K = h->K;
   fp.D = h->D;
   fp.DP = h->DP;
-  fp.exclude_empty = exclude_empty;
+  fp.exclude_empty = (rules & kmc::RULE_EXCLUDE_EMPTY) ? 1 : 0;
+  fp.keep_empty = (rules & kmc::RULE_KEEP_EMPTY) ? 1 : 0;
   fp.do_prep = 1;
   fp.prep = prep_params(h);
   fp.fixed = fixed ? 1 : 0;
@@ -472,13 +473,14 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int exclude_empty, hipEvent_t
   st.tol = h->cfg.tol;
   st.K = h->K;
   st.mode = mode;
+  st.cost_ratio = (rules & kmc::RULE_COST_RATIO) ? 1 : 0;
   KMC_CK(kmc::launch_finish(fp, h->stream));
   return KMC_OK;
 }
 
 // m iterations starting at global iteration `first`; events around the
 // distance kernel and the update kernel for iterations below ntimed.
-kmc_status enqueue_chunk(kmc_handle* h, uint64_t first, uint64_t m, int mode, int exclude,
+kmc_status enqueue_chunk(kmc_handle* h, uint64_t first, uint64_t m, int mode, int rules,
                          uint64_t ntimed) {
   for (uint64_t it = 0; it < m; ++it) {
     const uint64_t gi = first + it;
@@ -486,7 +488,7 @@ kmc_status enqueue_chunk(kmc_handle* h, uint64_t first, uint64_t m, int mode, in
     hipEvent_t* ev = tm ? &h->events[4 * gi] : nullptr;
     kmc_status st = enqueue_assign(h, tm ? ev[0] : nullptr, tm ? ev[1] : nullptr, it == 0);
     if (st == KMC_OK)
-      st = enqueue_update(h, mode, exclude, tm ? ev[2] : nullptr, tm ? ev[3] : nullptr);
+      st = enqueue_update(h, mode, rules, tm ? ev[2] : nullptr, tm ? ev[3] : nullptr);
     if (st != KMC_OK) return st;
   }
   return KMC_OK;
@@ -495,19 +497,19 @@ kmc_status enqueue_chunk(kmc_handle* h, uint64_t first, uint64_t m, int mode, in
 // Launch m iterations as one graph; the graph is captured on first use and
 // kept for later runs with the same shape. Returns KMC_ERR_UNSUPPORTED when
 // the stream cannot be captured (the caller then launches eagerly).
-kmc_status launch_chunk_graph(kmc_handle* h, uint64_t first, uint64_t m, int mode, int exclude,
+kmc_status launch_chunk_graph(kmc_handle* h, uint64_t first, uint64_t m, int mode, int rules,
                               uint64_t ntimed) {
   const int timed = first < ntimed ? 1 : 0;
   for (auto& g : h->graphs) {
     if (g.iters == m && g.ev_off == (timed ? first : 0) && g.mode == mode &&
-        g.exclude == exclude && g.variant == h->variant && g.timed == timed) {
+        g.rules == rules && g.variant == h->variant && g.timed == timed) {
       KMC_CK(hipGraphLaunch(g.exec, h->stream));
       return KMC_OK;
     }
   }
   if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return KMC_ERR_UNSUPPORTED;
-  kmc_status st = enqueue_chunk(h, first, m, mode, exclude, timed ? ntimed : 0);
+  kmc_status st = enqueue_chunk(h, first, m, mode, rules, timed ? ntimed : 0);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(h->stream, &graph);
   if (st != KMC_OK || ec != hipSuccess || !graph) {
@@ -522,7 +524,7 @@ kmc_status launch_chunk_graph(kmc_handle* h, uint64_t first, uint64_t m, int mod
     (void)hipGetLastError();
     return KMC_ERR_UNSUPPORTED;
   }
-  h->graphs.push_back({m, timed ? first : 0, mode, exclude, h->variant, timed, exec});
+  h->graphs.push_back({m, timed ? first : 0, mode, rules, h->variant, timed, exec});
   KMC_CK(hipGraphLaunch(exec, h->stream));
   return KMC_OK;
 }
@@ -889,7 +891,16 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
   const bool check = (flags & KMC_RUN_CHECK_CONVERGED) != 0;
   const bool timing = (flags & KMC_RUN_TIME_KERNELS) != 0;
   const bool exclude = (flags & KMC_RUN_EXCLUDE_EMPTY) != 0;
-  const int mode = !check ? kmc::CONV_NONE : (score_mode ? kmc::CONV_SCORE : kmc::CONV_SHIFT);
+  const bool sum_mode = (flags & KMC_RUN_CONVERGE_SHIFT_SUM) != 0;
+  const int mode = !check     ? kmc::CONV_NONE
+                   : sum_mode ? kmc::CONV_SHIFT_SUM
+                   : score_mode ? kmc::CONV_SCORE
+                                : kmc::CONV_SHIFT;
+  const int rules = (exclude ? kmc::RULE_EXCLUDE_EMPTY : 0) |
+                    ((flags & KMC_RUN_KEEP_EMPTY) ? kmc::RULE_KEEP_EMPTY : 0) |
+                    ((flags & KMC_RUN_COST_RATIO) ? kmc::RULE_COST_RATIO : 0);
+  // the cost-ratio test needs the host to look at the stop flag as well
+  const bool watch = check || (flags & KMC_RUN_COST_RATIO) != 0;
   if (!exclude) KMC_CK(kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream));
   const uint64_t kMaxTimed = 4096;
   const uint64_t ntimed = timing ? std::min<uint64_t>(maxiter, kMaxTimed) : 0;
@@ -906,19 +917,19 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
   DevState s{};
   bool use_graph = (flags & KMC_RUN_GRAPH) != 0;
   while (launched < maxiter) {
-    const uint64_t m = check ? std::min<uint64_t>(chunk, maxiter - launched) : maxiter - launched;
+    const uint64_t m = watch ? std::min<uint64_t>(chunk, maxiter - launched) : maxiter - launched;
     if (use_graph) {
-      st = launch_chunk_graph(h, launched, m, mode, exclude ? 1 : 0, ntimed);
+      st = launch_chunk_graph(h, launched, m, mode, rules, ntimed);
       if (st == KMC_ERR_UNSUPPORTED) {
         use_graph = false;  // capture refused: launch eagerly from here on
         st = KMC_OK;
       }
       if (st != KMC_OK) break;
     }
-    if (!use_graph) st = enqueue_chunk(h, launched, m, mode, exclude ? 1 : 0, ntimed);
+    if (!use_graph) st = enqueue_chunk(h, launched, m, mode, rules, ntimed);
     if (st != KMC_OK) break;
     launched += m;
-    if (check) {
+    if (watch) {
       st = read_state(h, &s);
       if (st != KMC_OK) break;
       if (s.done) break;

@@ -33,10 +33,14 @@ struct DevState {
   unsigned long long refined_all;
   double sum_x2;                    // FP64 sum_i ||x_i||^2 (set-up)
   double last_score;                // score of the previous iteration
+  double sum_shift2;                // FP64 sum_c ||c_new - c_old||^2
+  double prior_cost;                // FP64 inertia of the previous assignment
 };
 
 // convergence tests of the k_stats kernel
-enum { CONV_NONE = 0, CONV_SHIFT = 1, CONV_SCORE = 2, CONV_STABLE = 3 };
+enum { CONV_NONE = 0, CONV_SHIFT = 1, CONV_SCORE = 2, CONV_STABLE = 3, CONV_SHIFT_SUM = 4 };
+// per-run rules of the update and stop test (bit mask)
+enum { RULE_EXCLUDE_EMPTY = 1, RULE_KEEP_EMPTY = 2, RULE_COST_RATIO = 4 };
 
 struct PrepParams {
   const float* C;  // K x DP master centroids
@@ -164,7 +168,8 @@ struct StatsParams {
   DevState* state;
   const int* done;
   float tol;
-  int K, mode;  // mode: CONV_*
+  int K, mode;     // mode: CONV_*
+  int cost_ratio;  // also stop when cost_t / cost_{t-1} > 1 - tol (t >= 2)
 };
 
 struct FinishParams {
@@ -180,6 +185,7 @@ struct FinishParams {
   const int* done;
   float tol;
   int K, D, DP, exclude_empty;
+  int keep_empty;     // an empty cluster keeps its centroid
   int do_prep;        // write the next iteration's distance operands (prep)
   // fixed-point update: the partials to add; count and inertia are written
   int fixed;

@@ -1711,7 +1711,7 @@ __device__ void stats_reduce(const StatsParams& P) {
   __shared__ int sc[kStatsThreads];
   __shared__ unsigned long long sq2[kStatsThreads], sqa[kStatsThreads];
   const int t = threadIdx.x;
-  double in = 0.0, mx = 0.0;
+  double in = 0.0, mx = 0.0, sh = 0.0;
   int all = 1;
   unsigned long long r2 = 0, ra = 0;
   // eight loads in flight per thread and round (fixed order: deterministic)
@@ -1751,11 +1751,14 @@ __device__ void stats_reduce(const StatsParams& P) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       mx = fmax(mx, v[u]);
+      sh += v[u];
       all &= cv[u];
     }
   }
+  __shared__ double ss[kStatsThreads];
   sd[t] = in;
   sm[t] = mx;
+  ss[t] = sh;
   sc[t] = all;
   sq2[t] = r2;
   sqa[t] = ra;
@@ -1763,6 +1766,7 @@ __device__ void stats_reduce(const StatsParams& P) {
   for (int o = kStatsThreads / 2; o > 0; o >>= 1) {
     if (t < o) {
       sd[t] += sd[t + o];
+      ss[t] += ss[t + o];
       sm[t] = fmax(sm[t], sm[t + o]);
       sc[t] &= sc[t + o];
       sq2[t] += sq2[t + o];
@@ -1776,6 +1780,7 @@ __device__ void stats_reduce(const StatsParams& P) {
     s->iter = it;
     s->inertia = sd[0];
     s->max_shift2 = sm[0];
+    s->sum_shift2 = ss[0];
     s->refined_two = sq2[0];
     s->refined_all = sqa[0];
     const double score = sd[0] - s->sum_x2;
@@ -1792,6 +1797,20 @@ __device__ void stats_reduce(const StatsParams& P) {
       s->done = 1;
     }
     s->last_score = score;
+    if (P.mode == CONV_SHIFT_SUM && ss[0] < (double)P.tol && !s->done) {
+      // experiments/raft-bench/src/kmeans.cu:168-212: sum over all centroid
+      // entries of the squared change, tested every iteration
+      s->converged_at = it;
+      s->done = 1;
+    }
+    if (P.cost_ratio && it >= 2 && !s->done && s->prior_cost > 0.0 &&
+        sd[0] / s->prior_cost > 1.0 - (double)P.tol) {
+      // experiments/raft-bench/src/kmeans.cu:186-204: the clustering cost
+      // stopped falling by more than a tol fraction
+      s->converged_at = it;
+      s->done = 1;
+    }
+    s->prior_cost = sd[0];
     if (it >= s->maxiter) s->done = 1;
   }
 }
@@ -1925,6 +1944,14 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
     for (int j = lane; j < P.D; j += 64) ns[j] = nrow[j];
   }
   if (wave == 0) {
+    if (P.keep_empty && cnt == 0) {
+      // an empty cluster keeps its centroid (the explicit-Lloyd rule of
+      // experiments/raft-bench/src/kmeans.cu:158-166)
+      for (int j = lane; j < P.D; j += 64) {
+        ns[j] = cr[j];
+        nrow[j] = cr[j];
+      }
+    }
     __builtin_amdgcn_wave_barrier();
     // the squared terms of the float shift are independent: form them
     // lane-parallel (each is one rounded difference and one rounded product,

@@ -99,6 +99,11 @@ RUN_CONVERGE_SCORE = 0x4
 RUN_EXCLUDE_EMPTY = 0x8
 RUN_STOP_STABLE = 0x10
 RUN_GRAPH = 0x20
+RUN_CONVERGE_SHIFT_SUM = 0x40
+RUN_COST_RATIO = 0x80
+RUN_KEEP_EMPTY = 0x100
+CONVERGE = {"shift": 0, "score": RUN_CONVERGE_SCORE, "shift_sum": RUN_CONVERGE_SHIFT_SUM}
+EMPTY = {"zero": 0, "exclude": RUN_EXCLUDE_EMPTY, "keep": RUN_KEEP_EMPTY}
 
 _lib = None
 
@@ -238,9 +243,18 @@ class Engine:
                                                 out.ctypes.data), "init_plusplus")
         return out
 
-    def run(self, maxiter, check_converged=True, time_kernels=False, graph=False):
+    def run(self, maxiter, check_converged=True, time_kernels=False, graph=False,
+            converge="shift", cost_ratio=False, empty="zero"):
+        """Lloyd iterations. converge: "shift" (max per-centroid shift <= tol,
+        from iteration 2: the reference CPU k-means), "score" (|score change|
+        < tol: the reference GPU engine) or "shift_sum" (sum of squared
+        centroid changes < tol: the explicit-Lloyd benchmark); cost_ratio:
+        also stop when cost_t / cost_{t-1} > 1 - tol; empty: what an empty
+        cluster becomes ("zero" vector, "exclude"d from later assignments,
+        or "keep" its centroid)."""
         flags = (RUN_CHECK_CONVERGED if check_converged else 0) | \
-                (RUN_TIME_KERNELS if time_kernels else 0) | (RUN_GRAPH if graph else 0)
+                (RUN_TIME_KERNELS if time_kernels else 0) | (RUN_GRAPH if graph else 0) | \
+                CONVERGE[converge] | EMPTY[empty] | (RUN_COST_RATIO if cost_ratio else 0)
         r = _Result()
         self._check(self._lib.kmc_run(self._h, int(maxiter), flags, ctypes.byref(r)), "run")
         return r.as_dict()

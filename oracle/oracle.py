@@ -122,6 +122,37 @@ class Oracle:
         self.L.orc_plusplus(_ptr(X), X.shape[0], X.shape[1], k, _ptr(u), _ptr(chosen), _ptr(C))
         return chosen, C
 
+    def lloyd_explicit(self, X, C0, tol, maxiter, check):
+        """The explicit-Lloyd rules of the reference's RAFT benchmark
+        (experiments/raft-bench/src/kmeans.cu:95-212), over this oracle's
+        exact labels (the reference CPU float direct form, first minimum) and
+        FP64 means rounded once: per iteration t, cost_t = FP64 inertia of the
+        assignment, an empty cluster keeps its centroid, then stop when
+        (check and sum of squared centroid changes < tol) or (t >= 2 and
+        cost_t / cost_{t-1} > 1 - tol). Returns (stop iteration or maxiter,
+        labels, C, list of costs)."""
+        X = np.ascontiguousarray(X, np.float32)
+        C = np.array(C0, dtype=np.float32, copy=True)
+        k = C.shape[0]
+        prior = 0.0
+        costs = []
+        lab = None
+        for t in range(1, int(maxiter) + 1):
+            lab, _ = self.argmin(self.distances(X, C))
+            cost = self.f64_inertia(X, C, lab)
+            costs.append(cost)
+            Cn, cnt = self.f64_centroids(X, lab, k)
+            Cn[cnt == 0] = C[cnt == 0]
+            shift = float(((Cn.astype(np.float64) - C.astype(np.float64)) ** 2).sum())
+            C = Cn
+            done = bool(check) and shift < tol
+            if t >= 2 and prior > 0.0 and cost / prior > 1.0 - tol:
+                done = True
+            prior = cost
+            if done:
+                return t, lab, C, costs
+        return int(maxiter), lab, C, costs
+
     def f64_inertia(self, X, C, labels):
         X = np.ascontiguousarray(X, np.float32)
         C = np.ascontiguousarray(C, np.float32)

@@ -233,3 +233,30 @@ def test_plusplus_first_pick_and_distinct(oracle):
     v = splitmix_uniforms(0, 2)
     assert v[0] == (0xE220A8397B1DCDAF >> 11) * 2.0 ** -53
     assert v[1] == (0x6E789E6AA1B965F4 >> 11) * 2.0 ** -53
+
+
+def test_explicit_lloyd_known_answers(oracle):
+    """Hand-worked runs of the explicit-Lloyd rules (stop on the sum of
+    squared centroid changes, stop on the cost ratio, empty cluster keeps its
+    centroid), experiments/raft-bench/src/kmeans.cu:158-212."""
+    X = np.array([[0.0], [1.0], [10.0], [11.0]], np.float32)
+    C0 = np.array([[0.0], [1.0]], np.float32)
+    # t=1: labels 0,1,1,1 -> C = (0, 22/3), cost 181; t=2: labels 0,0,1,1 ->
+    # C = (0.5, 10.5), cost 1 + (10 - 22/3)^2 + (11 - 22/3)^2; t=3: same labels, C
+    # unchanged (shift sum 0), cost 1; t=4: cost 1 again, ratio 1
+    t, lab, C, costs = oracle.lloyd_explicit(X, C0, 1e-4, 50, check=True)
+    assert t == 3
+    assert np.array_equal(lab, np.array([0, 0, 1, 1], np.uint32))
+    assert np.array_equal(C, np.array([[0.5], [10.5]], np.float32))
+    c1 = float(np.float32(22.0 / 3.0))  # the float centroid of iteration 1
+    assert costs[0] == 181.0 and abs(costs[1] - (1 + (10 - c1) ** 2 + (11 - c1) ** 2)) < 1e-12
+    assert costs[2] == 1.0
+    t, _, _, costs = oracle.lloyd_explicit(X, C0, 1e-4, 50, check=False)
+    assert t == 4 and costs[3] == costs[2] == 1.0
+    # the second cluster is empty from the start and keeps its centroid
+    X2 = np.array([[0.0], [1.0], [2.0]], np.float32)
+    C2 = np.array([[1.0], [100.0]], np.float32)
+    t, lab, C, _ = oracle.lloyd_explicit(X2, C2, 1e-4, 50, check=True)
+    assert np.array_equal(lab, np.zeros(3, np.uint32))
+    assert np.array_equal(C, np.array([[1.0], [100.0]], np.float32))
+    assert t == 1  # nothing moved at iteration 1
