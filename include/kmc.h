@@ -243,15 +243,24 @@ typedef enum { KMC_STORE_FP32 = 0, KMC_STORE_BF16 = 1 } kmc_kstore;
                                      k-means): stop when no label changes,
                                      instead of the score test              */
 
+/* How S = K V^T (the kernel-space inner products) is formed each iteration. */
+typedef enum {
+  KMC_KK_AUTO = 0,    /* cluster-sorted (measured faster at every k)      */
+  KMC_KK_DENSE = 1,   /* one-hot MFMA product, k <= 64                    */
+  KMC_KK_SORTED = 2   /* cluster-sorted row sums (the reference's REORDER
+                         level, src/kmeans.cu:1125-1261), k <= 1024      */
+} kmc_kk_product;
+
 typedef struct {
   size_t n;
   uint32_t d;
-  uint32_t k;        /* clusters, at most 64                         */
+  uint32_t k;        /* clusters: at most 64 dense, 1024 sorted       */
   float tol;         /* score tolerance (src/kmeans.cu:1068-1073)    */
   int32_t kernel;    /* kmc_kernel                                   */
   int32_t store;     /* kmc_kstore                                   */
   float gamma;       /* RBF width; <= 0 selects 1/d                  */
   int32_t device;
+  int32_t product;   /* kmc_kk_product                               */
 } kmc_kk_config;
 
 typedef struct {

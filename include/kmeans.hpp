@@ -9,6 +9,9 @@
 //            Point::setCluster (src/kmeans.cu:1101-1103)
 //   float get_score() const                          (src/kmeans.cuh:138)
 //   enums InitMethod{random, plus_plus}, Kernel{linear, polynomial, sigmoid}
+//   a non-linear kernel runs kernel k-means on the n x n kernel matrix (the
+//   reference's Popcorn path, src/kmeans.cu:143-171,772-1122): initial labels
+//   i % k, score test with check_converged; the linear kernel runs Lloyd
 // Changed:
 //   the device-properties argument is a DeviceInfo* (kmc_query_device), not a
 //   cudaDeviceProp*; errors throw kmc::Error instead of exiting the process
@@ -97,6 +100,7 @@ class Kmeans {
   int level;
   Options opts;
   kmc_handle* h = nullptr;
+  kmc_kk_handle* kh = nullptr;  // kernel k-means (non-linear kernels)
   float score = 0.0f;
   double inertia = 0.0;
   uint64_t iterations = 0;

@@ -10,7 +10,11 @@ namespace kmc {
 
 enum { KK_LINEAR = 0, KK_POLY = 1, KK_SIGMOID = 2, KK_RBF = 3 };
 enum { KS_FP32 = 0, KS_BF16 = 1 };
-constexpr int KK_MAX_K = 64;
+// S = K H products: dense one-hot MFMA product, or cluster-sorted row sums
+enum { KK_DENSE = 0, KK_SORTED = 1 };
+constexpr int KK_MAX_K = 1024;       // clusters (cluster-sorted product)
+constexpr int KK_DENSE_MAX_K = 64;   // clusters of the dense product
+constexpr int kKKRows = 256;         // sorted rows per chunk of the sorted product
 
 struct KKState {
   unsigned long long iter, maxiter, converged_at;
@@ -37,8 +41,21 @@ struct KKParams {
   void* HT;          // Kp x Np one-hot (transposed), fp32 or bf16
   int* labels;       // n
   int* count;        // Kp
-  float* S;          // n x Kp
+  float* S;          // dense: n x Kp (S[i*Kp + c]); sorted: Kp x Np (S[c*Np + i])
+  long long sS_i, sS_c;  // strides of S along points and clusters
   int K, Kp;
+  int product;       // KK_DENSE or KK_SORTED
+  // cluster-sorted product: the stable sort of the points by label, chunk
+  // partial sums of K rows (row c + q for cluster c in chunk q), per-tile
+  // self-term partials
+  int* bcount;
+  int* start;
+  int* members;
+  int CH, nblk, lbits;
+  float* tpart;      // (nq + K) x Np
+  int nq;
+  double* gtile;     // K x ntile
+  int ntile;
   float* gpart;      // ceil(n/16) x Kp
   double* g;         // Kp
   double* opart;     // nassign

@@ -48,7 +48,8 @@ hipError_t dalloc(T** p, size_t count) {
 void free_kk(kmc_kk_handle* h) {
   kmc::KKParams& P = h->P;
   void* ptrs[] = {h->dX, P.Kmat, P.xn2, P.kdiag, P.HT, P.labels, P.count, P.S,
-                  P.gpart, P.g, P.opart, P.cpart, P.state};
+                  P.gpart, P.g, P.opart, P.cpart, P.state, P.bcount, P.start,
+                  P.members, P.tpart, P.gtile};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -70,7 +71,12 @@ kmc_status kmc_kk_create(const kmc_kk_config* cfg, kmc_kk_handle** out) {
   if (!cfg || !out) return KMC_ERR_ARG;
   *out = nullptr;
   if (cfg->n == 0 || cfg->d == 0 || cfg->k == 0 || cfg->k > cfg->n) return KMC_ERR_ARG;
-  if (cfg->k > (uint32_t)kmc::KK_MAX_K) return KMC_ERR_UNSUPPORTED;
+  if (cfg->product < KMC_KK_AUTO || cfg->product > KMC_KK_SORTED) return KMC_ERR_ARG;
+  // the cluster-sorted product streams K at a higher fraction of HBM
+  // bandwidth than the dense one-hot product (no operand traffic besides K)
+  const int product = cfg->product == KMC_KK_DENSE ? kmc::KK_DENSE : kmc::KK_SORTED;
+  if (cfg->k > (uint32_t)(product == kmc::KK_DENSE ? kmc::KK_DENSE_MAX_K : kmc::KK_MAX_K))
+    return KMC_ERR_UNSUPPORTED;
   if (cfg->kernel < KMC_KERNEL_LINEAR || cfg->kernel > KMC_KERNEL_RBF) return KMC_ERR_ARG;
   if (cfg->store != KMC_STORE_FP32 && cfg->store != KMC_STORE_BF16) return KMC_ERR_ARG;
   if (cfg->n > (size_t)0x7fffffff) return KMC_ERR_UNSUPPORTED;
@@ -91,7 +97,24 @@ kmc_status kmc_kk_create(const kmc_kk_config* cfg, kmc_kk_handle** out) {
   P.kind = cfg->kernel;
   P.gamma = cfg->gamma > 0.0f ? cfg->gamma : 1.0f / (float)cfg->d;
   P.K = (int)cfg->k;
-  P.Kp = P.K <= 16 ? 16 : (P.K <= 32 ? 32 : 64);
+  P.product = product;
+  if (product == kmc::KK_DENSE) {
+    P.Kp = P.K <= 16 ? 16 : (P.K <= 32 ? 32 : 64);
+    P.sS_i = P.Kp;
+    P.sS_c = 1;
+  } else {
+    P.Kp = (P.K + 15) / 16 * 16;
+    P.sS_i = 1;
+    P.sS_c = P.Np;
+    // sort blocking as the Lloyd path: ~4 points per cluster counter per block
+    const int ch = ((P.K + 255) / 256) * 256 * 4;
+    P.CH = std::max(1024, std::min(16384, ch));
+    P.nblk = (int)((P.n + P.CH - 1) / P.CH);
+    P.lbits = 0;
+    while ((1 << P.lbits) < P.K) ++P.lbits;
+    P.nq = (int)((P.n + kmc::kKKRows - 1) / kmc::kKKRows);
+    P.ntile = (int)((P.Np + 1023) / 1024);
+  }
   P.nassign = (int)((P.n + 255) / 256);
   P.tol = cfg->tol;
   kmc_status st = KMC_OK;
@@ -105,13 +128,22 @@ kmc_status kmc_kk_create(const kmc_kk_config* cfg, kmc_kk_handle** out) {
     KK_CK(dalloc(&h->dX, (size_t)P.n * P.d));
     KK_CK(dalloc(&P.xn2, (size_t)P.n));
     KK_CK(dalloc(&P.kdiag, (size_t)P.n));
-    void* hb = nullptr;
-    KK_CK(hipMalloc(&hb, (size_t)P.Kp * (size_t)P.Np * es));
-    P.HT = hb;
+    if (P.product == kmc::KK_DENSE) {
+      void* hb = nullptr;
+      KK_CK(hipMalloc(&hb, (size_t)P.Kp * (size_t)P.Np * es));
+      P.HT = hb;
+      KK_CK(dalloc(&P.S, (size_t)P.n * P.Kp));
+      KK_CK(dalloc(&P.gpart, (size_t)((P.n + 15) / 16) * P.Kp));
+    } else {
+      KK_CK(dalloc(&P.S, (size_t)P.Kp * (size_t)P.Np));
+      KK_CK(dalloc(&P.bcount, (size_t)P.K * (size_t)P.nblk));
+      KK_CK(dalloc(&P.start, (size_t)P.K));
+      KK_CK(dalloc(&P.members, (size_t)P.n));
+      KK_CK(dalloc(&P.tpart, ((size_t)P.nq + (size_t)P.K) * (size_t)P.Np));
+      KK_CK(dalloc(&P.gtile, (size_t)P.K * (size_t)P.ntile));
+    }
     KK_CK(dalloc(&P.labels, (size_t)P.n));
     KK_CK(dalloc(&P.count, (size_t)P.Kp));
-    KK_CK(dalloc(&P.S, (size_t)P.n * P.Kp));
-    KK_CK(dalloc(&P.gpart, (size_t)((P.n + 15) / 16) * P.Kp));
     KK_CK(dalloc(&P.g, (size_t)P.Kp));
     KK_CK(dalloc(&P.opart, (size_t)P.nassign));
     KK_CK(dalloc(&P.cpart, (size_t)P.nassign));

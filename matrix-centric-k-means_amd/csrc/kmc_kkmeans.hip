@@ -14,6 +14,16 @@
 //   k_kk_assign  d(i,c) = K_ii - 2 S[i,c]/|c| + g_c/|c|^2, first minimum,
 //                empty cluster (or g_c == 0) -> +inf; FP64 objective
 //   k_kk_stats   iteration counter, score test, stop flag
+// The cluster-sorted product (any number of clusters) replaces onehot,
+// count and spmm:
+//   launch_sort  stable counting sort of the points by label (the Lloyd
+//                path's kernels): counts, offsets, sorted member list
+//   k_kk_rowsum  K is symmetric, so S[:, c] = sum_{j in c} K[j, :]: rows of
+//                K in sorted order, summed per (chunk of 256 sorted rows,
+//                cluster) segment in fp32, one partial row per segment
+//   k_kk_colsum  S[c, :] = the cluster's chunk partials in chunk order
+//                (FP64), and the per-tile partials of g_c
+//   k_kk_gfin    g_c = its tile partials in order
 #include "kmc_device.h"
 #include "kmc_kk.h"
 
@@ -203,6 +213,144 @@ __global__ __launch_bounds__(256) void k_kk_spmm(KKParams P) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Cluster-sorted product (the reference's REORDER level, src/kmeans.cu:
+// 1125-1261, puts K's columns in cluster order so that V K has contiguous
+// blocks; here K stays in place): K is symmetric, so the sum over a
+// cluster's columns of row i equals the sum over the cluster's rows at
+// column i. Workgroup (column tile x, chunk q) stages the member ids and
+// labels of sorted positions [256 q, 256 q + 256) in LDS, then walks them
+// segment by segment (a segment = one cluster's part of the chunk), adding
+// the rows' 16-byte column slices in sorted order in fp32 (eight rows' loads
+// issued together); each segment leaves one partial row at index c + q,
+// injective because clusters occupy increasing chunk ranges. Every element
+// of K is read once, in whole-row runs, whatever the number of clusters.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int kk_seg_end(const int* lab_s, int r, int L, int c, int lane) {
+  for (int w = r + 1; w < L; w += 64) {
+    const int i = w + lane;
+    const uint64_t m = __ballot(i >= L || lab_s[i < L ? i : 0] != c);
+    if (m) return w + __ffsll((unsigned long long)m) - 1;
+  }
+  return L;
+}
+
+template <int STORE>
+__global__ __launch_bounds__(256) void k_kk_rowsum(KKParams P) {
+  if (*P.done) return;
+  __shared__ int mem_s[kKKRows], lab_s[kKKRows];
+  constexpr int ES = STORE == KS_FP32 ? 4 : 2;
+  constexpr int CPT = 16 / ES;  // columns per thread: one 16-byte load
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.y;
+  const long long p0 = (long long)q * kKKRows;
+  const int L = (int)min((long long)kKKRows, P.n - p0);
+  for (int t = threadIdx.x; t < kKKRows; t += 256) {
+    const int id = t < L ? P.members[p0 + t] : 0;
+    mem_s[t] = id;
+    lab_s[t] = t < L ? P.labels[id] : -1;
+  }
+  __syncthreads();
+  const long long col = ((long long)blockIdx.x * 256 + threadIdx.x) * CPT;
+  const bool act = col < P.Np;  // Np is a multiple of 64: whole loads in range
+  const unsigned char* Kb =
+      reinterpret_cast<const unsigned char*>(P.Kmat) + (size_t)(act ? col : 0) * ES;
+  const size_t rowb = (size_t)P.Np * ES;
+  int r = 0;
+  while (r < L) {
+    const int c = lab_s[r];
+    const int e = kk_seg_end(lab_s, r, L, c, lane);
+    float sum[CPT];
+#pragma unroll
+    for (int v = 0; v < CPT; ++v) sum[v] = 0.0f;
+    constexpr int U = 8;
+    for (int t = r; t < e; t += U) {
+      uint4 x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int tt = min(t + u, e - 1);
+        x[u] = *reinterpret_cast<const uint4*>(Kb + (size_t)mem_s[tt] * rowb);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (t + u < e) {  // segment-uniform
+          const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+          if (STORE == KS_FP32) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sum[v % CPT] += __uint_as_float(w[v]);
+          } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              sum[(2 * v) % CPT] += __uint_as_float(w[v] << 16);
+              sum[(2 * v + 1) % CPT] += __uint_as_float(w[v] & 0xffff0000u);
+            }
+          }
+        }
+      }
+    }
+    if (act) {
+      float* dst = P.tpart + (size_t)(c + q) * P.Np + col;
+#pragma unroll
+      for (int v = 0; v < CPT; v += 4)
+        *reinterpret_cast<float4*>(dst + v) = make_float4(sum[v], sum[v + 1 < CPT ? v + 1 : v],
+                                                          sum[v + 2 < CPT ? v + 2 : v],
+                                                          sum[v + 3 < CPT ? v + 3 : v]);
+    }
+    r = e;
+  }
+}
+
+// S[c, col] = sum of cluster c's chunk partials in chunk order (FP64, rounded
+// once); the columns of c's own points add S[c, i] to the tile's partial of
+// g_c (fixed-order block reduction). Workgroup (column tile, cluster), four
+// columns per thread.
+__global__ __launch_bounds__(256) void k_kk_colsum(KKParams P) {
+  if (*P.done) return;
+  __shared__ double red_s[256];
+  const int c = blockIdx.y;
+  const long long col = ((long long)blockIdx.x * 256 + threadIdx.x) * 4;
+  const int cnt = P.count[c];
+  double gs = 0.0;
+  if (col < P.Np) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    if (cnt > 0) {
+      const long long q0 = (long long)P.start[c] / kKKRows;
+      const long long q1 = ((long long)P.start[c] + cnt - 1) / kKKRows;
+      for (long long q = q0; q <= q1; ++q) {
+        const float4 v = *reinterpret_cast<const float4*>(P.tpart + (size_t)(c + q) * P.Np + col);
+        s[0] += v.x;
+        s[1] += v.y;
+        s[2] += v.z;
+        s[3] += v.w;
+      }
+    }
+    float out[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[v] = (float)s[v];
+    *reinterpret_cast<float4*>(P.S + (size_t)c * P.Np + col) =
+        make_float4(out[0], out[1], out[2], out[3]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (col + v < P.n && P.labels[col + v] == c) gs += (double)out[v];
+  }
+  red_s[threadIdx.x] = gs;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red_s[threadIdx.x] += red_s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) P.gtile[(size_t)c * P.ntile + blockIdx.x] = red_s[0];
+}
+
+__global__ __launch_bounds__(256) void k_kk_gfin(KKParams P) {
+  if (*P.done) return;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= P.K) return;
+  double g = 0.0;
+  for (int t = 0; t < P.ntile; ++t) g += P.gtile[(size_t)c * P.ntile + t];
+  P.g[c] = g;
+}
+
 // g_c = sum over row blocks (in order) of the partial self-terms, FP64.
 __global__ __launch_bounds__(256) void k_kk_gsum(KKParams P) {
   if (*P.done) return;
@@ -240,11 +388,11 @@ __global__ __launch_bounds__(256) void k_kk_assign(KKParams P) {
   double obj = 0.0;
   int changed = 0;
   if (i < P.n) {
-    const float* s = P.S + (size_t)i * P.Kp;
+    const float* s = P.S + (size_t)i * P.sS_i;
     float best = __builtin_inff();
     int bi = 0;
     for (int c = 0; c < P.K; ++c) {
-      const float v = cnorm_s[c] - inv_s[c] * s[c];
+      const float v = cnorm_s[c] - inv_s[c] * s[(size_t)c * P.sS_c];
       if (v < best) { best = v; bi = c; }
     }
     obj = (double)P.kdiag[i] + (double)best;
@@ -320,7 +468,7 @@ __global__ __launch_bounds__(256) void k_kk_dist_out(KKParams P) {
     float v = __builtin_inff();
     if (n_c != 0 && gc != 0.0)
       v = (float)(gc / ((double)n_c * (double)n_c)) -
-          (float)(2.0 / (double)n_c) * P.S[(size_t)i * P.Kp + c];
+          (float)(2.0 / (double)n_c) * P.S[(size_t)i * P.sS_i + (size_t)c * P.sS_c];
     P.Dout[(size_t)i * P.K + c] = v;
   }
 }
@@ -353,7 +501,42 @@ static void launch_spmm_t(const KKParams& P, unsigned grid, hipStream_t s) {
   }
 }
 
+// the cluster-sorted product: sort, chunk row sums, column sums, g
+static hipError_t launch_sorted_product(const KKParams& P, hipStream_t s) {
+  SortParams sp{};
+  sp.labels = P.labels;
+  sp.bcount = P.bcount;
+  sp.count = P.count;
+  sp.start = P.start;
+  sp.members = P.members;
+  sp.done = P.done;
+  sp.n = P.n;
+  sp.K = P.K;
+  sp.CH = P.CH;
+  sp.nblk = P.nblk;
+  sp.lbits = P.lbits;
+  hipError_t e = launch_sort(sp, s);
+  if (e != hipSuccess) return e;
+  return hipSuccess;
+}
+static hipError_t launch_sorted_sums(const KKParams& P, hipStream_t s) {
+  const int cpt = P.store == KS_FP32 ? 4 : 8;
+  const dim3 gr((unsigned)((P.Np + 256LL * cpt - 1) / (256LL * cpt)), (unsigned)P.nq);
+  if (P.store == KS_FP32) hipLaunchKernelGGL(k_kk_rowsum<KS_FP32>, gr, dim3(256), 0, s, P);
+  else hipLaunchKernelGGL(k_kk_rowsum<KS_BF16>, gr, dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_kk_colsum, dim3((unsigned)P.ntile, (unsigned)P.K), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_kk_gfin, dim3((unsigned)((P.K + 255) / 256)), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
 hipError_t kk_launch_distances(const KKParams& P, hipStream_t s) {
+  if (P.product == KK_SORTED) {
+    hipError_t e = launch_sorted_product(P, s);
+    if (e == hipSuccess) e = launch_sorted_sums(P, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_kk_dist_out, dim3(P.nassign), dim3(256), 0, s, P);
+    return hipGetLastError();
+  }
   const unsigned gj = (unsigned)((P.Np + 255) / 256);
   if (P.store == KS_FP32) hipLaunchKernelGGL(k_kk_onehot<KS_FP32>, dim3(gj), dim3(256), 0, s, P);
   else hipLaunchKernelGGL(k_kk_onehot<KS_BF16>, dim3(gj), dim3(256), 0, s, P);
@@ -368,6 +551,17 @@ hipError_t kk_launch_distances(const KKParams& P, hipStream_t s) {
 }
 
 hipError_t kk_launch_iter(const KKParams& P, hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+  if (P.product == KK_SORTED) {
+    hipError_t e = launch_sorted_product(P, s);
+    if (e != hipSuccess) return e;
+    if (e0) (void)hipEventRecord(e0, s);
+    e = launch_sorted_sums(P, s);
+    if (e != hipSuccess) return e;
+    if (e1) (void)hipEventRecord(e1, s);
+    hipLaunchKernelGGL(k_kk_assign, dim3(P.nassign), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_kk_stats, dim3(1), dim3(256), 0, s, P);
+    return hipGetLastError();
+  }
   const unsigned gj = (unsigned)((P.Np + 255) / 256);
   if (P.store == KS_FP32) hipLaunchKernelGGL(k_kk_onehot<KS_FP32>, dim3(gj), dim3(256), 0, s, P);
   else hipLaunchKernelGGL(k_kk_onehot<KS_BF16>, dim3(gj), dim3(256), 0, s, P);

@@ -33,6 +33,16 @@ void check(kmc_handle* h, kmc_status s, const char* what) {
   if (s != KMC_OK) raise(h, s, what);
 }
 
+void kk_check(const kmc_kk_handle* h, kmc_status s, const char* what) {
+  if (s == KMC_OK) return;
+  std::string msg = std::string(what) + ": " + kmc_status_string(s);
+  if (h) {
+    const char* e = kmc_kk_last_error(h);
+    if (e && *e) msg += std::string(" (") + e + ")";
+  }
+  throw kmc::Error(s, msg);
+}
+
 }  // namespace
 
 Kmeans::Kmeans(const size_t _n, const uint32_t _d, const uint32_t _k, const float _tol,
@@ -53,9 +63,32 @@ void Kmeans::init(const int* seed) {
   if (!points || n == 0 || d == 0 || k == 0 || k > n)
     throw kmc::Error(KMC_ERR_ARG, "Kmeans: invalid n/d/k or points");
   if (level < 0 || level > 4) throw kmc::Error(KMC_ERR_ARG, "Kmeans: level must be 0..4");
-  if (kernel != Kernel::linear)
-    throw kmc::Error(KMC_ERR_UNSUPPORTED,
-                     "Kmeans: kernel k-means runs through kmc_kernel_* (see kmc.h)");
+  if (kernel != Kernel::linear) {
+    // kernel k-means: K = f(X X^T) built on the device, labels i % k
+    // (src/kmeans.cu:143-171, 372-374); the bf16 forms store K in bf16
+    kmc_kk_config kc{};
+    kc.n = n;
+    kc.d = d;
+    kc.k = k;
+    kc.tol = tol;
+    kc.kernel = kernel == Kernel::polynomial ? KMC_KERNEL_POLYNOMIAL
+                : kernel == Kernel::sigmoid  ? KMC_KERNEL_SIGMOID
+                                             : KMC_KERNEL_RBF;
+    kc.store = opts.precision == Precision::fp32 ? KMC_STORE_FP32 : KMC_STORE_BF16;
+    kc.gamma = 0.0f;
+    kc.device = deviceProps ? deviceProps->ordinal : 0;
+    kc.product = KMC_KK_AUTO;
+    const kmc_status s = kmc_kk_create(&kc, &kh);
+    if (s != KMC_OK) throw kmc::Error(s, std::string("kmc_kk_create: ") + kmc_status_string(s));
+    std::vector<float> X(n * (size_t)d);
+    for (size_t i = 0; i < n; ++i)
+      for (uint32_t j = 0; j < d; ++j) X[i * d + j] = points[i]->get(j);
+    kk_check(kh, kmc_kk_set_points(kh, X.data(), d), "kmc_kk_set_points");
+    std::vector<int32_t> lab(n);
+    for (size_t i = 0; i < n; ++i) lab[i] = (int32_t)(i % k);
+    kk_check(kh, kmc_kk_set_labels(kh, lab.data()), "kmc_kk_set_labels");
+    return;
+  }
 
   kmc_config cfg{};
   cfg.n = n;
@@ -125,9 +158,22 @@ void Kmeans::init(const int* seed) {
 
 Kmeans::~Kmeans() {
   if (h) kmc_destroy(h);
+  if (kh) kmc_kk_destroy(kh);
 }
 
 uint64_t Kmeans::run(uint64_t maxiter, bool check_converged) {
+  if (kh) {
+    kmc_kk_result res{};
+    kk_check(kh, kmc_kk_run(kh, maxiter, check_converged ? KMC_RUN_CHECK_CONVERGED : 0u, &res),
+             "kmc_kk_run");
+    score = (float)res.score;
+    inertia = res.objective;
+    iterations = res.iterations;
+    std::vector<int32_t> lab(n);
+    kk_check(kh, kmc_kk_get_labels(kh, lab.data()), "kmc_kk_get_labels");
+    for (size_t i = 0; i < n; ++i) points[i]->setCluster((unsigned int)lab[i]);
+    return res.converged_at;
+  }
   uint32_t flags = check_converged ? KMC_RUN_CHECK_CONVERGED : 0u;
   if (opts.reference_gpu) flags |= KMC_RUN_CONVERGE_SCORE | KMC_RUN_EXCLUDE_EMPTY;
   kmc_result res{};
@@ -142,6 +188,9 @@ uint64_t Kmeans::run(uint64_t maxiter, bool check_converged) {
 }
 
 std::vector<float> Kmeans::get_centroids() const {
+  if (kh)
+    throw kmc::Error(KMC_ERR_UNSUPPORTED,
+                     "Kmeans: kernel k-means centroids live in the kernel's feature space");
   std::vector<float> C((size_t)k * d);
   check(h, kmc_get_centroids(h, C.data()), "kmc_get_centroids");
   return C;

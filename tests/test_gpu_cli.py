@@ -51,3 +51,35 @@ def test_cli_cpu_semantics_matches_reference(tmp_path, oracle, init):
     assert np.array_equal(lab, want.astype(np.int64))
     assert ("K-means converged at iteration %d" % conv) in text, text
     assert "GPU_Kmeans:" in text and "Score:" in text
+
+
+def test_cli_polynomial_kernel_runs_kernel_kmeans(tmp_path, kmc_mod):
+    """-f polynomial runs kernel k-means from labels i % k (the reference's
+    Popcorn path); its labels are those of the kernel k-means engine."""
+    rng = np.random.default_rng(3)
+    n, d, k = 600, 12, 4
+    centres = rng.uniform(0, 1, size=(k, d))
+    X = (centres[rng.integers(0, k, size=n)] + 0.05 * rng.standard_normal((n, d)))
+    X = X.astype(np.float32)
+    inp = tmp_path / "in.csv"
+    out = tmp_path / "out.csv"
+    with open(inp, "w") as f:
+        f.write("k," + ",".join("d%d" % j for j in range(d)) + "\n")
+        for i in range(n):
+            f.write("0," + ",".join("%.9g" % v for v in X[i]) + "\n")
+    Xr = np.loadtxt(inp, delimiter=",", skiprows=1, dtype=np.float64)[:, 1:].astype(np.float32)
+    cmd = [EXE, "-d", str(d), "-n", str(n), "-k", str(k), "-m", "30", "-t", "0.0001", "-c", "1",
+           "-b", "random", "-f", "polynomial", "-l", "2", "-r", "2", "-i", str(inp), "-o",
+           str(out), "--label-col", "first"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+    lab = np.loadtxt(out, delimiter=",", skiprows=1, usecols=0, dtype=np.int64)
+    with kmc_mod.KernelEngine(n, d, k, kernel="polynomial", store="fp32", tol=1e-4) as e:
+        e.set_points(Xr)
+        e.set_labels((np.arange(n) % k).astype(np.int32))
+        res = e.run(30, check_converged=True)
+        want = e.labels()
+    assert np.array_equal(lab, want.astype(np.int64))
+    if res["converged_at"] < 30:
+        assert ("K-means converged at iteration %d" % res["converged_at"]) in r.stdout
