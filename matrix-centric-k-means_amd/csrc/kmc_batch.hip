@@ -2,8 +2,8 @@
 // (kmc_batch_* of include/kmc.h).
 //
 // Problems are grouped by padded width DP (d rounded up to 16, 32, 64, 128 or
-// 256); each group is packed and launched on its own, the groups of one
-// iteration queued one after another on the batch's stream.
+// 256); each group is packed and launched on its own stream, so the groups'
+// iterations run side by side.
 //
 // Layout of a group: its problems' points are packed into one point space, problem p at
 // rows [poff_p, poff_p + n_p) with poff_p a multiple of kBatchAlign; the rows
@@ -349,6 +349,8 @@ using kmc::DevState;
 // padded width DP share one packed point space and one launch per stage, so
 // a problem of d=16 streams 64-byte rows, not the widest problem's.
 struct BatchGroup {
+  hipStream_t stream = nullptr;  // the group's iterations run on their own stream
+  hipEvent_t end = nullptr;
   uint32_t nprob = 0;           // problems in this group
   std::vector<uint32_t> ids;    // their indices in the batch
   int DP = 0, prec = 0, idx_bits = 0, lbits = 0, cus = 0;
@@ -429,7 +431,10 @@ void gfree_all(BatchGroup* g) {
 
 void bfree_all(kmc_batch* b) {
   for (BatchGroup* g : b->groups) {
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
     gfree_all(g);
+    if (g->end) (void)hipEventDestroy(g->end);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
   }
   b->groups.clear();
@@ -458,7 +463,7 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   pp.KTp = g->KTp;
   pp.DP = g->DP;
   pp.prec = g->prec;
-  hipLaunchKernelGGL(kmc::k_bprep, dim3((g->nops + 3) / 4), dim3(256), 0, b->stream, pp);
+  hipLaunchKernelGGL(kmc::k_bprep, dim3((g->nops + 3) / 4), dim3(256), 0, g->stream, pp);
   KMC_BCK(hipGetLastError());
 
   kmc::AssignParams ap{};
@@ -490,7 +495,7 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   ap.item_prob = g->item_prob;
   ap.pdone = g->pdone;
   ap.nitems = g->nitems;
-  KMC_BCK(kmc::launch_assign_batch(g->DP, g->prec, ap, g->grid, g->lds, b->stream));
+  KMC_BCK(kmc::launch_assign_batch(g->DP, g->prec, ap, g->grid, g->lds, g->stream));
 
   kmc::RefineParams rp{};
   rp.X = g->X;
@@ -511,7 +516,7 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   rp.DP = g->DP;
   rp.bp = g->bp;
   rp.blk_prob = g->blk_prob;
-  KMC_BCK(kmc::launch_refine(rp, b->stream));
+  KMC_BCK(kmc::launch_refine(rp, g->stream));
 
   kmc::BSortParams sp{};
   sp.labels = g->labels;
@@ -527,9 +532,9 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   sp.KT = g->KT;
   sp.lbits = g->lbits;
   const size_t hl = (size_t)(1 << g->lbits) * sizeof(int);
-  hipLaunchKernelGGL(kmc::k_bhist, dim3(g->nblk), dim3(64), hl, b->stream, sp);
-  hipLaunchKernelGGL(kmc::k_bscan_blocks, dim3((g->KT + 3) / 4), dim3(256), 0, b->stream, sp);
-  hipLaunchKernelGGL(kmc::k_bscatter, dim3(g->nblk), dim3(64), hl, b->stream, sp);
+  hipLaunchKernelGGL(kmc::k_bhist, dim3(g->nblk), dim3(64), hl, g->stream, sp);
+  hipLaunchKernelGGL(kmc::k_bscan_blocks, dim3((g->KT + 3) / 4), dim3(256), 0, g->stream, sp);
+  hipLaunchKernelGGL(kmc::k_bscatter, dim3(g->nblk), dim3(64), hl, g->stream, sp);
   KMC_BCK(hipGetLastError());
 
   kmc::UpdateParams up{};
@@ -547,7 +552,7 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   up.K = g->KT;
   up.D = g->DP;
   up.DP = g->DP;
-  KMC_BCK(kmc::launch_update(up, kmc::UPDATE_TREE, b->stream));
+  KMC_BCK(kmc::launch_update(up, kmc::UPDATE_TREE, g->stream));
 
   kmc::FinishParams fp{};
   fp.C = g->C;
@@ -567,7 +572,7 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   fp.exclude_empty = 0;
   fp.do_prep = 0;
   fp.stats.state = nullptr;  // the batch's own stop test follows
-  KMC_BCK(kmc::launch_finish(fp, b->stream));
+  KMC_BCK(kmc::launch_finish(fp, g->stream));
 
   kmc::BStatsParams st{};
   st.conv = g->conv;
@@ -580,7 +585,7 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   st.nprob = (int)g->nprob;
   st.check = check;
   st.maxiter = maxiter;
-  hipLaunchKernelGGL(kmc::k_bstats, dim3(1), dim3(kmc::kBStatsThreads), 0, b->stream, st);
+  hipLaunchKernelGGL(kmc::k_bstats, dim3(1), dim3(kmc::kBStatsThreads), 0, g->stream, st);
   KMC_BCK(hipGetLastError());
   return KMC_OK;
 }
@@ -809,6 +814,8 @@ kmc_status kmc_batch_create(const kmc_problem* probs, uint32_t count,
       BatchGroup* g = new (std::nothrow) BatchGroup();
       if (!g) return bfail(b, KMC_ERR_OOM, "kmc_batch_create: out of host memory");
       b->groups.push_back(g);
+      KMC_BCK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+      KMC_BCK(hipEventCreateWithFlags(&g->end, hipEventDisableTiming));
       g->ids = ids;
       for (size_t i = 0; i < ids.size(); ++i) {
         b->grp_of[ids[i]] = (int)b->groups.size() - 1;
@@ -837,24 +844,26 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
   KMC_BCK(hipSetDevice(b->dev));
   const int check = (flags & KMC_RUN_CHECK_CONVERGED) ? 1 : 0;
   // every run starts from the initial centroids
-  for (BatchGroup* g : b->groups) {
-    const size_t DP = (size_t)g->DP;
-    KMC_BCK(hipMemcpyAsync(g->C, g->C0, (size_t)g->KT * DP * sizeof(float),
-                           hipMemcpyDeviceToDevice, b->stream));
-    KMC_BCK(hipMemcpyAsync(g->Cnext, g->C0, (size_t)g->KT * DP * sizeof(float),
-                           hipMemcpyDeviceToDevice, b->stream));
-    hipLaunchKernelGGL(kmc::k_binit_labels, dim3((unsigned)((g->npad + 255) / 256)), dim3(256),
-                       0, b->stream, g->labels, g->bp, g->blk_prob, g->npad);
-    KMC_BCK(hipGetLastError());
-    KMC_BCK(hipMemsetAsync(g->pdone, 0, g->nprob * sizeof(int), b->stream));
-    KMC_BCK(hipMemsetAsync(g->piter, 0, g->nprob * sizeof(unsigned), b->stream));
-    KMC_BCK(hipMemsetAsync(g->pconv, 0, g->nprob * sizeof(unsigned), b->stream));
-    KMC_BCK(hipMemsetAsync(g->state, 0, sizeof(DevState), b->stream));
-  }
   hipEvent_t t0, t1;
   KMC_BCK(hipEventCreate(&t0));
   KMC_BCK(hipEventCreate(&t1));
   KMC_BCK(hipEventRecord(t0, b->stream));
+  // the groups are independent: each runs on its own stream from t0 on
+  for (BatchGroup* g : b->groups) {
+    const size_t DP = (size_t)g->DP;
+    KMC_BCK(hipStreamWaitEvent(g->stream, t0, 0));
+    KMC_BCK(hipMemcpyAsync(g->C, g->C0, (size_t)g->KT * DP * sizeof(float),
+                           hipMemcpyDeviceToDevice, g->stream));
+    KMC_BCK(hipMemcpyAsync(g->Cnext, g->C0, (size_t)g->KT * DP * sizeof(float),
+                           hipMemcpyDeviceToDevice, g->stream));
+    hipLaunchKernelGGL(kmc::k_binit_labels, dim3((unsigned)((g->npad + 255) / 256)), dim3(256),
+                       0, g->stream, g->labels, g->bp, g->blk_prob, g->npad);
+    KMC_BCK(hipGetLastError());
+    KMC_BCK(hipMemsetAsync(g->pdone, 0, g->nprob * sizeof(int), g->stream));
+    KMC_BCK(hipMemsetAsync(g->piter, 0, g->nprob * sizeof(unsigned), g->stream));
+    KMC_BCK(hipMemsetAsync(g->pconv, 0, g->nprob * sizeof(unsigned), g->stream));
+    KMC_BCK(hipMemsetAsync(g->state, 0, sizeof(DevState), g->stream));
+  }
   kmc_status st = KMC_OK;
   const size_t ng = b->groups.size();
   if (maxiter > 0) {
@@ -873,8 +882,9 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
       for (size_t gi = 0; gi < ng; ++gi)
         if (!gdone[gi])
           KMC_BCK(hipMemcpyAsync(&hs[gi], b->groups[gi]->state, sizeof(DevState),
-                                 hipMemcpyDeviceToHost, b->stream));
-      KMC_BCK(hipStreamSynchronize(b->stream));
+                                 hipMemcpyDeviceToHost, b->groups[gi]->stream));
+      for (size_t gi = 0; gi < ng; ++gi)
+        if (!gdone[gi]) KMC_BCK(hipStreamSynchronize(b->groups[gi]->stream));
       bool all = true;
       for (size_t gi = 0; gi < ng; ++gi) {
         if (!gdone[gi] && hs[gi].done) gdone[gi] = 1;
@@ -887,20 +897,22 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
     for (BatchGroup* g : b->groups) {
       const dim3 grid(g->nblk), block(kmc::kBInertiaThreads);
       switch (g->DP) {
-        case 16: hipLaunchKernelGGL(kmc::k_binertia_blocks<16>, grid, block, 0, b->stream,
+        case 16: hipLaunchKernelGGL(kmc::k_binertia_blocks<16>, grid, block, 0, g->stream,
                                     g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
-        case 32: hipLaunchKernelGGL(kmc::k_binertia_blocks<32>, grid, block, 0, b->stream,
+        case 32: hipLaunchKernelGGL(kmc::k_binertia_blocks<32>, grid, block, 0, g->stream,
                                     g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
-        case 64: hipLaunchKernelGGL(kmc::k_binertia_blocks<64>, grid, block, 0, b->stream,
+        case 64: hipLaunchKernelGGL(kmc::k_binertia_blocks<64>, grid, block, 0, g->stream,
                                     g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
-        case 128: hipLaunchKernelGGL(kmc::k_binertia_blocks<128>, grid, block, 0, b->stream,
+        case 128: hipLaunchKernelGGL(kmc::k_binertia_blocks<128>, grid, block, 0, g->stream,
                                      g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
-        default: hipLaunchKernelGGL(kmc::k_binertia_blocks<256>, grid, block, 0, b->stream,
+        default: hipLaunchKernelGGL(kmc::k_binertia_blocks<256>, grid, block, 0, g->stream,
                                     g->X, g->C, g->labels, g->bp, g->blk_prob, g->bpart); break;
       }
       hipLaunchKernelGGL(kmc::k_binertia_sum, dim3((g->nprob + 3) / 4), dim3(256), 0,
-                         b->stream, g->bpart, g->bp, (int)g->nprob, g->inertia);
+                         g->stream, g->bpart, g->bp, (int)g->nprob, g->inertia);
       KMC_BCK(hipGetLastError());
+      KMC_BCK(hipEventRecord(g->end, g->stream));
+      KMC_BCK(hipStreamWaitEvent(b->stream, g->end, 0));
     }
     KMC_BCK(hipEventRecord(t1, b->stream));
     KMC_BCK(hipEventSynchronize(t1));

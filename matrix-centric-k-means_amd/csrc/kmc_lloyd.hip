@@ -1402,7 +1402,7 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree_rows(UpdatePara
   if (*P.done) return;
   constexpr int LPR = DP / 4;    // lanes per row
   constexpr int G = 64 / LPR;    // rows per wave load
-  constexpr int U = 64 / G;      // loads per batch
+  constexpr int U = G >= 4 ? 64 / G : 16;  // loads per batch
   __shared__ int mem_s[kTreeWaves][kTreeRows], lab_s[kTreeWaves][kTreeRows];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long q = (long long)blockIdx.x * kTreeWaves + wv;
@@ -2263,7 +2263,7 @@ hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
     const dim3 g((unsigned)((waves + kTreeWaves - 1) / kTreeWaves));
     const dim3 b(64 * kTreeWaves);
     if (P.DP >= 256) hipLaunchKernelGGL(k_update_tree<4>, g, b, 0, s, P);
-    else if (P.DP >= 128) hipLaunchKernelGGL(k_update_tree<2>, g, b, 0, s, P);
+    else if (P.DP >= 128) hipLaunchKernelGGL(k_update_tree_rows<128>, g, b, 0, s, P);
     else if (P.DP == 64) hipLaunchKernelGGL(k_update_tree_rows<64>, g, b, 0, s, P);
     else if (P.DP == 32) hipLaunchKernelGGL(k_update_tree_rows<32>, g, b, 0, s, P);
     else if (P.DP == 16) hipLaunchKernelGGL(k_update_tree_rows<16>, g, b, 0, s, P);
