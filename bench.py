@@ -404,8 +404,8 @@ def main():
                    "algorithmic_flops_per_launch": flops,
                    "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches}
     ubytes = N * D * 4 + N * 4 + K * D * 4  # X rows once, member ids, centroids
-    uname = {"tree": "k_update_tree<%d>" % (dp // 64), "ordered": "k_update",
-             "fixed": "k_update_fx<64>"}[a.update]
+    uname = {"tree": "k_update_tree_rows<128>" if dp <= 128 else "k_update_tree<4>",
+             "ordered": "k_update", "fixed": "k_update_fx<64>"}[a.update]
     roof_update = {"bound": "hbm", "kernel": uname,
                    "achieved": round(ubytes / t_update / 1e9, 1), "peak": HBM_PEAK / 1e9,
                    "unit": "GB/s", "frac": round(ubytes / t_update / HBM_PEAK, 4),
