@@ -96,3 +96,31 @@ def test_batch_without_device(kmc_mod):
     with pytest.raises(kmc_mod.KmcError) as ei:
         kmc_mod.Batch([(X, C)])
     assert ei.value.status == kmc_mod.ERR_NO_DEVICE
+
+
+def test_kernel_kmeans_config_checks(kmc_mod):
+    """Kernel k-means: product selection and the cluster limits of each
+    product are checked before any device call."""
+    lib = kmc_mod.load_library()
+    h = ctypes.c_void_p()
+
+    def cfg(k, product):
+        return kmc_mod._KKConfig(1000, 8, k, 1e-4, 1, 0, 0.0, 0, product)
+
+    assert lib.kmc_kk_create(ctypes.byref(cfg(10, 3)), ctypes.byref(h)) == kmc_mod.ERR_ARG
+    assert lib.kmc_kk_create(ctypes.byref(cfg(65, kmc_mod.KK_PRODUCTS["dense"])),
+                             ctypes.byref(h)) == kmc_mod.ERR_UNSUPPORTED
+    assert lib.kmc_kk_create(ctypes.byref(cfg(1025, kmc_mod.KK_PRODUCTS["sorted"])),
+                             ctypes.byref(h)) == kmc_mod.ERR_UNSUPPORTED
+
+
+def test_batch_rejects_fp8(kmc_mod):
+    lib = kmc_mod.load_library()
+    X = np.zeros((100, 4), np.float32)
+    C0 = np.zeros((2, 4), np.float32)
+    arr = (kmc_mod._Problem * 1)()
+    arr[0] = kmc_mod._Problem(100, 4, 2, X.ctypes.data, 4, C0.ctypes.data)
+    bc = kmc_mod._BatchConfig(1e-4, kmc_mod.PRECISIONS["fp8"], 0, 0)
+    h = ctypes.c_void_p()
+    assert lib.kmc_batch_create(arr, 1, ctypes.byref(bc), ctypes.byref(h)) == \
+        kmc_mod.ERR_UNSUPPORTED
