@@ -105,7 +105,7 @@ def test_kernel_kmeans_config_checks(kmc_mod):
     h = ctypes.c_void_p()
 
     def cfg(k, product):
-        return kmc_mod._KKConfig(1000, 8, k, 1e-4, 1, 0, 0.0, 0, product)
+        return kmc_mod._KKConfig(5000, 8, k, 1e-4, 1, 0, 0.0, 0, product)
 
     assert lib.kmc_kk_create(ctypes.byref(cfg(10, 3)), ctypes.byref(h)) == kmc_mod.ERR_ARG
     assert lib.kmc_kk_create(ctypes.byref(cfg(65, kmc_mod.KK_PRODUCTS["dense"])),
