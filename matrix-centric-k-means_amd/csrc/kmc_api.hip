@@ -28,6 +28,9 @@ struct kmc_handle {
   uint16_t* Xlo = nullptr;
   uint8_t* Xq = nullptr;  // fp8 form: e4m3 plane and per-row E8M0 scales
   int* xsc = nullptr;
+  float* xdn = nullptr;   // single-product forms: measured operand rounding
+  float* cdn = nullptr;
+  float hop = 0.0f;
   float* xn2 = nullptr;
   // centroids
   float* C = nullptr;
@@ -115,7 +118,7 @@ double gam(double m, double u) { return m * u / (1.0 - m * u); }
 // bound b + E of each distance. Every constant carries a factor 1.25 of
 // headroom.
 void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, float* gref,
-                    float* xpk) {
+                    float* xpk, bool measured, float* hop) {
   const double u = std::ldexp(1.0, -24);   // fp32 unit roundoff
   const double ub = std::ldexp(1.0, -8);   // bf16 unit roundoff (8-bit significand)
   const double u8 = std::ldexp(1.0, -4);   // e4m3 unit roundoff (4-bit significand)
@@ -146,7 +149,11 @@ void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, fl
   }
   const double gD = gam(D, u);
   const double pk = std::ldexp(1.0, idx_bits - 23);
-  const double a = 2.0 * (u + gA) + 2.0 * (t_in + gA * (1.0 + t_in));
+  // measured: the single-product forms know each stored operand's rounding
+  // error norm; the distance kernel adds ||dc|| ||x|| + ||c'|| ||dx|| +
+  // ||dc|| ||dx|| (c' = -2c, Cauchy-Schwarz) in place of 2 t_in ||x|| c
+  const bool meas = measured && (prec == kmc::PREC_BF16 || prec == kmc::PREC_FP8);
+  const double a = 2.0 * (u + gA) + 2.0 * ((meas ? 0.0 : t_in) + gA * (1.0 + t_in));
   const double b = 3.0 * (u + gA) + gD;
   const double g = gam(D + 3, u) + pk + 2.0 * (u + gA);
   const double H = 1.25;  // headroom over the bound
@@ -157,6 +164,7 @@ void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, fl
   // afterwards: packing perturbation and the final add are then relative to
   // d^2 + ||x||^2
   *xpk = (prec == kmc::PREC_BF16 || prec == kmc::PREC_FP8) ? (float)(H * (pk + 2.0 * u)) : 0.0f;
+  if (hop) *hop = meas ? (float)H : 0.0f;
 }
 
 }  // namespace kmc
@@ -212,6 +220,10 @@ kmc_status alloc_all(kmc_handle* h) {
     KMC_CK(dalloc(&h->Xq, n * DP));
     KMC_CK(dalloc(&h->xsc, n));
   }
+  if (h->prec == kmc::PREC_FP8 || h->prec == kmc::PREC_BF16) {
+    KMC_CK(dalloc(&h->xdn, n));
+    KMC_CK(dalloc(&h->cdn, Kp));
+  }
   KMC_CK(dalloc(&h->xn2, n));
   KMC_CK(dalloc(&h->C, K * DP));
   KMC_CK(dalloc(&h->Cnext, K * DP));
@@ -263,7 +275,7 @@ void free_all(kmc_handle* h) {
                   h->bcount,
                   h->count, h->start, h->members, h->ipart, h->tpart, h->conv,
                   h->shift2, h->alive, h->state, h->xn64, h->fpart, h->qpart, h->cpart,
-                  h->Xq, h->xsc, h->Cq};
+                  h->Xq, h->xsc, h->Cq, h->xdn, h->cdn};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -312,6 +324,7 @@ kmc::PrepParams prep_params(kmc_handle* h) {
   pp.Chi = h->Chi;
   pp.Clo = h->Clo;
   pp.Cq = h->Cq;
+  pp.cdn = h->cdn;
   pp.cn = h->cn;
   pp.CT = h->CT;
   pp.alive = h->alive;
@@ -334,6 +347,9 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
   ap.Xq = h->Xq;
   ap.xsc = h->xsc;
   ap.Cq = h->Cq;
+  ap.xdn = h->xdn;
+  ap.cdn = h->cdn;
+  ap.hop = h->hop;
   ap.Xlo = h->Xlo;
   ap.xn2 = h->xn2;
   ap.n = (long long)h->n;
@@ -561,6 +577,7 @@ kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
   lp.Xhi = h->Xhi;
   lp.Xq = h->Xq;
   lp.xsc = h->xsc;
+  lp.xdn = h->xdn;
   lp.Xlo = h->Xlo;
   lp.xn2 = h->xn2;
   lp.xn2_64 = xn64;
@@ -751,7 +768,7 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     h->idx_bits = 0;
     while ((1 << h->idx_bits) < h->Kp) ++h->idx_bits;
     kmc::band_constants(h->prec, h->D, h->idx_bits, &h->alpha, &h->beta, &h->gref,
-                        &h->xpk);
+                        &h->xpk, true, &h->hop);
     st = alloc_all(h);
   } while (0);
   if (st != KMC_OK) {

@@ -53,6 +53,18 @@ __device__ __forceinline__ uint32_t fp8_e4m3_rne(float v) {
   return code | (v < 0.0f ? 0x80u : 0u);
 }
 
+// The value of an e4m3 code (inverse of fp8_e4m3_rne).
+__device__ __forceinline__ float fp8_e4m3_value(uint32_t code) {
+  const int e = (int)((code >> 3) & 15u), m = (int)(code & 7u);
+  const float a = e ? ldexpf(1.0f + 0.125f * (float)m, e - 7) : ldexpf((float)m, -9);
+  return (code & 0x80u) ? -a : a;
+}
+
+// sqrt of an FP64 sum of squares as a float no smaller than the true norm
+__device__ __forceinline__ float norm_up(double s2) {
+  return (float)(sqrt(s2) * (1.0 + 1e-6)) * (1.0f + 1e-6f);
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int lane = threadIdx.x & 63;
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));

@@ -48,6 +48,7 @@ struct PrepParams {
   uint16_t* Chi;   // Kp x DP  (bf16 forms)
   uint16_t* Clo;
   uint8_t* Cq;     // Kp x (DP + kFp8RowPad): e4m3 of -2c * 2^-e, scale e + 127 (fp8 form)
+  float* cdn;      // Kp: ||stored operand - (-2c)||, rounded up (single-product forms)
   float* cn;       // Kp, ||c||^2 (kFar for padding rows and excluded clusters)
   float* CT;       // DP x Kp transposed centroids (all-k refinement)
   const int* alive;  // K: 0 = cluster excluded from assignment (+inf)
@@ -83,6 +84,12 @@ struct AssignParams {
   const uint16_t* Clo;
   const uint8_t* Cq;
   const float* cn;
+  // measured operand rounding (single-product forms): per-point ||stored x -
+  // x||, per-centroid ||stored -2c - (-2c)||, and the factor applied to the
+  // operand term of the band (0: the worst-case constant in alpha is used)
+  const float* xdn;
+  const float* cdn;
+  float hop;
   int K, Kp, kchunk;
   uint32_t idx_mask;        // low mantissa bits that carry the centroid index
   float alpha, beta, gref;  // margin-band constants
@@ -210,13 +217,16 @@ struct LoadParams {
   uint16_t* Xlo;
   uint8_t* Xq;  // fp8 form: e4m3 plane and per-row scales
   int* xsc;
+  float* xdn;   // ||stored single-product operand - x|| per row, rounded up
   float* xn2;
   double* xn2_64;
 };
 
-// margin-band constants of a precision form (host; kmc_api.hip)
+// margin-band constants of a precision form (host; kmc_api.hip); with
+// measured = true the operand rounding is left out of alpha (the distance
+// kernel adds the measured operand term instead) and *hop is its factor
 void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, float* gref,
-                    float* xpk);
+                    float* xpk, bool measured = false, float* hop = nullptr);
 
 hipError_t launch_prep(const PrepParams& P, hipStream_t s);
 const void* assign_batch_kernel_ptr(int DP, int prec);

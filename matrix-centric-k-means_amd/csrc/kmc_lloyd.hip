@@ -53,20 +53,29 @@ __device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, G
     amax = wave_max_f32(amax);
     const int e = fp8_scale_exp(amax);
     uint8_t* q = P.Cq + (size_t)c * (P.DP + kFp8RowPad);
+    double d2 = 0.0;
 #pragma unroll
     for (int t = 0; t < kMaxDP / 64; ++t) {
       const int j = lane + 64 * t;
-      if (j < P.DP) q[j] = (uint8_t)fp8_e4m3_rne(ldexpf(w[t], -e));
+      if (j < P.DP) {
+        const uint32_t code = fp8_e4m3_rne(ldexpf(w[t], -e));
+        q[j] = (uint8_t)code;
+        const double dv = (double)ldexpf(fp8_e4m3_value(code), e) - (double)w[t];
+        d2 = fma(dv, dv, d2);
+      }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    d2 = wave_sum_f64(d2);
     if (lane == 0) {
       *reinterpret_cast<int*>(q + P.DP) = e + 127;
       P.cn[c] = (c < P.K && P.alive[c]) ? s : kFar;
+      if (P.cdn) P.cdn[c] = norm_up(d2);
     }
     return;
   }
   float s = 0.0f;
+  double d2 = 0.0;
   for (int j = lane; j < P.DP; j += 64) {
     const float v = (c < P.K) ? get(j) : 0.0f;
     const float w = -2.0f * v;  // exact scaling
@@ -78,11 +87,17 @@ __device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, G
       const uint16_t hi = bf16_rne(w);
       P.Chi[row + j] = hi;
       if (P.prec == PREC_BF16X3) P.Clo[row + j] = bf16_rne(w - bf16_to_f32(hi));
+      const double dv = (double)bf16_to_f32(hi) - (double)w;
+      d2 = fma(dv, dv, d2);
     }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0) P.cn[c] = (c < P.K && P.alive[c]) ? s : kFar;
+  if (P.cdn) d2 = wave_sum_f64(d2);
+  if (lane == 0) {
+    P.cn[c] = (c < P.K && P.alive[c]) ? s : kFar;
+    if (P.cdn) P.cdn[c] = norm_up(d2);
+  }
 }
 
 // k_prep: one wave per (padded) centroid row.
@@ -496,10 +511,22 @@ void k_assign(AssignParams P) {
     for (int w = 0; w < NW; ++w) cm_all = fmaxf(cm_all, red_s[w]);
     return sqrtf(cm_all);
   };
-  float cmax = 0.0f, Ecm = 0.0f;
+  float cmax = 0.0f, Ecm = 0.0f, cdmax = 0.0f;
   if constexpr (!BATCH) {
     cmax = cnorm_max(0, P.K);
     Ecm = P.beta * cmax * cmax;
+    if (P.hop > 0.0f) {
+      // max measured operand error of the centroid rows (all rows: padding
+      // rows carry 0, excluded clusters only make the bound larger)
+      float cd = 0.0f;
+      for (int c = threadIdx.x; c < P.Kp; c += blockDim.x) cd = fmaxf(cd, P.cdn[c]);
+      cd = wave_max_f32(cd);
+      __syncthreads();
+      if (lane == 0) red_s[wave] = cd;
+      __syncthreads();
+#pragma unroll
+      for (int w = 0; w < NW; ++w) cdmax = fmaxf(cdmax, red_s[w]);
+    }
   }
   const uint32_t keepv = opaque_vgpr(~P.idx_mask);
   const float ninf = __uint_as_float(opaque_vgpr(0xff800000u));
@@ -677,7 +704,13 @@ void k_assign(AssignParams P) {
       // distances.
       bool flag2 = false, flagA = false, few3 = false;
       if (h == 0 && valid) {
-        const float E = P.alpha * sqrtf(x2[m]) * cmax + Ecm + P.xpk * x2[m];
+        const float xn = sqrtf(x2[m]);
+        float E = P.alpha * xn * cmax + Ecm + P.xpk * x2[m];
+        if (!BATCH && P.hop > 0.0f) {
+          // |stored(-2c).stored(x) - (-2c).x| <= ||dc|| ||x|| + ||-2c|| ||dx|| + ||dc|| ||dx||
+          const float dx = P.xdn[prow];
+          E += P.hop * (cdmax * xn + 2.0f * cmax * dx + cdmax * dx);
+        }
         const float twoE = 2.0f * E;
         const float b1p = fmaxf(b1 + E, 0.0f);
         // clear(b): every centroid ranked at b or later loses to the first
@@ -2018,14 +2051,22 @@ __global__ __launch_bounds__(256) void k_load_points(LoadParams P) {
     for (int j = lane; j < P.D; j += 64) amax = fmaxf(amax, fabsf(src[j]));
     amax = wave_max_f32(amax);
     const int e = fp8_scale_exp(amax);
+    double d2 = 0.0;
     for (int j = lane; j < P.DP; j += 64) {
       const float v = j < P.D ? src[j] : 0.0f;
-      P.Xq[(size_t)i * P.DP + j] = (uint8_t)fp8_e4m3_rne(ldexpf(v, -e));
+      const uint32_t code = fp8_e4m3_rne(ldexpf(v, -e));
+      P.Xq[(size_t)i * P.DP + j] = (uint8_t)code;
+      const double dv = (double)ldexpf(fp8_e4m3_value(code), e) - (double)v;
+      d2 = fma(dv, dv, d2);
     }
-    if (lane == 0) P.xsc[i] = e + 127;
+    d2 = wave_sum_f64(d2);
+    if (lane == 0) {
+      P.xsc[i] = e + 127;
+      if (P.xdn) P.xdn[i] = norm_up(d2);
+    }
   }
   float s = 0.0f;
-  double s64 = 0.0;
+  double s64 = 0.0, d2 = 0.0;
   for (int j = lane; j < P.DP; j += 64) {
     const float v = j < P.D ? src[j] : 0.0f;
     P.X[(size_t)i * P.DP + j] = v;
@@ -2035,7 +2076,13 @@ __global__ __launch_bounds__(256) void k_load_points(LoadParams P) {
       const uint16_t hi = bf16_rne(v);
       P.Xhi[(size_t)i * P.DP + j] = hi;
       if (P.Xlo) P.Xlo[(size_t)i * P.DP + j] = bf16_rne(v - bf16_to_f32(hi));
+      const double dv = (double)bf16_to_f32(hi) - (double)v;
+      d2 = fma(dv, dv, d2);
     }
+  }
+  if (P.xdn && !P.Xq) {
+    d2 = wave_sum_f64(d2);
+    if (lane == 0) P.xdn[i] = norm_up(d2);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
