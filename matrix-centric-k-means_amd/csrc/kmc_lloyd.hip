@@ -825,7 +825,7 @@ __device__ __forceinline__ int region_of(const unsigned* pre_s, int nregions, un
 // strictly in dimension order. Padding dimensions are zero in X and C and add
 // exact zeros. An entry with two candidates repeats the second.
 constexpr int kR2Stride = 20;  // floats per LDS row of squared terms (b128 reads conflict-free)
-constexpr int kRefinePts = 4;  // points per all-k group
+constexpr int kRefinePts = 8;  // points per all-k group
 constexpr size_t kRefinePreBytes = ((size_t)(kMaxRegions + 1) * 4 + 15) / 16 * 16;
 constexpr size_t kRefineLds =
     kRefinePreBytes + (size_t)4 * 3 * 64 * kR2Stride * 4;  // two/three-candidate pass
