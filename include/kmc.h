@@ -166,6 +166,16 @@ kmc_status kmc_set_centroids(kmc_handle* h, const float* C0);
  * indices (may be NULL). */
 kmc_status kmc_init_plusplus(kmc_handle* h, uint64_t seed, int64_t* chosen);
 
+/* Greedy k-means++: each round draws `trials` candidates by D^2 sampling
+ * (round r >= 1, trial t uses uniform 1 + (r-1)*trials + t of the splitmix64
+ * sequence) and keeps the one whose potential sum_i min(D2_i, ||x_i - c||^2)
+ * (FP64, blocked fixed order) is smallest, the first on ties. This is the
+ * scheme of the reference's init_centroids_plus_plus (src/kmeans.cu:463-769,
+ * s = 2 + ceil(ln k) trials, compiled out there). trials = 0 selects that
+ * s; trials = 1 is kmc_init_plusplus; at most 16. */
+kmc_status kmc_init_plusplus_trials(kmc_handle* h, uint64_t seed, uint32_t trials,
+                                    int64_t* chosen);
+
 /* Initial labels (host, n values in [0,k)); the initial centroids become the
  * label means. Replaces init_centroids_rand (src/kmeans.cu:365-444), which
  * uses labels i % k. */

@@ -838,24 +838,35 @@ static double splitmix_uniform(uint64_t& state) {
 }
 
 kmc_status kmc_init_plusplus(kmc_handle* h, uint64_t seed, int64_t* chosen) {
+  return kmc_init_plusplus_trials(h, seed, 1, chosen);
+}
+
+kmc_status kmc_init_plusplus_trials(kmc_handle* h, uint64_t seed, uint32_t trials,
+                                    int64_t* chosen) {
   if (!h) return KMC_ERR_ARG;
   if (!h->have_points) return fail(h, KMC_ERR_STATE, "kmc_init_plusplus: points not set");
+  if (trials == 0) trials = 2u + (uint32_t)std::ceil(std::log((double)h->K));
+  if (trials > 16) return fail(h, KMC_ERR_ARG, "kmc_init_plusplus_trials: at most 16 trials");
   KMC_CK(hipSetDevice(h->dev));
-  std::vector<double> uni(h->K);
+  const size_t nu = 1 + (size_t)(h->K - 1) * trials;
+  std::vector<double> uni(nu);
   uint64_t st = seed;
-  for (int r = 0; r < h->K; ++r) uni[r] = splitmix_uniform(st);
+  for (size_t r = 0; r < nu; ++r) uni[r] = splitmix_uniform(st);
   const long long n = (long long)h->n;
   const size_t nb = (size_t)((n + 1023) / 1024);
-  double *du = nullptr, *D2 = nullptr, *bsum = nullptr;
-  long long* dch = nullptr;
-  hipError_t e = dalloc(&du, (size_t)h->K);
+  double *du = nullptr, *D2 = nullptr, *bsum = nullptr, *tsum = nullptr;
+  long long *dch = nullptr, *cand = nullptr;
+  hipError_t e = dalloc(&du, nu);
   if (e == hipSuccess) e = dalloc(&D2, (size_t)n);
   if (e == hipSuccess) e = dalloc(&bsum, nb);
   if (e == hipSuccess) e = dalloc(&dch, (size_t)h->K);
+  if (e == hipSuccess && trials > 1) e = dalloc(&cand, (size_t)trials);
+  if (e == hipSuccess && trials > 1) e = dalloc(&tsum, (size_t)trials * nb);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(du, uni.data(), h->K * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    e = hipMemcpyAsync(du, uni.data(), nu * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess)
-    e = kmc::launch_plusplus(h->X, n, h->DP, h->C, h->K, du, D2, bsum, dch, h->stream);
+    e = kmc::launch_plusplus(h->X, n, h->DP, h->C, h->K, du, D2, bsum, dch, h->stream,
+                             (int)trials, cand, tsum);
   if (e == hipSuccess && chosen)
     e = hipMemcpyAsync(chosen, dch, h->K * sizeof(long long), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream);
@@ -864,6 +875,8 @@ kmc_status kmc_init_plusplus(kmc_handle* h, uint64_t seed, int64_t* chosen) {
   (void)hipFree(D2);
   (void)hipFree(bsum);
   (void)hipFree(dch);
+  if (cand) (void)hipFree(cand);
+  if (tsum) (void)hipFree(tsum);
   KMC_CK(e);
   h->have_centroids = true;
   return KMC_OK;

@@ -250,11 +250,14 @@ hipError_t launch_absmax(const float* X, long long count, const double* xn64, lo
                          unsigned* amax, unsigned long long* qmax, hipStream_t s);
 hipError_t launch_load_points(const LoadParams& P, hipStream_t s);
 // k-means++ seeding of K centroid rows of C from the padded points X
-// (uni: K uniforms in [0,1); D2: n doubles; bsum: ceil(n/1024) doubles;
-// chosen: K point indices)
+// (uni: 1 + (K-1) * max(trials, 1) uniforms in [0,1); D2: n doubles; bsum:
+// ceil(n/1024) doubles; chosen: K point indices). trials > 1: greedy
+// k-means++ with that many D^2-sampled trials per round (at most 16; cand:
+// trials indices, tsum: trials * ceil(n/1024) doubles).
 hipError_t launch_plusplus(const float* X, long long n, int DP, float* C, int K,
                            const double* uni, double* D2, double* bsum, long long* chosen,
-                           hipStream_t s);
+                           hipStream_t s, int trials = 1, long long* cand = nullptr,
+                           double* tsum = nullptr);
 hipError_t launch_sum_f64(const double* v, long long n, double* out, double* part256,
                           hipStream_t s);
 hipError_t launch_fill_i32(int* p, long long n, int v, hipStream_t s);

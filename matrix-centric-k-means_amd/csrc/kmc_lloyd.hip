@@ -1718,17 +1718,138 @@ __global__ __launch_bounds__(256) void k_pp_pick(const double* bsum, int nb, con
   for (int j = threadIdx.x; j < DP; j += 256) C[(size_t)r * DP + j] = X[(size_t)idx * DP + j];
 }
 
+// Greedy k-means++ (several D^2-sampled trials per round, the one leaving
+// the smallest potential kept: the s = 2 + ceil(ln k) scheme of
+// src/kmeans.cu:463-769). k_pp_trials: workgroup t draws trial t of the round
+// as k_pp_pick does, from uniform uni[1 + (r-1) s + t]. k_pp_trial_cost:
+// per block of kPPBlock points, for every trial, sum_i min(D2_i, ||x_i -
+// trial||^2) in the k_pp_dist order. k_pp_choose: each trial's potential is
+// its block sums added in block order; the first smallest wins and becomes
+// centroid r.
+__global__ __launch_bounds__(256) void k_pp_trials(const double* bsum, int nb, const double* D2,
+                                                   long long n, int r, int s_trials,
+                                                   const double* uni, long long* cand) {
+  if (threadIdx.x != 0) return;
+  const int t = blockIdx.x;
+  const double u = uni[1 + (size_t)(r - 1) * s_trials + t];
+  long long idx = min((long long)(u * (double)n), n - 1);
+  double total = 0.0;
+  for (int b = 0; b < nb; ++b) total += bsum[b];
+  if (total > 0.0) {
+    const double target = u * total;
+    double run = 0.0;
+    int b = 0;
+    for (; b < nb - 1; ++b) {
+      if (run + bsum[b] > target) break;
+      run += bsum[b];
+    }
+    const long long lo = (long long)b * kPPBlock, hi = min(n, lo + kPPBlock);
+    long long pick = -1, last_pos = -1;
+    for (long long i = lo; i < hi; ++i) {
+      const double dv = D2[i];
+      if (dv > 0.0) last_pos = i;
+      run += dv;
+      if (run > target && dv > 0.0) {
+        pick = i;
+        break;
+      }
+    }
+    if (pick < 0) pick = last_pos;
+    if (pick >= 0) idx = pick;
+  }
+  cand[t] = idx;
+}
+
+constexpr int kPPMaxTrials = 16;
+__global__ __launch_bounds__(256) void k_pp_trial_cost(const float* X, long long n, int DP,
+                                                       const double* D2, const long long* cand,
+                                                       int s_trials, double* tsum) {
+  __shared__ float c_s[kPPMaxTrials][kMaxDP];
+  __shared__ double r_s[256];
+  for (int e = threadIdx.x; e < s_trials * DP; e += 256) {
+    const int t = e / DP, j = e - t * DP;
+    c_s[t][j] = X[(size_t)cand[t] * DP + j];
+  }
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kPPBlock;
+  for (int t = 0; t < s_trials; ++t) {
+    double part = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPPBlock / 256; ++u) {
+      const long long i = base + threadIdx.x + 256 * u;
+      if (i < n) {
+        const float* x = X + (size_t)i * DP;
+        double acc = 0.0;
+        for (int j = 0; j < DP; j += 4) {
+          const float4 v = *reinterpret_cast<const float4*>(x + j);
+          double d;
+          d = (double)v.x - (double)c_s[t][j];
+          acc = fma(d, d, acc);
+          d = (double)v.y - (double)c_s[t][j + 1];
+          acc = fma(d, d, acc);
+          d = (double)v.z - (double)c_s[t][j + 2];
+          acc = fma(d, d, acc);
+          d = (double)v.w - (double)c_s[t][j + 3];
+          acc = fma(d, d, acc);
+        }
+        part += fmin(D2[i], acc);
+      }
+    }
+    __syncthreads();  // r_s free
+    r_s[threadIdx.x] = part;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) r_s[threadIdx.x] += r_s[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) tsum[(size_t)t * gridDim.x + blockIdx.x] = r_s[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pp_choose(const double* tsum, int nb, int s_trials,
+                                                   const long long* cand, const float* X,
+                                                   int DP, float* C, int r, long long* chosen) {
+  __shared__ long long idx_s;
+  if (threadIdx.x == 0) {
+    double best = 0.0;
+    int bt = -1;
+    for (int t = 0; t < s_trials; ++t) {
+      double pot = 0.0;
+      for (int b = 0; b < nb; ++b) pot += tsum[(size_t)t * nb + b];
+      if (bt < 0 || pot < best) {
+        best = pot;
+        bt = t;
+      }
+    }
+    chosen[r] = cand[bt];
+    idx_s = cand[bt];
+  }
+  __syncthreads();
+  const long long idx = idx_s;
+  for (int j = threadIdx.x; j < DP; j += 256) C[(size_t)r * DP + j] = X[(size_t)idx * DP + j];
+}
+
 hipError_t launch_plusplus(const float* X, long long n, int DP, float* C, int K,
                            const double* uni, double* D2, double* bsum, long long* chosen,
-                           hipStream_t s) {
+                           hipStream_t s, int trials, long long* cand, double* tsum) {
+  if (trials > 1 && (trials > kPPMaxTrials || !cand || !tsum)) return hipErrorInvalidValue;
   const int nb = (int)((n + kPPBlock - 1) / kPPBlock);
   hipLaunchKernelGGL(k_pp_pick, dim3(1), dim3(256), 0, s, bsum, nb, D2, n, X, DP, C, 0, uni,
                      chosen);
   for (int r = 1; r < K; ++r) {
     hipLaunchKernelGGL(k_pp_dist, dim3(nb), dim3(256), 0, s, X, n, DP,
                        C + (size_t)(r - 1) * DP, r == 1 ? 1 : 0, D2, bsum);
-    hipLaunchKernelGGL(k_pp_pick, dim3(1), dim3(256), 0, s, bsum, nb, D2, n, X, DP, C, r, uni,
-                       chosen);
+    if (trials <= 1) {
+      hipLaunchKernelGGL(k_pp_pick, dim3(1), dim3(256), 0, s, bsum, nb, D2, n, X, DP, C, r, uni,
+                         chosen);
+    } else {
+      hipLaunchKernelGGL(k_pp_trials, dim3(trials), dim3(64), 0, s, bsum, nb, D2, n, r, trials,
+                         uni, cand);
+      hipLaunchKernelGGL(k_pp_trial_cost, dim3(nb), dim3(256), 0, s, X, n, DP, D2, cand, trials,
+                         tsum);
+      hipLaunchKernelGGL(k_pp_choose, dim3(1), dim3(256), 0, s, tsum, nb, trials, cand, X, DP, C,
+                         r, chosen);
+    }
   }
   return hipGetLastError();
 }

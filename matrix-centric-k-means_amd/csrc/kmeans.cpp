@@ -119,7 +119,9 @@ void Kmeans::init(const int* seed) {
       std::random_device rd;
       s = ((uint64_t)rd() << 32) ^ rd();
     }
-    check(h, kmc_init_plusplus(h, s, nullptr), "kmc_init_plusplus");
+    // greedy, 2 + ceil(ln k) trials per round: the scheme the reference's
+    // version implements (src/kmeans.cu:463-769)
+    check(h, kmc_init_plusplus_trials(h, s, 0, nullptr), "kmc_init_plusplus_trials");
   } else if (opts.reference_gpu) {
     // init_centroids_rand (src/kmeans.cu:365-444): labels i % k, the
     // initial centroids are the label means.

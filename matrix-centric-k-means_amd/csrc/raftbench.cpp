@@ -9,7 +9,7 @@
 // label), a libsvm file ("idx:value", 1-based), or, with no file (or "-1"),
 // uniform values in [-1e5, 1e5); 10 trials, the first a warm-up; per trial
 // the initial centroids (random: k distinct rows; plusplus: the engine's
-// k-means++), one explicit Lloyd run with the reference program's rules --
+// greedy k-means++), one explicit Lloyd run with the reference program's rules --
 // cost-ratio stop (cost_t / cost_{t-1} > 1 - tol from iteration 2), with
 // check = 1 also sum_c ||c_new - c_old||^2 < tol, empty clusters keep their
 // centroid -- and the final clustering cost. Printed at the end, as the
@@ -161,8 +161,9 @@ int main(int argc, char** argv) {
       st = kmc_set_centroids(h, C0.data());
       if (st != KMC_OK) die("kmc_set_centroids", st, h);
     } else {
-      st = kmc_init_plusplus(h, trial_seed, nullptr);
-      if (st != KMC_OK) die("kmc_init_plusplus", st, h);
+      // greedy k-means++ with 2 + ceil(ln k) trials, as RAFT's init_plus_plus
+      st = kmc_init_plusplus_trials(h, trial_seed, 0, nullptr);
+      if (st != KMC_OK) die("kmc_init_plusplus_trials", st, h);
     }
     auto s1 = std::chrono::system_clock::now();
     if (t > 0) init_time += std::chrono::duration<double>(s1 - s0).count();

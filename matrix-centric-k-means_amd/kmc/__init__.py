@@ -128,6 +128,7 @@ def load_library(path=LIB_PATH):
         "kmc_set_centroids": (ctypes.c_int, [vp, vp]),
         "kmc_set_labels": (ctypes.c_int, [vp, vp]),
         "kmc_init_plusplus": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
+        "kmc_init_plusplus_trials": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32, vp]),
         "kmc_run": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32, P(_Result)]),
         "kmc_get_labels": (ctypes.c_int, [vp, vp]),
         "kmc_get_centroids": (ctypes.c_int, [vp, vp]),
@@ -237,12 +238,14 @@ class Engine:
         lab = np.ascontiguousarray(labels, dtype=np.int32)
         self._check(self._lib.kmc_set_labels(self._h, lab.ctypes.data), "set_labels")
 
-    def init_plusplus(self, seed):
+    def init_plusplus(self, seed, trials=1):
         """k-means++ seeding of the initial centroids; returns the chosen
-        point indices."""
+        point indices. trials > 1: greedy k-means++ with that many D^2
+        trials per round; 0: 2 + ceil(ln k) trials."""
         out = np.empty(self.k, dtype=np.int64)
-        self._check(self._lib.kmc_init_plusplus(self._h, int(seed) & ((1 << 64) - 1),
-                                                out.ctypes.data), "init_plusplus")
+        self._check(self._lib.kmc_init_plusplus_trials(self._h, int(seed) & ((1 << 64) - 1),
+                                                       int(trials), out.ctypes.data),
+                    "init_plusplus")
         return out
 
     def run(self, maxiter, check_converged=True, time_kernels=False, graph=False,

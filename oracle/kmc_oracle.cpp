@@ -12,7 +12,7 @@
 //   3. orc_kk_*   : kernel k-means in the Popcorn formulation of the GPU
 //                   reference (src/kmeans.cu, src/kernels/distances.cu,
 //                   src/kernels/kernels.cuh), computed in FP64.
-//   4. orc_plusplus: k-means++ (D^2) seeding with the engine's fixed
+//   4. orc_plusplus(_trials): k-means++ (D^2) seeding, plain or greedy, with the engine's fixed
 //                   summation order (the reference's own, src/kmeans.cu:
 //                   447-769, is compiled out and broken, so the order is the
 //                   engine's; the mathematics is Arthur & Vassilvitskii 2007).
@@ -357,47 +357,50 @@ uint64_t orc_kk_lloyd(const double* K, size_t n, uint32_t k, uint64_t maxiter,
 // exceeds u_r * total (else the last block), then the first point of it with
 // D^2 > 0 at which the running sum exceeds the target (else the block's last
 // point with D^2 > 0); a zero total falls back to floor(u_r * n).
-void orc_plusplus(const float* X, size_t n, uint32_t d, uint32_t k, const double* u,
-                  int64_t* chosen, float* C) {
+// Greedy k-means++ (kmc_init_plusplus_trials; src/kmeans.cu:463-769): round r
+// >= 1 draws `trials` candidates by D^2 sampling from uniforms
+// u[1 + (r-1)*trials + t], each candidate's potential sum_i min(D2_i,
+// ||x_i - cand||^2) is summed in the same blocked order as D2, and the first
+// smallest becomes centroid r. trials = 1 is plain k-means++ (u[r]).
+void orc_plusplus_trials(const float* X, size_t n, uint32_t d, uint32_t k, uint32_t trials,
+                         const double* u, int64_t* chosen, float* C) {
   const size_t nb = (n + 1023) / 1024;
   std::vector<double> D2(n, 0.0), bsum(nb, 0.0);
   auto pick_row = [&](uint32_t r, int64_t idx) {
     chosen[r] = idx;
     for (uint32_t j = 0; j < d; ++j) C[(size_t)r * d + j] = X[(size_t)idx * d + j];
   };
-  int64_t first = (int64_t)(u[0] * (double)n);
-  if (first > (int64_t)n - 1) first = (int64_t)n - 1;
-  pick_row(0, first);
-  for (uint32_t r = 1; r < k; ++r) {
-    const float* c = C + (size_t)(r - 1) * d;
-    for (size_t b = 0; b < nb; ++b) {
-      double part[256];
-      for (int t = 0; t < 256; ++t) {
-        double ps = 0.0;
-        for (int q = 0; q < 4; ++q) {
-          const size_t i = b * 1024 + (size_t)t + 256 * (size_t)q;
-          if (i >= n) continue;
-          double acc = 0.0;
-          for (uint32_t j = 0; j < d; ++j) {
-            const double tt = (double)X[i * d + j] - (double)c[j];
-            acc = std::fma(tt, tt, acc);
-          }
-          const double dv = (r == 1) ? acc : std::fmin(D2[i], acc);
-          D2[i] = dv;
-          ps += dv;
-        }
-        part[t] = ps;
-      }
-      for (int o = 128; o > 0; o >>= 1)
-        for (int t = 0; t < o; ++t) part[t] += part[t + o];
-      bsum[b] = part[0];
+  auto dist = [&](size_t i, const float* c) {
+    double acc = 0.0;
+    for (uint32_t j = 0; j < d; ++j) {
+      const double tt = (double)X[i * d + j] - (double)c[j];
+      acc = std::fma(tt, tt, acc);
     }
-    int64_t idx = (int64_t)(u[r] * (double)n);
+    return acc;
+  };
+  // block b's sum of f(i) over its 1024 points: thread t adds points t,
+  // t+256, t+512, t+768 in order, then a pairwise tree over the threads
+  auto block_sum = [&](size_t b, auto&& f) {
+    double part[256];
+    for (int t = 0; t < 256; ++t) {
+      double ps = 0.0;
+      for (int q = 0; q < 4; ++q) {
+        const size_t i = b * 1024 + (size_t)t + 256 * (size_t)q;
+        if (i < n) ps += f(i);
+      }
+      part[t] = ps;
+    }
+    for (int o = 128; o > 0; o >>= 1)
+      for (int t = 0; t < o; ++t) part[t] += part[t + o];
+    return part[0];
+  };
+  auto sample = [&](double uu) {
+    int64_t idx = (int64_t)(uu * (double)n);
     if (idx > (int64_t)n - 1) idx = (int64_t)n - 1;
     double total = 0.0;
     for (size_t b = 0; b < nb; ++b) total += bsum[b];
     if (total > 0.0) {
-      const double target = u[r] * total;
+      const double target = uu * total;
       double run = 0.0;
       size_t b = 0;
       for (; b + 1 < nb; ++b) {
@@ -418,8 +421,45 @@ void orc_plusplus(const float* X, size_t n, uint32_t d, uint32_t k, const double
       if (pick < 0) pick = last_pos;
       if (pick >= 0) idx = pick;
     }
-    pick_row(r, idx);
+    return idx;
+  };
+  int64_t first = (int64_t)(u[0] * (double)n);
+  if (first > (int64_t)n - 1) first = (int64_t)n - 1;
+  pick_row(0, first);
+  const uint32_t s = trials < 1 ? 1 : trials;
+  for (uint32_t r = 1; r < k; ++r) {
+    const float* c = C + (size_t)(r - 1) * d;
+    for (size_t b = 0; b < nb; ++b)
+      bsum[b] = block_sum(b, [&](size_t i) {
+        const double acc = dist(i, c);
+        const double dv = (r == 1) ? acc : std::fmin(D2[i], acc);
+        D2[i] = dv;
+        return dv;
+      });
+    if (s == 1) {
+      pick_row(r, sample(u[r]));
+      continue;
+    }
+    int64_t best_idx = -1;
+    double best = 0.0;
+    for (uint32_t t = 0; t < s; ++t) {
+      const int64_t cand = sample(u[1 + (size_t)(r - 1) * s + t]);
+      const float* cr = X + (size_t)cand * d;
+      double pot = 0.0;
+      for (size_t b = 0; b < nb; ++b)
+        pot += block_sum(b, [&](size_t i) { return std::fmin(D2[i], dist(i, cr)); });
+      if (best_idx < 0 || pot < best) {
+        best = pot;
+        best_idx = cand;
+      }
+    }
+    pick_row(r, best_idx);
   }
+}
+
+void orc_plusplus(const float* X, size_t n, uint32_t d, uint32_t k, const double* u,
+                  int64_t* chosen, float* C) {
+  orc_plusplus_trials(X, n, d, k, 1, u, chosen, C);
 }
 
 }  // extern "C"

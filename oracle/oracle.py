@@ -45,6 +45,7 @@ class Oracle:
         L.orc_cpu_lloyd.argtypes = [_P, _sz, _u32, _u32, ctypes.c_float, _u64, _P, _P, _P, _P]
         L.orc_f64_row_norms.argtypes = [_P, _sz, _u32, _P]
         L.orc_plusplus.argtypes = [_P, _sz, _u32, _u32, _P, _P, _P]
+        L.orc_plusplus_trials.argtypes = [_P, _sz, _u32, _u32, _u32, _P, _P, _P]
         L.orc_f64_distances.argtypes = [_P, _sz, _u32, _P, _u32, _P]
         L.orc_f64_inertia.restype = ctypes.c_double
         L.orc_f64_inertia.argtypes = [_P, _sz, _u32, _P, _P]
@@ -113,13 +114,16 @@ class Oracle:
         self.L.orc_f64_distances(_ptr(X), X.shape[0], X.shape[1], _ptr(C), C.shape[0], _ptr(D))
         return D
 
-    def plusplus(self, X, k, uniforms):
-        """k-means++ seeding with the given uniforms: (chosen indices, C)."""
+    def plusplus(self, X, k, uniforms, trials=1):
+        """k-means++ seeding with the given uniforms (1 + (k-1)*trials of
+        them): (chosen indices, C). trials > 1: greedy k-means++."""
         X = np.ascontiguousarray(X, np.float32)
         u = np.ascontiguousarray(uniforms, np.float64)
+        assert u.size >= 1 + (k - 1) * max(1, trials)
         chosen = np.empty(k, np.int64)
         C = np.empty((k, X.shape[1]), np.float32)
-        self.L.orc_plusplus(_ptr(X), X.shape[0], X.shape[1], k, _ptr(u), _ptr(chosen), _ptr(C))
+        self.L.orc_plusplus_trials(_ptr(X), X.shape[0], X.shape[1], k, trials, _ptr(u),
+                                   _ptr(chosen), _ptr(C))
         return chosen, C
 
     def lloyd_explicit(self, X, C0, tol, maxiter, check):

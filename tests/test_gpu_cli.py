@@ -46,7 +46,10 @@ def test_cli_cpu_semantics_matches_reference(tmp_path, oracle, init):
     if init == "random":
         C0 = oracle.init_centroids(X, k, seed)
     else:
-        chosen, C0 = oracle.plusplus(X, k, splitmix_uniforms(seed, k))
+        # greedy k-means++ with 2 + ceil(ln k) trials per round
+        import math
+        st = 2 + math.ceil(math.log(k))
+        chosen, C0 = oracle.plusplus(X, k, splitmix_uniforms(seed, 1 + (k - 1) * st), trials=st)
     conv, want, C, cnt, ran = oracle.lloyd(X, C0, tol, maxiter)
     assert np.array_equal(lab, want.astype(np.int64))
     assert ("K-means converged at iteration %d" % conv) in text, text

@@ -258,3 +258,23 @@ def test_plusplus_matches_oracle(kmc_mod, oracle, n, d, k, seed):
     assert np.array_equal(chosen, want), (chosen, want)
     assert np.array_equal(C, X[want])
     assert len(set(chosen.tolist())) == k
+
+
+@pytest.mark.parametrize("n,d,k,seed,trials", [(5000, 20, 16, 3, 0), (70000, 64, 40, 11, 4),
+                                               (1797, 64, 10, 42, 0)])
+def test_greedy_plusplus_matches_oracle(kmc_mod, oracle, n, d, k, seed, trials):
+    """Greedy k-means++ (trials D^2 candidates per round, 0 = 2 + ceil(ln k))
+    on the GPU picks exactly the oracle's points."""
+    import math
+    from kmc import datagen
+    from oracle import splitmix_uniforms
+    X, _, _ = datagen.blobs(n, d, max(2, k // 2), seed, -8.0, 8.0, 0.5, 2.0)
+    s = trials if trials > 0 else 2 + math.ceil(math.log(k))
+    _say("greedy k-means++", n, d, k, seed, s)
+    with kmc_mod.Engine(n, d, k) as e:
+        e.set_points(X)
+        chosen = e.init_plusplus(seed, trials=trials)
+        C = e.centroids()
+    want, _ = oracle.plusplus(X, k, splitmix_uniforms(seed, 1 + (k - 1) * s), trials=s)
+    assert np.array_equal(chosen, want), (chosen, want)
+    assert np.array_equal(C, X[want])

@@ -220,6 +220,47 @@ def test_plusplus_matches_plain_sampler_on_exact_data(oracle):
         assert len(set(got.tolist())) == k
 
 
+def _plain_greedy_plusplus(X, k, u, s):
+    """Greedy k-means++ (s D^2 trials per round, the candidate leaving the
+    smallest potential kept, the first on ties) with plain cumulative sums."""
+    Xd = X.astype(np.float64)
+    n = X.shape[0]
+    chosen = [min(int(u[0] * n), n - 1)]
+    D2 = None
+    for r in range(1, k):
+        d = ((Xd - Xd[chosen[-1]]) ** 2).sum(1)
+        D2 = d if D2 is None else np.minimum(D2, d)
+        total = D2.sum()
+        cs = np.cumsum(D2)
+        best, best_pot = None, None
+        for t in range(s):
+            uu = u[1 + (r - 1) * s + t]
+            cand = int(np.nonzero((cs > uu * total) & (D2 > 0))[0][0])
+            pot = np.minimum(D2, ((Xd - Xd[cand]) ** 2).sum(1)).sum()
+            if best is None or pot < best_pot:
+                best, best_pot = cand, pot
+        chosen.append(best)
+    return np.array(chosen)
+
+
+def test_greedy_plusplus_matches_plain_sampler_on_exact_data(oracle):
+    """Greedy k-means++ (the reference's s = 2 + ceil(ln k) scheme,
+    src/kmeans.cu:463-769): on integer-valued data the oracle's blocked sums
+    must pick what plain cumulative sums pick; one trial is plain k-means++."""
+    from oracle import splitmix_uniforms
+    rng = np.random.default_rng(5)
+    for n, d, k, s, seed in [(900, 4, 10, 5, 11), (3000, 6, 16, 3, 12)]:
+        X = rng.integers(-40, 41, size=(n, d)).astype(np.float32)
+        u = splitmix_uniforms(seed, 1 + (k - 1) * s)
+        got, C = oracle.plusplus(X, k, u, trials=s)
+        want = _plain_greedy_plusplus(X, k, u, s)
+        assert np.array_equal(got, want), (n, d, k, s, got, want)
+        assert np.array_equal(C, X[got])
+    X = rng.standard_normal((1500, 8)).astype(np.float32)
+    u = splitmix_uniforms(3, 12)
+    assert np.array_equal(oracle.plusplus(X, 12, u, trials=1)[0], oracle.plusplus(X, 12, u)[0])
+
+
 def test_plusplus_first_pick_and_distinct(oracle):
     from oracle import splitmix_uniforms
     rng = np.random.default_rng(4)
