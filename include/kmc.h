@@ -225,6 +225,15 @@ kmc_status kmc_autotune(kmc_handle* h, kmc_tuning* out);
 /* Select a launch variant of the fused distance kernel directly. */
 kmc_status kmc_set_assign_variant(kmc_handle* h, int variant);
 
+/* Points whose nearest MFMA candidates all fall inside the margin band are
+ * refined against every centroid in the reference's float arithmetic. From
+ * min_points such points per assignment on (default 32768), that pass first
+ * narrows each point's centroids with a bf16x3 MFMA sweep and evaluates only
+ * the survivors; below it every centroid is evaluated. 1 selects the pruned
+ * pass always, 0 never. Labels are the same either way. A tuning knob of
+ * this library's refinement, with no counterpart in the reference. */
+kmc_status kmc_set_prune_min(kmc_handle* h, int32_t min_points);
+
 /* Release every device buffer (src/kmeans.cu:307-359). */
 kmc_status kmc_destroy(kmc_handle* h);
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/kmc.h"
+#include "kmc_copy.h"
 #include "kmc_kernels.h"
 
 using kmc::DevState;
@@ -80,6 +81,8 @@ struct kmc_handle {
   long long vqstride[8] = {0};
   int max_regions = 0;
   float alpha = 0, beta = 0, gref = 0, xpk = 0;
+  float alpha3 = 0, beta3 = 0, gref3 = 0;  // bf16x3 constants (pruned all-k pass)
+  int prune_min = kmc::kPruneMinPoints;
   int idx_bits = 0;
 
   bool have_points = false, have_centroids = false;
@@ -293,7 +296,9 @@ kmc_status reset_state(kmc_handle* h, uint64_t maxiter) {
   s.maxiter = maxiter;
   s.converged_at = maxiter;
   s.done = 0;
+  // s lives on this stack frame: the copy must be done before it returns
   KMC_CK(hipMemcpyAsync(h->state, &s, sizeof(DevState), hipMemcpyHostToDevice, h->stream));
+  KMC_CK(hipStreamSynchronize(h->stream));
   return KMC_OK;
 }
 
@@ -394,6 +399,12 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
   rp.Kp = h->Kp;
   rp.D = h->D;
   rp.DP = h->DP;
+  rp.cn = h->cn;
+  rp.xn2 = h->xn2;
+  rp.alpha3 = h->alpha3;
+  rp.beta3 = h->beta3;
+  rp.gref3 = h->gref3;
+  rp.prune_min = h->prune_min;
   KMC_CK(kmc::launch_refine(rp, h->stream));
   return KMC_OK;
 }
@@ -769,6 +780,9 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     while ((1 << h->idx_bits) < h->Kp) ++h->idx_bits;
     kmc::band_constants(h->prec, h->D, h->idx_bits, &h->alpha, &h->beta, &h->gref,
                         &h->xpk, true, &h->hop);
+    float xpk3 = 0.0f;
+    kmc::band_constants(kmc::PREC_BF16X3, h->D, h->idx_bits, &h->alpha3, &h->beta3, &h->gref3,
+                        &xpk3);
     st = alloc_all(h);
   } while (0);
   if (st != KMC_OK) {
@@ -819,8 +833,8 @@ kmc_status kmc_set_points_device(kmc_handle* h, const float* dX, size_t ld) {
 kmc_status kmc_set_centroids(kmc_handle* h, const float* C0) {
   if (!h || !C0) return fail(h, KMC_ERR_ARG, "kmc_set_centroids: bad argument");
   KMC_CK(hipSetDevice(h->dev));
-  KMC_CK(hipMemcpy2DAsync(h->C, h->DP * sizeof(float), C0, h->D * sizeof(float),
-                          h->D * sizeof(float), h->K, hipMemcpyHostToDevice, h->stream));
+  KMC_CK(kmc::rows_to_device(h->C, (size_t)h->DP, C0, (size_t)h->D, (size_t)h->D,
+                             (size_t)h->K, h->stream));
   KMC_CK(kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream));
   KMC_CK(hipStreamSynchronize(h->stream));
   h->have_centroids = true;
@@ -1007,6 +1021,17 @@ kmc_status kmc_set_assign_variant(kmc_handle* h, int variant) {
   return KMC_OK;
 }
 
+kmc_status kmc_set_prune_min(kmc_handle* h, int32_t min_points) {
+  if (!h || min_points < 0) return fail(h, KMC_ERR_ARG, "kmc_set_prune_min: bad argument");
+  h->prune_min = min_points;
+  // captured graphs carry the old value in their refinement launches
+  KMC_CK(hipSetDevice(h->dev));
+  KMC_CK(hipStreamSynchronize(h->stream));
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+  h->graphs.clear();
+  return KMC_OK;
+}
+
 kmc_status kmc_autotune(kmc_handle* h, kmc_tuning* out) {
   if (!h) return KMC_ERR_ARG;
   if (!h->have_points || !h->have_centroids)
@@ -1065,9 +1090,8 @@ kmc_status kmc_get_labels(kmc_handle* h, int32_t* labels) {
 kmc_status kmc_get_centroids(kmc_handle* h, float* C) {
   if (!h || !C) return fail(h, KMC_ERR_ARG, "kmc_get_centroids: bad argument");
   KMC_CK(hipSetDevice(h->dev));
-  KMC_CK(hipMemcpy2DAsync(C, h->D * sizeof(float), h->C, h->DP * sizeof(float),
-                          h->D * sizeof(float), h->K, hipMemcpyDeviceToHost, h->stream));
-  KMC_CK(hipStreamSynchronize(h->stream));
+  KMC_CK(kmc::rows_to_host(C, (size_t)h->D, h->C, (size_t)h->DP, (size_t)h->D, (size_t)h->K,
+                           h->stream));
   return KMC_OK;
 }
 

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "../../include/kmc.h"
+#include "kmc_copy.h"
 #include "kmc_device.h"
 #include "kmc_kernels.h"
 
@@ -415,6 +416,8 @@ hipError_t upload(T** p, const std::vector<T>& v, hipStream_t s) {
   hipError_t e = balloc(p, v.size());
   if (e == hipSuccess && !v.empty())
     e = hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+  // v may be a local of the caller that an early error return destroys
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e;
 }
 
@@ -731,6 +734,7 @@ kmc_status create_group(kmc_batch* b, BatchGroup* g, const std::vector<kmc_probl
   }
   KMC_BCK(hipMemcpyAsync(g->C0, c0.data(), c0.size() * sizeof(float), hipMemcpyHostToDevice,
                          b->stream));
+  KMC_BCK(hipStreamSynchronize(b->stream));  // c0 dies on an early error return
   // points: each problem's rows as one contiguous copy into a staging buffer
   // (a pitched copy from pageable memory goes row by row), then the norms,
   // padded rows and bf16 planes at the problem's offset
@@ -961,9 +965,8 @@ kmc_status kmc_batch_get_centroids(kmc_batch* b, uint32_t problem, float* C) {
   const int loc = b->loc_of[problem];
   const BatchProblem& e = g->hp[loc];
   const size_t d = g->d[loc];
-  KMC_BCK(hipMemcpy2D(C, d * sizeof(float), g->C + (size_t)e.coff * g->DP,
-                      (size_t)g->DP * sizeof(float), d * sizeof(float), (size_t)e.k,
-                      hipMemcpyDeviceToHost));
+  KMC_BCK(kmc::rows_to_host(C, d, g->C + (size_t)e.coff * g->DP, (size_t)g->DP, d,
+                            (size_t)e.k, b->stream));
   return KMC_OK;
 }
 

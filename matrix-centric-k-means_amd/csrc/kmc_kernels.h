@@ -125,7 +125,17 @@ struct RefineParams {
   // batch mode: the all-k pass is restricted to the point's own problem
   const BatchProblem* bp;  // null for a single problem
   const int* blk_prob;     // problem of each kBatchAlign-point block
+  // MFMA-pruned all-k pass (single problem; null cn: every centroid is
+  // evaluated exactly): ||c||^2 as prepared (kFar for padding rows and
+  // excluded clusters), ||x||^2, and the bf16x3 form's margin constants
+  const float* cn;
+  const float* xn2;
+  float alpha3, beta3, gref3;
+  int prune_min;  // all-k queues this long or longer take the pruned pass (0: never)
 };
+// all-k queue length from which the pruned pass runs: one 128-point group
+// per CU of the chip
+constexpr int kPruneMinPoints = 256 * 128;
 
 struct SortParams {
   const int* labels;

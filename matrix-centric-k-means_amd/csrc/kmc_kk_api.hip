@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/kmc.h"
+#include "kmc_copy.h"
 #include "kmc_kk.h"
 
 using kmc::KKState;
@@ -59,7 +60,9 @@ kmc_status reset_kk_state(kmc_kk_handle* h, uint64_t maxiter) {
   s.maxiter = maxiter;
   s.converged_at = maxiter;
   s.sum_diag = h->sum_diag;
+  // s lives on this stack frame: the copy must be done before it returns
   KK_CK(hipMemcpyAsync(h->P.state, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+  KK_CK(hipStreamSynchronize(h->stream));
   return KMC_OK;
 }
 
@@ -182,12 +185,7 @@ kmc_status kmc_kk_set_points(kmc_kk_handle* h, const float* X, size_t ld) {
   if (!h || !X || ld < (size_t)h->P.d) return kfail(h, KMC_ERR_ARG, "kmc_kk_set_points: bad argument");
   kmc::KKParams& P = h->P;
   KK_CK(hipSetDevice(h->dev));
-  if (ld == (size_t)P.d)  // one contiguous copy (a pitched copy from pageable memory goes row by row)
-    KK_CK(hipMemcpyAsync(h->dX, X, (size_t)P.n * P.d * sizeof(float), hipMemcpyHostToDevice,
-                         h->stream));
-  else
-    KK_CK(hipMemcpy2DAsync(h->dX, P.d * sizeof(float), X, ld * sizeof(float),
-                           P.d * sizeof(float), (size_t)P.n, hipMemcpyHostToDevice, h->stream));
+  KK_CK(kmc::rows_to_device(h->dX, (size_t)P.d, X, ld, (size_t)P.d, (size_t)P.n, h->stream));
   hipEvent_t e0, e1;
   KK_CK(hipEventCreate(&e0));
   KK_CK(hipEventCreate(&e1));
@@ -321,15 +319,14 @@ kmc_status kmc_kk_get_kernel_rows(kmc_kk_handle* h, size_t row0, size_t nrows, f
   const kmc::KKParams& P = h->P;
   KK_CK(hipSetDevice(h->dev));
   if (P.store == kmc::KS_FP32) {
-    KK_CK(hipMemcpy2D(out, (size_t)P.n * sizeof(float),
-                      reinterpret_cast<const float*>(P.Kmat) + row0 * (size_t)P.Np,
-                      (size_t)P.Np * sizeof(float), (size_t)P.n * sizeof(float), nrows,
-                      hipMemcpyDeviceToHost));
+    KK_CK(kmc::rows_to_host(out, (size_t)P.n,
+                            reinterpret_cast<const float*>(P.Kmat) + row0 * (size_t)P.Np,
+                            (size_t)P.Np, (size_t)P.n, nrows, h->stream));
   } else {
     std::vector<uint16_t> tmp(nrows * (size_t)P.n);
-    KK_CK(hipMemcpy2D(tmp.data(), (size_t)P.n * 2,
-                      reinterpret_cast<const uint16_t*>(P.Kmat) + row0 * (size_t)P.Np,
-                      (size_t)P.Np * 2, (size_t)P.n * 2, nrows, hipMemcpyDeviceToHost));
+    KK_CK(kmc::rows_to_host(tmp.data(), (size_t)P.n,
+                            reinterpret_cast<const uint16_t*>(P.Kmat) + row0 * (size_t)P.Np,
+                            (size_t)P.Np, (size_t)P.n, nrows, h->stream));
     for (size_t i = 0; i < tmp.size(); ++i) {
       const uint32_t u = ((uint32_t)tmp[i]) << 16;
       float f;

@@ -941,7 +941,8 @@ __device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsi
   int* const pid_s = reinterpret_cast<int*>(smem + kRefinePreBytes + kRefinePts * kMaxDP * 4 +
                                             32 * kRefinePts);
   const unsigned total = region_prefix(P.qacnt, P.nregions, pre_s);
-  if (total == 0) return;
+  // an MFMA-pruned pass (k_refine_prune) takes queues of prune_min or more
+  if (total == 0 || (P.prune_min > 0 && total >= (unsigned)P.prune_min)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned ngroups = (total + kRefinePts - 1) / kRefinePts;
   for (unsigned g = bid; g < ngroups; g += nblk) {
@@ -1073,6 +1074,351 @@ __global__ __launch_bounds__(256) void k_refine_all(RefineParams P) {
   if (*P.done) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
   refine_all_body<16>(P, rsm, blockIdx.x, gridDim.x);  // 32-dimension steps measured slower
+}
+
+// ---------------------------------------------------------------------------
+// All-k refinement with MFMA pruning (one problem; batches use k_refine_all).
+// A workgroup takes kPrunePts queued points, 32 per wave. Stage 1 forms the
+// squared distance of each point to every centroid in the bf16x3 distance
+// form -- x and -2c split into bf16 hi + lo on the fly, products hi.hi +
+// hi.lo + lo.hi, accumulator started at ||c||^2 + ||x||^2 -- whose error the
+// bf16x3 margin constants bound: E = alpha3 ||x|| cmax + beta3 cmax^2, with
+// gref3 covering the reference's own float error. The centroids pass through
+// LDS in chunks of KC rows; a first sweep finds each point's smallest value
+// b1, a second collects every centroid the band test cannot clear against
+// b1, so the reference's first minimum is among the candidates. Stage 2
+// evaluates the candidates in the reference's float arithmetic and keeps the
+// smallest distance, the lower index among equal ones. A point with more
+// than kPruneCap candidates, or without a finite stage-1 value (NaN data, no
+// live cluster), gets every live centroid evaluated exactly by the whole
+// workgroup instead.
+// ---------------------------------------------------------------------------
+constexpr int kPruneWaves = 4;
+constexpr int kPrunePts = 32 * kPruneWaves;
+constexpr int kPruneCap = 16;
+constexpr int kPruneBlocks = 256;
+template <int DP>
+constexpr int prune_chunk() {
+  return DP <= 64 ? 256 : (DP == 128 ? 128 : 32);
+}
+template <int DP>
+constexpr size_t prune_lds() {
+  using T = AssignTraits<DP, PREC_BF16X3>;
+  return kRefinePreBytes + (size_t)prune_chunk<DP>() * 4 +
+         (size_t)prune_chunk<DP>() * T::kRowBytes * 2 +
+         (size_t)kPrunePts * (4 + 4 + 8 + 4 + 4 * kPruneCap);
+}
+
+__device__ __forceinline__ unsigned long long dist_key(float d, int c) {
+  return ((unsigned long long)__float_as_uint(d) << 32) | (unsigned)c;
+}
+__device__ __forceinline__ unsigned long long min_key_xor(unsigned long long k, int o) {
+  const unsigned lo = __shfl_xor((unsigned)k, o, 64);
+  const unsigned hi = __shfl_xor((unsigned)(k >> 32), o, 64);
+  const unsigned long long ok = ((unsigned long long)hi << 32) | lo;
+  return ok < k ? ok : k;
+}
+
+// Squared distance in the reference's float arithmetic over the DP padded
+// dimensions (padding is zero in X and C and adds exact zeros): in index
+// order, one rounding for each subtraction, product and addition; the next
+// 32 dimensions (16 for DP = 256) of both rows are loaded before the current
+// ones are summed.
+template <int DP>
+__device__ __forceinline__ float exact_dist_rows(const float* __restrict__ x,
+                                                 const float* __restrict__ c) {
+#pragma clang fp contract(off)
+  constexpr int CH = DP < 32 ? DP : (DP >= 256 ? 16 : 32);
+  constexpr int NV = CH / 4;
+  float4 xa[NV], ca[NV], xb[NV], cb[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    xa[v] = *reinterpret_cast<const float4*>(x + 4 * v);
+    ca[v] = *reinterpret_cast<const float4*>(c + 4 * v);
+  }
+  float acc = 0.0f;
+#pragma unroll
+  for (int j0 = 0; j0 < DP; j0 += CH) {
+    if (j0 + CH < DP) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        xb[v] = *reinterpret_cast<const float4*>(x + j0 + CH + 4 * v);
+        cb[v] = *reinterpret_cast<const float4*>(c + j0 + CH + 4 * v);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      float t, sq;
+      t = xa[v].x - ca[v].x; sq = t * t; acc = acc + sq;
+      t = xa[v].y - ca[v].y; sq = t * t; acc = acc + sq;
+      t = xa[v].z - ca[v].z; sq = t * t; acc = acc + sq;
+      t = xa[v].w - ca[v].w; sq = t * t; acc = acc + sq;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      xa[v] = xb[v];
+      ca[v] = cb[v];
+    }
+  }
+  return acc;
+}
+
+// bf16x3 chain of one 32-centroid block of the LDS chunk against a wave's
+// 32 points (the order of mfma_chain_lds): A fragments are read four k-steps
+// at a time, with a scheduling barrier between groups, so that no more than
+// four steps' fragments are live beside the register-resident points.
+template <int DP>
+__device__ __forceinline__ kmc_f32x16 prune_chain(const unsigned char* rows, int cb, int p, int h,
+                                                  const XFrag<DP, PREC_BF16X3>& x,
+                                                  kmc_f32x16 acc) {
+  using T = AssignTraits<DP, PREC_BF16X3>;
+  constexpr int KC = prune_chunk<DP>();
+  constexpr int NS = DP / 16;
+  constexpr int G = NS < 4 ? NS : 4;
+  const uint16_t* ah = reinterpret_cast<const uint16_t*>(rows + (size_t)(cb + p) * T::kRowBytes);
+  const uint16_t* al = reinterpret_cast<const uint16_t*>(rows + (size_t)KC * T::kRowBytes +
+                                                         (size_t)(cb + p) * T::kRowBytes);
+#pragma unroll
+  for (int s0 = 0; s0 < NS; s0 += G) {
+    uint4 hv[G], lv[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      hv[i] = *reinterpret_cast<const uint4*>(ah + 16 * (s0 + i) + 8 * h);
+      lv[i] = *reinterpret_cast<const uint4*>(al + 16 * (s0 + i) + 8 * h);
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int st = s0 + i;
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, hv[i]),
+                                                    __builtin_bit_cast(kmc_bf16x8, x.hi[st]),
+                                                    acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, hv[i]),
+                                                    __builtin_bit_cast(kmc_bf16x8, x.lo[st]),
+                                                    acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, lv[i]),
+                                                    __builtin_bit_cast(kmc_bf16x8, x.hi[st]),
+                                                    acc, 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  return acc;
+}
+
+template <int DP>
+__global__ __launch_bounds__(256) void k_refine_prune(RefineParams P) {
+  if (*P.done) return;
+  using T = AssignTraits<DP, PREC_BF16X3>;
+  constexpr int KC = prune_chunk<DP>();
+  extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
+  unsigned* const pre_s = reinterpret_cast<unsigned*>(rsm);
+  float* const cn_s = reinterpret_cast<float*>(rsm + kRefinePreBytes);
+  unsigned char* const rows = rsm + kRefinePreBytes + (size_t)KC * 4;
+  unsigned long long* const key_s =
+      reinterpret_cast<unsigned long long*>(rows + (size_t)KC * T::kRowBytes * 2);
+  int* const cnt_s = reinterpret_cast<int*>(key_s + kPrunePts);
+  int* const pid_s = cnt_s + kPrunePts;
+  int* const ovf_s = pid_s + kPrunePts;
+  int* const cand_s = ovf_s + kPrunePts;
+  __shared__ float red_s[kPruneWaves];
+
+  const unsigned total = region_prefix(P.qacnt, P.nregions, pre_s);
+  // shorter queues go to k_refine_all: a group's sweeps are long, so the
+  // pruned pass pays only once its groups fill the chip
+  if (total < (unsigned)max(P.prune_min, 1)) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = lane & 31, h = lane >> 5;
+  const float kInf = __builtin_inff();
+
+  // max ||c|| over the live centroids (excluded ones carry kFar)
+  float cm = 0.0f;
+  for (int c = threadIdx.x; c < P.K; c += 256) {
+    const float v = P.cn[c];
+    if (v < kFarTest) cm = fmaxf(cm, v);
+  }
+  cm = wave_max_f32(cm);
+  if (lane == 0) red_s[wave] = cm;
+  __syncthreads();
+  float cmax = 0.0f;
+#pragma unroll
+  for (int w = 0; w < kPruneWaves; ++w) cmax = fmaxf(cmax, red_s[w]);
+  cmax = sqrtf(cmax);
+  const float Ecm = P.beta3 * cmax * cmax;
+
+  const unsigned ngroups = (total + kPrunePts - 1) / kPrunePts;
+  const int nchunks = (P.Kp + KC - 1) / KC;
+  // chunk rows [c0, c0 + KC): thread t holds float4 t + 256 i of the chunk
+  constexpr int kFill = KC * (DP / 4) / 256;
+  static_assert(KC * (DP / 4) % 256 == 0, "chunk fill splits evenly");
+  float4 pf[kFill];
+  auto fetch_chunk = [&](int c0) {
+#pragma unroll
+    for (int i = 0; i < kFill; ++i) {
+      const int v = threadIdx.x + 256 * i;
+      const int r = v / (DP / 4), q = v - r * (DP / 4);
+      const int c = c0 + r;
+      pf[i] = c < P.K ? *reinterpret_cast<const float4*>(P.C + (size_t)c * DP + 4 * q)
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  // -2c as bf16 hi and lo planes, ||c||^2 (kFar past Kp)
+  auto store_chunk = [&](int c0) {
+#pragma unroll
+    for (int i = 0; i < kFill; ++i) {
+      const int v = threadIdx.x + 256 * i;
+      const int r = v / (DP / 4), q = v - r * (DP / 4);
+      const float w[4] = {-2.0f * pf[i].x, -2.0f * pf[i].y, -2.0f * pf[i].z, -2.0f * pf[i].w};
+      uint16_t hb[4], lb[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        hb[e] = bf16_rne(w[e]);
+        lb[e] = bf16_rne(w[e] - bf16_to_f32(hb[e]));
+      }
+      unsigned char* rp = rows + (size_t)r * T::kRowBytes + 8 * q;
+      *reinterpret_cast<uint2*>(rp) = make_uint2((uint32_t)hb[0] | ((uint32_t)hb[1] << 16),
+                                                 (uint32_t)hb[2] | ((uint32_t)hb[3] << 16));
+      *reinterpret_cast<uint2*>(rp + (size_t)KC * T::kRowBytes) =
+          make_uint2((uint32_t)lb[0] | ((uint32_t)lb[1] << 16),
+                     (uint32_t)lb[2] | ((uint32_t)lb[3] << 16));
+      __builtin_amdgcn_sched_barrier(0);  // convert and store one float4 at a time
+    }
+    for (int r = threadIdx.x; r < KC; r += 256) cn_s[r] = c0 + r < P.Kp ? P.cn[c0 + r] : kFar;
+  };
+  for (unsigned g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    __syncthreads();  // the previous group is done with the LDS
+    if (threadIdx.x < kPrunePts) {
+      const unsigned t = g * kPrunePts + threadIdx.x;
+      int pid = -1;
+      if (t < total) {
+        const int r = region_of(pre_s, P.nregions, t);
+        pid = P.qa[(size_t)r * P.qstride + (t - pre_s[r])];
+      }
+      pid_s[threadIdx.x] = pid;
+      cnt_s[threadIdx.x] = 0;
+      key_s[threadIdx.x] = ~0ull;
+    }
+    __syncthreads();
+    const int slot = wave * 32 + p;
+    const int pid = pid_s[slot];
+    const bool valid = pid >= 0;
+    const float* const xr = P.X + (size_t)(valid ? pid : 0) * DP;
+    // B fragments: dims 16 s + 8 h + [0, 8) of the point, x = hi + lo
+    XFrag<DP, PREC_BF16X3> xf;
+#pragma unroll
+    for (int s = 0; s < DP / 16; ++s) {
+      float v[8];
+      const float4 a = *reinterpret_cast<const float4*>(xr + 16 * s + 8 * h);
+      const float4 b = *reinterpret_cast<const float4*>(xr + 16 * s + 8 * h + 4);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+      uint32_t hw[4], lw[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float x0 = valid ? v[2 * e] : 0.0f, x1 = valid ? v[2 * e + 1] : 0.0f;
+        const uint16_t h0 = bf16_rne(x0), h1 = bf16_rne(x1);
+        const uint16_t l0 = bf16_rne(x0 - bf16_to_f32(h0)), l1 = bf16_rne(x1 - bf16_to_f32(h1));
+        hw[e] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+        lw[e] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+      }
+      xf.hi[s] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+      xf.lo[s] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    }
+    const float x2 = valid ? P.xn2[pid] : 0.0f;
+
+    float b1 = kInf, E = 0.0f, twoE = 0.0f, b1p = 0.0f;
+    // the chunk sequence of the two sweeps (a single chunk is loaded once and
+    // stays resident); the next chunk's rows are fetched into registers
+    // while the current one feeds the MFMAs
+    const int nfill = nchunks == 1 ? 1 : 2 * nchunks;
+    fetch_chunk(0);
+    int fill = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int ch = 0; ch < nchunks; ++ch) {
+        const int c0 = ch * KC;
+        if (pass == 0 || nchunks > 1) {
+          __syncthreads();  // the previous chunk's MFMAs are done with the LDS
+          store_chunk(c0);
+          __syncthreads();
+          ++fill;
+          if (fill < nfill) fetch_chunk((fill % nchunks) * KC);
+        }
+#pragma unroll 1
+        for (int cb = 0; cb < KC && c0 + cb < P.Kp; cb += 32) {
+          kmc_f32x16 acc = init_acc<true>(cn_s, cb, h, x2);
+          acc = prune_chain<DP>(rows, cb, p, h, xf, acc);
+          if (pass == 0) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) b1 = fminf(b1, acc[i]);
+          } else {
+#pragma unroll
+            for (int gq = 0; gq < 4; ++gq) {
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                const float v = acc[4 * gq + j];
+                const int c = c0 + cb + 8 * gq + 4 * h + j;
+                const bool cand = valid && c < P.K && v < kFarTest &&
+                                  !(v - b1 > twoE + P.gref3 * (fmaxf(v + E, 0.0f) + b1p));
+                if (cand) {
+                  const int pos = atomicAdd(&cnt_s[slot], 1);
+                  if (pos < kPruneCap) cand_s[slot * kPruneCap + pos] = c;
+                }
+              }
+            }
+          }
+        }
+      }
+      if (pass == 0) {
+        b1 = fminf(b1, __shfl_xor(b1, 32, 64));
+        E = P.alpha3 * sqrtf(x2) * cmax + Ecm;
+        twoE = 2.0f * E;
+        b1p = fmaxf(b1 + E, 0.0f);
+      }
+    }
+    __syncthreads();
+
+    // stage 2: lanes p and p + 32 take the even and odd candidates
+    const int nc = cnt_s[slot];
+    const bool ovf = valid && (nc > kPruneCap || !(b1 < kFarTest));
+    unsigned long long best = ~0ull;
+    if (valid && !ovf) {
+#pragma unroll 1
+      for (int i = h; i < nc; i += 2) {
+        const int c = cand_s[slot * kPruneCap + i];
+        const float d = exact_dist_rows<DP>(xr, P.C + (size_t)c * DP);
+        if (d < kInf) {
+          const unsigned long long k = dist_key(d, c);
+          best = k < best ? k : best;
+        }
+      }
+    }
+    best = min_key_xor(best, 32);
+    if (h == 0) {
+      ovf_s[slot] = ovf ? 1 : 0;
+      if (valid && !ovf) P.labels[pid] = best == ~0ull ? 0 : (int)(unsigned)best;
+    }
+    __syncthreads();
+
+    // points the pruning could not narrow: every live centroid, exactly
+    for (int q = 0; q < kPrunePts; ++q) {
+      if (!ovf_s[q]) continue;
+      const int qp = pid_s[q];
+      const float* const xq = P.X + (size_t)qp * DP;
+      unsigned long long bq = ~0ull;
+#pragma unroll 1
+      for (int c = threadIdx.x; c < P.K; c += 256) {
+        if (!P.alive[c]) continue;
+        const float d = exact_dist_rows<DP>(xq, P.C + (size_t)c * DP);
+        if (d < kInf) {
+          const unsigned long long k = dist_key(d, c);
+          bq = k < bq ? k : bq;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) bq = min_key_xor(bq, o);
+      if (lane == 0) atomicMin(&key_s[q], bq);
+      __syncthreads();
+      if (threadIdx.x == 0) P.labels[qp] = key_s[q] == ~0ull ? 0 : (int)(unsigned)key_s[q];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2395,6 +2741,20 @@ hipError_t launch_prep(const PrepParams& P, hipStream_t s) {
   hipLaunchKernelGGL(k_prep, dim3((P.Kp + 3) / 4), dim3(256), 0, s, P);
   return hipGetLastError();
 }
+template <int DP>
+static hipError_t launch_prune(const RefineParams& P, hipStream_t s) {
+  static bool lds_set = false;
+  if (!lds_set) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_refine_prune<DP>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)prune_lds<DP>());
+    if (e != hipSuccess) return e;
+    lds_set = true;
+  }
+  hipLaunchKernelGGL(k_refine_prune<DP>, dim3(kPruneBlocks), dim3(256), prune_lds<DP>(), s, P);
+  return hipGetLastError();
+}
+
 hipError_t launch_refine(const RefineParams& P, hipStream_t s) {
   static bool lds_set = false;  // opt in to the large dynamic LDS once per process
   if (!lds_set) {
@@ -2406,7 +2766,15 @@ hipError_t launch_refine(const RefineParams& P, hipStream_t s) {
   }
   hipLaunchKernelGGL(k_refine2, dim3(kRefineBlocks), dim3(256), kRefineLds, s, P);
   hipLaunchKernelGGL(k_refine_all, dim3(kRefineBlocks), dim3(256), kRefineAllLds, s, P);
-  return hipGetLastError();
+  if (P.bp || !P.cn || !P.xn2 || P.prune_min <= 0) return hipGetLastError();
+  switch (P.DP) {
+    case 16: return launch_prune<16>(P, s);
+    case 32: return launch_prune<32>(P, s);
+    case 64: return launch_prune<64>(P, s);
+    case 128: return launch_prune<128>(P, s);
+    case 256: return launch_prune<256>(P, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 hipError_t launch_sort(const SortParams& P, hipStream_t s) {
   const size_t lds = (size_t)P.K * sizeof(int);

@@ -140,6 +140,7 @@ def load_library(path=LIB_PATH):
         "kmc_memory_footprint": (ctypes.c_int, [ctypes.c_int, P(ctypes.c_size_t)]),
         "kmc_autotune": (ctypes.c_int, [vp, P(_Tuning)]),
         "kmc_set_assign_variant": (ctypes.c_int, [vp, ctypes.c_int]),
+        "kmc_set_prune_min": (ctypes.c_int, [vp, ctypes.c_int32]),
         "kmc_kk_create": (ctypes.c_int, [P(_KKConfig), P(vp)]),
         "kmc_kk_destroy": (ctypes.c_int, [vp]),
         "kmc_kk_set_points": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
@@ -275,6 +276,11 @@ class Engine:
 
     def set_variant(self, v):
         self._check(self._lib.kmc_set_assign_variant(self._h, int(v)), "set_assign_variant")
+
+    def set_prune_min(self, n):
+        """All-k refinement queues of n or more points take the MFMA-pruned
+        pass (1: always, 0: never; default 32768)."""
+        self._check(self._lib.kmc_set_prune_min(self._h, int(n)), "set_prune_min")
 
     def labels(self):
         out = np.empty(self.n, dtype=np.int32)

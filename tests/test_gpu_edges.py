@@ -10,10 +10,12 @@ pytestmark = pytest.mark.gpu
 PRECS = ("fp32", "bf16x3", "bf16", "fp8")
 
 
-def _check(kmc_mod, oracle, X, C0, prec, tol=1e-4, maxiter=50, update="ordered"):
+def _check(kmc_mod, oracle, X, C0, prec, tol=1e-4, maxiter=50, update="ordered", prune=None):
     want_conv, want_lab, want_C, _, _ = oracle.lloyd(X, C0, tol, maxiter)
     with kmc_mod.Engine(X.shape[0], X.shape[1], C0.shape[0], tol=tol, precision=prec,
                         update=update) as e:
+        if prune is not None:
+            e.set_prune_min(prune)
         e.set_points(X)
         e.set_centroids(C0)
         r = e.run(maxiter, check_converged=True)
@@ -39,35 +41,66 @@ def test_fewer_points_than_a_tile(kmc_mod, oracle, prec):
     _check(kmc_mod, oracle, X, X[[0, 2]].copy(), prec)
 
 
+@pytest.mark.parametrize("prune", [0, 1])
 @pytest.mark.parametrize("prec", PRECS)
-def test_one_dimension(kmc_mod, oracle, prec):
+def test_one_dimension(kmc_mod, oracle, prec, prune):
     rng = np.random.default_rng(2)
     X = np.concatenate([rng.normal(c, 0.5, size=(300, 1)) for c in (-5, 0, 5)]).astype(np.float32)
-    _check(kmc_mod, oracle, X, X[[0, 300, 600]].copy(), prec)
+    _check(kmc_mod, oracle, X, X[[0, 300, 600]].copy(), prec, prune=prune)
 
 
+@pytest.mark.parametrize("prune", [0, 1])
 @pytest.mark.parametrize("prec", PRECS)
-def test_identical_points_tie_to_first_centroid(kmc_mod, oracle, prec):
+def test_identical_points_tie_to_first_centroid(kmc_mod, oracle, prec, prune):
     X = np.full((2000, 16), 3.25, np.float32)
     C0 = np.stack([np.full(16, 3.0, np.float32), np.full(16, 3.5, np.float32)])
-    lab = _check(kmc_mod, oracle, X, C0, prec)
+    lab = _check(kmc_mod, oracle, X, C0, prec, prune=prune)
     assert not lab.any()  # equidistant: the first index wins (strict <)
 
 
+@pytest.mark.parametrize("prune", [0, 1])
 @pytest.mark.parametrize("prec", PRECS)
-def test_duplicate_centroids(kmc_mod, oracle, prec):
+def test_duplicate_centroids(kmc_mod, oracle, prec, prune):
     rng = np.random.default_rng(3)
     X = rng.normal(0, 1, size=(3000, 24)).astype(np.float32)
     C0 = np.concatenate([X[:5], X[:5]])  # clusters 5..9 duplicate 0..4
-    lab = _check(kmc_mod, oracle, X, C0, prec, maxiter=1)
+    lab = _check(kmc_mod, oracle, X, C0, prec, maxiter=1, prune=prune)
     assert lab.max() < 5  # a duplicate never beats its first copy
 
 
+@pytest.mark.parametrize("prune", [0, 1])
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("d", [3, 33, 130])
-def test_padded_widths(kmc_mod, oracle, prec, d):
+def test_padded_widths(kmc_mod, oracle, prec, d, prune):
     rng = np.random.default_rng(d)
     X = rng.normal(0, 2, size=(4000, d)).astype(np.float32)
     C0 = X[rng.choice(4000, 12, replace=False)]
-    _check(kmc_mod, oracle, X, C0, prec, maxiter=20)
-    _check(kmc_mod, oracle, X, C0, prec, maxiter=20, update="tree")
+    _check(kmc_mod, oracle, X, C0, prec, maxiter=20, prune=prune)
+    _check(kmc_mod, oracle, X, C0, prec, maxiter=20, update="tree", prune=prune)
+
+
+@pytest.mark.parametrize("prune", [0, 1])
+@pytest.mark.parametrize("prec", PRECS)
+def test_many_tied_candidates(kmc_mod, oracle, prec, prune):
+    """Twelve copies of one centroid among twenty others: points near the
+    copies have twelve exactly tied candidates, which the all-k pass narrows
+    and evaluates one by one (the first copy wins)."""
+    rng = np.random.default_rng(11)
+    X = rng.normal(0, 1, size=(4000, 48)).astype(np.float32)
+    others = rng.normal(0, 3, size=(20, 48)).astype(np.float32)
+    C0 = np.concatenate([others[:7], np.repeat(X[:1], 12, axis=0), others[7:]])
+    lab = _check(kmc_mod, oracle, X, C0, prec, maxiter=1, prune=prune)
+    assert not np.any((lab > 7) & (lab < 19))  # never a later copy
+
+
+@pytest.mark.parametrize("prune", [0, 1])
+@pytest.mark.parametrize("prec", PRECS)
+def test_all_centroids_tied(kmc_mod, oracle, prec, prune):
+    """Forty identical centroids: every point has forty tied candidates, more
+    than the pruned all-k pass keeps, so every centroid is evaluated exactly
+    and the first one wins."""
+    rng = np.random.default_rng(12)
+    X = rng.normal(0, 1, size=(3000, 40)).astype(np.float32)
+    C0 = np.repeat(X[5:6], 40, axis=0)
+    lab = _check(kmc_mod, oracle, X, C0, prec, maxiter=1, prune=prune)
+    assert not lab.any()

@@ -188,6 +188,25 @@ def test_every_launch_variant_gives_the_same_labels(kmc_mod, oracle, c2_data, pr
             assert np.array_equal(lab, want), v
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_pruned_all_k_pass_gives_the_same_labels(kmc_mod, oracle, c2_data, prec):
+    """The all-k refinement with and without its bf16x3 MFMA pruning stage
+    reproduces the reference's labels on a 200K-point slice; centroids
+    perturbed towards each other put many points in the band."""
+    X, C0 = c2_data
+    Xs = np.ascontiguousarray(X[:200_000])
+    C = C0.copy()
+    C[1::2] = (0.9 * C0[1::2] + 0.1 * C0[0::2]).astype(np.float32)  # pairs closer together
+    want, _ = _reference_labels(oracle, Xs, C)
+    with kmc_mod.Engine(Xs.shape[0], 128, 256, precision=prec) as e:
+        e.set_points(Xs)
+        for pm in (1, 0):
+            e.set_prune_min(pm)
+            lab, info = e.assign(C)
+            _say("prune_min", pm, prec, "refined", info["refined_two"], info["refined_full"])
+            assert np.array_equal(lab, want), pm
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("update", ["ordered", "tree", "fixed"])
 def test_graph_replays_match_eager(kmc_mod, prec, update):
