@@ -47,6 +47,8 @@ struct kmc_handle {
   int* q2 = nullptr;
   int* qa = nullptr;
   unsigned* q2cnt = nullptr;
+  uint16_t* Phi = nullptr;  // bf16 hi / lo of -2c for the pruned all-k passes
+  uint16_t* Plo = nullptr;  // (the bf16x3 form uses Chi / Clo)
   unsigned* qacnt = nullptr;
   long long qstride = 0;
   // sort / update
@@ -235,6 +237,10 @@ kmc_status alloc_all(kmc_handle* h) {
   else KMC_CK(dalloc(&h->Chi, Kp * DP));
   if (h->prec == kmc::PREC_BF16X3) KMC_CK(dalloc(&h->Clo, Kp * DP));
   KMC_CK(dalloc(&h->cn, Kp));
+  if (h->prec != kmc::PREC_BF16X3) {
+    KMC_CK(dalloc(&h->Phi, Kp * DP));
+    KMC_CK(dalloc(&h->Plo, Kp * DP));
+  }
   KMC_CK(dalloc(&h->CT, Kp * DP));
   KMC_CK(dalloc(&h->labels, n));
   size_t qtot = 0;
@@ -278,7 +284,7 @@ void free_all(kmc_handle* h) {
                   h->bcount,
                   h->count, h->start, h->members, h->ipart, h->tpart, h->conv,
                   h->shift2, h->alive, h->state, h->xn64, h->fpart, h->qpart, h->cpart,
-                  h->Xq, h->xsc, h->Cq, h->xdn, h->cdn};
+                  h->Xq, h->xsc, h->Cq, h->xdn, h->cdn, h->Phi, h->Plo};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -331,6 +337,8 @@ kmc::PrepParams prep_params(kmc_handle* h) {
   pp.Cq = h->Cq;
   pp.cdn = h->cdn;
   pp.cn = h->cn;
+  pp.Phi = h->Phi;
+  pp.Plo = h->Plo;
   pp.CT = h->CT;
   pp.alive = h->alive;
   pp.done = done_ptr(h);
@@ -401,6 +409,8 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
   rp.DP = h->DP;
   rp.cn = h->cn;
   rp.xn2 = h->xn2;
+  rp.Phi = h->prec == kmc::PREC_BF16X3 ? h->Chi : h->Phi;
+  rp.Plo = h->prec == kmc::PREC_BF16X3 ? h->Clo : h->Plo;
   rp.alpha3 = h->alpha3;
   rp.beta3 = h->beta3;
   rp.gref3 = h->gref3;

@@ -51,6 +51,10 @@ struct PrepParams {
   float* cdn;      // Kp: ||stored operand - (-2c)||, rounded up (single-product forms)
   float* cn;       // Kp, ||c||^2 (kFar for padding rows and excluded clusters)
   float* CT;       // DP x Kp transposed centroids (all-k refinement)
+  // Kp x DP bf16 hi and lo of -2c for the pruned all-k passes (null: not
+  // written; the bf16x3 form's Chi / Clo hold the same values)
+  uint16_t* Phi;
+  uint16_t* Plo;
   const int* alive;  // K: 0 = cluster excluded from assignment (+inf)
   const int* done;
   int K, Kp, DP, prec;
@@ -130,6 +134,8 @@ struct RefineParams {
   // excluded clusters), ||x||^2, and the bf16x3 form's margin constants
   const float* cn;
   const float* xn2;
+  const uint16_t* Phi;  // Kp x DP bf16 hi and lo of -2c
+  const uint16_t* Plo;
   float alpha3, beta3, gref3;
   int prune_min;  // all-k queues this long or longer take the pruned pass (0: never)
 };

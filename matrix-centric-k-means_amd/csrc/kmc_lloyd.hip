@@ -9,6 +9,9 @@
 //   k_refine2    band points with two or three candidates: their distances
 //                in the reference's float arithmetic, first-min rule
 //   k_refine_all band points with four or more candidates: all k distances
+//   k_refine_prune
+//                long all-k queues instead: bf16x3 MFMA sweeps narrow each
+//                point's centroids before the exact distances
 //   k_hist / k_scan_blocks / k_scatter
 //                stable counting sort of point ids by label (the one-hot
 //                V matrix in CSR form: row offsets + column ids)
@@ -48,7 +51,14 @@ __device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, G
       w[t] = -2.0f * v;  // exact scaling
       s = fmaf(v, v, s);
       amax = fmaxf(amax, fabsf(w[t]));
-      if (j < P.DP) P.CT[(size_t)j * P.Kp + c] = v;
+      if (j < P.DP) {
+        P.CT[(size_t)j * P.Kp + c] = v;
+        if (P.Phi) {
+          const uint16_t hb = bf16_rne(w[t]);
+          P.Phi[row + j] = hb;
+          P.Plo[row + j] = bf16_rne(w[t] - bf16_to_f32(hb));
+        }
+      }
     }
     amax = wave_max_f32(amax);
     const int e = fp8_scale_exp(amax);
@@ -81,6 +91,11 @@ __device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, G
     const float w = -2.0f * v;  // exact scaling
     s = fmaf(v, v, s);
     P.CT[(size_t)j * P.Kp + c] = v;
+    if (P.Phi) {
+      const uint16_t hb = bf16_rne(w);
+      P.Phi[row + j] = hb;
+      P.Plo[row + j] = bf16_rne(w - bf16_to_f32(hb));
+    }
     if (P.prec == PREC_FP32) {
       P.Cw[row + j] = w;
     } else {
@@ -802,6 +817,12 @@ __device__ unsigned region_prefix(const unsigned* cnt, int nregions, unsigned* p
   return pre_s[nregions];
 }
 
+// the pruned all-k kernel (k_refine_prune) is launched, and k_refine_all
+// leaves queues of prune_min or more entries to it, exactly when this holds
+__host__ __device__ inline bool prune_active(const RefineParams& P) {
+  return !P.bp && P.cn && P.xn2 && P.Phi && P.Plo && P.prune_min > 0;
+}
+
 // region holding flattened entry t (pre is nondecreasing, pre[0] = 0)
 __device__ __forceinline__ int region_of(const unsigned* pre_s, int nregions, unsigned t) {
   int lo = 0, hi = nregions - 1;
@@ -941,8 +962,8 @@ __device__ void refine_all_body(const RefineParams& P, unsigned char* smem, unsi
   int* const pid_s = reinterpret_cast<int*>(smem + kRefinePreBytes + kRefinePts * kMaxDP * 4 +
                                             32 * kRefinePts);
   const unsigned total = region_prefix(P.qacnt, P.nregions, pre_s);
-  // an MFMA-pruned pass (k_refine_prune) takes queues of prune_min or more
-  if (total == 0 || (P.prune_min > 0 && total >= (unsigned)P.prune_min)) return;
+  // k_refine_prune takes queues of prune_min or more entries
+  if (total == 0 || (prune_active(P) && total >= (unsigned)P.prune_min)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned ngroups = (total + kRefinePts - 1) / kRefinePts;
   for (unsigned g = bid; g < ngroups; g += nblk) {
@@ -1078,20 +1099,24 @@ __global__ __launch_bounds__(256) void k_refine_all(RefineParams P) {
 
 // ---------------------------------------------------------------------------
 // All-k refinement with MFMA pruning (one problem; batches use k_refine_all).
-// A workgroup takes kPrunePts queued points, 32 per wave. Stage 1 forms the
-// squared distance of each point to every centroid in the bf16x3 distance
-// form -- x and -2c split into bf16 hi + lo on the fly, products hi.hi +
-// hi.lo + lo.hi, accumulator started at ||c||^2 + ||x||^2 -- whose error the
-// bf16x3 margin constants bound: E = alpha3 ||x|| cmax + beta3 cmax^2, with
-// gref3 covering the reference's own float error. The centroids pass through
-// LDS in chunks of KC rows; a first sweep finds each point's smallest value
-// b1, a second collects every centroid the band test cannot clear against
-// b1, so the reference's first minimum is among the candidates. Stage 2
-// evaluates the candidates in the reference's float arithmetic and keeps the
-// smallest distance, the lower index among equal ones. A point with more
-// than kPruneCap candidates, or without a finite stage-1 value (NaN data, no
-// live cluster), gets every live centroid evaluated exactly by the whole
-// workgroup instead.
+// Stage 1 forms the squared distance of each queued point to every centroid
+// in the bf16x3 distance form -- x split into bf16 hi + lo on the fly, the
+// prepared hi and lo planes of -2c (prep_row), products hi.hi + hi.lo +
+// lo.hi, accumulator started at ||c||^2 + ||x||^2 -- whose error the bf16x3
+// margin constants bound: E = alpha3 ||x|| cmax + beta3 cmax^2, with gref3
+// covering the reference's own float error. A first sweep finds each point's
+// smallest value b1, a second collects every centroid the band test cannot
+// clear against b1, so the reference's first minimum is among the
+// candidates. Stage 2 evaluates the candidates in the reference's float
+// arithmetic and keeps the smallest distance, the lower index among equal
+// ones. A point with more than kPruneCap candidates, or without a finite
+// stage-1 value (NaN data, no live cluster), gets every live centroid
+// evaluated exactly by the whole workgroup instead.
+// A workgroup takes kPrunePts points, 32 per wave, and the centroid planes
+// pass through LDS in chunks of KC rows shared by the four waves, the next
+// chunk fetched into registers while the current one feeds the MFMAs. A
+// group's two sweeps run on one CU, so the pass only pays once the queue
+// fills the chip: shorter queues stay with k_refine_all.
 // ---------------------------------------------------------------------------
 constexpr int kPruneWaves = 4;
 constexpr int kPrunePts = 32 * kPruneWaves;
@@ -1122,13 +1147,13 @@ __device__ __forceinline__ unsigned long long min_key_xor(unsigned long long k, 
 // Squared distance in the reference's float arithmetic over the DP padded
 // dimensions (padding is zero in X and C and adds exact zeros): in index
 // order, one rounding for each subtraction, product and addition; the next
-// 32 dimensions (16 for DP = 256) of both rows are loaded before the current
+// 16 dimensions (8 for DP = 256) of both rows are loaded before the current
 // ones are summed.
 template <int DP>
 __device__ __forceinline__ float exact_dist_rows(const float* __restrict__ x,
                                                  const float* __restrict__ c) {
 #pragma clang fp contract(off)
-  constexpr int CH = DP < 32 ? DP : (DP >= 256 ? 16 : 32);
+  constexpr int CH = DP >= 256 ? 8 : 16;
   constexpr int NV = CH / 4;
   float4 xa[NV], ca[NV], xb[NV], cb[NV];
 #pragma unroll
@@ -1137,7 +1162,7 @@ __device__ __forceinline__ float exact_dist_rows(const float* __restrict__ x,
     ca[v] = *reinterpret_cast<const float4*>(c + 4 * v);
   }
   float acc = 0.0f;
-#pragma unroll
+#pragma unroll 1  // rolled: one chunk in use and one in flight, not every load hoisted
   for (int j0 = 0; j0 < DP; j0 += CH) {
     if (j0 + CH < DP) {
 #pragma unroll
@@ -1246,40 +1271,32 @@ __global__ __launch_bounds__(256) void k_refine_prune(RefineParams P) {
 
   const unsigned ngroups = (total + kPrunePts - 1) / kPrunePts;
   const int nchunks = (P.Kp + KC - 1) / KC;
-  // chunk rows [c0, c0 + KC): thread t holds float4 t + 256 i of the chunk
-  constexpr int kFill = KC * (DP / 4) / 256;
-  static_assert(KC * (DP / 4) % 256 == 0, "chunk fill splits evenly");
-  float4 pf[kFill];
+  // chunk rows [c0, c0 + KC) of the prepared hi and lo planes of -2c, in
+  // 16-byte pieces: thread t holds piece t + 256 i of the chunk
+  constexpr int kPieces = KC * (DP / 8);  // per plane
+  constexpr int kFill = 2 * kPieces / 256;
+  static_assert((2 * kPieces) % 256 == 0, "chunk fill splits evenly");
+  uint4 pf[kFill];
   auto fetch_chunk = [&](int c0) {
 #pragma unroll
     for (int i = 0; i < kFill; ++i) {
       const int v = threadIdx.x + 256 * i;
-      const int r = v / (DP / 4), q = v - r * (DP / 4);
+      const int plane = v / kPieces, u = v - plane * kPieces;
+      const int r = u / (DP / 8), q = u - r * (DP / 8);
       const int c = c0 + r;
-      pf[i] = c < P.K ? *reinterpret_cast<const float4*>(P.C + (size_t)c * DP + 4 * q)
-                      : make_float4(0.f, 0.f, 0.f, 0.f);
+      const uint16_t* src = plane ? P.Plo : P.Phi;
+      pf[i] = c < P.Kp ? *reinterpret_cast<const uint4*>(src + (size_t)c * DP + 8 * q)
+                       : make_uint4(0u, 0u, 0u, 0u);
     }
   };
-  // -2c as bf16 hi and lo planes, ||c||^2 (kFar past Kp)
   auto store_chunk = [&](int c0) {
 #pragma unroll
     for (int i = 0; i < kFill; ++i) {
       const int v = threadIdx.x + 256 * i;
-      const int r = v / (DP / 4), q = v - r * (DP / 4);
-      const float w[4] = {-2.0f * pf[i].x, -2.0f * pf[i].y, -2.0f * pf[i].z, -2.0f * pf[i].w};
-      uint16_t hb[4], lb[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        hb[e] = bf16_rne(w[e]);
-        lb[e] = bf16_rne(w[e] - bf16_to_f32(hb[e]));
-      }
-      unsigned char* rp = rows + (size_t)r * T::kRowBytes + 8 * q;
-      *reinterpret_cast<uint2*>(rp) = make_uint2((uint32_t)hb[0] | ((uint32_t)hb[1] << 16),
-                                                 (uint32_t)hb[2] | ((uint32_t)hb[3] << 16));
-      *reinterpret_cast<uint2*>(rp + (size_t)KC * T::kRowBytes) =
-          make_uint2((uint32_t)lb[0] | ((uint32_t)lb[1] << 16),
-                     (uint32_t)lb[2] | ((uint32_t)lb[3] << 16));
-      __builtin_amdgcn_sched_barrier(0);  // convert and store one float4 at a time
+      const int plane = v / kPieces, u = v - plane * kPieces;
+      const int r = u / (DP / 8), q = u - r * (DP / 8);
+      *reinterpret_cast<uint4*>(rows + (size_t)plane * KC * T::kRowBytes +
+                                (size_t)r * T::kRowBytes + 16 * q) = pf[i];
     }
     for (int r = threadIdx.x; r < KC; r += 256) cn_s[r] = c0 + r < P.Kp ? P.cn[c0 + r] : kFar;
   };
@@ -2742,7 +2759,7 @@ hipError_t launch_prep(const PrepParams& P, hipStream_t s) {
   return hipGetLastError();
 }
 template <int DP>
-static hipError_t launch_prune(const RefineParams& P, hipStream_t s) {
+static hipError_t launch_pruned(const RefineParams& P, hipStream_t s) {
   static bool lds_set = false;
   if (!lds_set) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_refine_prune<DP>),
@@ -2766,13 +2783,13 @@ hipError_t launch_refine(const RefineParams& P, hipStream_t s) {
   }
   hipLaunchKernelGGL(k_refine2, dim3(kRefineBlocks), dim3(256), kRefineLds, s, P);
   hipLaunchKernelGGL(k_refine_all, dim3(kRefineBlocks), dim3(256), kRefineAllLds, s, P);
-  if (P.bp || !P.cn || !P.xn2 || P.prune_min <= 0) return hipGetLastError();
+  if (!prune_active(P)) return hipGetLastError();
   switch (P.DP) {
-    case 16: return launch_prune<16>(P, s);
-    case 32: return launch_prune<32>(P, s);
-    case 64: return launch_prune<64>(P, s);
-    case 128: return launch_prune<128>(P, s);
-    case 256: return launch_prune<256>(P, s);
+    case 16: return launch_pruned<16>(P, s);
+    case 32: return launch_pruned<32>(P, s);
+    case 64: return launch_pruned<64>(P, s);
+    case 128: return launch_pruned<128>(P, s);
+    case 256: return launch_pruned<256>(P, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -1,7 +1,9 @@
 """Edge cases of the Lloyd path on the GPU, each against the oracle's
 restatement of the reference CPU k-means: one cluster, fewer points than a
 tile, one dimension, all points identical (every distance ties: the first
-centroid wins), exact duplicate centroids, and a width that needs padding."""
+centroid wins), exact duplicate centroids, a width that needs padding, and
+many tied candidates -- the tie-prone cases with both all-k refinement
+paths (prune 0: every centroid exactly; 1: the bf16x3 MFMA-pruned pass)."""
 import numpy as np
 import pytest
 

@@ -200,7 +200,7 @@ def test_pruned_all_k_pass_gives_the_same_labels(kmc_mod, oracle, c2_data, prec)
     want, _ = _reference_labels(oracle, Xs, C)
     with kmc_mod.Engine(Xs.shape[0], 128, 256, precision=prec) as e:
         e.set_points(Xs)
-        for pm in (1, 0):
+        for pm in (1, 32768, 0):  # always pruned, the default threshold, never pruned
             e.set_prune_min(pm)
             lab, info = e.assign(C)
             _say("prune_min", pm, prec, "refined", info["refined_two"], info["refined_full"])
