@@ -1,14 +1,15 @@
 """Lloyd-iteration throughput of the gfx950 k-means engine on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--prec bf16] [--update tree]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--prec bf16] [--update fixed]
 
 Workload (the large plain-vs-BF16 configuration): N=1,000,000 points, D=128,
 K=256 Gaussian blobs (centres U[-10,10]^128, per-blob sigma U[0.5,2.0]),
 float32, seeded; 256 distinct seed-chosen rows as initial centroids; no
 convergence test (tol=0, every iteration runs). A step is one Lloyd
-iteration: fused MFMA distance/argmin + margin refinement + stable counting
-sort + centroid update + shift, next-iteration operands and FP64 inertia.
-Points are resident in HBM before the timed region.
+iteration: fused MFMA distance/argmin + margin refinement + centroid update
++ shift, next-iteration operands and FP64 inertia.
+Points are resident in HBM before the timed region; every timed pass runs
+iterations 1..K from the initial centroids.
 
 The headline form (``--prec``, default bf16: single-product bf16 MFMA with
 exact recomputation of the margin-band points), the split form (bf16x3: hi+lo
@@ -18,14 +19,21 @@ centroids;
 the line reports the headline's iterations/s as ``value`` and the other forms
 beside it, checks that every form ends with bit-identical centroids and
 labels, and measures the inertia error of the headline form against an FP64
-oracle. ``--update tree`` (default) sums each cluster in FP64 and rounds once;
-``--update ordered`` reproduces the reference's float summation order.
+oracle. ``--update fixed`` (default) sums each cluster exactly in 64-bit
+fixed point and keeps the totals across the run: an iteration streams every
+point only at the start of the run or after more than an eighth of the points
+moved, and otherwise subtracts and adds the rows of the moved points (the
+totals equal those of a full pass, bit for bit); ``--update tree`` sorts the
+members and sums each cluster in FP64 every iteration; ``--update ordered``
+reproduces the reference's float summation order.
 
 The roofline object covers the dominant kernel of the iteration (the one
 with the larger mean launch time of the two large ones), the other is given
 beside it: the fused distance kernel against the MFMA peak (achieved =
 2*N*K*D algorithmic flops per launch / mean launch time) and the centroid
-update against HBM (achieved = N*D*4 + N*4 + K*D*4 algorithmic bytes / mean
+update against HBM (achieved = the bytes the update must move per iteration
+-- N*D*4 + N*4 + K*D*4 for a full pass; labels, previous labels and the moved
+points' rows for an incremental one, averaged over the run -- / mean
 launch time). Both times are taken live with HIP events recorded around
 every launch on the engine's stream, in a second timed pass of the same
 steps; ``value`` and ``ms_per_step`` come from the first pass, which records
@@ -51,11 +59,13 @@ sys.path.insert(0, os.path.join(ROOT, "matrix-centric-k-means_amd"))
 # (kernel k-means, its own line)
 WORKLOADS = {
     "c2": dict(n=1_000_000, d=128, k=256, seed=20261203, gen="large_blobs", steps=50,
-               cpu_sample=100_000, cpu_iters=2,
+               cpu_sample=100_000, cpu_iters=2, update="fixed",
                metric="Lloyd iter/s, N=1M D=128 K=256: BF16 MFMA vs FP32 at <=1e-5 inertia err",
                text="Gaussian blobs N=1000000 D=128 K=256"),
+    # K = 4096: the fixed-point update's per-workgroup accumulators do not fit
+    # the LDS, so the tree update runs
     "c5": dict(n=4_000_000, d=256, k=4096, seed=20261204, gen="huge_blobs", steps=20,
-               cpu_sample=10_000, cpu_iters=1,
+               cpu_sample=10_000, cpu_iters=1, update="tree",
                metric="Lloyd iter/s, N=4M D=256 K=4096: BF16 MFMA vs FP32",
                text="Gaussian blobs N=4000000 D=256 K=4096"),
 }
@@ -82,17 +92,20 @@ def run_form(kmc, X, C0, prec, steps, warmup, sync, update, graph):
                                                [round(v, 4) for v in tuning["ms"]]))
         if warmup:
             e.run(warmup, check_converged=False, graph=graph)
-        # capture the timed chunks once (the capture is set-up, not a step):
-        # run each, then restore the starting centroids and replay it timed.
-        # The step rate comes from a pass without kernel events (an event
-        # record node in a graph costs ~6 us of idle GPU); the per-kernel
-        # durations come from a second pass of the same steps with events
-        # around every distance and update launch.
-        C_start = e.centroids()
+        # every timed pass runs the first `steps` iterations from the initial
+        # centroids, as a run of the engine would. Capture the timed chunks
+        # once (the capture is set-up, not a step): run each, then restore
+        # the initial centroids and replay it timed. The step rate comes from
+        # a pass without kernel events (an event record node in a graph
+        # costs ~6 us of idle GPU); the per-kernel durations come from a
+        # second pass of the same steps with events around every distance
+        # and update launch.
+        C_start = C0
         if graph:
             for tk in (False, True):
-                e.run(steps, check_converged=False, time_kernels=tk, graph=True)
                 e.set_centroids(C_start)
+                e.run(steps, check_converged=False, time_kernels=tk, graph=True)
+        e.set_centroids(C_start)
         sync()
         t0 = time.perf_counter()
         e.run(steps, check_converged=False, graph=graph)
@@ -339,10 +352,11 @@ def main():
                     help="timed iterations (default: 50 for c2, 20 for c5, 100 for c3)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--prec", default="bf16", choices=["bf16x3", "bf16", "fp32", "fp8"])
-    ap.add_argument("--update", default="tree", choices=["ordered", "tree", "fixed"],
-                    help="centroid update: FP64 tree over the sorted members, exact "
-                         "fixed-point streaming (no sort), or the reference's float "
-                         "summation order")
+    ap.add_argument("--update", default=None, choices=["ordered", "tree", "fixed"],
+                    help="centroid update: exact fixed-point sums kept across the run "
+                         "(moved points only once few move; the C2 default), FP64 tree "
+                         "over the sorted members (the C5 default), or the reference's "
+                         "float summation order")
     ap.add_argument("--no-x3", action="store_true", help="skip the bf16x3 comparison form")
     ap.add_argument("--no-fp8", action="store_true", help="skip the fp8 comparison form")
     ap.add_argument("--eager", action="store_true",
@@ -370,6 +384,8 @@ def main():
     N, D, K, DATA_SEED = W["n"], W["d"], W["k"], W["seed"]
     if a.steps is None:
         a.steps = W["steps"]
+    if a.update is None:
+        a.update = W["update"]
 
     import torch
     import kmc
@@ -426,6 +442,19 @@ def main():
     ubytes = N * D * 4 + N * 4 + K * D * 4  # X rows once, member ids, centroids
     uname = {"tree": "k_update_tree_rows<128>" if dp <= 128 else "k_update_tree<4>",
              "ordered": "k_update", "fixed": "k_update_fx<64>"}[a.update]
+    update_note = None
+    if a.update == "fixed" and int(r["full_passes"]) < iters:
+        # incremental: full passes stream X, labels and the previous labels;
+        # the other iterations read labels and previous labels and the rows
+        # of the points that moved
+        full = int(r["full_passes"])
+        total = full * N * (D * 4 + 12) + (iters - full) * N * 8 + \
+            int(r["moved_points"]) * D * 4
+        ubytes = total / max(1, iters)
+        uname = "k_update_fx<64> (incremental)"
+        update_note = ("%d of %d iterations streamed all points; the others updated the "
+                       "cluster totals from %d moved points in all" %
+                       (full, iters, int(r["moved_points"])))
     roof_update = {"bound": "hbm", "kernel": uname,
                    "achieved": round(ubytes / t_update / 1e9, 1), "peak": HBM_PEAK / 1e9,
                    "unit": "GB/s", "frac": round(ubytes / t_update / HBM_PEAK, 4),
@@ -433,6 +462,8 @@ def main():
                    "algorithmic_bytes_per_launch": ubytes,
                    "mean_launch_ms": round(t_update * 1e3, 5),
                    "launches_timed": int(r["update_launches"])}
+    if update_note:
+        roof_update["incremental"] = update_note
     if t_update > t_assign:
         roof, roof_other = roof_update, roof_assign
     else:
@@ -457,7 +488,8 @@ def main():
                    "init": "%d distinct seed-chosen rows (seed %d)" % (K, INIT_SEED),
                    "update": {"ordered": "ordered (reference summation order)",
                               "tree": "tree (FP64 chunk partials)",
-                              "fixed": "fixed (exact fixed-point streaming sums)"}[a.update]},
+                              "fixed": "fixed (exact fixed-point totals, moved points "
+                                       "only once at most N/8 move)"}[a.update]},
         "roofline": roof,
         "roofline_second_kernel": roof_other,
         "assign_variant": {"chosen": head["tuning"]["chosen"],

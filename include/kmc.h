@@ -135,6 +135,11 @@ typedef struct {
   uint64_t assign_launches; /* distance-kernel launches timed                */
   double ms_update;      /* sum of centroid-update kernel times (same flag)   */
   uint64_t update_launches; /* update-kernel launches timed                   */
+  /* KMC_UPDATE_FIXED keeps exact cluster totals across the iterations of a
+     run and, once at most an eighth of the points moved in the previous
+     iteration, updates them from the moved points only: */
+  uint64_t full_passes;  /* iterations whose update streamed every point      */
+  uint64_t moved_points; /* points moved, summed over the other iterations    */
 } kmc_result;
 
 typedef struct kmc_handle kmc_handle;

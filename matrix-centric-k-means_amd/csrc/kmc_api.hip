@@ -64,6 +64,11 @@ struct kmc_handle {
   // fixed-point streaming update
   double* xn64 = nullptr;
   long long* fpart = nullptr;
+  // incremental fixed-point update: labels of the last update, exact totals
+  int* prev = nullptr;
+  long long* Stot = nullptr;
+  long long* Qtot = nullptr;
+  int* Ntot = nullptr;
   long long* qpart = nullptr;
   int* cpart = nullptr;
   int fx_ds = 0, fx_nslice = 0, fx_wgps = 0;
@@ -261,6 +266,10 @@ kmc_status alloc_all(kmc_handle* h) {
     const size_t Kp = (size_t)h->Kp, ws = (size_t)h->fx_wgps;
     KMC_CK(dalloc(&h->xn64, n));
     KMC_CK(dalloc(&h->fpart, (size_t)h->fx_nslice * ws * Kp * (size_t)h->fx_ds));
+    KMC_CK(dalloc(&h->prev, n));
+    KMC_CK(dalloc(&h->Stot, Kp * DP));
+    KMC_CK(dalloc(&h->Qtot, Kp));
+    KMC_CK(dalloc(&h->Ntot, Kp));
     KMC_CK(dalloc(&h->qpart, ws * Kp));
     KMC_CK(dalloc(&h->cpart, ws * Kp));
   }
@@ -284,6 +293,7 @@ void free_all(kmc_handle* h) {
                   h->bcount,
                   h->count, h->start, h->members, h->ipart, h->tpart, h->conv,
                   h->shift2, h->alive, h->state, h->xn64, h->fpart, h->qpart, h->cpart,
+                  h->prev, h->Stot, h->Qtot, h->Ntot,
                   h->Xq, h->xsc, h->Cq, h->xdn, h->cdn, h->Phi, h->Plo};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -454,6 +464,11 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nu
   up.DP = h->DP;
   up.xn64 = h->xn64;
   up.fpart = h->fpart;
+  up.prev = h->prev;
+  up.Stot = h->Stot;
+  up.Qtot = h->Qtot;
+  up.Ntot = h->Ntot;
+  up.state = h->state;
   up.qpart = h->qpart;
   up.cpart = h->cpart;
   up.fscale = h->fx_scale;
@@ -487,6 +502,10 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nu
   fp.prep = prep_params(h);
   fp.fixed = fixed ? 1 : 0;
   fp.fpart = h->fpart;
+  fp.Stot = h->Stot;
+  fp.Qtot = h->Qtot;
+  fp.Ntot = h->Ntot;
+  fp.n = (long long)h->n;
   fp.qpart = h->qpart;
   fp.cpart = h->cpart;
   fp.count_out = h->count;
@@ -511,6 +530,8 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nu
   st.K = h->K;
   st.mode = mode;
   st.cost_ratio = (rules & kmc::RULE_COST_RATIO) ? 1 : 0;
+  st.incremental = h->Stot ? 1 : 0;
+  st.n = (long long)h->n;
   KMC_CK(kmc::launch_finish(fp, h->stream));
   return KMC_OK;
 }
@@ -581,6 +602,8 @@ void fill_result(const kmc_handle* h, const DevState& s, kmc_result* r) {
   r->max_shift = std::sqrt(s.max_shift2);
   r->refined_two = s.refined_two;
   r->refined_full = s.refined_all;
+  r->full_passes = s.full_passes;
+  r->moved_points = s.moved_total;
   (void)h;
 }
 

@@ -35,7 +35,24 @@ struct DevState {
   double last_score;                // score of the previous iteration
   double sum_shift2;                // FP64 sum_c ||c_new - c_old||^2
   double prior_cost;                // FP64 inertia of the previous assignment
+  // fixed-point update, incremental form: points whose label changed in the
+  // last iteration, the count being formed by this one, and whether the
+  // cluster totals match the labels of the last iteration (0 at run start)
+  unsigned long long moved;
+  unsigned long long moved_acc;
+  int sums_valid;
+  int pad2;
+  unsigned long long full_passes;  // iterations of this run with a full pass
+  unsigned long long moved_total;  // points moved in the other iterations
 };
+
+// The fixed-point update of an iteration is a full streaming pass (and it
+// rebuilds the cluster totals) unless the totals are valid and at most an
+// eighth of the points changed cluster in the last iteration; then only the
+// moved points' rows are subtracted from and added to the totals.
+__host__ __device__ inline bool fx_full_pass(const DevState* s, long long n) {
+  return !s->sums_valid || s->moved > (unsigned long long)(n / 8);
+}
 
 // convergence tests of the k_stats kernel
 enum { CONV_NONE = 0, CONV_SHIFT = 1, CONV_SCORE = 2, CONV_STABLE = 3, CONV_SHIFT_SUM = 4 };
@@ -177,6 +194,14 @@ struct UpdateParams {
   float fscale;        // x -> round(x * fscale), a power of two
   double qscale;       // ||x||^2 -> round(||x||^2 * qscale), a power of two
   int Kp, nslice, wgps, ds;
+  // incremental form (null Stot: every iteration is a full pass): labels of
+  // the last update, exact cluster totals (Kp x DP, Kp, Kp) and the state
+  // that chooses between a full pass and the moved points only
+  int* prev;
+  long long* Stot;
+  long long* Qtot;
+  int* Ntot;
+  DevState* state;
 };
 
 
@@ -193,6 +218,8 @@ struct StatsParams {
   float tol;
   int K, mode;     // mode: CONV_*
   int cost_ratio;  // also stop when cost_t / cost_{t-1} > 1 - tol (t >= 2)
+  int incremental;  // fixed-point update keeps cluster totals across iterations
+  long long n;      // points (the full-pass test)
 };
 
 struct FinishParams {
@@ -219,6 +246,12 @@ struct FinishParams {
   double* ipart_out;   // FP64 inertia of the assignment, per cluster
   int Kp, nslice, wgps, ds;
   double inv_fscale, inv_qscale;
+  // incremental fixed-point update: the cluster totals (written after a full
+  // pass, read otherwise) and the number of points
+  long long* Stot;
+  long long* Qtot;
+  int* Ntot;
+  long long n;
   PrepParams prep;
   StatsParams stats;  // k_stats, launched after k_finish
 };

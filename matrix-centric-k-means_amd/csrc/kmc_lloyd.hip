@@ -1878,9 +1878,63 @@ size_t update_fx_lds(int Kp, int ds) {
   return (size_t)Kp * fx_stride(ds) * 8 + (size_t)Kp * 8 + (size_t)Kp * 4;
 }
 
+// Incremental form of the fixed-point update (the iterations that are not a
+// full pass; run by k_update_fx's grid): every point whose label differs
+// from the one the cluster totals stand for has its converted row (the same
+// round(x * fscale) integers as the full pass) subtracted from the old
+// cluster's totals and added to the new one's, with its fixed-point ||x||^2
+// and count, by 64-bit integer atomics. Integer sums do not depend on the
+// order, so the totals equal those a full pass would form. A wave scans 64
+// points at a time and takes its moved points one by one, a lane per
+// dimension.
+__device__ void update_moved_points(const UpdateParams& P) {
+  const int lane = threadIdx.x & 63;
+  const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+  const long long w0 = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const float fs = P.fscale;
+  for (long long base = w0 * 64; base < P.n; base += nw * 64) {
+    const long long i = base + lane;
+    int lab = 0, old = 0;
+    if (i < P.n) {
+      lab = P.labels[i];
+      old = P.prev[i];
+    }
+    const bool mv = i < P.n && lab != old;
+    uint64_t m = __ballot(mv);
+    if (!m) continue;
+    if (lane == 0) atomicAdd(&P.state->moved_acc, (unsigned long long)__popcll(m));
+    if (mv) P.prev[i] = lab;
+    while (m) {
+      const int src = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const long long pi = __shfl(i, src, 64);
+      const int nl = __shfl(lab, src, 64), ol = __shfl(old, src, 64);
+      const float* xr = P.X + (size_t)pi * P.DP;
+      unsigned long long* sn = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)nl * P.DP);
+      unsigned long long* so = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)ol * P.DP);
+      for (int j = lane; j < P.D; j += 64) {
+        const long long q = __float2ll_rn(xr[j] * fs);
+        atomicAdd(sn + j, (unsigned long long)q);
+        atomicAdd(so + j, (unsigned long long)(-q));
+      }
+      if (lane == 0) {
+        const long long qq = __double2ll_rn(P.xn64[pi] * P.qscale);
+        atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + nl), (unsigned long long)qq);
+        atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + ol), (unsigned long long)(-qq));
+        atomicAdd(P.Ntot + nl, 1);
+        atomicAdd(P.Ntot + ol, -1);
+      }
+    }
+  }
+}
+
 template <int DS>
 __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   if (*P.done) return;
+  if (P.Stot && !fx_full_pass(P.state, P.n)) {  // the moved points only
+    update_moved_points(P);
+    return;
+  }
   constexpr int ST = fx_stride(DS);
   extern __shared__ __attribute__((aligned(16))) unsigned char fsm[];
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(fsm);
@@ -1931,6 +1985,7 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
       atomicAdd(a + 3, (unsigned long long)__float2ll_rn(v[u].w * fs));
     }
     if (slice == 0 && dg == 0) {
+      unsigned mv = 0;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const long long i = i0 + u * PPW + pp;
@@ -1938,7 +1993,12 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
         const double q = ok ? P.xn64[i] : 0.0;
         atomicAdd(&qacc[lab[u]], (unsigned long long)__double2ll_rn(q * P.qscale));
         atomicAdd(&cacc[lab[u]], ok ? 1u : 0u);
+        if (ok && P.Stot) {  // the labels the totals will stand for
+          mv += P.prev[i] != lab[u] ? 1u : 0u;
+          P.prev[i] = lab[u];
+        }
       }
+      if (P.Stot && mv) atomicAdd(&P.state->moved_acc, (unsigned long long)mv);
     }
     if (more) {
 #pragma unroll
@@ -2223,6 +2283,17 @@ hipError_t launch_plusplus(const float* X, long long n, int DP, float* C, int K,
 // ---------------------------------------------------------------------------
 constexpr int kStatsThreads = 512;
 __device__ void stats_reduce(const StatsParams& P) {
+  if (P.incremental && threadIdx.x == 0) {
+    // the iteration's update is complete: its moved count decides the next
+    // one, and the cluster totals now stand for its labels
+    if (fx_full_pass(P.state, P.n))
+      P.state->full_passes += 1;
+    else
+      P.state->moved_total += P.state->moved_acc;
+    P.state->moved = P.state->moved_acc;
+    P.state->moved_acc = 0ull;
+    P.state->sums_valid = 1;
+  }
   __shared__ double sd[kStatsThreads];
   __shared__ double sm[kStatsThreads];
   __shared__ int sc[kStatsThreads];
@@ -2360,14 +2431,17 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
     // exact integer sums of the workgroup partials (any order), then the
     // means rounded once and the FP64 inertia of the assignment:
     //   sum ||x - c||^2 = Q - 2 c.S + n ||c||^2
+    // In the incremental form an iteration without a full pass reads the
+    // cluster totals instead, and a full pass stores them.
     long long(*fs_s)[kMaxDP] = reinterpret_cast<long long(*)[kMaxDP]>(part_s);
     __shared__ long long q_s[kFinishWaves];
     __shared__ int n_s[kFinishWaves];
+    const bool from_totals = P.Stot && !fx_full_pass(P.stats.state, P.n);
     long long qsum = 0;
     int nsum = 0;
     // wave w adds the partials of workgroups w, w+8, ...; four loads in
     // flight per dimension (integer sums: the order does not matter)
-    for (int j = lane; j < P.D; j += 64) {
+    for (int j = lane; j < P.D && !from_totals; j += 64) {
       const int sl = j / P.ds, dj = j - sl * P.ds;
       const long long* fp = P.fpart + ((size_t)sl * P.wgps * P.Kp + c) * P.ds + dj;
       const size_t wst = (size_t)P.Kp * P.ds;  // one workgroup further
@@ -2383,7 +2457,7 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
       for (; w < P.wgps; w += kFinishWaves) acc += fp[(size_t)w * wst];
       fs_s[wave][j] = acc;
     }
-    if (lane == 0) {
+    if (lane == 0 && !from_totals) {
       for (int w = wave; w < P.wgps; w += kFinishWaves) {
         qsum += P.qpart[(size_t)w * P.Kp + c];
         nsum += P.cpart[(size_t)w * P.Kp + c];
@@ -2397,16 +2471,30 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
     if (wave == 0) {
       long long qt = 0;
       int nt = 0;
+      if (from_totals) {
+        qt = P.Qtot[c];
+        nt = P.Ntot[c];
+      } else {
 #pragma unroll
-      for (int w = 0; w < kFinishWaves; ++w) {
-        qt += q_s[w];
-        nt += n_s[w];
+        for (int w = 0; w < kFinishWaves; ++w) {
+          qt += q_s[w];
+          nt += n_s[w];
+        }
+        if (P.Stot && lane == 0) {
+          P.Qtot[c] = qt;
+          P.Ntot[c] = nt;
+        }
       }
       double cs = 0.0, cc = 0.0;
       for (int j = lane; j < P.D; j += 64) {
         long long sj = 0;
+        if (from_totals) {
+          sj = P.Stot[(size_t)c * P.DP + j];
+        } else {
 #pragma unroll
-        for (int w = 0; w < kFinishWaves; ++w) sj += fs_s[w][j];
+          for (int w = 0; w < kFinishWaves; ++w) sj += fs_s[w][j];
+          if (P.Stot) P.Stot[(size_t)c * P.DP + j] = sj;
+        }
         const double S = (double)sj * P.inv_fscale;
         const float v = (float)(S / (double)(nt > 1 ? nt : 1));
         ns[j] = v;

@@ -46,7 +46,8 @@ class _Result(ctypes.Structure):
                 ("max_shift", ctypes.c_double), ("refined_two", ctypes.c_uint64),
                 ("refined_full", ctypes.c_uint64), ("ms_total", ctypes.c_double),
                 ("ms_assign", ctypes.c_double), ("assign_launches", ctypes.c_uint64),
-                ("ms_update", ctypes.c_double), ("update_launches", ctypes.c_uint64)]
+                ("ms_update", ctypes.c_double), ("update_launches", ctypes.c_uint64),
+                ("full_passes", ctypes.c_uint64), ("moved_points", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

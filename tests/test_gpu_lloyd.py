@@ -207,6 +207,31 @@ def test_pruned_all_k_pass_gives_the_same_labels(kmc_mod, oracle, c2_data, prec)
             assert np.array_equal(lab, want), pm
 
 
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
+def test_fixed_update_from_moved_points_equals_full_passes(kmc_mod, c2_data, prec, graph):
+    """The fixed-point update keeps exact cluster totals across the
+    iterations of a run and, once few points move, adds and subtracts only
+    the moved points' rows. Single-iteration runs (each starts with a full
+    pass) must give the same labels, centroid bits and inertia as one run
+    over the same iterations."""
+    X, C0 = c2_data
+    Xs = np.ascontiguousarray(X[:200_000])
+    iters = 12
+    with kmc_mod.Engine(Xs.shape[0], 128, 256, tol=0.0, precision=prec, update="fixed") as e:
+        e.set_points(Xs)
+        e.set_centroids(C0)
+        r = e.run(iters, check_converged=False, graph=graph)
+        C_run, L_run = e.centroids(), e.labels()
+        e.set_centroids(C0)
+        for _ in range(iters):
+            r1 = e.run(1, check_converged=False)
+        _say("fixed", prec, graph, r["inertia"], r1["inertia"])
+        assert np.array_equal(e.labels(), L_run)
+        assert np.array_equal(e.centroids(), C_run)
+        assert r1["inertia"] == r["inertia"]
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("update", ["ordered", "tree", "fixed"])
 def test_graph_replays_match_eager(kmc_mod, prec, update):
