@@ -234,9 +234,11 @@ kmc_status kmc_set_assign_variant(kmc_handle* h, int variant);
  * refined against every centroid in the reference's float arithmetic. From
  * min_points such points per assignment on (default 32768), that pass first
  * narrows each point's centroids with a bf16x3 MFMA sweep and evaluates only
- * the survivors; below it every centroid is evaluated. 1 selects the pruned
- * pass always, 0 never. Labels are the same either way. A tuning knob of
- * this library's refinement, with no counterpart in the reference. */
+ * the survivors; below it every centroid is evaluated. The pruned pass is
+ * launched in a run only when the handle's previous run ended with an all-k
+ * queue of at least min_points / 4 points (or none ran yet). 1 selects the
+ * pruned pass always, 0 never. Labels are the same either way. A tuning
+ * knob of this library's refinement, with no counterpart in the reference. */
 kmc_status kmc_set_prune_min(kmc_handle* h, int32_t min_points);
 
 /* Release every device buffer (src/kmeans.cu:307-359). */

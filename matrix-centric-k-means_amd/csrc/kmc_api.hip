@@ -90,6 +90,12 @@ struct kmc_handle {
   float alpha = 0, beta = 0, gref = 0, xpk = 0;
   float alpha3 = 0, beta3 = 0, gref3 = 0;  // bf16x3 constants (pruned all-k pass)
   int prune_min = kmc::kPruneMinPoints;
+  // launch the pruned all-k pass in this run: pruning is on and the last run
+  // ended with an all-k queue within a quarter of the threshold (or none
+  // ran yet); otherwise the exact pass takes every queue (same labels)
+  bool prune_launch() const {
+    return prune_min > 0 && (last_all_k < 0 || last_all_k >= prune_min / 4);
+  }
   int idx_bits = 0;
 
   bool have_points = false, have_centroids = false;
@@ -98,9 +104,11 @@ struct kmc_handle {
   // captured iteration chunks (KMC_RUN_GRAPH), reused by later runs
   struct GraphEntry {
     uint64_t iters, ev_off;
-    int mode, rules, variant, timed;
+    int mode, rules, variant, timed, prune;
     hipGraphExec_t exec;
   };
+  // all-k queue length of the last assignment of the last run (-1: none yet)
+  long long last_all_k = -1;
   std::vector<GraphEntry> graphs;
 };
 
@@ -424,7 +432,7 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
   rp.alpha3 = h->alpha3;
   rp.beta3 = h->beta3;
   rp.gref3 = h->gref3;
-  rp.prune_min = h->prune_min;
+  rp.prune_min = h->prune_launch() ? h->prune_min : 0;
   KMC_CK(kmc::launch_refine(rp, h->stream));
   return KMC_OK;
 }
@@ -560,7 +568,8 @@ kmc_status launch_chunk_graph(kmc_handle* h, uint64_t first, uint64_t m, int mod
   const int timed = first < ntimed ? 1 : 0;
   for (auto& g : h->graphs) {
     if (g.iters == m && g.ev_off == (timed ? first : 0) && g.mode == mode &&
-        g.rules == rules && g.variant == h->variant && g.timed == timed) {
+        g.rules == rules && g.variant == h->variant && g.timed == timed &&
+        g.prune == (h->prune_launch() ? 1 : 0)) {
       KMC_CK(hipGraphLaunch(g.exec, h->stream));
       return KMC_OK;
     }
@@ -582,7 +591,8 @@ kmc_status launch_chunk_graph(kmc_handle* h, uint64_t first, uint64_t m, int mod
     (void)hipGetLastError();
     return KMC_ERR_UNSUPPORTED;
   }
-  h->graphs.push_back({m, timed ? first : 0, mode, rules, h->variant, timed, exec});
+  h->graphs.push_back({m, timed ? first : 0, mode, rules, h->variant, timed,
+                       h->prune_launch() ? 1 : 0, exec});
   KMC_CK(hipGraphLaunch(exec, h->stream));
   return KMC_OK;
 }
@@ -1020,6 +1030,7 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
     if (e != hipSuccess) st = fail(h, KMC_ERR_HIP, std::string("run timing: ") + hipGetErrorString(e));
     if (st == KMC_OK) st = read_state(h, &s);
+    if (st == KMC_OK) h->last_all_k = (long long)s.refined_all;
     if (st == KMC_OK) {
       fill_result(h, s, res);
       if (res) {
@@ -1057,6 +1068,7 @@ kmc_status kmc_set_assign_variant(kmc_handle* h, int variant) {
 kmc_status kmc_set_prune_min(kmc_handle* h, int32_t min_points) {
   if (!h || min_points < 0) return fail(h, KMC_ERR_ARG, "kmc_set_prune_min: bad argument");
   h->prune_min = min_points;
+  h->last_all_k = -1;
   // captured graphs carry the old value in their refinement launches
   KMC_CK(hipSetDevice(h->dev));
   KMC_CK(hipStreamSynchronize(h->stream));
