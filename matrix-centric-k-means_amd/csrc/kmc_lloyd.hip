@@ -1888,6 +1888,9 @@ size_t update_fx_lds(int Kp, int ds) {
 // points at a time and takes its moved points one by one, a lane per
 // dimension.
 __device__ void update_moved_points(const UpdateParams& P) {
+  __shared__ unsigned moved_s;  // this workgroup's moved points (one global add)
+  if (threadIdx.x == 0) moved_s = 0u;
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
   const long long w0 = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1902,7 +1905,7 @@ __device__ void update_moved_points(const UpdateParams& P) {
     const bool mv = i < P.n && lab != old;
     uint64_t m = __ballot(mv);
     if (!m) continue;
-    if (lane == 0) atomicAdd(&P.state->moved_acc, (unsigned long long)__popcll(m));
+    if (lane == 0) atomicAdd(&moved_s, (unsigned)__popcll(m));
     if (mv) P.prev[i] = lab;
     while (m) {
       const int src = __ffsll((unsigned long long)m) - 1;
@@ -1926,6 +1929,8 @@ __device__ void update_moved_points(const UpdateParams& P) {
       }
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && moved_s) atomicAdd(&P.state->moved_acc, (unsigned long long)moved_s);
 }
 
 template <int DS>
@@ -1941,6 +1946,8 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   unsigned long long* qacc = acc + (size_t)P.Kp * ST;
   unsigned* cacc = reinterpret_cast<unsigned*>(qacc + P.Kp);
   const int slice = blockIdx.x % P.nslice, wg = blockIdx.x / P.nslice;
+  __shared__ unsigned moved_s;  // points whose label changed (one global add)
+  if (threadIdx.x == 0) moved_s = 0u;
   for (int t = threadIdx.x; t < P.Kp * ST + P.Kp; t += kFxThreads) acc[t] = 0ull;
   for (int t = threadIdx.x; t < P.Kp; t += kFxThreads) cacc[t] = 0u;
   __syncthreads();
@@ -1998,7 +2005,7 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
           P.prev[i] = lab[u];
         }
       }
-      if (P.Stot && mv) atomicAdd(&P.state->moved_acc, (unsigned long long)mv);
+      if (P.Stot && mv) atomicAdd(&moved_s, mv);
     }
     if (more) {
 #pragma unroll
@@ -2009,6 +2016,7 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
     }
   }
   __syncthreads();
+  if (threadIdx.x == 0 && moved_s) atomicAdd(&P.state->moved_acc, (unsigned long long)moved_s);
   long long* out = P.fpart + ((size_t)slice * P.wgps + wg) * (size_t)P.Kp * DS;
   for (int t = threadIdx.x; t < P.Kp * DS; t += kFxThreads) {
     const int c = t / DS, d = t - c * DS;
