@@ -23,6 +23,8 @@
 //
 // Every kernel returns at once when the device stop flag is set, so the host
 // may queue iterations ahead of the convergence decision.
+#include <algorithm>
+
 #include "kmc.h"
 #include "kmc_device.h"
 #include "kmc_kernels.h"
@@ -2869,15 +2871,27 @@ static hipError_t launch_pruned(const RefineParams& P, hipStream_t s) {
 }
 
 hipError_t launch_refine(const RefineParams& P, hipStream_t s) {
-  static bool lds_set = false;  // opt in to the large dynamic LDS once per process
-  if (!lds_set) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_refine2),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)kRefineLds);
+  // k_refine2's grid: the workgroups resident at once for its LDS (a second
+  // round of workgroups would only add a tail: 1024 measured slower than
+  // the 512 that fit on 256 CUs); set up once per process
+  static int grid2 = 0;
+  if (!grid2) {
+    const void* fn = reinterpret_cast<const void*>(k_refine2);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kRefineLds);
     if (e != hipSuccess) return e;
-    lds_set = true;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, kRefineLds) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    grid2 = std::min(kRefineBlocks, per_cu * cus);
   }
-  hipLaunchKernelGGL(k_refine2, dim3(kRefineBlocks), dim3(256), kRefineLds, s, P);
+  hipLaunchKernelGGL(k_refine2, dim3(grid2), dim3(256), kRefineLds, s, P);
   hipLaunchKernelGGL(k_refine_all, dim3(kRefineBlocks), dim3(256), kRefineAllLds, s, P);
   if (!prune_active(P)) return hipGetLastError();
   switch (P.DP) {
