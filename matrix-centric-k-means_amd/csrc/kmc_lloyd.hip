@@ -1969,7 +1969,13 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   const long long stride = (long long)NW * PPW * U;
   float4 v[U], vn[U];
   int lab[U], labn[U];
-  auto load_step = [&](long long i0, float4(&vv)[U], int(&ll)[U]) {
+  // the lanes that add a point's ||x||^2 and count (slice 0, first lane of
+  // the point) also load its ||x||^2 and previous label with the step
+  const bool pt_lane = slice == 0 && dg == 0;
+  double q[U], qn[U];
+  int pv[U], pvn[U];
+  auto load_step = [&](long long i0, float4(&vv)[U], int(&ll)[U], double(&qq)[U],
+                       int(&pq)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const long long i = i0 + u * PPW + pp;
@@ -1978,13 +1984,16 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
       ll[u] = ok ? P.labels[ic] : 0;
       vv[u] = *reinterpret_cast<const float4*>(P.X + (size_t)ic * P.DP + col);
       if (!ok) vv[u] = make_float4(0.f, 0.f, 0.f, 0.f);  // adds exact zeros
+      qq[u] = (pt_lane && ok) ? P.xn64[ic] : 0.0;
+      pq[u] = (pt_lane && ok && P.Stot) ? P.prev[ic] : 0;
     }
   };
+  unsigned mv_lane = 0;  // this lane's moved points, added to moved_s once
   long long i0 = lo + (long long)wave * PPW * U;
-  if (i0 < hi) load_step(i0, v, lab);
+  if (i0 < hi) load_step(i0, v, lab, q, pv);
   for (; i0 < hi; i0 += stride) {
     const bool more = i0 + stride < hi;
-    if (more) load_step(i0 + stride, vn, labn);
+    if (more) load_step(i0 + stride, vn, labn, qn, pvn);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       unsigned long long* a = acc + (size_t)lab[u] * ST + 4 * dg;
@@ -1993,30 +2002,30 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
       atomicAdd(a + 2, (unsigned long long)__float2ll_rn(v[u].z * fs));
       atomicAdd(a + 3, (unsigned long long)__float2ll_rn(v[u].w * fs));
     }
-    if (slice == 0 && dg == 0) {
-      unsigned mv = 0;
+    if (pt_lane) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const long long i = i0 + u * PPW + pp;
         const bool ok = i < hi;
-        const double q = ok ? P.xn64[i] : 0.0;
-        atomicAdd(&qacc[lab[u]], (unsigned long long)__double2ll_rn(q * P.qscale));
+        atomicAdd(&qacc[lab[u]], (unsigned long long)__double2ll_rn(q[u] * P.qscale));
         atomicAdd(&cacc[lab[u]], ok ? 1u : 0u);
         if (ok && P.Stot) {  // the labels the totals will stand for
-          mv += P.prev[i] != lab[u] ? 1u : 0u;
+          mv_lane += pv[u] != lab[u] ? 1u : 0u;
           P.prev[i] = lab[u];
         }
       }
-      if (P.Stot && mv) atomicAdd(&moved_s, mv);
     }
     if (more) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         v[u] = vn[u];
         lab[u] = labn[u];
+        q[u] = qn[u];
+        pv[u] = pvn[u];
       }
     }
   }
+  if (mv_lane) atomicAdd(&moved_s, mv_lane);
   __syncthreads();
   if (threadIdx.x == 0 && moved_s) atomicAdd(&P.state->moved_acc, (unsigned long long)moved_s);
   long long* out = P.fpart + ((size_t)slice * P.wgps + wg) * (size_t)P.Kp * DS;
