@@ -1897,37 +1897,48 @@ __device__ void update_moved_points(const UpdateParams& P) {
   const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
   const long long w0 = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const float fs = P.fscale;
-  for (long long base = w0 * 64; base < P.n; base += nw * 64) {
-    const long long i = base + lane;
-    int lab = 0, old = 0;
-    if (i < P.n) {
-      lab = P.labels[i];
-      old = P.prev[i];
+  // one moved point: its converted row out of the old cluster's totals and
+  // into the new one's, a lane per dimension
+  auto move_point = [&](long long pi, int nl, int ol) {
+    const float* xr = P.X + (size_t)pi * P.DP;
+    unsigned long long* sn = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)nl * P.DP);
+    unsigned long long* so = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)ol * P.DP);
+    for (int j = lane; j < P.D; j += 64) {
+      const long long q = __float2ll_rn(xr[j] * fs);
+      atomicAdd(sn + j, (unsigned long long)q);
+      atomicAdd(so + j, (unsigned long long)(-q));
     }
-    const bool mv = i < P.n && lab != old;
-    uint64_t m = __ballot(mv);
-    if (!m) continue;
-    if (lane == 0) atomicAdd(&moved_s, (unsigned)__popcll(m));
-    if (mv) P.prev[i] = lab;
-    while (m) {
-      const int src = __ffsll((unsigned long long)m) - 1;
-      m &= m - 1;
-      const long long pi = __shfl(i, src, 64);
-      const int nl = __shfl(lab, src, 64), ol = __shfl(old, src, 64);
-      const float* xr = P.X + (size_t)pi * P.DP;
-      unsigned long long* sn = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)nl * P.DP);
-      unsigned long long* so = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)ol * P.DP);
-      for (int j = lane; j < P.D; j += 64) {
-        const long long q = __float2ll_rn(xr[j] * fs);
-        atomicAdd(sn + j, (unsigned long long)q);
-        atomicAdd(so + j, (unsigned long long)(-q));
-      }
-      if (lane == 0) {
-        const long long qq = __double2ll_rn(P.xn64[pi] * P.qscale);
-        atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + nl), (unsigned long long)qq);
-        atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + ol), (unsigned long long)(-qq));
-        atomicAdd(P.Ntot + nl, 1);
-        atomicAdd(P.Ntot + ol, -1);
+    if (lane == 0) {
+      const long long qq = __double2ll_rn(P.xn64[pi] * P.qscale);
+      atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + nl), (unsigned long long)qq);
+      atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + ol), (unsigned long long)(-qq));
+      atomicAdd(P.Ntot + nl, 1);
+      atomicAdd(P.Ntot + ol, -1);
+    }
+  };
+  // a wave takes points base + u * nw * 64 + lane, u < UD: the labels and
+  // previous labels of all UD groups are loaded before any is compared
+  constexpr int UD = 8;
+  for (long long base = w0 * 64; base < P.n; base += nw * 64 * UD) {
+    int labv[UD], oldv[UD];
+#pragma unroll
+    for (int u = 0; u < UD; ++u) {
+      const long long i = base + u * nw * 64 + lane;
+      labv[u] = i < P.n ? P.labels[i] : 0;
+      oldv[u] = i < P.n ? P.prev[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < UD; ++u) {
+      const long long i = base + u * nw * 64 + lane;
+      const bool mv = i < P.n && labv[u] != oldv[u];
+      uint64_t m = __ballot(mv);
+      if (!m) continue;
+      if (lane == 0) atomicAdd(&moved_s, (unsigned)__popcll(m));
+      if (mv) P.prev[i] = labv[u];
+      while (m) {
+        const int src = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        move_point(__shfl(i, src, 64), __shfl(labv[u], src, 64), __shfl(oldv[u], src, 64));
       }
     }
   }
