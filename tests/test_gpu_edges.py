@@ -79,6 +79,7 @@ def test_padded_widths(kmc_mod, oracle, prec, d, prune):
     C0 = X[rng.choice(4000, 12, replace=False)]
     _check(kmc_mod, oracle, X, C0, prec, maxiter=20, prune=prune)
     _check(kmc_mod, oracle, X, C0, prec, maxiter=20, update="tree", prune=prune)
+    _check(kmc_mod, oracle, X, C0, prec, maxiter=20, update="fixed", prune=prune)
 
 
 @pytest.mark.parametrize("prune", [0, 1])

@@ -20,9 +20,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "matrix-centric-k-means_amd", "bin", "raftbench")
 
 
-def _run_engine(kmc_mod, X, C0, tol, maxiter, check, prec="fp32"):
+def _run_engine(kmc_mod, X, C0, tol, maxiter, check, prec="fp32", update="tree"):
     with kmc_mod.Engine(X.shape[0], X.shape[1], C0.shape[0], tol=tol, precision=prec,
-                        update="tree") as e:
+                        update=update) as e:
         e.set_points(X)
         e.set_centroids(C0)
         r = e.run(maxiter, check_converged=check, converge="shift_sum", cost_ratio=True,
@@ -30,33 +30,42 @@ def _run_engine(kmc_mod, X, C0, tol, maxiter, check, prec="fp32"):
         return r, e.labels(), e.centroids()
 
 
+UPDATES = ("tree", "fixed")
+
+
+@pytest.mark.parametrize("update", UPDATES)
 @pytest.mark.parametrize("check", [True, False])
-def test_known_answers(kmc_mod, check):
+def test_known_answers(kmc_mod, check, update):
     X = np.array([[0.0], [1.0], [10.0], [11.0]], np.float32)
     C0 = np.array([[0.0], [1.0]], np.float32)
-    r, lab, C = _run_engine(kmc_mod, X, C0, 1e-4, 50, check)
+    r, lab, C = _run_engine(kmc_mod, X, C0, 1e-4, 50, check, update=update)
     assert int(r["converged_at"]) == (3 if check else 4)
     assert np.array_equal(lab, [0, 0, 1, 1])
     assert np.array_equal(C, np.array([[0.5], [10.5]], np.float32))
 
 
-def test_empty_cluster_keeps_centroid(kmc_mod):
+@pytest.mark.parametrize("update", UPDATES)
+def test_empty_cluster_keeps_centroid(kmc_mod, update):
     X = np.array([[0.0], [1.0], [2.0]], np.float32)
     C0 = np.array([[1.0], [100.0]], np.float32)
-    r, lab, C = _run_engine(kmc_mod, X, C0, 1e-4, 50, True)
+    r, lab, C = _run_engine(kmc_mod, X, C0, 1e-4, 50, True, update=update)
     assert int(r["converged_at"]) == 1
     assert np.array_equal(lab, [0, 0, 0])
     assert np.array_equal(C, C0)
 
 
+@pytest.mark.parametrize("update", UPDATES)
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
 @pytest.mark.parametrize("check", [True, False])
-def test_matches_oracle_on_digits_like_data(kmc_mod, oracle, prec, check):
+def test_matches_oracle_on_digits_like_data(kmc_mod, oracle, prec, check, update):
+    """Integer-valued data: FP64 and fixed-point cluster sums are both exact,
+    so every update form must give the oracle's centroid bits (the fixed-point
+    one from totals kept across the run once few points move)."""
     g = load_golden("ref_c1_blobs.npz")
     X, C0 = g["X"], g["C0"]
     tol = 1e-3
     want_t, want_lab, want_C, costs = oracle.lloyd_explicit(X, C0, tol, 300, check)
-    r, lab, C = _run_engine(kmc_mod, X, C0, tol, 300, check, prec)
+    r, lab, C = _run_engine(kmc_mod, X, C0, tol, 300, check, prec, update)
     assert int(r["converged_at"]) == want_t
     assert np.array_equal(lab, want_lab.astype(np.int32))
     assert np.array_equal(C, want_C)
