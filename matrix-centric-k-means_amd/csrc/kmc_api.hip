@@ -90,6 +90,7 @@ struct kmc_handle {
   float alpha = 0, beta = 0, gref = 0, xpk = 0;
   float alpha3 = 0, beta3 = 0, gref3 = 0;  // bf16x3 constants (pruned all-k pass)
   int prune_min = kmc::kPruneMinPoints;
+  bool short_prune = true;  // shorter all-k queues take the single-sweep pruned pass
   // launch the pruned all-k pass in this run: pruning is on and the last run
   // ended with an all-k queue within a quarter of the threshold (or none
   // ran yet); otherwise the exact pass takes every queue (same labels)
@@ -433,6 +434,7 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
   rp.beta3 = h->beta3;
   rp.gref3 = h->gref3;
   rp.prune_min = h->prune_launch() ? h->prune_min : 0;
+  rp.pq = h->short_prune ? 1 : 0;
   KMC_CK(kmc::launch_refine(rp, h->stream));
   return KMC_OK;
 }
@@ -1070,6 +1072,17 @@ kmc_status kmc_set_prune_min(kmc_handle* h, int32_t min_points) {
   h->prune_min = min_points;
   h->last_all_k = -1;
   // captured graphs carry the old value in their refinement launches
+  KMC_CK(hipSetDevice(h->dev));
+  KMC_CK(hipStreamSynchronize(h->stream));
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+  h->graphs.clear();
+  return KMC_OK;
+}
+
+kmc_status kmc_set_short_prune(kmc_handle* h, int32_t enable) {
+  if (!h) return fail(h, KMC_ERR_ARG, "kmc_set_short_prune: null handle");
+  h->short_prune = enable != 0;
+  // captured graphs carry the old choice in their refinement launches
   KMC_CK(hipSetDevice(h->dev));
   KMC_CK(hipStreamSynchronize(h->stream));
   for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);

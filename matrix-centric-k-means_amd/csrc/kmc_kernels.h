@@ -155,7 +155,9 @@ struct RefineParams {
   const uint16_t* Plo;
   float alpha3, beta3, gref3;
   int prune_min;  // all-k queues this long or longer take the pruned pass (0: never)
+  int pq;         // shorter all-k queues take the single-sweep pruned pass (K <= kPqMaxK)
 };
+constexpr int kPqMaxK = 512;
 // all-k queue length from which the pruned pass runs: one 128-point group
 // per CU of the chip
 constexpr int kPruneMinPoints = 256 * 128;

@@ -824,6 +824,10 @@ __device__ unsigned region_prefix(const unsigned* cnt, int nregions, unsigned* p
 __host__ __device__ inline bool prune_active(const RefineParams& P) {
   return !P.bp && P.cn && P.xn2 && P.Phi && P.Plo && P.prune_min > 0;
 }
+// the short-queue pruned kernel (k_refine_pq) replaces k_refine_all
+__host__ __device__ inline bool pq_active(const RefineParams& P) {
+  return !P.bp && P.cn && P.xn2 && P.Phi && P.Plo && P.pq && P.K <= kPqMaxK;
+}
 
 // region holding flattened entry t (pre is nondecreasing, pre[0] = 0)
 __device__ __forceinline__ int region_of(const unsigned* pre_s, int nregions, unsigned t) {
@@ -1190,6 +1194,55 @@ __device__ __forceinline__ float exact_dist_rows(const float* __restrict__ x,
   return acc;
 }
 
+// B fragments of a point for the bf16x3 form: dims 16 s + 8 h + [0, 8) of
+// the float row, x = hi + lo (zero for an invalid slot)
+template <int DP>
+__device__ __forceinline__ void split_x(const float* xr, bool valid, int h,
+                                        XFrag<DP, PREC_BF16X3>& xf) {
+#pragma unroll
+  for (int s = 0; s < DP / 16; ++s) {
+    float v[8];
+    const float4 a = *reinterpret_cast<const float4*>(xr + 16 * s + 8 * h);
+    const float4 b = *reinterpret_cast<const float4*>(xr + 16 * s + 8 * h + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    uint32_t hw[4], lw[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float x0 = valid ? v[2 * e] : 0.0f, x1 = valid ? v[2 * e + 1] : 0.0f;
+      const uint16_t h0 = bf16_rne(x0), h1 = bf16_rne(x1);
+      const uint16_t l0 = bf16_rne(x0 - bf16_to_f32(h0)), l1 = bf16_rne(x1 - bf16_to_f32(h1));
+      hw[e] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+      lw[e] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+    }
+    xf.hi[s] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+    xf.lo[s] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+  }
+}
+
+// split_x on a row already in registers (xv[2 s], xv[2 s + 1]: dims
+// 16 s + 8 h + [0, 8))
+template <int DP>
+__device__ __forceinline__ void split_x_regs(const float4* xv, bool valid,
+                                             XFrag<DP, PREC_BF16X3>& xf) {
+#pragma unroll
+  for (int s = 0; s < DP / 16; ++s) {
+    const float4 a = xv[2 * s], b = xv[2 * s + 1];
+    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t hw[4], lw[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float x0 = valid ? v[2 * e] : 0.0f, x1 = valid ? v[2 * e + 1] : 0.0f;
+      const uint16_t h0 = bf16_rne(x0), h1 = bf16_rne(x1);
+      const uint16_t l0 = bf16_rne(x0 - bf16_to_f32(h0)), l1 = bf16_rne(x1 - bf16_to_f32(h1));
+      hw[e] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+      lw[e] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+    }
+    xf.hi[s] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+    xf.lo[s] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+  }
+}
+
 // bf16x3 chain of one 32-centroid block of the LDS chunk against a wave's
 // 32 points (the order of mfma_chain_lds): A fragments are read four k-steps
 // at a time, with a scheduling barrier between groups, so that no more than
@@ -1320,27 +1373,8 @@ __global__ __launch_bounds__(256) void k_refine_prune(RefineParams P) {
     const int pid = pid_s[slot];
     const bool valid = pid >= 0;
     const float* const xr = P.X + (size_t)(valid ? pid : 0) * DP;
-    // B fragments: dims 16 s + 8 h + [0, 8) of the point, x = hi + lo
     XFrag<DP, PREC_BF16X3> xf;
-#pragma unroll
-    for (int s = 0; s < DP / 16; ++s) {
-      float v[8];
-      const float4 a = *reinterpret_cast<const float4*>(xr + 16 * s + 8 * h);
-      const float4 b = *reinterpret_cast<const float4*>(xr + 16 * s + 8 * h + 4);
-      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-      uint32_t hw[4], lw[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float x0 = valid ? v[2 * e] : 0.0f, x1 = valid ? v[2 * e + 1] : 0.0f;
-        const uint16_t h0 = bf16_rne(x0), h1 = bf16_rne(x1);
-        const uint16_t l0 = bf16_rne(x0 - bf16_to_f32(h0)), l1 = bf16_rne(x1 - bf16_to_f32(h1));
-        hw[e] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-        lw[e] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-      }
-      xf.hi[s] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-      xf.lo[s] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
-    }
+    split_x<DP>(xr, valid, h, xf);
     const float x2 = valid ? P.xn2[pid] : 0.0f;
 
     float b1 = kInf, E = 0.0f, twoE = 0.0f, b1p = 0.0f;
@@ -1438,6 +1472,313 @@ __global__ __launch_bounds__(256) void k_refine_prune(RefineParams P) {
       if (threadIdx.x == 0) P.labels[qp] = key_s[q] == ~0ull ? 0 : (int)(unsigned)key_s[q];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// All-k refinement of queues too short for k_refine_prune (one problem,
+// K <= kPqMaxK): the same bf16x3 candidate test and exact stage 2, shaped for
+// latency. A workgroup takes 32 points, the four waves split the centroids
+// (NB 32-centroid blocks each) and keep their MFMA values in registers, so
+// one sweep serves both the minimum b1 (exchanged in LDS) and the band test.
+// Stage 2 gives each point eight lanes for its candidates; a point with more
+// than kPqCap candidates or without a finite value gets every live centroid
+// evaluated exactly by the whole workgroup.
+// ---------------------------------------------------------------------------
+constexpr int kPqCap = 16;
+constexpr int kPqBlocks = 1024;
+
+// A fragments of G steps (from s0) of the 32-centroid block at c0, straight
+// from the prepared planes in global memory
+template <int DP>
+struct PqA {
+  static constexpr int NS = DP / 16;
+  static constexpr int G = NS < 8 ? NS : 8;
+  uint4 hv[G], lv[G];
+};
+template <int DP>
+__device__ __forceinline__ void pq_load(const uint16_t* Phi, const uint16_t* Plo, int c0, int p,
+                                        int h, int s0, PqA<DP>& a) {
+  const uint16_t* ah = Phi + (size_t)(c0 + p) * DP;
+  const uint16_t* al = Plo + (size_t)(c0 + p) * DP;
+#pragma unroll
+  for (int i = 0; i < PqA<DP>::G; ++i) {
+    a.hv[i] = *reinterpret_cast<const uint4*>(ah + 16 * (s0 + i) + 8 * h);
+    a.lv[i] = *reinterpret_cast<const uint4*>(al + 16 * (s0 + i) + 8 * h);
+  }
+}
+template <int DP>
+__device__ __forceinline__ kmc_f32x16 pq_mfma(const PqA<DP>& a, int s0,
+                                              const XFrag<DP, PREC_BF16X3>& x, kmc_f32x16 acc) {
+#pragma unroll
+  for (int i = 0; i < PqA<DP>::G; ++i) {
+    const int st = s0 + i;
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a.hv[i]),
+                                                  __builtin_bit_cast(kmc_bf16x8, x.hi[st]),
+                                                  acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a.hv[i]),
+                                                  __builtin_bit_cast(kmc_bf16x8, x.lo[st]),
+                                                  acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a.lv[i]),
+                                                  __builtin_bit_cast(kmc_bf16x8, x.hi[st]),
+                                                  acc, 0, 0, 0);
+  }
+  return acc;
+}
+// bf16x3 chain of the block at c0 (the order of prune_chain), its first G
+// steps already requested in a0; later groups are requested whole before
+// their MFMAs (the scheduling barrier keeps the compiler from interleaving
+// loads and waits, which would pay one memory latency per step)
+template <int DP>
+__device__ __forceinline__ kmc_f32x16 pq_chain(const uint16_t* Phi, const uint16_t* Plo, int c0,
+                                               int p, int h, const XFrag<DP, PREC_BF16X3>& x,
+                                               kmc_f32x16 acc, const PqA<DP>& a0) {
+  acc = pq_mfma<DP>(a0, 0, x, acc);
+#pragma unroll
+  for (int s0 = PqA<DP>::G; s0 < PqA<DP>::NS; s0 += PqA<DP>::G) {
+    PqA<DP> a;
+    pq_load<DP>(Phi, Plo, c0, p, h, s0, a);
+    __builtin_amdgcn_sched_barrier(0);
+    acc = pq_mfma<DP>(a, s0, x, acc);
+  }
+  return acc;
+}
+
+// exact_dist_rows with the point row in LDS: the centroid row is requested
+// 128 dimensions at a time (64 at DP = 256), all before the first is used
+template <int DP>
+__device__ __forceinline__ float exact_dist_lds(const float* xs, const float* __restrict__ c) {
+#pragma clang fp contract(off)
+  constexpr int CH = DP <= 128 ? DP : 64;
+  float acc = 0.0f;
+#pragma unroll
+  for (int j0 = 0; j0 < DP; j0 += CH) {
+    float4 cv[CH / 4];
+#pragma unroll
+    for (int v = 0; v < CH / 4; ++v) cv[v] = *reinterpret_cast<const float4*>(c + j0 + 4 * v);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int v = 0; v < CH / 4; ++v) {
+      const float4 xv = *reinterpret_cast<const float4*>(xs + j0 + 4 * v);
+      float t, sq;
+      t = xv.x - cv[v].x; sq = t * t; acc = acc + sq;
+      t = xv.y - cv[v].y; sq = t * t; acc = acc + sq;
+      t = xv.z - cv[v].z; sq = t * t; acc = acc + sq;
+      t = xv.w - cv[v].w; sq = t * t; acc = acc + sq;
+    }
+  }
+  return acc;
+}
+
+template <int DP>
+constexpr size_t pq_lds() {
+  return kRefinePreBytes + (size_t)32 * (DP + 4) * sizeof(float);
+}
+
+template <int DP, int NB>
+__global__ __launch_bounds__(256) void k_refine_pq(RefineParams P) {
+  if (*P.done) return;
+  constexpr int XS = DP + 4;  // LDS row stride of the staged point rows (floats)
+  extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
+  unsigned* const pre_s = reinterpret_cast<unsigned*>(rsm);
+  float* const x_s = reinterpret_cast<float*>(rsm + kRefinePreBytes);
+  __shared__ float b1_s[4][32];
+  __shared__ int pid_s[32], cnt_s[32], cand_s[32 * kPqCap];
+  __shared__ unsigned long long key_s[32];
+  __shared__ float red_s[4];
+  __shared__ unsigned ovf_s;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = lane & 31, h = lane >> 5;
+  const float kInf = __builtin_inff();
+
+  // the centroid norms (for max ||c||) are requested before the region
+  // prefix is formed
+  constexpr int NCN = (4 * 32 * NB + 255) / 256;
+  float cnv[NCN];
+#pragma unroll
+  for (int i = 0; i < NCN; ++i) {
+    const int c = threadIdx.x + 256 * i;
+    cnv[i] = c < P.K ? P.cn[c] : 0.0f;
+  }
+  const unsigned total = region_prefix(P.qacnt, P.nregions, pre_s);
+  if (total == 0 || (prune_active(P) && total >= (unsigned)P.prune_min)) return;
+  // max ||c|| over the live centroids (excluded ones carry kFar)
+  float cm = 0.0f;
+#pragma unroll
+  for (int i = 0; i < NCN; ++i)
+    if (cnv[i] < kFarTest) cm = fmaxf(cm, cnv[i]);
+  cm = wave_max_f32(cm);
+  if (lane == 0) red_s[wave] = cm;
+  __syncthreads();
+  const float cmax = sqrtf(fmaxf(fmaxf(red_s[0], red_s[1]), fmaxf(red_s[2], red_s[3])));
+  const float Ecm = P.beta3 * cmax * cmax;
+  const int cw = 32 * NB * wave;  // this wave's first centroid
+
+  const unsigned ngroups = (total + 31) / 32;
+  for (unsigned g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    __syncthreads();  // the previous group is done with the LDS
+    if (threadIdx.x < 32) {
+      const unsigned t = g * 32 + threadIdx.x;
+      int pid = -1;
+      if (t < total) {
+        const int r = region_of(pre_s, P.nregions, t);
+        pid = P.qa[(size_t)r * P.qstride + (t - pre_s[r])];
+      }
+      pid_s[threadIdx.x] = pid;
+      cnt_s[threadIdx.x] = 0;
+      key_s[threadIdx.x] = ~0ull;
+    }
+    if (threadIdx.x == 0) ovf_s = 0u;
+    // the first block's planes do not depend on the points (at DP = 256 the
+    // registers do not hold them beside the point's row and its split)
+    constexpr bool kEarly = DP <= 128;
+    PqA<DP> a[2];
+    if (kEarly && cw < P.Kp) pq_load<DP>(P.Phi, P.Plo, cw, p, h, 0, a[0]);
+    __syncthreads();
+    const int pid = pid_s[p];
+    const bool valid = pid >= 0;
+    // the point's row (this lane's half of each 16-dimension step) is
+    // requested in full before any of it is split into bf16 hi + lo
+    const float* const xr = P.X + (size_t)(valid ? pid : 0) * DP;
+    float4 xv[DP / 8];
+#pragma unroll
+    for (int st = 0; st < DP / 16; ++st) {
+      xv[2 * st] = *reinterpret_cast<const float4*>(xr + 16 * st + 8 * h);
+      xv[2 * st + 1] = *reinterpret_cast<const float4*>(xr + 16 * st + 8 * h + 4);
+    }
+    const float x2 = valid ? P.xn2[pid] : 0.0f;
+    __builtin_amdgcn_sched_barrier(0);
+    XFrag<DP, PREC_BF16X3> xf;
+    split_x_regs<DP>(xv, valid, xf);
+    if (wave == 0) {  // the rows for stage 2
+#pragma unroll
+      for (int st = 0; st < DP / 16; ++st) {
+        *reinterpret_cast<float4*>(x_s + p * XS + 16 * st + 8 * h) = xv[2 * st];
+        *reinterpret_cast<float4*>(x_s + p * XS + 16 * st + 8 * h + 4) = xv[2 * st + 1];
+      }
+    }
+
+    // stage 1: this wave's centroid blocks, values kept for the band test;
+    // the next block's first planes are requested before this block's MFMAs
+    kmc_f32x16 acc[NB];
+    float bl = kInf;
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) {
+      const int c0 = cw + 32 * cb;
+      if (c0 < P.Kp) {
+        if (!kEarly) {
+          pq_load<DP>(P.Phi, P.Plo, c0, p, h, 0, a[cb & 1]);
+          __builtin_amdgcn_sched_barrier(0);
+        } else if (cb + 1 < NB && c0 + 32 < P.Kp) {
+          pq_load<DP>(P.Phi, P.Plo, c0 + 32, p, h, 0, a[(cb + 1) & 1]);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        acc[cb] = pq_chain<DP>(P.Phi, P.Plo, c0, p, h, xf, init_acc<true>(P.cn, c0, h, x2),
+                               a[cb & 1]);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) bl = fminf(bl, acc[cb][i]);
+      }
+    }
+    bl = fminf(bl, __shfl_xor(bl, 32, 64));
+    if (h == 0) b1_s[wave][p] = bl;
+    __syncthreads();
+    const float b1 = fminf(fminf(b1_s[0][p], b1_s[1][p]), fminf(b1_s[2][p], b1_s[3][p]));
+    const float E = P.alpha3 * sqrtf(x2) * cmax + Ecm;
+    const float twoE = 2.0f * E;
+    const float b1p = fmaxf(b1 + E, 0.0f);
+    // candidates as a bit set (bit 16 cb + 4 gq + j), one LDS atomic per lane
+    uint64_t bits = 0;
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) {
+      const int c0 = cw + 32 * cb;
+      if (c0 < P.Kp) {
+#pragma unroll
+        for (int gq = 0; gq < 4; ++gq) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float v = acc[cb][4 * gq + j];
+            const int c = c0 + 8 * gq + 4 * h + j;
+            const bool cand = valid && c < P.K && v < kFarTest &&
+                              !(v - b1 > twoE + P.gref3 * (fmaxf(v + E, 0.0f) + b1p));
+            if (cand) bits |= 1ull << (16 * cb + 4 * gq + j);
+          }
+        }
+      }
+    }
+    if (bits) {
+      int pos = atomicAdd(&cnt_s[p], __popcll(bits));
+      while (bits) {
+        const int b = __ffsll((long long)bits) - 1;
+        bits &= bits - 1;
+        const int r = b & 15;
+        if (pos < kPqCap) cand_s[p * kPqCap + pos] = cw + 32 * (b >> 4) + 8 * (r >> 2) + 4 * h + (r & 3);
+        ++pos;
+      }
+    }
+    __syncthreads();
+
+    // stage 2: lanes 8 q + [0, 8) take point q's candidates
+    {
+      const int q = threadIdx.x >> 3, sub = threadIdx.x & 7;
+      const int qp = pid_s[q];
+      const int nc = cnt_s[q];
+      const float bq1 = fminf(fminf(b1_s[0][q], b1_s[1][q]), fminf(b1_s[2][q], b1_s[3][q]));
+      const bool ovf = qp >= 0 && (nc > kPqCap || !(bq1 < kFarTest));
+      if (ovf && sub == 0) atomicOr(&ovf_s, 1u << q);
+      unsigned long long best = ~0ull;
+      if (qp >= 0 && !ovf) {
+#pragma unroll 1
+        for (int i = sub; i < nc; i += 8) {
+          const int c = cand_s[q * kPqCap + i];
+          const float d = exact_dist_lds<DP>(x_s + q * XS, P.C + (size_t)c * DP);
+          if (d < kInf) {
+            const unsigned long long k = dist_key(d, c);
+            best = k < best ? k : best;
+          }
+        }
+      }
+      best = min_key_xor(best, 4);
+      best = min_key_xor(best, 2);
+      best = min_key_xor(best, 1);
+      if (sub == 0 && qp >= 0 && !ovf) P.labels[qp] = best == ~0ull ? 0 : (int)(unsigned)best;
+    }
+    __syncthreads();
+
+    // points the pruning could not narrow: every live centroid, exactly
+    unsigned m = ovf_s;
+    while (m) {
+      const int q = __ffs(m) - 1;
+      m &= m - 1;
+      const int qp = pid_s[q];
+      unsigned long long bq = ~0ull;
+#pragma unroll 1
+      for (int c = threadIdx.x; c < P.K; c += 256) {
+        if (!P.alive[c]) continue;
+        const float d = exact_dist_lds<DP>(x_s + q * XS, P.C + (size_t)c * DP);
+        if (d < kInf) {
+          const unsigned long long k = dist_key(d, c);
+          bq = k < bq ? k : bq;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) bq = min_key_xor(bq, o);
+      if (lane == 0) atomicMin(&key_s[q], bq);
+      __syncthreads();
+      if (threadIdx.x == 0) P.labels[qp] = key_s[q] == ~0ull ? 0 : (int)(unsigned)key_s[q];
+    }
+  }
+}
+
+template <int DP>
+hipError_t launch_pq(const RefineParams& P, hipStream_t s) {
+  if (P.K <= 128)
+    hipLaunchKernelGGL((k_refine_pq<DP, 1>), dim3(kPqBlocks), dim3(256), pq_lds<DP>(), s, P);
+  else if (P.K <= 256)
+    hipLaunchKernelGGL((k_refine_pq<DP, 2>), dim3(kPqBlocks), dim3(256), pq_lds<DP>(), s, P);
+  else if (P.K <= kPqMaxK)
+    hipLaunchKernelGGL((k_refine_pq<DP, 4>), dim3(kPqBlocks), dim3(256), pq_lds<DP>(), s, P);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -2912,7 +3253,20 @@ hipError_t launch_refine(const RefineParams& P, hipStream_t s) {
     grid2 = std::min(kRefineBlocks, per_cu * cus);
   }
   hipLaunchKernelGGL(k_refine2, dim3(grid2), dim3(256), kRefineLds, s, P);
-  hipLaunchKernelGGL(k_refine_all, dim3(kRefineBlocks), dim3(256), kRefineAllLds, s, P);
+  if (pq_active(P)) {
+    hipError_t e = hipSuccess;
+    switch (P.DP) {
+      case 16: e = launch_pq<16>(P, s); break;
+      case 32: e = launch_pq<32>(P, s); break;
+      case 64: e = launch_pq<64>(P, s); break;
+      case 128: e = launch_pq<128>(P, s); break;
+      case 256: e = launch_pq<256>(P, s); break;
+      default: e = hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+  } else {
+    hipLaunchKernelGGL(k_refine_all, dim3(kRefineBlocks), dim3(256), kRefineAllLds, s, P);
+  }
   if (!prune_active(P)) return hipGetLastError();
   switch (P.DP) {
     case 16: return launch_pruned<16>(P, s);

@@ -142,6 +142,7 @@ def load_library(path=LIB_PATH):
         "kmc_autotune": (ctypes.c_int, [vp, P(_Tuning)]),
         "kmc_set_assign_variant": (ctypes.c_int, [vp, ctypes.c_int]),
         "kmc_set_prune_min": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "kmc_set_short_prune": (ctypes.c_int, [vp, ctypes.c_int32]),
         "kmc_kk_create": (ctypes.c_int, [P(_KKConfig), P(vp)]),
         "kmc_kk_destroy": (ctypes.c_int, [vp]),
         "kmc_kk_set_points": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
@@ -282,6 +283,13 @@ class Engine:
         """All-k refinement queues of n or more points take the MFMA-pruned
         pass (1: always, 0: never; default 32768)."""
         self._check(self._lib.kmc_set_prune_min(self._h, int(n)), "set_prune_min")
+
+    def set_short_prune(self, enable):
+        """All-k queues below the prune_min threshold (all of them when it is
+        0) take the single-sweep MFMA-pruned pass when K <= 512 (default on);
+        off: every centroid is evaluated exactly for them."""
+        self._check(self._lib.kmc_set_short_prune(self._h, int(bool(enable))),
+                    "set_short_prune")
 
     def labels(self):
         out = np.empty(self.n, dtype=np.int32)

@@ -2,8 +2,9 @@
 restatement of the reference CPU k-means: one cluster, fewer points than a
 tile, one dimension, all points identical (every distance ties: the first
 centroid wins), exact duplicate centroids, a width that needs padding, and
-many tied candidates -- the tie-prone cases with both all-k refinement
-paths (prune 0: every centroid exactly; 1: the bf16x3 MFMA-pruned pass)."""
+many tied candidates -- the tie-prone cases with each all-k refinement
+path (prune 0: every centroid exactly; 1: the bf16x3 MFMA-pruned pass;
+"short": its single-sweep form for short queues)."""
 import numpy as np
 import pytest
 
@@ -16,8 +17,10 @@ def _check(kmc_mod, oracle, X, C0, prec, tol=1e-4, maxiter=50, update="ordered",
     want_conv, want_lab, want_C, _, _ = oracle.lloyd(X, C0, tol, maxiter)
     with kmc_mod.Engine(X.shape[0], X.shape[1], C0.shape[0], tol=tol, precision=prec,
                         update=update) as e:
+        if prune == 0:
+            e.set_short_prune(False)
         if prune is not None:
-            e.set_prune_min(prune)
+            e.set_prune_min(0 if prune == "short" else prune)
         e.set_points(X)
         e.set_centroids(C0)
         r = e.run(maxiter, check_converged=True)
@@ -43,7 +46,7 @@ def test_fewer_points_than_a_tile(kmc_mod, oracle, prec):
     _check(kmc_mod, oracle, X, X[[0, 2]].copy(), prec)
 
 
-@pytest.mark.parametrize("prune", [0, 1])
+@pytest.mark.parametrize("prune", [0, 1, "short"])
 @pytest.mark.parametrize("prec", PRECS)
 def test_one_dimension(kmc_mod, oracle, prec, prune):
     rng = np.random.default_rng(2)
@@ -51,7 +54,7 @@ def test_one_dimension(kmc_mod, oracle, prec, prune):
     _check(kmc_mod, oracle, X, X[[0, 300, 600]].copy(), prec, prune=prune)
 
 
-@pytest.mark.parametrize("prune", [0, 1])
+@pytest.mark.parametrize("prune", [0, 1, "short"])
 @pytest.mark.parametrize("prec", PRECS)
 def test_identical_points_tie_to_first_centroid(kmc_mod, oracle, prec, prune):
     X = np.full((2000, 16), 3.25, np.float32)
@@ -60,7 +63,7 @@ def test_identical_points_tie_to_first_centroid(kmc_mod, oracle, prec, prune):
     assert not lab.any()  # equidistant: the first index wins (strict <)
 
 
-@pytest.mark.parametrize("prune", [0, 1])
+@pytest.mark.parametrize("prune", [0, 1, "short"])
 @pytest.mark.parametrize("prec", PRECS)
 def test_duplicate_centroids(kmc_mod, oracle, prec, prune):
     rng = np.random.default_rng(3)
@@ -70,7 +73,7 @@ def test_duplicate_centroids(kmc_mod, oracle, prec, prune):
     assert lab.max() < 5  # a duplicate never beats its first copy
 
 
-@pytest.mark.parametrize("prune", [0, 1])
+@pytest.mark.parametrize("prune", [0, 1, "short"])
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("d", [3, 33, 130])
 def test_padded_widths(kmc_mod, oracle, prec, d, prune):
@@ -82,7 +85,7 @@ def test_padded_widths(kmc_mod, oracle, prec, d, prune):
     _check(kmc_mod, oracle, X, C0, prec, maxiter=20, update="fixed", prune=prune)
 
 
-@pytest.mark.parametrize("prune", [0, 1])
+@pytest.mark.parametrize("prune", [0, 1, "short"])
 @pytest.mark.parametrize("prec", PRECS)
 def test_many_tied_candidates(kmc_mod, oracle, prec, prune):
     """Twelve copies of one centroid among twenty others: points near the
@@ -96,7 +99,7 @@ def test_many_tied_candidates(kmc_mod, oracle, prec, prune):
     assert not np.any((lab > 7) & (lab < 19))  # never a later copy
 
 
-@pytest.mark.parametrize("prune", [0, 1])
+@pytest.mark.parametrize("prune", [0, 1, "short"])
 @pytest.mark.parametrize("prec", PRECS)
 def test_all_centroids_tied(kmc_mod, oracle, prec, prune):
     """Forty identical centroids: every point has forty tied candidates, more

@@ -191,7 +191,7 @@ def test_every_launch_variant_gives_the_same_labels(kmc_mod, oracle, c2_data, pr
 @pytest.mark.parametrize("prec", PRECS)
 def test_pruned_all_k_pass_gives_the_same_labels(kmc_mod, oracle, c2_data, prec):
     """The all-k refinement with and without its bf16x3 MFMA pruning stage
-    reproduces the reference's labels on a 200K-point slice; centroids
+    (in both its forms) reproduces the reference's labels on a 200K-point slice; centroids
     perturbed towards each other put many points in the band."""
     X, C0 = c2_data
     Xs = np.ascontiguousarray(X[:200_000])
@@ -200,11 +200,15 @@ def test_pruned_all_k_pass_gives_the_same_labels(kmc_mod, oracle, c2_data, prec)
     want, _ = _reference_labels(oracle, Xs, C)
     with kmc_mod.Engine(Xs.shape[0], 128, 256, precision=prec) as e:
         e.set_points(Xs)
-        for pm in (1, 32768, 0):  # always pruned, the default threshold, never pruned
+        # always pruned, the default threshold, the short-queue form for every
+        # queue, never pruned
+        for short, pm in ((True, 1), (True, 32768), (True, 0), (False, 0)):
+            e.set_short_prune(short)
             e.set_prune_min(pm)
             lab, info = e.assign(C)
-            _say("prune_min", pm, prec, "refined", info["refined_two"], info["refined_full"])
-            assert np.array_equal(lab, want), pm
+            _say("prune_min", pm, short, prec, "refined", info["refined_two"],
+                 info["refined_full"])
+            assert np.array_equal(lab, want), (short, pm)
 
 
 @pytest.mark.parametrize("graph", [False, True])
