@@ -1194,6 +1194,18 @@ __device__ __forceinline__ float exact_dist_rows(const float* __restrict__ x,
   return acc;
 }
 
+// x0, x1 = hi + lo, each half rounded to nearest even (the packed hardware
+// conversion: for finite values the same as bf16_rne; a NaN row gets no
+// finite MFMA value and its point every centroid evaluated exactly)
+typedef __bf16 kmc_bf16x2v __attribute__((ext_vector_type(2)));
+typedef float kmc_f32x2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split_pair(float x0, float x1, uint32_t& hw, uint32_t& lw) {
+  const kmc_f32x2v xv = {x0, x1};
+  hw = __builtin_bit_cast(uint32_t, __builtin_convertvector(xv, kmc_bf16x2v));
+  const kmc_f32x2v rv = {x0 - __uint_as_float(hw << 16), x1 - __uint_as_float(hw & 0xffff0000u)};
+  lw = __builtin_bit_cast(uint32_t, __builtin_convertvector(rv, kmc_bf16x2v));
+}
+
 // B fragments of a point for the bf16x3 form: dims 16 s + 8 h + [0, 8) of
 // the float row, x = hi + lo (zero for an invalid slot)
 template <int DP>
@@ -1208,13 +1220,8 @@ __device__ __forceinline__ void split_x(const float* xr, bool valid, int h,
     v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
     uint32_t hw[4], lw[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const float x0 = valid ? v[2 * e] : 0.0f, x1 = valid ? v[2 * e + 1] : 0.0f;
-      const uint16_t h0 = bf16_rne(x0), h1 = bf16_rne(x1);
-      const uint16_t l0 = bf16_rne(x0 - bf16_to_f32(h0)), l1 = bf16_rne(x1 - bf16_to_f32(h1));
-      hw[e] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-      lw[e] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-    }
+    for (int e = 0; e < 4; ++e)
+      split_pair(valid ? v[2 * e] : 0.0f, valid ? v[2 * e + 1] : 0.0f, hw[e], lw[e]);
     xf.hi[s] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
     xf.lo[s] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
   }
@@ -1231,13 +1238,8 @@ __device__ __forceinline__ void split_x_regs(const float4* xv, bool valid,
     const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     uint32_t hw[4], lw[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const float x0 = valid ? v[2 * e] : 0.0f, x1 = valid ? v[2 * e + 1] : 0.0f;
-      const uint16_t h0 = bf16_rne(x0), h1 = bf16_rne(x1);
-      const uint16_t l0 = bf16_rne(x0 - bf16_to_f32(h0)), l1 = bf16_rne(x1 - bf16_to_f32(h1));
-      hw[e] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-      lw[e] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-    }
+    for (int e = 0; e < 4; ++e)
+      split_pair(valid ? v[2 * e] : 0.0f, valid ? v[2 * e + 1] : 0.0f, hw[e], lw[e]);
     xf.hi[s] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
     xf.lo[s] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
   }
