@@ -241,11 +241,12 @@ kmc_status kmc_set_assign_variant(kmc_handle* h, int variant);
  * knob of this library's refinement, with no counterpart in the reference. */
 kmc_status kmc_set_prune_min(kmc_handle* h, int32_t min_points);
 
-/* With K <= 512, all-k queues shorter than that threshold (every queue when
- * it is 0) take a single-sweep form of the same MFMA pruning, shaped for
- * short queues (default on); 0 sends them to the exact pass over every
- * centroid. Labels are the same either way. A tuning knob with no
- * counterpart in the reference. */
+/* With K <= 512 at D <= 64, or K <= 256 at 64 < D <= 128, all-k queues
+ * shorter than that threshold (every queue when it is 0) take a single-sweep
+ * form of the same MFMA pruning, shaped for short queues and run in one
+ * launch beside the two/three-candidate refinement (default on); 0 sends them
+ * to the exact pass over every centroid. Labels are the same either way. A
+ * tuning knob with no counterpart in the reference. */
 kmc_status kmc_set_short_prune(kmc_handle* h, int32_t enable);
 
 /* Release every device buffer (src/kmeans.cu:307-359). */

@@ -155,9 +155,13 @@ struct RefineParams {
   const uint16_t* Plo;
   float alpha3, beta3, gref3;
   int prune_min;  // all-k queues this long or longer take the pruned pass (0: never)
-  int pq;         // shorter all-k queues take the single-sweep pruned pass (K <= kPqMaxK)
+  int pq;         // shorter all-k queues take the single-sweep pruned pass (pq_shape)
 };
-constexpr int kPqMaxK = 512;
+// shapes of the single-sweep pruned all-k pass: K <= 512 up to padded D = 64,
+// K <= 256 at padded D = 128
+__host__ __device__ constexpr bool pq_shape(int K, int DP) {
+  return DP <= 64 ? K <= 512 : (DP == 128 && K <= 256);
+}
 // all-k queue length from which the pruned pass runs: one 128-point group
 // per CU of the chip
 constexpr int kPruneMinPoints = 256 * 128;

@@ -819,14 +819,50 @@ __device__ unsigned region_prefix(const unsigned* cnt, int nregions, unsigned* p
   return pre_s[nregions];
 }
 
+// region_prefix of cnt, and in the same pass the total of cnt2 (*total2)
+__device__ unsigned region_prefix2(const unsigned* cnt, const unsigned* cnt2, int nregions,
+                                   unsigned* pre_s, unsigned* total2) {
+  __shared__ unsigned part_s[256];
+  __shared__ unsigned t2_s[4];
+  const int per = (nregions + 255) / 256;
+  const int lo = min(nregions, (int)threadIdx.x * per), hi = min(nregions, lo + per);
+  unsigned s = 0, s2 = 0;
+  for (int r = lo; r < hi; ++r) {
+    s += cnt[r];
+    s2 += cnt2[r];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o, 64);
+  if ((threadIdx.x & 63) == 0) t2_s[threadIdx.x >> 6] = s2;
+  part_s[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const unsigned v = threadIdx.x >= (unsigned)o ? part_s[threadIdx.x - o] : 0u;
+    __syncthreads();
+    part_s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned run = part_s[threadIdx.x] - s;
+  for (int r = lo; r < hi; ++r) {
+    pre_s[r] = run;
+    run += cnt[r];
+  }
+  if (threadIdx.x == 255) pre_s[nregions] = part_s[255];
+  __syncthreads();
+  *total2 = t2_s[0] + t2_s[1] + t2_s[2] + t2_s[3];
+  return pre_s[nregions];
+}
+
 // the pruned all-k kernel (k_refine_prune) is launched, and k_refine_all
 // leaves queues of prune_min or more entries to it, exactly when this holds
 __host__ __device__ inline bool prune_active(const RefineParams& P) {
   return !P.bp && P.cn && P.xn2 && P.Phi && P.Plo && P.prune_min > 0;
 }
-// the short-queue pruned kernel (k_refine_pq) replaces k_refine_all
+// the fused refinement kernel (k_refine_fused) replaces k_refine2 and
+// k_refine_all; the shapes are those whose all-k path fits 256 registers
+// per lane without scratch
 __host__ __device__ inline bool pq_active(const RefineParams& P) {
-  return !P.bp && P.cn && P.xn2 && P.Phi && P.Plo && P.pq && P.K <= kPqMaxK;
+  return !P.bp && P.cn && P.xn2 && P.Phi && P.Plo && P.pq && pq_shape(P.K, P.DP);
 }
 
 // region holding flattened entry t (pre is nondecreasing, pre[0] = 0)
@@ -860,14 +896,16 @@ static_assert(kRefinePts * kMaxDP * 4 + 48 * kRefinePts <= 4 * 3 * 64 * kR2Strid
               "all-k pass needs less LDS than the two/three-candidate pass");
 template <int CH>
 __device__ void refine2_body(const RefineParams& P, unsigned char* smem, unsigned bid,
-                             unsigned nblk) {
+                             unsigned nblk, unsigned known_total = ~0u) {
   constexpr int LPE = CH / 4;     // lanes per entry in the load phase (16 B each)
   constexpr int EPW = 256 / LPE;  // entries per wave (each lane loads 4 rows)
   constexpr int RS = CH + 4;      // LDS row stride in floats (b128 reads conflict-free)
   static_assert(3 * EPW * RS <= 3 * 64 * kR2Stride, "LDS slice of a wave");
   unsigned* const pre_s = reinterpret_cast<unsigned*>(smem);
   float* const sq_all = reinterpret_cast<float*>(smem + kRefinePreBytes);
-  const unsigned total = region_prefix(P.q2cnt, P.nregions, pre_s);
+  // known_total: the caller has left the queue's region prefix in pre_s
+  const unsigned total =
+      known_total != ~0u ? known_total : region_prefix(P.q2cnt, P.nregions, pre_s);
   if (total == 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float* const sq = sq_all + wave * (3 * 64 * kR2Stride);
@@ -1478,7 +1516,7 @@ __global__ __launch_bounds__(256) void k_refine_prune(RefineParams P) {
 
 // ---------------------------------------------------------------------------
 // All-k refinement of queues too short for k_refine_prune (one problem,
-// K <= kPqMaxK): the same bf16x3 candidate test and exact stage 2, shaped for
+// pq_shape): the same bf16x3 candidate test and exact stage 2, shaped for
 // latency. A workgroup takes 32 points, the four waves split the centroids
 // (NB 32-centroid blocks each) and keep their MFMA values in registers, so
 // one sweep serves both the minimum b1 (exchanged in LDS) and the band test.
@@ -1487,7 +1525,6 @@ __global__ __launch_bounds__(256) void k_refine_prune(RefineParams P) {
 // evaluated exactly by the whole workgroup.
 // ---------------------------------------------------------------------------
 constexpr int kPqCap = 16;
-constexpr int kPqBlocks = 1024;
 
 // A fragments of G steps (from s0) of the 32-centroid block at c0, straight
 // from the prepared planes in global memory
@@ -1571,29 +1608,41 @@ __device__ __forceinline__ float exact_dist_lds(const float* xs, const float* __
   return acc;
 }
 
+// the all-k groups' LDS arrays, after the staged point rows
+struct PqShared {
+  float b1_s[4][32];
+  int pid_s[32], cnt_s[32], cand_s[32 * kPqCap];
+  unsigned long long key_s[32];
+  float red_s[4];
+  unsigned ovf_s;
+};
 template <int DP>
 constexpr size_t pq_lds() {
-  return kRefinePreBytes + (size_t)32 * (DP + 4) * sizeof(float);
+  return kRefinePreBytes + (size_t)32 * (DP + 4) * sizeof(float) + sizeof(PqShared);
 }
 
-template <int DP, int NB>
-__global__ __launch_bounds__(256) void k_refine_pq(RefineParams P) {
-  if (*P.done) return;
+// The all-k groups g = bid, bid + nblk, ... of a queue of `total` points
+// whose region prefix is in the LDS (kEarly: the first centroid block's
+// planes are requested before the points' rows, when the registers allow).
+template <int DP, int NB, bool kEarly>
+__device__ void pq_body(const RefineParams& P, unsigned char* rsm, unsigned bid, unsigned nblk,
+                        unsigned total) {
   constexpr int XS = DP + 4;  // LDS row stride of the staged point rows (floats)
-  extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
   unsigned* const pre_s = reinterpret_cast<unsigned*>(rsm);
   float* const x_s = reinterpret_cast<float*>(rsm + kRefinePreBytes);
-  __shared__ float b1_s[4][32];
-  __shared__ int pid_s[32], cnt_s[32], cand_s[32 * kPqCap];
-  __shared__ unsigned long long key_s[32];
-  __shared__ float red_s[4];
-  __shared__ unsigned ovf_s;
+  PqShared& sh = *reinterpret_cast<PqShared*>(rsm + kRefinePreBytes +
+                                              (size_t)32 * XS * sizeof(float));
+  float(&b1_s)[4][32] = sh.b1_s;
+  int(&pid_s)[32] = sh.pid_s;
+  int(&cnt_s)[32] = sh.cnt_s;
+  int(&cand_s)[32 * kPqCap] = sh.cand_s;
+  unsigned long long(&key_s)[32] = sh.key_s;
+  float(&red_s)[4] = sh.red_s;
+  unsigned& ovf_s = sh.ovf_s;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int p = lane & 31, h = lane >> 5;
   const float kInf = __builtin_inff();
 
-  // the centroid norms (for max ||c||) are requested before the region
-  // prefix is formed
   constexpr int NCN = (4 * 32 * NB + 255) / 256;
   float cnv[NCN];
 #pragma unroll
@@ -1601,8 +1650,6 @@ __global__ __launch_bounds__(256) void k_refine_pq(RefineParams P) {
     const int c = threadIdx.x + 256 * i;
     cnv[i] = c < P.K ? P.cn[c] : 0.0f;
   }
-  const unsigned total = region_prefix(P.qacnt, P.nregions, pre_s);
-  if (total == 0 || (prune_active(P) && total >= (unsigned)P.prune_min)) return;
   // max ||c|| over the live centroids (excluded ones carry kFar)
   float cm = 0.0f;
 #pragma unroll
@@ -1616,7 +1663,7 @@ __global__ __launch_bounds__(256) void k_refine_pq(RefineParams P) {
   const int cw = 32 * NB * wave;  // this wave's first centroid
 
   const unsigned ngroups = (total + 31) / 32;
-  for (unsigned g = blockIdx.x; g < ngroups; g += gridDim.x) {
+  for (unsigned g = bid; g < ngroups; g += nblk) {
     __syncthreads();  // the previous group is done with the LDS
     if (threadIdx.x < 32) {
       const unsigned t = g * 32 + threadIdx.x;
@@ -1630,9 +1677,7 @@ __global__ __launch_bounds__(256) void k_refine_pq(RefineParams P) {
       key_s[threadIdx.x] = ~0ull;
     }
     if (threadIdx.x == 0) ovf_s = 0u;
-    // the first block's planes do not depend on the points (at DP = 256 the
-    // registers do not hold them beside the point's row and its split)
-    constexpr bool kEarly = DP <= 128;
+    // the first block's planes do not depend on the points
     PqA<DP> a[2];
     if (kEarly && cw < P.Kp) pq_load<DP>(P.Phi, P.Plo, cw, p, h, 0, a[0]);
     __syncthreads();
@@ -1770,17 +1815,64 @@ __global__ __launch_bounds__(256) void k_refine_pq(RefineParams P) {
   }
 }
 
-template <int DP>
-hipError_t launch_pq(const RefineParams& P, hipStream_t s) {
-  if (P.K <= 128)
-    hipLaunchKernelGGL((k_refine_pq<DP, 1>), dim3(kPqBlocks), dim3(256), pq_lds<DP>(), s, P);
-  else if (P.K <= 256)
-    hipLaunchKernelGGL((k_refine_pq<DP, 2>), dim3(kPqBlocks), dim3(256), pq_lds<DP>(), s, P);
-  else if (P.K <= kPqMaxK)
-    hipLaunchKernelGGL((k_refine_pq<DP, 4>), dim3(kPqBlocks), dim3(256), pq_lds<DP>(), s, P);
-  else
-    return hipErrorInvalidValue;
+// Both refinement passes in one launch (one problem, pq_shape): the first G
+// workgroups take the all-k groups (G: the group count, at most a quarter of
+// the grid; none when the all-k queue goes to k_refine_prune), the others
+// the two/three-candidate batches, so that the
+// all-k chain runs beside the batches instead of after them. The grid is the
+// workgroups resident at once, two per CU for the two/three-candidate LDS;
+// the all-k path fits that budget without its early plane requests.
+template <int DP, int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_refine_fused(RefineParams P) {
+  if (*P.done) return;
+  extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
+  unsigned* const pre_s = reinterpret_cast<unsigned*>(rsm);
+  unsigned ta = 0;
+  const unsigned t2 = region_prefix2(P.q2cnt, P.qacnt, P.nregions, pre_s, &ta);
+  unsigned G = (ta + 31) / 32;
+  if (ta == 0 || (prune_active(P) && ta >= (unsigned)P.prune_min)) G = 0;
+  G = min(G, gridDim.x / 4);
+  if (blockIdx.x < G) {
+    __syncthreads();  // every thread has read the two/three-candidate total
+    unsigned dummy;
+    const unsigned tq = region_prefix2(P.qacnt, P.qacnt, P.nregions, pre_s, &dummy);
+    pq_body<DP, NB, false>(P, rsm, blockIdx.x, G, tq);
+  } else if (t2 > 0) {
+    refine2_body<(DP % 32 == 0) ? 32 : 16>(P, rsm, blockIdx.x - G, gridDim.x - G, t2);
+  }
+}
+static_assert(pq_lds<kMaxDP>() <= kRefineLds, "the all-k groups fit the fused LDS");
+
+template <int DP, int NB>
+hipError_t launch_fused1(const RefineParams& P, hipStream_t s) {
+  static int grid = 0;
+  if (!grid) {
+    const void* fn = reinterpret_cast<const void*>(k_refine_fused<DP, NB>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kRefineLds);
+    if (e != hipSuccess) return e;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, kRefineLds) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    grid = std::min(kRefineBlocks, per_cu * cus);
+  }
+  hipLaunchKernelGGL((k_refine_fused<DP, NB>), dim3(grid), dim3(256), kRefineLds, s, P);
   return hipGetLastError();
+}
+template <int DP>
+hipError_t launch_fused(const RefineParams& P, hipStream_t s) {
+  if (!pq_shape(P.K, DP)) return hipErrorInvalidValue;
+  if (P.K <= 128) return launch_fused1<DP, 1>(P, s);
+  if (P.K <= 256) return launch_fused1<DP, 2>(P, s);
+  if constexpr (DP <= 64) return launch_fused1<DP, 4>(P, s);
+  return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------
@@ -3254,19 +3346,18 @@ hipError_t launch_refine(const RefineParams& P, hipStream_t s) {
       per_cu = 1;
     grid2 = std::min(kRefineBlocks, per_cu * cus);
   }
-  hipLaunchKernelGGL(k_refine2, dim3(grid2), dim3(256), kRefineLds, s, P);
   if (pq_active(P)) {
     hipError_t e = hipSuccess;
     switch (P.DP) {
-      case 16: e = launch_pq<16>(P, s); break;
-      case 32: e = launch_pq<32>(P, s); break;
-      case 64: e = launch_pq<64>(P, s); break;
-      case 128: e = launch_pq<128>(P, s); break;
-      case 256: e = launch_pq<256>(P, s); break;
+      case 16: e = launch_fused<16>(P, s); break;
+      case 32: e = launch_fused<32>(P, s); break;
+      case 64: e = launch_fused<64>(P, s); break;
+      case 128: e = launch_fused<128>(P, s); break;
       default: e = hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
   } else {
+    hipLaunchKernelGGL(k_refine2, dim3(grid2), dim3(256), kRefineLds, s, P);
     hipLaunchKernelGGL(k_refine_all, dim3(kRefineBlocks), dim3(256), kRefineAllLds, s, P);
   }
   if (!prune_active(P)) return hipGetLastError();

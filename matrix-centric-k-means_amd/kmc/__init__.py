@@ -286,7 +286,9 @@ class Engine:
 
     def set_short_prune(self, enable):
         """All-k queues below the prune_min threshold (all of them when it is
-        0) take the single-sweep MFMA-pruned pass when K <= 512 (default on);
+        0) take the single-sweep MFMA-pruned pass, in one launch with the
+        two/three-candidate refinement (default on; K <= 512 at D <= 64,
+        K <= 256 at D <= 128);
         off: every centroid is evaluated exactly for them."""
         self._check(self._lib.kmc_set_short_prune(self._h, int(bool(enable))),
                     "set_short_prune")
