@@ -985,6 +985,107 @@ __device__ void refine2_body(const RefineParams& P, unsigned char* smem, unsigne
   }
 }
 
+// refine2_body at a compile-time width with the latency hidden deeper, for
+// the fused kernel (whose register budget is 256 per lane): the dimension
+// chunks are requested two ahead, and the next batch's queue entry while the
+// current batch's first chunks are in flight. The queue's region prefix is
+// in the LDS; same arithmetic and order as refine2_body.
+template <int CH, int DP>
+__device__ void refine2_deep(const RefineParams& P, unsigned char* smem, unsigned bid,
+                             unsigned nblk, unsigned total) {
+  constexpr int LPE = CH / 4;
+  constexpr int EPW = 256 / LPE;
+  constexpr int RS = CH + 4;
+  constexpr int NCH = DP / CH;
+  static_assert(3 * EPW * RS <= 3 * 64 * kR2Stride, "LDS slice of a wave");
+  static_assert(DP % CH == 0, "whole chunks");
+  unsigned* const pre_s = reinterpret_cast<unsigned*>(smem);
+  float* const sq_all = reinterpret_cast<float*>(smem + kRefinePreBytes);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float* const sq = sq_all + wave * (3 * 64 * kR2Stride);
+  const int sub = lane % LPE;
+  const unsigned nbatch = (total + EPW - 1) / EPW;
+  const unsigned stride = nblk * 4u;
+  auto entry = [&](unsigned b) {
+    const unsigned t = min(b * EPW + min(lane, EPW - 1), total - 1);
+    const int r = region_of(pre_s, P.nregions, t);
+    return *reinterpret_cast<const int4*>(P.q2 + 4 * ((size_t)r * P.qstride + (t - pre_s[r])));
+  };
+  unsigned bt = bid * 4u + wave;
+  int4 qn = make_int4(0, 0, 0, -1);
+  if (bt < nbatch) qn = entry(bt);
+  for (; bt < nbatch; bt += stride) {
+    const int4 q = qn;
+    const int cand0 = q.y, cand1 = q.z, cand2 = q.w >= 0 ? q.w : q.z;
+    const float* xp[4];
+    const float* cp[4][3];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int e = lane / LPE + (64 / LPE) * rr;
+      xp[rr] = P.X + (size_t)__shfl(q.x, e, 64) * DP + 4 * sub;
+      cp[rr][0] = P.C + (size_t)__shfl(cand0, e, 64) * DP + 4 * sub;
+      cp[rr][1] = P.C + (size_t)__shfl(cand1, e, 64) * DP + 4 * sub;
+      cp[rr][2] = P.C + (size_t)__shfl(cand2, e, 64) * DP + 4 * sub;
+    }
+    float4 xv[2][4], cv[2][4][3];
+    auto load_chunk = [&](int b, int j0) {
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        xv[b][rr] = *reinterpret_cast<const float4*>(xp[rr] + j0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          cv[b][rr][k] = *reinterpret_cast<const float4*>(cp[rr][k] + j0);
+      }
+    };
+    load_chunk(0, 0);
+    if (NCH > 1) load_chunk(1, CH);
+    if (bt + stride < nbatch) qn = entry(bt + stride);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      const int b = ch & 1;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int e = lane / LPE + (64 / LPE) * rr;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+#pragma clang fp contract(off)
+          float4 sv;
+          float tt;
+          tt = xv[b][rr].x - cv[b][rr][k].x; sv.x = tt * tt;
+          tt = xv[b][rr].y - cv[b][rr][k].y; sv.y = tt * tt;
+          tt = xv[b][rr].z - cv[b][rr][k].z; sv.z = tt * tt;
+          tt = xv[b][rr].w - cv[b][rr][k].w; sv.w = tt * tt;
+          *reinterpret_cast<float4*>(sq + (k * EPW + e) * RS + 4 * sub) = sv;
+        }
+      }
+      if (ch + 2 < NCH) load_chunk(b, (ch + 2) * CH);
+      __builtin_amdgcn_wave_barrier();
+      if (lane < EPW) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float* row = sq + (k * EPW + lane) * RS;
+#pragma unroll
+          for (int v4 = 0; v4 < CH / 4; ++v4) {
+#pragma clang fp contract(off)
+            const float4 sv = *reinterpret_cast<const float4*>(row + 4 * v4);
+            acc[k] = acc[k] + sv.x;
+            acc[k] = acc[k] + sv.y;
+            acc[k] = acc[k] + sv.z;
+            acc[k] = acc[k] + sv.w;
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    int best = cand0;
+    float rbest = acc[0];
+    if (acc[1] < rbest || (acc[1] == rbest && cand1 < best)) { rbest = acc[1]; best = cand1; }
+    if (acc[2] < rbest || (acc[2] == rbest && cand2 < best)) { rbest = acc[2]; best = cand2; }
+    if (lane < EPW && bt * EPW + lane < total) P.labels[q.x] = best;
+  }
+}
+
 // Band points whose four nearest MFMA candidates are all ambiguous: every
 // centroid in the reference's float arithmetic. A workgroup takes
 // kRefinePts consecutive queue entries and stages their rows in LDS; wave w
@@ -1839,7 +1940,7 @@ void k_refine_fused(RefineParams P) {
     const unsigned tq = region_prefix2(P.qacnt, P.qacnt, P.nregions, pre_s, &dummy);
     pq_body<DP, NB, false>(P, rsm, blockIdx.x, G, tq);
   } else if (t2 > 0) {
-    refine2_body<(DP % 32 == 0) ? 32 : 16>(P, rsm, blockIdx.x - G, gridDim.x - G, t2);
+    refine2_deep<(DP % 32 == 0) ? 32 : 16, DP>(P, rsm, blockIdx.x - G, gridDim.x - G, t2);
   }
 }
 static_assert(pq_lds<kMaxDP>() <= kRefineLds, "the all-k groups fit the fused LDS");
