@@ -326,3 +326,34 @@ def test_greedy_plusplus_matches_oracle(kmc_mod, oracle, n, d, k, seed, trials):
     want, _ = oracle.plusplus(X, k, splitmix_uniforms(seed, 1 + (k - 1) * s), trials=s)
     assert np.array_equal(chosen, want), (chosen, want)
     assert np.array_equal(C, X[want])
+
+
+@pytest.mark.parametrize("d,k", [(16, 512), (48, 400), (64, 100), (100, 200), (128, 256)])
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_short_queue_pruned_groups_over_shapes(kmc_mod, oracle, prec, d, k):
+    """The single-sweep MFMA-pruned all-k groups, run beside the
+    two/three-candidate refinement in one launch, for each count of
+    32-centroid blocks per wave (K up to 128, 256 and 512) and padded widths
+    16 to 128: with the default threshold, with every queue (prune_min 0: a
+    long queue has more groups than its quarter of the grid, so workgroups
+    take several), and against the exact pass over every centroid. The
+    centroids come in groups of four near-duplicates around a quarter of the
+    blob centres, so a point has four or more candidates inside the band
+    (its own blob's group, or for the other blobs' points many far
+    groups at similar distances)."""
+    from kmc import datagen
+    n = 60_000
+    X, _, centres = datagen.blobs(n, d, k, 4000 + d + k, -10.0, 10.0, 0.5, 2.0)
+    rng = np.random.default_rng(d * 1000 + k)
+    C = np.repeat(centres[: k // 4], 4, axis=0)
+    C = np.ascontiguousarray(C + rng.normal(0.0, 0.02, C.shape), dtype=np.float32)
+    want, _ = _reference_labels(oracle, X, C)
+    with kmc_mod.Engine(n, d, k, precision=prec) as e:
+        e.set_points(X)
+        for short, pm in ((True, 32768), (True, 0), (False, 0)):
+            e.set_short_prune(short)
+            e.set_prune_min(pm)
+            lab, info = e.assign(C)
+            _say("shape", d, k, prec, short, pm, "refined", info["refined_two"],
+                 info["refined_full"])
+            assert np.array_equal(lab, want), (d, k, prec, short, pm)
