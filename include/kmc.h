@@ -231,7 +231,9 @@ kmc_status kmc_autotune(kmc_handle* h, kmc_tuning* out);
 kmc_status kmc_set_assign_variant(kmc_handle* h, int variant);
 
 /* Points whose nearest MFMA candidates all fall inside the margin band are
- * refined against every centroid in the reference's float arithmetic. From
+ * refined against every centroid in the reference's float arithmetic. Where
+ * kmc_set_short_prune's single-sweep form applies (its shapes, on by
+ * default) it takes every such queue; otherwise, from
  * min_points such points per assignment on (default 32768), that pass first
  * narrows each point's centroids with a bf16x3 MFMA sweep and evaluates only
  * the survivors; below it every centroid is evaluated. The pruned pass is
@@ -241,12 +243,13 @@ kmc_status kmc_set_assign_variant(kmc_handle* h, int variant);
  * knob of this library's refinement, with no counterpart in the reference. */
 kmc_status kmc_set_prune_min(kmc_handle* h, int32_t min_points);
 
-/* With K <= 512 at D <= 64, or K <= 256 at 64 < D <= 128, all-k queues
- * shorter than that threshold (every queue when it is 0) take a single-sweep
- * form of the same MFMA pruning, shaped for short queues and run in one
- * launch beside the two/three-candidate refinement (default on); 0 sends them
- * to the exact pass over every centroid. Labels are the same either way. A
- * tuning knob with no counterpart in the reference. */
+/* With K <= 512 at D <= 64, or K <= 256 at 64 < D <= 128, the all-k queue
+ * takes a single-sweep form of the same MFMA pruning, run in one launch beside
+ * the two/three-candidate refinement on a share of the grid sized by the two
+ * queues (default on); 0 returns these shapes to kmc_set_prune_min's choice
+ * between the two-sweep pruned pass and the exact pass over every centroid.
+ * Labels are the same either way. A tuning knob with no counterpart in the
+ * reference. */
 kmc_status kmc_set_short_prune(kmc_handle* h, int32_t enable);
 
 /* Release every device buffer (src/kmeans.cu:307-359). */

@@ -433,8 +433,12 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
   rp.alpha3 = h->alpha3;
   rp.beta3 = h->beta3;
   rp.gref3 = h->gref3;
-  rp.prune_min = h->prune_launch() ? h->prune_min : 0;
   rp.pq = h->short_prune ? 1 : 0;
+  // with the fused shape the single-sweep groups take all-k queues of any
+  // length; k_refine_prune serves the other shapes, or with the fused form off
+  rp.prune_min = (rp.pq && kmc::pq_shape(h->K, h->DP)) ? 0
+                 : h->prune_launch()                  ? h->prune_min
+                                                      : 0;
   KMC_CK(kmc::launch_refine(rp, h->stream));
   return KMC_OK;
 }

@@ -1917,8 +1917,9 @@ __device__ void pq_body(const RefineParams& P, unsigned char* rsm, unsigned bid,
 }
 
 // Both refinement passes in one launch (one problem, pq_shape): the first G
-// workgroups take the all-k groups (G: the group count, at most a quarter of
-// the grid; none when the all-k queue goes to k_refine_prune), the others
+// workgroups take the all-k groups (G: the group count up to a quarter of
+// the grid, for a longer queue the share of the grid its estimated work
+// asks for; none when the all-k queue goes to k_refine_prune), the others
 // the two/three-candidate batches, so that the
 // all-k chain runs beside the batches instead of after them. The grid is the
 // workgroups resident at once, two per CU for the two/three-candidate LDS;
@@ -1931,9 +1932,21 @@ void k_refine_fused(RefineParams P) {
   unsigned* const pre_s = reinterpret_cast<unsigned*>(rsm);
   unsigned ta = 0;
   const unsigned t2 = region_prefix2(P.q2cnt, P.qacnt, P.nregions, pre_s, &ta);
-  unsigned G = (ta + 31) / 32;
-  if (ta == 0 || (prune_active(P) && ta >= (unsigned)P.prune_min)) G = 0;
-  G = min(G, gridDim.x / 4);
+  const unsigned ngA = (ta + 31) / 32;
+  unsigned G = 0;
+  if (ta > 0 && !(prune_active(P) && ta >= (unsigned)P.prune_min)) {
+    constexpr unsigned EPW2 = 256 / ((DP % 32 == 0 ? 32 : 16) / 4);
+    const unsigned nb2 = (t2 + EPW2 - 1) / EPW2;  // two/three-candidate wave batches
+    G = min(ngA, gridDim.x / 4);
+    if (ngA > G) {
+      // a long all-k queue: the grid is split by estimated work, an all-k
+      // group costing about five wave batches (the measured chain of one
+      // group against a batch's share of a workgroup)
+      const unsigned long long wA = 5ull * ngA;
+      G = max(G, (unsigned)((unsigned long long)gridDim.x * wA / (wA + nb2)));
+      G = min(G, nb2 > 0 ? gridDim.x - 1 : gridDim.x);
+    }
+  }
   if (blockIdx.x < G) {
     __syncthreads();  // every thread has read the two/three-candidate total
     unsigned dummy;

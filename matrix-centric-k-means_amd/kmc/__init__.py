@@ -285,11 +285,10 @@ class Engine:
         self._check(self._lib.kmc_set_prune_min(self._h, int(n)), "set_prune_min")
 
     def set_short_prune(self, enable):
-        """All-k queues below the prune_min threshold (all of them when it is
-        0) take the single-sweep MFMA-pruned pass, in one launch with the
-        two/three-candidate refinement (default on; K <= 512 at D <= 64,
-        K <= 256 at D <= 128);
-        off: every centroid is evaluated exactly for them."""
+        """The all-k queue takes the single-sweep MFMA-pruned groups, in one
+        launch with the two/three-candidate refinement (default on; K <= 512
+        at D <= 64, K <= 256 at D <= 128); off: set_prune_min chooses between
+        the two-sweep pruned pass and the exact pass."""
         self._check(self._lib.kmc_set_short_prune(self._h, int(bool(enable))),
                     "set_short_prune")
 

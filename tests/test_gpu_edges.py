@@ -17,7 +17,7 @@ def _check(kmc_mod, oracle, X, C0, prec, tol=1e-4, maxiter=50, update="ordered",
     want_conv, want_lab, want_C, _, _ = oracle.lloyd(X, C0, tol, maxiter)
     with kmc_mod.Engine(X.shape[0], X.shape[1], C0.shape[0], tol=tol, precision=prec,
                         update=update) as e:
-        if prune == 0:
+        if prune in (0, 1):
             e.set_short_prune(False)
         if prune is not None:
             e.set_prune_min(0 if prune == "short" else prune)

@@ -200,9 +200,10 @@ def test_pruned_all_k_pass_gives_the_same_labels(kmc_mod, oracle, c2_data, prec)
     want, _ = _reference_labels(oracle, Xs, C)
     with kmc_mod.Engine(Xs.shape[0], 128, 256, precision=prec) as e:
         e.set_points(Xs)
-        # always pruned, the default threshold, the short-queue form for every
-        # queue, never pruned
-        for short, pm in ((True, 1), (True, 32768), (True, 0), (False, 0)):
+        # the single-sweep groups (the default at this shape), then without
+        # them: the two-sweep pruned pass always, from the default threshold,
+        # never
+        for short, pm in ((True, 32768), (False, 1), (False, 32768), (False, 0)):
             e.set_short_prune(short)
             e.set_prune_min(pm)
             lab, info = e.assign(C)
@@ -334,9 +335,9 @@ def test_short_queue_pruned_groups_over_shapes(kmc_mod, oracle, prec, d, k):
     """The single-sweep MFMA-pruned all-k groups, run beside the
     two/three-candidate refinement in one launch, for each count of
     32-centroid blocks per wave (K up to 128, 256 and 512) and padded widths
-    16 to 128: with the default threshold, with every queue (prune_min 0: a
-    long queue has more groups than its quarter of the grid, so workgroups
-    take several), and against the exact pass over every centroid. The
+    16 to 128, on a long queue (every point: the grid is split by estimated
+    work and workgroups take several groups), against the two-sweep pruned
+    pass and the exact pass over every centroid. The
     centroids come in groups of four near-duplicates around a quarter of the
     blob centres, so a point has four or more candidates inside the band
     (its own blob's group, or for the other blobs' points many far
@@ -350,7 +351,7 @@ def test_short_queue_pruned_groups_over_shapes(kmc_mod, oracle, prec, d, k):
     want, _ = _reference_labels(oracle, X, C)
     with kmc_mod.Engine(n, d, k, precision=prec) as e:
         e.set_points(X)
-        for short, pm in ((True, 32768), (True, 0), (False, 0)):
+        for short, pm in ((True, 32768), (False, 1), (False, 0)):
             e.set_short_prune(short)
             e.set_prune_min(pm)
             lab, info = e.assign(C)
