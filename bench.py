@@ -147,10 +147,28 @@ def cpu_baseline(X, C0, n_sample=100_000, iters=2):
                       (n_sample, N, D, K, iters, secs)}
 
 
-def traffic_from_profiles(role, form, workload="c2"):
+# Launch shapes of the distance kernel's variants in the autotuner's order:
+# (waves per workgroup, point sets per wave, register prefetch, waves-per-SIMD
+# target), the kVariants table of csrc/kmc_lloyd.hip (a CPU test compares them).
+ASSIGN_VARIANTS = [(8, 1, True, 0), (8, 1, False, 0), (12, 1, False, 0), (16, 1, False, 0),
+                   (12, 1, False, 6), (4, 2, False, 0), (8, 2, False, 0), (8, 1, False, 8)]
+FORM_CODE = {"fp32": 0, "bf16x3": 1, "bf16": 2, "fp8": 3}
+
+
+def assign_kernel_name(dp, prec, variant):
+    """The distance kernel's instantiation for a form and variant, as the
+    profiler names it (without the return type and parameters)."""
+    nw, ms, pf, wpe = ASSIGN_VARIANTS[variant]
+    return "k_assign<%d, %d, %d, %d, %s, %d, false>" % (dp, FORM_CODE[prec], nw, ms,
+                                                        "true" if pf else "false", wpe)
+
+
+def traffic_from_profiles(role, form, workload="c2", kernel=None):
     """HBM bytes per launch of a kernel ("assign" per form, "update_tree",
     "update_ordered") from the committed rocprofv3 --pmc summary of the
-    workload (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factors)."""
+    workload (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factors); for
+    the distance kernel, the measured instantiation `kernel` when the summary
+    holds it (the autotuner's choice can differ between runs)."""
     if workload != "c2":
         return None
     p = os.path.join(ROOT, "profiles", "lloyd_n1m_d128_k256_hbm_traffic.json")
@@ -159,6 +177,8 @@ def traffic_from_profiles(role, form, workload="c2"):
             t = json.load(f)
     except (OSError, ValueError):
         return None
+    if role == "assign" and kernel and kernel in t.get("assign_variants", {}):
+        return t["assign_variants"][kernel]
     e = t.get(role, {})
     if role == "assign":
         e = e.get(form, {})
@@ -433,10 +453,11 @@ def main():
     achieved = flops / t_assign
     # the two largest kernels of an iteration, each against its own roof
     dp = 128 if D <= 128 else 256
-    roof_assign = {"bound": "mfma", "kernel": "k_assign<%d,%s>" % (dp, a.prec),
+    kname = assign_kernel_name(dp, a.prec, head["tuning"]["chosen"])
+    roof_assign = {"bound": "mfma", "kernel": kname,
                    "achieved": round(achieved / 1e12, 3), "peak": PEAK[a.prec] / 1e12,
                    "unit": "TFLOP/s", "frac": round(achieved / PEAK[a.prec], 4),
-                   "traffic": traffic_from_profiles("assign", a.prec, a.workload),
+                   "traffic": traffic_from_profiles("assign", a.prec, a.workload, kname),
                    "algorithmic_flops_per_launch": flops,
                    "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches}
     ubytes = N * D * 4 + N * 4 + K * D * 4  # X rows once, member ids, centroids

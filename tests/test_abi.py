@@ -124,3 +124,18 @@ def test_batch_rejects_fp8(kmc_mod):
     h = ctypes.c_void_p()
     assert lib.kmc_batch_create(arr, 1, ctypes.byref(bc), ctypes.byref(h)) == \
         kmc_mod.ERR_UNSUPPORTED
+
+
+def test_bench_variant_table_matches_the_kernel_source():
+    """bench.py names the distance kernel instantiation it timed from its copy
+    of the launch-variant table; the copy must match kVariants in the HIP
+    source (same shapes, same order as kmc_autotune reports them)."""
+    src = open(os.path.join(ROOT, "matrix-centric-k-means_amd", "csrc", "kmc_lloyd.hip")).read()
+    body = re.search(r"kVariants\[\]\s*=\s*\{(.*?)\};", src, flags=re.S).group(1)
+    shapes = [(int(a), int(b), c == "true", int(d))
+              for a, b, c, d in re.findall(r"\{\s*(\d+),\s*(\d+),\s*(true|false),\s*(\d+)\s*\}", body)]
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+    assert shapes == bench.ASSIGN_VARIANTS
+    assert bench.assign_kernel_name(128, "bf16", 1) == "k_assign<128, 2, 8, 1, false, 0, false>"
