@@ -329,7 +329,8 @@ def test_greedy_plusplus_matches_oracle(kmc_mod, oracle, n, d, k, seed, trials):
     assert np.array_equal(C, X[want])
 
 
-@pytest.mark.parametrize("d,k", [(16, 512), (48, 400), (64, 100), (100, 200), (128, 256)])
+@pytest.mark.parametrize("d,k", [(16, 512), (48, 400), (64, 100), (100, 200), (128, 256),
+                                 (40, 600), (128, 300), (200, 64)])
 @pytest.mark.parametrize("prec", ["bf16", "fp8"])
 def test_short_queue_pruned_groups_over_shapes(kmc_mod, oracle, prec, d, k):
     """The single-sweep MFMA-pruned all-k groups, run beside the
@@ -337,7 +338,9 @@ def test_short_queue_pruned_groups_over_shapes(kmc_mod, oracle, prec, d, k):
     32-centroid blocks per wave (K up to 128, 256 and 512) and padded widths
     16 to 128, on a long queue (every point: the grid is split by estimated
     work and workgroups take several groups), against the two-sweep pruned
-    pass and the exact pass over every centroid. The
+    pass and the exact pass over every centroid; shapes beyond them (K over
+    512, over 256 at D <= 128, any K at D = 256) take the other two paths
+    whatever the setting. The
     centroids come in groups of four near-duplicates around a quarter of the
     blob centres, so a point has four or more candidates inside the band
     (its own blob's group, or for the other blobs' points many far
@@ -351,7 +354,7 @@ def test_short_queue_pruned_groups_over_shapes(kmc_mod, oracle, prec, d, k):
     want, _ = _reference_labels(oracle, X, C)
     with kmc_mod.Engine(n, d, k, precision=prec) as e:
         e.set_points(X)
-        for short, pm in ((True, 32768), (False, 1), (False, 0)):
+        for short, pm in ((True, 32768), (True, 0), (False, 1), (False, 0)):
             e.set_short_prune(short)
             e.set_prune_min(pm)
             lab, info = e.assign(C)
