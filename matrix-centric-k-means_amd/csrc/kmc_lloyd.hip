@@ -794,35 +794,46 @@ void k_assign(AssignParams P) {
 // region r with pre[r] <= t < pre[r+1], so the work spreads evenly over the
 // grid however the band points cluster in memory.
 // ---------------------------------------------------------------------------
+// Exclusive prefix of one value per thread over a 256-thread workgroup
+// (shuffle scan inside each wave, the four wave totals through LDS: one
+// barrier); *total receives the sum. Every thread of the workgroup calls it.
+__device__ __forceinline__ unsigned block_excl_scan256(unsigned s, unsigned* total) {
+  __shared__ unsigned wsum_s[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned incl = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wsum_s[wave] = incl;
+  __syncthreads();
+  unsigned base = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) base += w < wave ? wsum_s[w] : 0u;
+  *total = wsum_s[0] + wsum_s[1] + wsum_s[2] + wsum_s[3];
+  return base + incl - s;
+}
+
 __device__ unsigned region_prefix(const unsigned* cnt, int nregions, unsigned* pre_s) {
-  __shared__ unsigned part_s[256];
   const int per = (nregions + 255) / 256;
   const int lo = min(nregions, (int)threadIdx.x * per), hi = min(nregions, lo + per);
   unsigned s = 0;
   for (int r = lo; r < hi; ++r) s += cnt[r];
-  part_s[threadIdx.x] = s;
-  __syncthreads();
-  // inclusive scan of 256 slice sums (Hillis-Steele)
-  for (int o = 1; o < 256; o <<= 1) {
-    const unsigned v = threadIdx.x >= (unsigned)o ? part_s[threadIdx.x - o] : 0u;
-    __syncthreads();
-    part_s[threadIdx.x] += v;
-    __syncthreads();
-  }
-  unsigned run = part_s[threadIdx.x] - s;
+  unsigned total;
+  unsigned run = block_excl_scan256(s, &total);
   for (int r = lo; r < hi; ++r) {
     pre_s[r] = run;
     run += cnt[r];
   }
-  if (threadIdx.x == 255) pre_s[nregions] = part_s[255];
+  if (threadIdx.x == 0) pre_s[nregions] = total;
   __syncthreads();
-  return pre_s[nregions];
+  return total;
 }
 
 // region_prefix of cnt, and in the same pass the total of cnt2 (*total2)
 __device__ unsigned region_prefix2(const unsigned* cnt, const unsigned* cnt2, int nregions,
                                    unsigned* pre_s, unsigned* total2) {
-  __shared__ unsigned part_s[256];
   __shared__ unsigned t2_s[4];
   const int per = (nregions + 255) / 256;
   const int lo = min(nregions, (int)threadIdx.x * per), hi = min(nregions, lo + per);
@@ -834,23 +845,16 @@ __device__ unsigned region_prefix2(const unsigned* cnt, const unsigned* cnt2, in
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o, 64);
   if ((threadIdx.x & 63) == 0) t2_s[threadIdx.x >> 6] = s2;
-  part_s[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {
-    const unsigned v = threadIdx.x >= (unsigned)o ? part_s[threadIdx.x - o] : 0u;
-    __syncthreads();
-    part_s[threadIdx.x] += v;
-    __syncthreads();
-  }
-  unsigned run = part_s[threadIdx.x] - s;
+  unsigned total;
+  unsigned run = block_excl_scan256(s, &total);
   for (int r = lo; r < hi; ++r) {
     pre_s[r] = run;
     run += cnt[r];
   }
-  if (threadIdx.x == 255) pre_s[nregions] = part_s[255];
+  if (threadIdx.x == 0) pre_s[nregions] = total;
   __syncthreads();
   *total2 = t2_s[0] + t2_s[1] + t2_s[2] + t2_s[3];
-  return pre_s[nregions];
+  return total;
 }
 
 // the pruned all-k kernel (k_refine_prune) is launched, and k_refine_all
