@@ -2934,28 +2934,46 @@ __device__ void stats_reduce(const StatsParams& P) {
   sq2[t] = r2;
   sqa[t] = ra;
   __syncthreads();
-  for (int o = kStatsThreads / 2; o > 0; o >>= 1) {
-    if (t < o) {
-      sd[t] += sd[t + o];
-      ss[t] += ss[t + o];
-      sm[t] = fmax(sm[t], sm[t + o]);
-      sc[t] &= sc[t + o];
-      sq2[t] += sq2[t + o];
-      sqa[t] += sqa[t + o];
-    }
-    __syncthreads();
+  if (t >= 64) return;
+  // The FP64 sums keep the order of the halving tree over the 512 values
+  // (value t + o added to value t for o = 256, 128, ..., 1): wave 0 forms
+  // its lanes' first three levels from LDS, shuffles the other six. The
+  // integer sums, the maximum and the conjunction are exact in any order.
+  static_assert(kStatsThreads == 512, "three levels through LDS");
+  auto tree8 = [&](const double* v) {
+    return ((v[t] + v[t + 256]) + (v[t + 128] + v[t + 384])) +
+           ((v[t + 64] + v[t + 320]) + (v[t + 192] + v[t + 448]));
+  };
+  double vd = tree8(sd), vs = tree8(ss), vm = sm[t];
+  int vc = sc[t];
+  unsigned long long v2 = sq2[t], va = sqa[t];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    vm = fmax(vm, sm[t + 64 * k]);
+    vc &= sc[t + 64 * k];
+    v2 += sq2[t + 64 * k];
+    va += sqa[t + 64 * k];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    vd += __shfl_down(vd, o, 64);
+    vs += __shfl_down(vs, o, 64);
+    vm = fmax(vm, __shfl_down(vm, o, 64));
+    vc &= __shfl_down(vc, o, 64);
+    v2 += __shfl_down(v2, o, 64);
+    va += __shfl_down(va, o, 64);
   }
   if (t == 0) {
     DevState* s = P.state;
     const unsigned long long it = s->iter + 1;
     s->iter = it;
-    s->inertia = sd[0];
-    s->max_shift2 = sm[0];
-    s->sum_shift2 = ss[0];
-    s->refined_two = sq2[0];
-    s->refined_all = sqa[0];
-    const double score = sd[0] - s->sum_x2;
-    if (P.mode == CONV_SHIFT && it >= 2 && sc[0]) {
+    s->inertia = vd;
+    s->max_shift2 = vm;
+    s->sum_shift2 = vs;
+    s->refined_two = v2;
+    s->refined_all = va;
+    const double score = vd - s->sum_x2;
+    if (P.mode == CONV_SHIFT && it >= 2 && vc) {
       // comparisons/CPU_Kmeans/src/kmeans.cpp:205-208: tested from iteration 2
       s->converged_at = it;
       s->done = 1;
@@ -2968,20 +2986,20 @@ __device__ void stats_reduce(const StatsParams& P) {
       s->done = 1;
     }
     s->last_score = score;
-    if (P.mode == CONV_SHIFT_SUM && ss[0] < (double)P.tol && !s->done) {
+    if (P.mode == CONV_SHIFT_SUM && vs < (double)P.tol && !s->done) {
       // experiments/raft-bench/src/kmeans.cu:168-212: sum over all centroid
       // entries of the squared change, tested every iteration
       s->converged_at = it;
       s->done = 1;
     }
     if (P.cost_ratio && it >= 2 && !s->done && s->prior_cost > 0.0 &&
-        sd[0] / s->prior_cost > 1.0 - (double)P.tol) {
+        vd / s->prior_cost > 1.0 - (double)P.tol) {
       // experiments/raft-bench/src/kmeans.cu:186-204: the clustering cost
       // stopped falling by more than a tol fraction
       s->converged_at = it;
       s->done = 1;
     }
-    s->prior_cost = sd[0];
+    s->prior_cost = vd;
     if (it >= s->maxiter) s->done = 1;
   }
 }
