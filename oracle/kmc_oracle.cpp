@@ -12,6 +12,9 @@
 //   3. orc_kk_*   : kernel k-means in the Popcorn formulation of the GPU
 //                   reference (src/kmeans.cu, src/kernels/distances.cu,
 //                   src/kernels/kernels.cuh), computed in FP64.
+//   3b. orc_gpu_lloyd: the GPU reference engine's loop rules on the linear
+//                   path (labels i % k -> means, empty cluster -> +inf,
+//                   float score test), over the float arithmetic of 1.
 //   4. orc_plusplus(_trials): k-means++ (D^2) seeding, plain or greedy, with the engine's fixed
 //                   summation order (the reference's own, src/kmeans.cu:
 //                   447-769, is compiled out and broken, so the order is the
@@ -34,6 +37,15 @@
 //                 (tests/unit_kernels.cu:365-395) restated independently in
 //                 the test, and the linear-kernel identity
 //                 d_K(x_i, mu_c) = ||x_i - mu_c||^2.
+//   orc_kk_lloyd -> a hand-worked 1-D case (labels and iteration count), and
+//                 explicit FP64 Lloyd on the kernel's finite feature map:
+//                 phi(x) = x for the linear kernel, phi(x) = (vec(x x^T),
+//                 sqrt(2) x, 1) for (x.y + 1)^2, run from the same labels to
+//                 the same stable labels in the same number of iterations.
+//   orc_gpu_lloyd -> hand-worked 1-D cases: the score sequence, the stop
+//                 iteration on either side of tol, an emptied cluster that is
+//                 never chosen again (the zero-vector rule would re-pick it),
+//                 and the no-test-at-maxiter rule.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -344,6 +356,75 @@ uint64_t orc_kk_lloyd(const double* K, size_t n, uint32_t k, uint64_t maxiter,
     if (!changed) break;
   }
   return it;
+}
+
+// ---------------------------------------------------------------------------
+// 3b. The reference GPU engine's Lloyd loop rules on the linear path
+// ---------------------------------------------------------------------------
+
+// The loop of Kmeans::run (src/kmeans.cu:772-1122) with the linear kernel,
+// from initial labels (init_centroids_rand gives labels i % k,
+// src/kmeans.cu:365-444). The engine keeps no explicit centroids: its
+// distance D[i,c] = ||mu_c||^2 - 2 x_i . mu_c of the label means mu_c comes
+// from the kernel matrix (src/kernels/distances.cu:895-1002) and ranks the
+// centroids exactly as ||x_i - mu_c||^2 does. This restatement uses the
+// reference CPU k-means' float arithmetic for that ranking (orc_cpu_distances:
+// direct form, first minimum) and its float means (orc_cpu_centroids), so it
+// shares labels with the rest of this oracle wherever the top-2 margin is
+// above float rounding. Rules restated:
+//   * a cluster empty under the current labels has c_tilde = 0 and distance
+//     +inf (src/kernels/distances.cu:588-598): it is never chosen again;
+//   * score_t = sum_i min_c D[i,c] = FP64 inertia of the assignment minus
+//     sum_i ||x_i||^2 (the kernel form omits ||x||^2), held in float
+//     (DATA_TYPE score, src/kmeans.cu:1053-1061);
+//   * after iteration t: stop (without a test) when t == maxiter; with
+//     check, stop when t > 1 and |score_t - score_{t-1}| < tol (float) and
+//     return t (src/kmeans.cu:1064-1073); otherwise return maxiter.
+// labels: in = initial labels, out = labels of the last assignment.
+// C (k x d, out): means of those labels (empty cluster: zero vector).
+// scores (optional, maxiter entries): float score of each iteration run.
+// iters_run (optional): iterations run.
+uint64_t orc_gpu_lloyd(const float* X, size_t n, uint32_t d, uint32_t k, float tol,
+                       uint64_t maxiter, int check, uint32_t* labels, float* C,
+                       float* scores, uint64_t* iters_run) {
+  std::vector<uint32_t> cnt(k, 0);
+  for (size_t i = 0; i < n; ++i) cnt[labels[i]]++;
+  std::vector<char> alive(k);
+  for (uint32_t c = 0; c < k; ++c) alive[c] = cnt[c] > 0;
+  orc_cpu_centroids(X, n, d, labels, cnt.data(), k, C);
+  std::vector<double> xn(n);
+  orc_f64_row_norms(X, n, d, xn.data());
+  double sum_x2 = 0.0;
+  for (size_t i = 0; i < n; ++i) sum_x2 += xn[i];
+  std::vector<float> D(n * (size_t)k);
+  uint64_t converged = maxiter, iter = 0;
+  float last_score = 0.0f;
+  while (iter++ < maxiter) {
+    orc_cpu_distances(X, n, d, C, k, D.data());
+    std::fill(cnt.begin(), cnt.end(), 0u);
+    for (size_t i = 0; i < n; ++i) {
+      float best = std::numeric_limits<float>::infinity();
+      uint32_t idx = 0;
+      for (uint32_t c = 0; c < k; ++c)
+        if (alive[c] && D[i * k + c] < best) { best = D[i * k + c]; idx = c; }
+      labels[i] = idx;
+      cnt[idx]++;
+    }
+    const double inert = orc_f64_inertia(X, n, d, C, labels);
+    const float score = (float)(inert - sum_x2);
+    if (scores) scores[iter - 1] = score;
+    for (uint32_t c = 0; c < k; ++c)
+      if (cnt[c] == 0) alive[c] = 0;
+    orc_cpu_centroids(X, n, d, labels, cnt.data(), k, C);
+    if (iter == maxiter) break;
+    if (check && iter > 1 && std::fabs(score - last_score) < tol) {
+      converged = iter;
+      break;
+    }
+    last_score = score;
+  }
+  if (iters_run) *iters_run = iter > maxiter ? maxiter : iter;
+  return converged;
 }
 
 

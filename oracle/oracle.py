@@ -56,6 +56,9 @@ class Oracle:
         L.orc_kk_distances.argtypes = [_P, _sz, _P, _u32, ctypes.c_int, _P]
         L.orc_kk_lloyd.restype = _u64
         L.orc_kk_lloyd.argtypes = [_P, _sz, _u32, _u64, _P]
+        L.orc_gpu_lloyd.restype = _u64
+        L.orc_gpu_lloyd.argtypes = [_P, _sz, _u32, _u32, ctypes.c_float, _u64, ctypes.c_int,
+                                    _P, _P, _P, _P]
         self.L = L
 
     # -- reference CPU k-means restatement (float) --
@@ -105,6 +108,21 @@ class Oracle:
         conv = self.L.orc_cpu_lloyd(_ptr(X), n, d, k, ctypes.c_float(tol), maxiter, _ptr(C),
                                     _ptr(lab), _ptr(cnt), _ptr(ran))
         return int(conv), lab, C, cnt, int(ran[0])
+
+    def gpu_lloyd(self, X, labels0, k, tol, maxiter, check=True):
+        """The reference GPU engine's loop rules (src/kmeans.cu:772-1122) on the
+        linear path from initial labels: returns (converged, labels, C, scores,
+        iterations run); scores are the float per-iteration scores."""
+        X = np.ascontiguousarray(X, np.float32)
+        n, d = X.shape
+        lab = np.array(labels0, dtype=np.uint32, copy=True)
+        C = np.empty((k, d), np.float32)
+        sc = np.zeros(max(1, int(maxiter)), np.float32)
+        ran = np.zeros(1, np.uint64)
+        conv = self.L.orc_gpu_lloyd(_ptr(X), n, d, k, ctypes.c_float(tol), maxiter,
+                                    int(bool(check)), _ptr(lab), _ptr(C), _ptr(sc), _ptr(ran))
+        r = int(ran[0])
+        return int(conv), lab, C, sc[:r].copy(), r
 
     # -- FP64 --
     def f64_distances(self, X, C):

@@ -8,7 +8,10 @@ checker of the HIP path.
     (files/GPU_Computing_Report.pdf p.6);
   * FP64 parts against closed forms, brute force and Lloyd monotonicity;
   * kernel k-means against the reference unit test's O(n^3) host formula
-    (tests/unit_kernels.cu:365-395) and the linear-kernel identity.
+    (tests/unit_kernels.cu:365-395) and the linear-kernel identity; its Lloyd
+    loop against a hand-worked case and explicit Lloyd on finite feature maps;
+  * the reference GPU engine's loop rules (orc_gpu_lloyd) against hand-worked
+    1-D cases and run invariants.
 """
 import numpy as np
 import pytest
@@ -301,3 +304,143 @@ def test_explicit_lloyd_known_answers(oracle):
     assert np.array_equal(lab, np.zeros(3, np.uint32))
     assert np.array_equal(C, np.array([[1.0], [100.0]], np.float32))
     assert t == 1  # nothing moved at iteration 1
+
+
+# ---- the reference GPU engine's loop rules (orc_gpu_lloyd) -----------------
+# Hand-worked 1-D cases. Scores are sum_i min_c ||x_i - mu_c||^2 - sum_i x_i^2.
+
+def _x(vals):
+    return np.array(vals, np.float32).reshape(-1, 1)
+
+
+def test_gpu_rules_known_answers_two_clusters(oracle):
+    # X = 0 1 10 11 20 21, labels i % 2 -> mu = (10, 11).
+    # it 1: labels 0 0 0 1 1 1, sum min d = 100+81+0+0+81+100 = 362,
+    #       score = 362 - 1063 = -701; mu = (11/3, 52/3)
+    # it 2: labels unchanged, sum min d = 364/3, score = -2825/3
+    # it 3: same score -> |delta| = 0
+    X = _x([0, 1, 10, 11, 20, 21])
+    lab0 = np.arange(6) % 2
+    conv, lab, C, sc, ran = oracle.gpu_lloyd(X, lab0, 2, 1e-4, 10)
+    assert lab.tolist() == [0, 0, 0, 1, 1, 1]
+    assert (conv, ran) == (3, 3)
+    assert sc.tolist() == [np.float32(-701.0), np.float32(-2825.0 / 3.0),
+                           np.float32(-2825.0 / 3.0)]
+    np.testing.assert_allclose(C[:, 0], [11 / 3, 52 / 3], rtol=1e-6)
+    # |score_2 - score_1| = 240.67: tol 300 stops at 2, tol 240 does not
+    assert oracle.gpu_lloyd(X, lab0, 2, 300.0, 10)[0] == 2
+    assert oracle.gpu_lloyd(X, lab0, 2, 240.0, 10)[0] == 3
+    # no test without check, and none at the last iteration
+    conv, _, _, _, ran = oracle.gpu_lloyd(X, lab0, 2, 1e-4, 10, check=False)
+    assert (conv, ran) == (10, 10)
+    assert oracle.gpu_lloyd(X, lab0, 2, 300.0, 2)[:1] == (2,)
+    conv, _, _, _, ran = oracle.gpu_lloyd(X, lab0, 2, 300.0, 1)
+    assert (conv, ran) == (1, 1)
+
+
+def test_gpu_rules_empty_cluster_is_never_chosen_again(oracle):
+    # X = 0 1 2 6 7, k = 3, labels i % 3 -> mu = (3, 4, 2).
+    # it 1: labels 2 2 2 1 1 (cluster 0 empties), sum min d = 4+1+0+4+9 = 18,
+    #       score = 18 - 90 = -72; mu_1 = 6.5, mu_2 = 1
+    # it 2: cluster 0 has distance +inf, so x = 0 stays with cluster 2:
+    #       sum min d = 1+0+1+0.25+0.25 = 2.5, score = -87.5
+    # The reference CPU rule (empty -> zero vector, still selectable) would
+    # move x = 0 to cluster 0 in iteration 2.
+    X = _x([0, 1, 2, 6, 7])
+    lab0 = np.arange(5) % 3
+    conv, lab, C, sc, ran = oracle.gpu_lloyd(X, lab0, 3, 1e-4, 20)
+    assert lab.tolist() == [2, 2, 2, 1, 1]
+    assert (conv, ran) == (3, 3)
+    assert sc.tolist() == [-72.0, -87.5, -87.5]
+    assert C[:, 0].tolist() == [0.0, 6.5, 1.0]
+    assert oracle.gpu_lloyd(X, lab0, 3, 20.0, 20)[0] == 2
+    # the CPU rule from the same means does re-pick the zero vector
+    C1 = np.array([[0.0], [6.5], [1.0]], np.float32)
+    lab_cpu, _ = oracle.argmin(oracle.distances(X, C1))
+    assert lab_cpu.tolist() == [0, 2, 2, 1, 1]
+
+
+def test_gpu_rules_invariants_on_blobs(oracle):
+    """Invariants of a long run from labels i % k (check off): the scores never
+    rise (Lloyd monotonicity; dropping an empty cluster cannot raise the
+    cost), the last score is the FP64 inertia of the last labels against the
+    means they were assigned to, minus sum ||x||^2, and the last labels are a
+    fixed point: assigning against the final means of the clusters still in
+    use (FP64, first minimum) reproduces them."""
+    rng = np.random.default_rng(3)
+    cen = rng.uniform(-20, 20, (5, 3))
+    X = (cen[rng.integers(0, 5, 400)] + rng.normal(0, 1.0, (400, 3))).astype(np.float32)
+    lab0 = np.arange(400) % 5
+    conv, lab, C, sc, ran = oracle.gpu_lloyd(X, lab0, 5, 0.0, 40, check=False)
+    assert ran == 40 and conv == 40
+    s64 = sc.astype(np.float64)
+    assert np.all(np.diff(s64) <= 1e-6 * np.abs(s64[:-1]))
+    live = np.bincount(lab.astype(np.int64), minlength=5) > 0
+    D = oracle.f64_distances(X, C)
+    D[:, ~live] = np.inf
+    assert np.array_equal(D.argmin(1), lab)
+    sx = float((X.astype(np.float64) ** 2).sum())
+    # the fixed point: the last assignment was made against these same means
+    assert sc[-1] == np.float32(oracle.f64_inertia(X, C, lab) - sx)
+
+
+# ---- kernel k-means Lloyd (orc_kk_lloyd) ------------------------------------
+
+def test_kk_lloyd_known_answer(oracle):
+    # linear kernel, X = 0 1 10 11 20 21, labels i % 2: iteration 1 moves to
+    # 0 0 0 1 1 1, iteration 2 changes nothing -> 2 iterations
+    X = _x([0, 1, 10, 11, 20, 21])
+    K = oracle.kernel_matrix(X, 0)
+    it, lab = oracle.kk_lloyd(K, np.arange(6) % 2, 2, 50)
+    assert (it, lab.tolist()) == (2, [0, 0, 0, 1, 1, 1])
+    it, lab = oracle.kk_lloyd(K, np.arange(6) % 2, 2, 1)
+    assert (it, lab.tolist()) == (1, [0, 0, 0, 1, 1, 1])
+
+
+def _explicit_lloyd_to_stable(F, lab0, k, maxiter):
+    """FP64 Lloyd on explicit features F from labels: centroids are label
+    means, first-minimum assignment over non-empty clusters, stop when no
+    label changes. Returns (iterations, labels)."""
+    lab = np.array(lab0, np.int64)
+    it = 0
+    while it < maxiter:
+        it += 1
+        mu = np.zeros((k, F.shape[1]))
+        cnt = np.bincount(lab, minlength=k)
+        np.add.at(mu, lab, F)
+        live = cnt > 0
+        mu[live] /= cnt[live][:, None]
+        D = ((F[:, None, :] - mu[None, :, :]) ** 2).sum(2)
+        D[:, ~live] = np.inf
+        new = D.argmin(1)
+        changed = np.any(new != lab)
+        lab = new
+        if not changed:
+            break
+    return it, lab
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_kk_lloyd_equals_explicit_lloyd_on_feature_map(oracle, kind):
+    """Kernel k-means with a kernel whose feature map is finite is explicit
+    Lloyd on the mapped points: linear phi(x) = x; polynomial (x.y + 1)^2
+    phi(x) = (vec(x x^T), sqrt(2) x, 1). Same labels, same iteration count,
+    on blob data whose FP64 top-2 margins are far above rounding."""
+    rng = np.random.default_rng(11 + kind)
+    n, d, k = 240, 3, 4
+    cen = rng.uniform(-3, 3, (k, d))
+    X = (cen[rng.integers(0, k, n)] + rng.normal(0, 0.6, (n, d))).astype(np.float32)
+    X64 = X.astype(np.float64)
+    if kind == 0:
+        F = X64
+    else:
+        F = np.concatenate([(X64[:, :, None] * X64[:, None, :]).reshape(n, -1),
+                            np.sqrt(2.0) * X64, np.ones((n, 1))], axis=1)
+    lab0 = rng.integers(0, k, n)
+    it_e, lab_e = _explicit_lloyd_to_stable(F, lab0, k, 100)
+    K = oracle.kernel_matrix(X, kind)
+    np.testing.assert_allclose(K, F @ F.T, rtol=1e-12, atol=1e-9)
+    it_k, lab_k = oracle.kk_lloyd(K, lab0, k, 100)
+    assert it_e < 100
+    assert it_k == it_e
+    assert np.array_equal(lab_k.astype(np.int64), lab_e)
