@@ -212,6 +212,36 @@ kmc_status kmc_assign(kmc_handle* h, const float* C, int32_t* labels,
 kmc_status kmc_update(kmc_handle* h, const int32_t* labels, const float* C_old,
                       float* C_new, uint32_t* counts, kmc_result* res);
 
+/* Device times of the handle's set-up steps, ms (HIP events): the H2D copy of
+ * the points (0 for kmc_set_points_device), the derived point operands
+ * (norms, bf16/fp8 planes; the reference's B = -2 X X^T build, "b-mult"),
+ * and the last initial-centroid step (kmc_set_centroids, kmc_set_labels or
+ * k-means++). Replace the memcpy, b-mult and init_centroids timers of
+ * src/kmeans.cu:117-136, 148-205, 267-297. */
+typedef struct {
+  double memcpy_ms;
+  double operands_ms;
+  double init_ms;
+} kmc_setup_times;
+
+kmc_status kmc_get_setup_times(const kmc_handle* h, kmc_setup_times* out);
+
+/* Per-iteration stage times, ms, of the last kmc_run with
+ * KMC_RUN_TIME_KERNELS (its first 4096 iterations): the distance stage (fused
+ * distance GEMM + top-4), the argmin stage (exact refinement of the margin
+ * band) and the centroid stage (sort/update/finish). Replace the
+ * compute_distances, clusters_argmin_shfl and compute_centroids timers of
+ * src/kmeans.cu:837-904, 938-958, 968-1044. Writes min(cap, count) entries;
+ * *count (may be NULL) receives the number of timed iterations. */
+typedef struct {
+  float distances_ms;
+  float argmin_ms;
+  float centroids_ms;
+} kmc_stage_times;
+
+kmc_status kmc_get_stage_times(kmc_handle* h, kmc_stage_times* out, uint64_t cap,
+                               uint64_t* count);
+
 /* Launch-variant autotuner for the fused distance kernel (no counterpart in
  * the reference, whose stages are fixed library calls): times every variant
  * (waves per workgroup, register prefetch) on the handle's points and
@@ -331,6 +361,16 @@ kmc_status kmc_kk_set_labels(kmc_kk_handle* h, const int32_t* labels0);
 kmc_status kmc_kk_run(kmc_kk_handle* h, uint64_t maxiter, uint32_t flags, kmc_kk_result* res);
 
 kmc_status kmc_kk_get_labels(kmc_kk_handle* h, int32_t* labels);
+
+/* Set-up times of a kernel k-means handle (memcpy: the H2D copy of X;
+ * operands: the kernel-matrix build, the reference's b-mult; init: the last
+ * kmc_kk_set_labels), and per-iteration stage times of the last kmc_kk_run
+ * with KMC_RUN_TIME_KERNELS (distances: the K product; argmin: assignment and
+ * statistics; centroids: the rebuild of V, i.e. the sort by label). Same
+ * reference timers as kmc_get_setup_times / kmc_get_stage_times. */
+kmc_status kmc_kk_get_setup_times(const kmc_kk_handle* h, kmc_setup_times* out);
+kmc_status kmc_kk_get_stage_times(kmc_kk_handle* h, kmc_stage_times* out, uint64_t cap,
+                                  uint64_t* count);
 
 /* Stage hook: kernel-space distances for host labels, host n x k, as the
  * reference materialises them (no K_ii term): replaces

@@ -66,6 +66,12 @@ class Kmeans {
     // per-centroid shift test, empty cluster -> zero vector).
     bool reference_gpu = true;
     Precision precision = Precision::fp32;
+    // Print the reference's "[PERFORMANCE] <stage> time: <s>" lines: memcpy,
+    // b-mult and init_centroids from the constructor, compute_distances,
+    // clusters_argmin_shfl and compute_centroids per iteration from run()
+    // (src/kmeans.cu:136,205,297,904,958,1044, compiled in there by the
+    // PERFORMANCES_* macros), timed with HIP events on the device.
+    bool report_stage_times = false;
   };
 
   Kmeans(const size_t n, const uint32_t d, const uint32_t k, const float tol, const int* seed,
@@ -90,6 +96,8 @@ class Kmeans {
 
  private:
   void init(const int* seed);
+  void print_setup_times() const;
+  void print_stage_times();
   const size_t n;
   const uint32_t d, k;
   const float tol;

@@ -12,6 +12,9 @@
 //   --cpu-semantics                   reference CPU k-means semantics (seeded
 //                                     distinct rows, shift test) instead of the
 //                                     reference GPU engine's
+//   --no-stage-times                  omit the [PERFORMANCE] stage-time lines
+//                                     (printed by default, as the reference's
+//                                     harness expects them)
 #include <cfloat>
 #include <chrono>
 #include <cstdio>
@@ -46,7 +49,8 @@ const Flag kFlags[] = {
     {"i", "in-file", true},      {"r", "runs", true},         {"s", "seed", true},
     {"t", "tolerance", true},    {"c", "check", true},        {"b", "init", true},
     {"f", "kernel", true},       {"l", "level", true},        {"p", "precision", true},
-    {nullptr, "label-col", true}, {nullptr, "cpu-semantics", false}};
+    {nullptr, "label-col", true}, {nullptr, "cpu-semantics", false},
+    {nullptr, "no-stage-times", false}};
 
 void usage() {
   std::cout
@@ -68,7 +72,8 @@ void usage() {
          "  -l, --level arg          Optimization level\n"
          "  -p, --precision arg      fp32 | bf16x3 | bf16 | fp8 (default: fp32)\n"
          "      --label-col arg      first | last | none (default: first)\n"
-         "      --cpu-semantics      reference CPU k-means semantics\n";
+         "      --cpu-semantics      reference CPU k-means semantics\n"
+         "      --no-stage-times     omit the [PERFORMANCE] stage-time lines\n";
 }
 
 [[noreturn]] void missing(const char* arg) {
@@ -278,6 +283,8 @@ int main(int argc, char** argv) {
   }
   Kmeans::Options opts;
   opts.reference_gpu = !args.count("cpu-semantics");
+  // the [PERFORMANCE] stage lines that experiments/driver.py:108-124 parses
+  opts.report_stage_times = !args.count("no-stage-times");
   if (prec == "fp32") opts.precision = Kmeans::Precision::fp32;
   else if (prec == "bf16x3") opts.precision = Kmeans::Precision::bf16x3;
   else if (prec == "fp8") opts.precision = Kmeans::Precision::fp8;

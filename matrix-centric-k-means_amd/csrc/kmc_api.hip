@@ -100,7 +100,11 @@ struct kmc_handle {
   int idx_bits = 0;
 
   bool have_points = false, have_centroids = false;
+  // per timed iteration kEvPerIter events: distance stage start and end,
+  // refinement end, update kernel start and end, iteration end
   std::vector<hipEvent_t> events;
+  uint64_t timed_iters = 0;  // iterations of the last run with stage events
+  kmc_setup_times setup{};
 
   // captured iteration chunks (KMC_RUN_GRAPH), reused by later runs
   struct GraphEntry {
@@ -189,6 +193,8 @@ void band_constants(int prec, int D, int idx_bits, float* alpha, float* beta, fl
 }  // namespace kmc
 
 namespace {
+
+constexpr int kEvPerIter = 6;
 
 const char* kStatusText[] = {"ok",
                              "invalid or missing argument",
@@ -369,8 +375,11 @@ kmc::PrepParams prep_params(kmc_handle* h) {
 }
 
 // prep: form the distance operands from C first (the first iteration of a
-// run; later iterations get them from the previous iteration's k_finish)
-kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep = true) {
+// run; later iterations get them from the previous iteration's k_finish).
+// e0/e1 bracket the distance stage, r1 marks the end of the refinement.
+kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep = true,
+                          hipEvent_t r1 = nullptr) {
+  if (e0) KMC_CK(record_event(h->stream, e0));
   if (prep) KMC_CK(kmc::launch_prep(prep_params(h), h->stream));
 
   kmc::AssignParams ap{};
@@ -404,7 +413,6 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
   ap.qacnt = h->qacnt;
   ap.qstride = h->qstride;
   ap.done = done_ptr(h);
-  if (e0) KMC_CK(record_event(h->stream, e0));
   KMC_CK(kmc::launch_assign(h->DP, h->prec, h->variant, ap, h->assign_grid, h->assign_lds,
                             h->stream));
   if (e1) KMC_CK(record_event(h->stream, e1));
@@ -440,12 +448,14 @@ kmc_status enqueue_assign(kmc_handle* h, hipEvent_t e0, hipEvent_t e1, bool prep
                  : h->prune_launch()                  ? h->prune_min
                                                       : 0;
   KMC_CK(kmc::launch_refine(rp, h->stream));
+  if (r1) KMC_CK(record_event(h->stream, r1));
   return KMC_OK;
 }
 
-// u0/u1: events around the centroid-update kernel (timed iterations)
+// u0/u1: events around the centroid-update kernel (timed iterations); f1:
+// after the finishing kernels (end of the iteration)
 kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nullptr,
-                          hipEvent_t u1 = nullptr) {
+                          hipEvent_t u1 = nullptr, hipEvent_t f1 = nullptr) {
   kmc::SortParams sp{};
   sp.labels = h->labels;
   sp.bcount = h->bcount;
@@ -547,6 +557,7 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nu
   st.incremental = h->Stot ? 1 : 0;
   st.n = (long long)h->n;
   KMC_CK(kmc::launch_finish(fp, h->stream));
+  if (f1) KMC_CK(record_event(h->stream, f1));
   return KMC_OK;
 }
 
@@ -557,10 +568,12 @@ kmc_status enqueue_chunk(kmc_handle* h, uint64_t first, uint64_t m, int mode, in
   for (uint64_t it = 0; it < m; ++it) {
     const uint64_t gi = first + it;
     const bool tm = gi < ntimed;
-    hipEvent_t* ev = tm ? &h->events[4 * gi] : nullptr;
-    kmc_status st = enqueue_assign(h, tm ? ev[0] : nullptr, tm ? ev[1] : nullptr, it == 0);
+    hipEvent_t* ev = tm ? &h->events[kEvPerIter * gi] : nullptr;
+    kmc_status st = enqueue_assign(h, tm ? ev[0] : nullptr, tm ? ev[1] : nullptr, it == 0,
+                                   tm ? ev[2] : nullptr);
     if (st == KMC_OK)
-      st = enqueue_update(h, mode, rules, tm ? ev[2] : nullptr, tm ? ev[3] : nullptr);
+      st = enqueue_update(h, mode, rules, tm ? ev[3] : nullptr, tm ? ev[4] : nullptr,
+                          tm ? ev[5] : nullptr);
     if (st != KMC_OK) return st;
   }
   return KMC_OK;
@@ -623,7 +636,31 @@ void fill_result(const kmc_handle* h, const DevState& s, kmc_result* r) {
   (void)h;
 }
 
+// Milliseconds between two events recorded on the handle's stream around a
+// synchronous set-up step (0 when timing fails: timing never fails a call).
+struct StepTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s;
+  explicit StepTimer(hipStream_t st) : s(st) {
+    if (hipEventCreate(&a) != hipSuccess) a = nullptr;
+    if (hipEventCreate(&b) != hipSuccess) b = nullptr;
+    if (a) (void)hipEventRecord(a, s);
+  }
+  double stop() {
+    float ms = 0.f;
+    if (a && b && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+        hipEventElapsedTime(&ms, a, b) == hipSuccess)
+      return ms;
+    return 0.0;
+  }
+  ~StepTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
 kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
+  StepTimer tm(h->stream);
   const bool keep64 = h->update_mode == kmc::UPDATE_FIXED;
   double* xn64 = keep64 ? h->xn64 : nullptr;
   if (!keep64) KMC_CK(dalloc(&xn64, h->n + 256));
@@ -678,6 +715,7 @@ kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
   if (part) (void)hipFree(part);
   if (!keep64) (void)hipFree(xn64);
   KMC_CK(e);
+  h->setup.operands_ms = tm.stop();
   h->have_points = true;
   return KMC_OK;
 }
@@ -863,7 +901,9 @@ kmc_status kmc_set_points(kmc_handle* h, const float* X, size_t ld) {
   const size_t bytes = ((h->n - 1) * ld + h->D) * sizeof(float);
   float* tmp = nullptr;
   KMC_CK(hipMalloc(reinterpret_cast<void**>(&tmp), bytes));
+  StepTimer tm(h->stream);
   hipError_t e = hipMemcpyAsync(tmp, X, bytes, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) h->setup.memcpy_ms = tm.stop();
   kmc_status st = KMC_OK;
   if (e == hipSuccess) st = load_points(h, tmp, ld);
   else st = fail(h, KMC_ERR_HIP, std::string("H2D copy: ") + hipGetErrorString(e));
@@ -876,16 +916,19 @@ kmc_status kmc_set_points_device(kmc_handle* h, const float* dX, size_t ld) {
   if (!h || !dX || ld < (size_t)h->D)
     return fail(h, KMC_ERR_ARG, "kmc_set_points_device: bad argument");
   KMC_CK(hipSetDevice(h->dev));
+  h->setup.memcpy_ms = 0.0;
   return load_points(h, dX, ld);
 }
 
 kmc_status kmc_set_centroids(kmc_handle* h, const float* C0) {
   if (!h || !C0) return fail(h, KMC_ERR_ARG, "kmc_set_centroids: bad argument");
   KMC_CK(hipSetDevice(h->dev));
+  StepTimer tm(h->stream);
   KMC_CK(kmc::rows_to_device(h->C, (size_t)h->DP, C0, (size_t)h->D, (size_t)h->D,
                              (size_t)h->K, h->stream));
   KMC_CK(kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream));
   KMC_CK(hipStreamSynchronize(h->stream));
+  h->setup.init_ms = tm.stop();
   h->have_centroids = true;
   return KMC_OK;
 }
@@ -911,6 +954,7 @@ kmc_status kmc_init_plusplus_trials(kmc_handle* h, uint64_t seed, uint32_t trial
   if (trials == 0) trials = 2u + (uint32_t)std::ceil(std::log((double)h->K));
   if (trials > 16) return fail(h, KMC_ERR_ARG, "kmc_init_plusplus_trials: at most 16 trials");
   KMC_CK(hipSetDevice(h->dev));
+  StepTimer tm(h->stream);
   const size_t nu = 1 + (size_t)(h->K - 1) * trials;
   std::vector<double> uni(nu);
   uint64_t st = seed;
@@ -941,6 +985,7 @@ kmc_status kmc_init_plusplus_trials(kmc_handle* h, uint64_t seed, uint32_t trial
   if (cand) (void)hipFree(cand);
   if (tsum) (void)hipFree(tsum);
   KMC_CK(e);
+  h->setup.init_ms = tm.stop();
   h->have_centroids = true;
   return KMC_OK;
 }
@@ -952,6 +997,7 @@ kmc_status kmc_set_labels(kmc_handle* h, const int32_t* labels0) {
     if (labels0[i] < 0 || labels0[i] >= h->K)
       return fail(h, KMC_ERR_OUT_OF_BOUNDS, "kmc_set_labels: label out of range");
   KMC_CK(hipSetDevice(h->dev));
+  StepTimer tm(h->stream);
   KMC_CK(hipMemcpyAsync(h->labels, labels0, h->n * sizeof(int32_t), hipMemcpyHostToDevice,
                         h->stream));
   kmc_status st = reset_state(h, 1);
@@ -962,6 +1008,7 @@ kmc_status kmc_set_labels(kmc_handle* h, const int32_t* labels0) {
   st = enqueue_update(h, kmc::CONV_NONE, 1);
   if (st != KMC_OK) return st;
   KMC_CK(hipStreamSynchronize(h->stream));
+  h->setup.init_ms = tm.stop();
   h->have_centroids = true;
   return KMC_OK;
 }
@@ -997,7 +1044,7 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
   if (!exclude) KMC_CK(kmc::launch_fill_i32(h->alive, (long long)h->K, 1, h->stream));
   const uint64_t kMaxTimed = 4096;
   const uint64_t ntimed = timing ? std::min<uint64_t>(maxiter, kMaxTimed) : 0;
-  while (h->events.size() < 4 * ntimed) {
+  while (h->events.size() < kEvPerIter * ntimed) {
     hipEvent_t e;
     KMC_CK(hipEventCreate(&e));
     h->events.push_back(e);
@@ -1037,6 +1084,7 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
     if (e != hipSuccess) st = fail(h, KMC_ERR_HIP, std::string("run timing: ") + hipGetErrorString(e));
     if (st == KMC_OK) st = read_state(h, &s);
     if (st == KMC_OK) h->last_all_k = (long long)s.refined_all;
+    if (st == KMC_OK) h->timed_iters = std::min<uint64_t>(s.iter, ntimed);
     if (st == KMC_OK) {
       fill_result(h, s, res);
       if (res) {
@@ -1045,10 +1093,9 @@ kmc_status kmc_run(kmc_handle* h, uint64_t maxiter, uint32_t flags, kmc_result* 
         double acc = 0.0, accu = 0.0;
         for (uint64_t i = 0; i < ran; ++i) {
           float a = 0.f, u = 0.f;
-          if (hipEventElapsedTime(&a, h->events[4 * i], h->events[4 * i + 1]) == hipSuccess)
-            acc += a;
-          if (hipEventElapsedTime(&u, h->events[4 * i + 2], h->events[4 * i + 3]) == hipSuccess)
-            accu += u;
+          const hipEvent_t* ev = &h->events[kEvPerIter * i];
+          if (hipEventElapsedTime(&a, ev[0], ev[1]) == hipSuccess) acc += a;
+          if (hipEventElapsedTime(&u, ev[3], ev[4]) == hipSuccess) accu += u;
         }
         res->ms_assign = acc;
         res->assign_launches = ran;
@@ -1117,8 +1164,8 @@ kmc_status kmc_autotune(kmc_handle* h, kmc_tuning* out) {
     if (st != KMC_OK) break;
     // one untimed launch, then two timed ones
     st = enqueue_assign(h, nullptr, nullptr);
-    if (st == KMC_OK) st = enqueue_assign(h, e[0], e[1]);
-    if (st == KMC_OK) st = enqueue_assign(h, e[2], e[3]);
+    if (st == KMC_OK) st = enqueue_assign(h, e[0], e[1], false);
+    if (st == KMC_OK) st = enqueue_assign(h, e[2], e[3], false);
     if (st != KMC_OK) break;
     KMC_CK(hipEventSynchronize(e[3]));
     float a = 0.f, b = 0.f;
@@ -1178,6 +1225,31 @@ kmc_status kmc_assign(kmc_handle* h, const float* C, int32_t* labels, kmc_result
     for (unsigned v : q2) res->refined_two += v;
     for (unsigned v : qa) res->refined_full += v;
   }
+  return KMC_OK;
+}
+
+kmc_status kmc_get_setup_times(const kmc_handle* h, kmc_setup_times* out) {
+  if (!h || !out) return KMC_ERR_ARG;
+  *out = h->setup;
+  return KMC_OK;
+}
+
+kmc_status kmc_get_stage_times(kmc_handle* h, kmc_stage_times* out, uint64_t cap,
+                               uint64_t* count) {
+  if (!h || (!out && cap > 0)) return fail(h, KMC_ERR_ARG, "kmc_get_stage_times: bad argument");
+  KMC_CK(hipSetDevice(h->dev));
+  const uint64_t m = std::min<uint64_t>(cap, h->timed_iters);
+  for (uint64_t i = 0; i < m; ++i) {
+    const hipEvent_t* ev = &h->events[kEvPerIter * i];
+    float a = 0.f, r = 0.f, c = 0.f;
+    KMC_CK(hipEventElapsedTime(&a, ev[0], ev[1]));
+    KMC_CK(hipEventElapsedTime(&r, ev[1], ev[2]));
+    KMC_CK(hipEventElapsedTime(&c, ev[2], ev[5]));
+    out[i].distances_ms = a;
+    out[i].argmin_ms = r;
+    out[i].centroids_ms = c;
+  }
+  if (count) *count = h->timed_iters;
   return KMC_OK;
 }
 

@@ -22,7 +22,11 @@ struct kmc_kk_handle {
   double sum_diag = 0.0;
   bool have_points = false, have_labels = false;
   float ms_build = 0.f;
+  // per timed iteration 4 events: start, after the V rebuild (sort or
+  // one-hot), after the K product, end
   std::vector<hipEvent_t> events;
+  uint64_t timed_iters = 0;
+  kmc_setup_times setup{};
 };
 
 namespace {
@@ -185,15 +189,22 @@ kmc_status kmc_kk_set_points(kmc_kk_handle* h, const float* X, size_t ld) {
   if (!h || !X || ld < (size_t)h->P.d) return kfail(h, KMC_ERR_ARG, "kmc_kk_set_points: bad argument");
   kmc::KKParams& P = h->P;
   KK_CK(hipSetDevice(h->dev));
-  KK_CK(kmc::rows_to_device(h->dX, (size_t)P.d, X, ld, (size_t)P.d, (size_t)P.n, h->stream));
   hipEvent_t e0, e1;
   KK_CK(hipEventCreate(&e0));
   KK_CK(hipEventCreate(&e1));
+  KK_CK(hipEventRecord(e0, h->stream));
+  KK_CK(kmc::rows_to_device(h->dX, (size_t)P.d, X, ld, (size_t)P.d, (size_t)P.n, h->stream));
+  KK_CK(hipEventRecord(e1, h->stream));
+  KK_CK(hipEventSynchronize(e1));
+  float mc = 0.f;
+  KK_CK(hipEventElapsedTime(&mc, e0, e1));
+  h->setup.memcpy_ms = mc;
   KK_CK(hipEventRecord(e0, h->stream));
   KK_CK(kmc::kk_launch_setup(P, h->stream));
   KK_CK(hipEventRecord(e1, h->stream));
   KK_CK(hipEventSynchronize(e1));
   KK_CK(hipEventElapsedTime(&h->ms_build, e0, e1));
+  h->setup.operands_ms = h->ms_build;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::vector<float> diag((size_t)P.n);
@@ -211,9 +222,19 @@ kmc_status kmc_kk_set_labels(kmc_kk_handle* h, const int32_t* labels0) {
     if (labels0[i] < 0 || labels0[i] >= h->P.K)
       return kfail(h, KMC_ERR_OUT_OF_BOUNDS, "kmc_kk_set_labels: label out of range");
   KK_CK(hipSetDevice(h->dev));
+  hipEvent_t e0, e1;
+  KK_CK(hipEventCreate(&e0));
+  KK_CK(hipEventCreate(&e1));
+  KK_CK(hipEventRecord(e0, h->stream));
   KK_CK(hipMemcpyAsync(h->P.labels, labels0, (size_t)h->P.n * sizeof(int32_t),
                        hipMemcpyHostToDevice, h->stream));
-  KK_CK(hipStreamSynchronize(h->stream));
+  KK_CK(hipEventRecord(e1, h->stream));
+  KK_CK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  KK_CK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->setup.init_ms = ms;
   h->have_labels = true;
   return KMC_OK;
 }
@@ -232,7 +253,7 @@ kmc_status kmc_kk_run(kmc_kk_handle* h, uint64_t maxiter, uint32_t flags, kmc_kk
   if (st != KMC_OK) return st;
   const bool timing = (flags & KMC_RUN_TIME_KERNELS) != 0;
   const uint64_t ntimed = timing ? std::min<uint64_t>(maxiter, 4096) : 0;
-  while (h->events.size() < 2 * ntimed) {
+  while (h->events.size() < 4 * ntimed) {
     hipEvent_t e;
     KK_CK(hipEventCreate(&e));
     h->events.push_back(e);
@@ -247,8 +268,11 @@ kmc_status kmc_kk_run(kmc_kk_handle* h, uint64_t maxiter, uint32_t flags, kmc_kk
     const uint64_t m = check ? std::min<uint64_t>(chunk, maxiter - launched) : maxiter - launched;
     for (uint64_t it = 0; it < m; ++it) {
       const uint64_t gi = launched + it;
-      KK_CK(kmc::kk_launch_iter(P, gi < ntimed ? h->events[2 * gi] : nullptr,
-                                gi < ntimed ? h->events[2 * gi + 1] : nullptr, h->stream));
+      const bool tm = gi < ntimed;
+      if (tm) KK_CK(hipEventRecord(h->events[4 * gi], h->stream));
+      KK_CK(kmc::kk_launch_iter(P, tm ? h->events[4 * gi + 1] : nullptr,
+                                tm ? h->events[4 * gi + 2] : nullptr, h->stream));
+      if (tm) KK_CK(hipEventRecord(h->events[4 * gi + 3], h->stream));
     }
     launched += m;
     if (check) {
@@ -265,6 +289,7 @@ kmc_status kmc_kk_run(kmc_kk_handle* h, uint64_t maxiter, uint32_t flags, kmc_kk
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   KK_CK(hipMemcpy(&s, P.state, sizeof(s), hipMemcpyDeviceToHost));
+  h->timed_iters = std::min<uint64_t>(s.iter, ntimed);
   if (res) {
     res->iterations = s.iter;
     res->converged_at = s.converged_at;
@@ -277,11 +302,37 @@ kmc_status kmc_kk_run(kmc_kk_handle* h, uint64_t maxiter, uint32_t flags, kmc_kk
     double acc = 0.0;
     for (uint64_t i = 0; i < ran; ++i) {
       float a = 0.f;
-      if (hipEventElapsedTime(&a, h->events[2 * i], h->events[2 * i + 1]) == hipSuccess) acc += a;
+      if (hipEventElapsedTime(&a, h->events[4 * i + 1], h->events[4 * i + 2]) == hipSuccess)
+        acc += a;
     }
     res->ms_spmm = acc;
     res->spmm_launches = ran;
   }
+  return KMC_OK;
+}
+
+kmc_status kmc_kk_get_setup_times(const kmc_kk_handle* h, kmc_setup_times* out) {
+  if (!h || !out) return KMC_ERR_ARG;
+  *out = h->setup;
+  return KMC_OK;
+}
+
+kmc_status kmc_kk_get_stage_times(kmc_kk_handle* h, kmc_stage_times* out, uint64_t cap,
+                                  uint64_t* count) {
+  if (!h || (!out && cap > 0)) return kfail(h, KMC_ERR_ARG, "kmc_kk_get_stage_times: bad argument");
+  KK_CK(hipSetDevice(h->dev));
+  const uint64_t m = std::min<uint64_t>(cap, h->timed_iters);
+  for (uint64_t i = 0; i < m; ++i) {
+    const hipEvent_t* ev = &h->events[4 * i];
+    float c = 0.f, a = 0.f, r = 0.f;
+    KK_CK(hipEventElapsedTime(&c, ev[0], ev[1]));
+    KK_CK(hipEventElapsedTime(&a, ev[1], ev[2]));
+    KK_CK(hipEventElapsedTime(&r, ev[2], ev[3]));
+    out[i].distances_ms = a;
+    out[i].argmin_ms = r;
+    out[i].centroids_ms = c;
+  }
+  if (count) *count = h->timed_iters;
   return KMC_OK;
 }
 

@@ -1,6 +1,7 @@
 // kmeans.cpp -- the drop-in Kmeans class (include/kmeans.hpp) over the C ABI.
 #include "kmeans.hpp"
 
+#include <cstdio>
 #include <random>
 #include <string>
 #include <vector>
@@ -57,6 +58,40 @@ Kmeans::Kmeans(const size_t _n, const uint32_t _d, const uint32_t _k, const floa
     : n(_n), d(_d), k(_k), tol(_tol), points(_points), deviceProps(_deviceProps),
       initMethod(_initMethod), kernel(_kernel), level(_level), opts(_opts) {
   init(seed);
+  if (opts.report_stage_times) print_setup_times();
+}
+
+namespace {
+const char* kCyan = "\033[36m";
+const char* kReset = "\033[0m";
+void perf_line(const char* stage, double ms) {
+  std::printf("%s[PERFORMANCE]%s %s time: %.8f\n", kCyan, kReset, stage, ms / 1000.0);
+}
+}  // namespace
+
+void Kmeans::print_setup_times() const {
+  kmc_setup_times t{};
+  const kmc_status s = kh ? kmc_kk_get_setup_times(kh, &t) : kmc_get_setup_times(h, &t);
+  if (s != KMC_OK) return;
+  perf_line("memcpy", t.memcpy_ms);
+  perf_line("b-mult", t.operands_ms);
+  perf_line("init_centroids", t.init_ms);
+}
+
+void Kmeans::print_stage_times() {
+  uint64_t cnt = 0;
+  kmc_status s = kh ? kmc_kk_get_stage_times(kh, nullptr, 0, &cnt)
+                    : kmc_get_stage_times(h, nullptr, 0, &cnt);
+  if (s != KMC_OK || cnt == 0) return;
+  std::vector<kmc_stage_times> t(cnt);
+  s = kh ? kmc_kk_get_stage_times(kh, t.data(), cnt, &cnt)
+         : kmc_get_stage_times(h, t.data(), cnt, &cnt);
+  if (s != KMC_OK) return;
+  for (const kmc_stage_times& it : t) {
+    perf_line("compute_distances", it.distances_ms);
+    perf_line("clusters_argmin_shfl", it.argmin_ms);
+    perf_line("compute_centroids", it.centroids_ms);
+  }
 }
 
 void Kmeans::init(const int* seed) {
@@ -166,8 +201,10 @@ Kmeans::~Kmeans() {
 uint64_t Kmeans::run(uint64_t maxiter, bool check_converged) {
   if (kh) {
     kmc_kk_result res{};
-    kk_check(kh, kmc_kk_run(kh, maxiter, check_converged ? KMC_RUN_CHECK_CONVERGED : 0u, &res),
-             "kmc_kk_run");
+    uint32_t kflags = check_converged ? KMC_RUN_CHECK_CONVERGED : 0u;
+    if (opts.report_stage_times) kflags |= KMC_RUN_TIME_KERNELS;
+    kk_check(kh, kmc_kk_run(kh, maxiter, kflags, &res), "kmc_kk_run");
+    if (opts.report_stage_times) print_stage_times();
     score = (float)res.score;
     inertia = res.objective;
     iterations = res.iterations;
@@ -178,8 +215,10 @@ uint64_t Kmeans::run(uint64_t maxiter, bool check_converged) {
   }
   uint32_t flags = check_converged ? KMC_RUN_CHECK_CONVERGED : 0u;
   if (opts.reference_gpu) flags |= KMC_RUN_CONVERGE_SCORE | KMC_RUN_EXCLUDE_EMPTY;
+  if (opts.report_stage_times) flags |= KMC_RUN_TIME_KERNELS;
   kmc_result res{};
   check(h, kmc_run(h, maxiter, flags, &res), "kmc_run");
+  if (opts.report_stage_times) print_stage_times();
   score = (float)res.score;
   inertia = res.inertia;
   iterations = res.iterations;

@@ -53,6 +53,24 @@ class _Result(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class _SetupTimes(ctypes.Structure):
+    _fields_ = [("memcpy_ms", ctypes.c_double), ("operands_ms", ctypes.c_double),
+                ("init_ms", ctypes.c_double)]
+
+
+class _StageTimes(ctypes.Structure):
+    _fields_ = [("distances_ms", ctypes.c_float), ("argmin_ms", ctypes.c_float),
+                ("centroids_ms", ctypes.c_float)]
+
+
+def _stage_rows(fn, h, check):
+    cnt = ctypes.c_uint64(0)
+    check(fn(h, None, 0, ctypes.byref(cnt)), "stage_times")
+    arr = (_StageTimes * max(1, cnt.value))()
+    check(fn(h, arr, cnt.value, ctypes.byref(cnt)), "stage_times")
+    return [(a.distances_ms, a.argmin_ms, a.centroids_ms) for a in arr[:cnt.value]]
+
+
 MAX_VARIANTS = 8
 
 
@@ -160,6 +178,11 @@ def load_library(path=LIB_PATH):
         "kmc_batch_get_centroids": (ctypes.c_int, [vp, ctypes.c_uint32, vp]),
         "kmc_batch_last_error": (ctypes.c_char_p, [vp]),
         "kmc_batch_destroy": (ctypes.c_int, [vp]),
+        "kmc_get_setup_times": (ctypes.c_int, [vp, P(_SetupTimes)]),
+        "kmc_get_stage_times": (ctypes.c_int, [vp, vp, ctypes.c_uint64, P(ctypes.c_uint64)]),
+        "kmc_kk_get_setup_times": (ctypes.c_int, [vp, P(_SetupTimes)]),
+        "kmc_kk_get_stage_times": (ctypes.c_int, [vp, vp, ctypes.c_uint64,
+                                                  P(ctypes.c_uint64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -266,6 +289,18 @@ class Engine:
         r = _Result()
         self._check(self._lib.kmc_run(self._h, int(maxiter), flags, ctypes.byref(r)), "run")
         return r.as_dict()
+
+    def setup_times(self):
+        """Device ms of the H2D copy, the derived operands and the initial
+        centroids (the reference's memcpy / b-mult / init_centroids timers)."""
+        t = _SetupTimes()
+        self._check(self._lib.kmc_get_setup_times(self._h, ctypes.byref(t)), "setup_times")
+        return {"memcpy_ms": t.memcpy_ms, "operands_ms": t.operands_ms, "init_ms": t.init_ms}
+
+    def stage_times(self):
+        """Per-iteration (distances, argmin, centroids) ms of the last run
+        with time_kernels=True."""
+        return _stage_rows(self._lib.kmc_get_stage_times, self._h, self._check)
 
     def autotune(self):
         """Time every launch variant of the distance kernel on the current
@@ -387,6 +422,14 @@ class KernelEngine:
         r = _KKResult()
         self._check(self._lib.kmc_kk_run(self._h, int(maxiter), flags, ctypes.byref(r)), "kk_run")
         return r.as_dict()
+
+    def setup_times(self):
+        t = _SetupTimes()
+        self._check(self._lib.kmc_kk_get_setup_times(self._h, ctypes.byref(t)), "setup_times")
+        return {"memcpy_ms": t.memcpy_ms, "operands_ms": t.operands_ms, "init_ms": t.init_ms}
+
+    def stage_times(self):
+        return _stage_rows(self._lib.kmc_kk_get_stage_times, self._h, self._check)
 
     def labels(self):
         out = np.empty(self.n, dtype=np.int32)

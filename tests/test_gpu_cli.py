@@ -86,3 +86,101 @@ def test_cli_polynomial_kernel_runs_kernel_kmeans(tmp_path, kmc_mod):
     assert np.array_equal(lab, want.astype(np.int64))
     if res["converged_at"] < 30:
         assert ("K-means converged at iteration %d" % res["converged_at"]) in r.stdout
+
+
+def _driver_fields(out, n_trials, maxiters):
+    """The fields experiments/driver.py:55-141 extracts from gpukmeans'
+    output, with its patterns and its averaging (first match skipped,
+    statistics.mean / stdev of the rest)."""
+    import re
+    import statistics as stats
+
+    def avg(pattern):
+        lst = [float(t) for t in re.findall(pattern, out)[1:]]
+        return stats.mean(lst), stats.stdev(lst)
+
+    f = {
+        "runtime": avg(r"Time: (\d+\.\d+)"),
+        "mem": float(re.search(r"MEMORY FOOTPRINT: (\d+) MB", out).group(1)),
+        "memcpy": avg(r"memcpy time: (\d+\.\d+)"),
+        "argmin": avg(r"clusters_argmin_shfl time: (\d+\.\d+)"),
+        "centroids": avg(r"compute_centroids time: (\d+\.\d+)"),
+        "dist": avg(r"compute_distances time: (\d+\.\d+)"),
+        "bmult": avg(r"b-mult time: (\d+\.\d+)"),
+        "init": avg(r"init_centroids time: (\d+\.\d+)"),
+        "score": float(re.search(r"Score: -?(\d+\.\d+)", out).group(1)),
+    }
+    iters = n = 0
+    for line in out.splitlines():
+        if "converged at iteration" in line:
+            iters += int(line.split("iteration")[1].split("-")[0])
+            n += 1
+        elif "NOT" in line:
+            iters += maxiters
+            n += 1
+    f["iterations"] = iters // n if n else 0
+    return f
+
+
+@pytest.mark.parametrize("kernel", ["linear", "polynomial"])
+def test_cli_output_parses_with_harness_patterns(tmp_path, kernel):
+    """gpukmeans -r 3 prints everything the reference harness parses: the
+    six [PERFORMANCE] stage lines (three per run, three per iteration), the
+    Time / MEMORY FOOTPRINT / Score lines and the convergence lines."""
+    rng = np.random.default_rng(5)
+    n, d, k, maxiter, runs = 4000, 16, 8, 12, 3
+    cen = rng.uniform(-5, 5, (k, d))
+    X = (cen[rng.integers(0, k, n)] + rng.normal(0, 1, (n, d))).astype(np.float32)
+    inp = tmp_path / "in.csv"
+    with open(inp, "w") as f:
+        f.write("k," + ",".join("d%d" % j for j in range(d)) + "\n")
+        for i in range(n):
+            f.write("0," + ",".join("%.9g" % v for v in X[i]) + "\n")
+    cmd = [EXE, "-d", str(d), "-n", str(n), "-k", str(k), "-m", str(maxiter), "-t", "0.0001",
+           "-c", "0", "-b", "random", "-f", kernel, "-l", "2", "-r", str(runs), "-i", str(inp)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+    out = r.stdout
+    f = _driver_fields(out, runs, maxiter)
+    assert f["iterations"] == maxiter
+    assert out.count("memcpy time:") == runs
+    assert out.count("b-mult time:") == runs
+    assert out.count("init_centroids time:") == runs
+    for stage in ("compute_distances", "clusters_argmin_shfl", "compute_centroids"):
+        assert out.count(stage + " time:") == runs * maxiter, stage
+    assert f["dist"][0] > 0 and f["centroids"][0] > 0 and f["mem"] > 0
+    # stage lines carry seconds: a stage of this size takes well under 1 s
+    assert f["dist"][0] < 1.0 and f["argmin"][0] < 1.0 and f["centroids"][0] < 1.0
+    with_flag = subprocess.run(cmd + ["--no-stage-times"], stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert with_flag.returncode == 0 and "[PERFORMANCE]" not in with_flag.stdout
+
+
+def test_cli_reads_libsvm(tmp_path, oracle):
+    """A libsvm input (label, then 1-based idx:value pairs with zeros left
+    out; src/include/input_parser.hpp:76-100) gives the same labels as the
+    same points read from CSV, and the oracle's labels (CPU semantics)."""
+    g = load_golden("ref_c1_blobs.npz")
+    X = g["X"]
+    n, d = X.shape
+    k, seed, tol, maxiter = 10, 42, 1e-4, 300
+    svm = tmp_path / "in.svm"
+    out = tmp_path / "out.csv"
+    with open(svm, "w") as f:
+        for i in range(n):
+            toks = ["%d:%.9g" % (j + 1, X[i, j]) for j in range(d) if X[i, j] != 0]
+            f.write("1 " + " ".join(toks) + "\n")
+    cmd = [EXE, "-d", str(d), "-n", str(n), "-k", str(k), "-m", str(maxiter), "-t", repr(tol),
+           "-c", "1", "-b", "random", "-f", "linear", "-l", "2", "-s", str(seed), "-r", "2",
+           "-i", str(svm), "-o", str(out), "--cpu-semantics", "--no-stage-times"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+    lab = np.loadtxt(out, delimiter=",", skiprows=1, usecols=0, dtype=np.int64)
+    Xw = np.loadtxt(out, delimiter=",", skiprows=1, dtype=np.float64)[:, 1:].astype(np.float32)
+    assert np.array_equal(Xw, X)  # integer-valued points survive the text round trip
+    C0 = oracle.init_centroids(X, k, seed)
+    conv, want, C, cnt, ran = oracle.lloyd(X, C0, tol, maxiter)
+    assert np.array_equal(lab, want.astype(np.int64))
+    assert ("K-means converged at iteration %d" % conv) in r.stdout, r.stdout
