@@ -242,6 +242,26 @@ typedef struct {
 kmc_status kmc_get_stage_times(kmc_handle* h, kmc_stage_times* out, uint64_t cap,
                                uint64_t* count);
 
+/* Allocation-free assignment. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same stage as
+ * kmc_assign in the fp32 form (fused distance GEMM + top-4 + margin band,
+ * then the exact refinement), with the same labels, without a handle and
+ * without any allocation: everything besides the inputs and the output
+ * lives in a fixed device array of the library (under 20 MiB, one per GPU,
+ * held until the process ends).
+ *   X       device, n rows of ld floats; C device, k rows of ld floats;
+ *           ld = d rounded up to 16, 32, 64, 128 or 256 (KMC_ERR_ARG
+ *           otherwise) and the columns d..ld-1 of both must be zero
+ *   labels  device, n int32 (output)
+ *   stream  a hipStream_t, or NULL for the null stream; the call only
+ *           enqueues work (synchronise the stream before reading labels)
+ * Limits: n <= 524288 and k <= 1024 (KMC_ERR_UNSUPPORTED beyond). Uses the
+ * current HIP device. Calls that overlap on one GPU share the array and race;
+ * calls on one stream are ordered and safe. */
+kmc_status kmc_assign_static(const float* X, size_t n, uint32_t d, size_t ld, const float* C,
+                             uint32_t k, int32_t* labels, void* stream);
+
 /* Launch-variant autotuner for the fused distance kernel (no counterpart in
  * the reference, whose stages are fixed library calls): times every variant
  * (waves per workgroup, register prefetch) on the handle's points and

@@ -840,8 +840,7 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     for (int v = 0; v < kmc::assign_variants() && ok; ++v) {
       const void* fn = kmc::assign_kernel_ptr(DP, h->prec, v);
       if (!fn) continue;  // variant not built for this (DP, precision)
-      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)h->assign_lds) != hipSuccess) {
+      if (kmc::raise_lds_limit(fn, h->assign_lds) != hipSuccess) {
         st = fail(h, KMC_ERR_HIP, "hipFuncSetAttribute(assign)");
         ok = false;
         break;

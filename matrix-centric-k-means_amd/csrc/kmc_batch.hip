@@ -667,7 +667,7 @@ kmc_status create_group(kmc_batch* b, BatchGroup* g, const std::vector<kmc_probl
   // launch shape of the distance kernel
   const void* fn = kmc::assign_batch_kernel_ptr(g->DP, g->prec);
   if (!fn) return bfail(b, KMC_ERR_UNSUPPORTED, "kmc_batch_create: no kernel for this shape");
-  KMC_BCK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g->lds));
+  KMC_BCK(kmc::raise_lds_limit(fn, g->lds));
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kmc::kBatchAssignThreads,
                                                    g->lds) != hipSuccess ||

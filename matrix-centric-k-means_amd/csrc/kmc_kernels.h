@@ -294,6 +294,8 @@ int assign_variants();
 int assign_threads(int variant);
 int assign_tile(int variant);
 const void* assign_kernel_ptr(int DP, int prec, int variant);
+// raise (never lower) a kernel's dynamic-LDS limit to bytes
+hipError_t raise_lds_limit(const void* fn, size_t bytes);
 hipError_t launch_refine(const RefineParams& P, hipStream_t s);
 hipError_t launch_sort(const SortParams& P, hipStream_t s);
 hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s);

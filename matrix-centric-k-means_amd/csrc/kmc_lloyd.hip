@@ -24,6 +24,8 @@
 // Every kernel returns at once when the device stop flag is set, so the host
 // may queue iterations ahead of the convergence decision.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "kmc.h"
 #include "kmc_device.h"
@@ -3242,7 +3244,7 @@ __global__ __launch_bounds__(256) void k_load_points(LoadParams P) {
   double s64 = 0.0, d2 = 0.0;
   for (int j = lane; j < P.DP; j += 64) {
     const float v = j < P.D ? src[j] : 0.0f;
-    P.X[(size_t)i * P.DP + j] = v;
+    if (P.X) P.X[(size_t)i * P.DP + j] = v;  // null: norms only (rows already padded)
     s = fmaf(v, v, s);
     s64 += (double)v * (double)v;
     if (P.Xhi) {
@@ -3399,6 +3401,20 @@ const void* assign_kernel_ptr(int DP, int prec, int v) {
     case PREC_FP8: return kernel_of_dv<PREC_FP8>(DP, v);
     default: return nullptr;
   }
+}
+
+// The dynamic-LDS limit of a kernel is process-wide state: handles of other
+// shapes launch the same kernel, so it is only ever raised.
+hipError_t raise_lds_limit(const void* fn, size_t bytes) {
+  static std::mutex mu;
+  static std::map<const void*, size_t> set;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = set.find(fn);
+  if (it != set.end() && it->second >= bytes) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)bytes);
+  if (e == hipSuccess) set[fn] = bytes;
+  return e;
 }
 
 int assign_variants() { return kNumVariants; }

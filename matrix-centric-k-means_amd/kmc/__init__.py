@@ -178,6 +178,8 @@ def load_library(path=LIB_PATH):
         "kmc_batch_get_centroids": (ctypes.c_int, [vp, ctypes.c_uint32, vp]),
         "kmc_batch_last_error": (ctypes.c_char_p, [vp]),
         "kmc_batch_destroy": (ctypes.c_int, [vp]),
+        "kmc_assign_static": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.c_uint32,
+                                             ctypes.c_size_t, vp, ctypes.c_uint32, vp, vp]),
         "kmc_get_setup_times": (ctypes.c_int, [vp, P(_SetupTimes)]),
         "kmc_get_stage_times": (ctypes.c_int, [vp, vp, ctypes.c_uint64, P(ctypes.c_uint64)]),
         "kmc_kk_get_setup_times": (ctypes.c_int, [vp, P(_SetupTimes)]),
@@ -355,6 +357,17 @@ class Engine:
                                          C_new.ctypes.data, counts.ctypes.data,
                                          ctypes.byref(r)), "update")
         return C_new, counts, r.as_dict()
+
+
+def assign_static(x_ptr, n, d, ld, c_ptr, k, labels_ptr, stream=None):
+    """kmc_assign_static on device pointers (allocation-free assignment;
+    enqueues on `stream`, a hipStream_t as an int, or the null stream)."""
+    st = load_library().kmc_assign_static(ctypes.c_void_p(x_ptr), int(n), int(d), int(ld),
+                                           ctypes.c_void_p(c_ptr), int(k),
+                                           ctypes.c_void_p(labels_ptr),
+                                           ctypes.c_void_p(stream) if stream else None)
+    if st != OK:
+        raise KmcError(st, "kmc_assign_static(n=%d, d=%d, k=%d)" % (n, d, k))
 
 
 def fit(X, C0, maxiter, tol=1e-4, precision="fp32", check_converged=True, update="ordered"):
