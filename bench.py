@@ -272,17 +272,9 @@ def bench_kernel_kmeans(a):
 def c4_problems(count=512, seed=20261205):
     """512 independent problems: N log-uniform in [1e3, 2e5], D in
     {16, 32, 64, 128}, K uniform in [4, 128], Gaussian blobs with per-instance
-    seeds, K seed-chosen rows as initial centroids."""
+    seeds, K seed-chosen rows as initial centroids (kmc.datagen)."""
     from kmc import datagen
-    rng = np.random.default_rng(seed)
-    probs = []
-    for i in range(count):
-        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))))
-        d = int(rng.choice([16, 32, 64, 128]))
-        k = int(rng.integers(4, 129))
-        X, _, _ = datagen.blobs(n, d, k, seed + 1 + i, -10.0, 10.0, 0.5, 2.0)
-        probs.append((X, datagen.init_rows(X, k, seed + 10_000 + i)))
-    return probs
+    return datagen.batch_problems(count, seed)
 
 
 def bench_batch(a):

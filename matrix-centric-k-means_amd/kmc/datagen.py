@@ -69,3 +69,19 @@ def init_rows(X, k, seed):
     if len(chosen) < k:
         raise ValueError("fewer than k distinct rows")
     return X[np.array(chosen)].copy()
+
+
+def batch_problems(count=512, seed=20261205):
+    """Independent Lloyd problems of uneven shape: N log-uniform in [1e3, 2e5],
+    D in {16, 32, 64, 128}, K uniform in [4, 128], Gaussian blobs (centres
+    U[-10,10]^D, sigma U[0.5, 2]) with per-instance seeds, K seed-chosen
+    distinct rows as initial centroids. Returns [(X, C0), ...]."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))))
+        d = int(rng.choice([16, 32, 64, 128]))
+        k = int(rng.integers(4, 129))
+        X, _, _ = blobs(n, d, k, seed + 1 + i, -10.0, 10.0, 0.5, 2.0)
+        probs.append((X, init_rows(X, k, seed + 10_000 + i)))
+    return probs
