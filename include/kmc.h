@@ -262,6 +262,19 @@ kmc_status kmc_get_stage_times(kmc_handle* h, kmc_stage_times* out, uint64_t cap
 kmc_status kmc_assign_static(const float* X, size_t n, uint32_t d, size_t ld, const float* C,
                              uint32_t k, int32_t* labels, void* stream);
 
+/* Diagnostic, no counterpart in the reference: one 32 x 32 block of the
+ * distance GEMM's matrix-core products, out[i*32 + j] = A_i . B_j, for host
+ * fp32 operands A, B (32 x d each) stored as the form stores them (fp32 as
+ * is; bf16 round to nearest even; fp8 e4m3 round to nearest even after one
+ * power-of-two scale per row) and multiplied by the distance kernel's MFMA
+ * chain (same instructions, k-step order and lane layout) from a zero
+ * accumulator. Aq, Bq (32 x d) receive the stored operand values. It lets a
+ * caller check the error model behind the margin band (operand rounding and
+ * accumulation bounds per form). precision: KMC_PREC_FP32, _BF16 or _FP8;
+ * d in {16, 32, 64, 128, 256} (fp8: d >= 64). */
+kmc_status kmc_probe_mfma(int32_t precision, uint32_t d, const float* A, const float* B,
+                          float* out, float* Aq, float* Bq);
+
 /* Launch-variant autotuner for the fused distance kernel (no counterpart in
  * the reference, whose stages are fixed library calls): times every variant
  * (waves per workgroup, register prefetch) on the handle's points and

@@ -29,7 +29,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 HIP_SOURCES = ["kmc_lloyd.hip", "kmc_api.hip", "kmc_kkmeans.hip", "kmc_kk_api.hip",
-               "kmc_batch.hip", "kmc_static.hip"]
+               "kmc_batch.hip", "kmc_static.hip", "kmc_probe.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

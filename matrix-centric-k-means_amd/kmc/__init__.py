@@ -180,6 +180,8 @@ def load_library(path=LIB_PATH):
         "kmc_batch_destroy": (ctypes.c_int, [vp]),
         "kmc_assign_static": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.c_uint32,
                                              ctypes.c_size_t, vp, ctypes.c_uint32, vp, vp]),
+        "kmc_probe_mfma": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint32, vp, vp, vp, vp,
+                                          vp]),
         "kmc_get_setup_times": (ctypes.c_int, [vp, P(_SetupTimes)]),
         "kmc_get_stage_times": (ctypes.c_int, [vp, vp, ctypes.c_uint64, P(ctypes.c_uint64)]),
         "kmc_kk_get_setup_times": (ctypes.c_int, [vp, P(_SetupTimes)]),
@@ -368,6 +370,23 @@ def assign_static(x_ptr, n, d, ld, c_ptr, k, labels_ptr, stream=None):
                                            ctypes.c_void_p(stream) if stream else None)
     if st != OK:
         raise KmcError(st, "kmc_assign_static(n=%d, d=%d, k=%d)" % (n, d, k))
+
+
+def probe_mfma(precision, A, B):
+    """kmc_probe_mfma: (out 32 x 32, Aq, Bq) for 32 x d operands A, B."""
+    A = _f32(A)
+    B = _f32(B)
+    if A.shape != B.shape or A.shape[0] != 32:
+        raise ValueError("A and B must both be 32 x d")
+    d = A.shape[1]
+    out = np.empty((32, 32), np.float32)
+    Aq = np.empty_like(A)
+    Bq = np.empty_like(B)
+    st = load_library().kmc_probe_mfma(PRECISIONS[precision], d, A.ctypes.data, B.ctypes.data,
+                                       out.ctypes.data, Aq.ctypes.data, Bq.ctypes.data)
+    if st != OK:
+        raise KmcError(st, "kmc_probe_mfma(%s, d=%d)" % (precision, d))
+    return out, Aq, Bq
 
 
 def fit(X, C0, maxiter, tol=1e-4, precision="fp32", check_converged=True, update="ordered"):
