@@ -59,7 +59,7 @@ sys.path.insert(0, os.path.join(ROOT, "matrix-centric-k-means_amd"))
 # (kernel k-means, its own line)
 WORKLOADS = {
     "c2": dict(n=1_000_000, d=128, k=256, seed=20261203, gen="large_blobs", steps=50,
-               cpu_sample=100_000, cpu_iters=2, update="fixed",
+               cpu_sample=100_000, cpu_iters=3, update="fixed",
                metric="Lloyd iter/s, N=1M D=128 K=256: BF16 MFMA vs FP32 at <=1e-5 inertia err",
                text="Gaussian blobs N=1000000 D=128 K=256"),
     # K = 4096: the fixed-point update's per-workgroup accumulators do not fit
@@ -77,6 +77,32 @@ HBM_PEAK = 8.0e12
 
 
 _T0 = time.perf_counter()
+
+
+def hip_runtime():
+    """The HIP runtime libkmc.so is linked against (loaded through it), for
+    device-wide synchronisation and the plain device copy of --profile-copy:
+    one runtime in the process, the library's own."""
+    import ctypes
+    import kmc
+    kmc.load_library()
+    L = ctypes.CDLL("libamdhip64.so.7")
+    L.hipDeviceSynchronize.argtypes = []
+    L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    L.hipFree.argtypes = [ctypes.c_void_p]
+    L.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return L
+
+
+def device_sync():
+    """Wait for all work on the device (hipDeviceSynchronize)."""
+    rt = hip_runtime()
+
+    def sync():
+        if rt.hipDeviceSynchronize() != 0:
+            sys.exit("bench.py: hipDeviceSynchronize failed")
+    return sync
 
 
 def log(*a):
@@ -141,10 +167,15 @@ def cpu_baseline(X, C0, n_sample=100_000, iters=2):
         secs = time.perf_counter() - t0
         kind = "port"
     per_iter_full = secs / iters * (N / n_sample)
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
     return {"value": 1.0 / per_iter_full, "unit": "iter/s", "cores": 1, "kind": kind,
+            "host_cpus": {"nproc": usable, "os_cpu_count": os.cpu_count()},
             "sample": "first %d of the %d points, D=%d, K=%d, %d iterations in %.2f s "
-                      "(single thread, g++ -O2), scaled linearly in N" %
-                      (n_sample, N, D, K, iters, secs)}
+                      "(single thread, g++ -O2), scaled linearly in N; %.3f s per iteration "
+                      "on the sample" % (n_sample, N, D, K, iters, secs, secs / iters)}
 
 
 # Launch shapes of the distance kernel's variants in the autotuner's order:
@@ -186,13 +217,19 @@ def traffic_from_profiles(role, form, workload="c2", kernel=None):
 
 
 def profile_copy(nbytes):
-    """A plain device copy of known size, for calibrating the PMC byte counters."""
-    import torch
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
-    b = torch.empty_like(a)
+    """A plain device copy of known size (the runtime's blit kernel), for
+    calibrating the PMC byte counters."""
+    import ctypes
+    rt = hip_runtime()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if rt.hipMalloc(ctypes.byref(a), nbytes) or rt.hipMalloc(ctypes.byref(b), nbytes):
+        sys.exit("bench.py: hipMalloc failed")
+    rt.hipMemset(a, 1, nbytes)
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
+        rt.hipMemcpy(b, a, nbytes, 3)  # device to device
+    rt.hipDeviceSynchronize()
+    rt.hipFree(a)
+    rt.hipFree(b)
     print(json.dumps({"copy_bytes": nbytes, "launches": 3}))
 
 
@@ -205,9 +242,9 @@ def bench_kernel_kmeans(a):
     cluster segment), the self terms, the distances and the argmin; the dense
     one-hot MFMA product is timed beside it. The kernel matrix is built once,
     outside the timed region (the reference builds it in the constructor)."""
-    import torch
     import kmc
     from kmc import datagen
+    sync = device_sync()
     n, d, k = 65536, 784, 10
     steps = a.steps if a.steps is not None else 100
     X = datagen.sparse_unit_blobs(n, d, k)[0]
@@ -221,23 +258,23 @@ def bench_kernel_kmeans(a):
                               gamma=1.0 / d if kernel == "rbf" else 0.0) as e:
             t0 = time.perf_counter()
             e.set_points(X)
-            torch.cuda.synchronize()
+            sync()
             build = time.perf_counter() - t0
             e.set_labels(lab0)
             e.run(a.warmup, check_converged=False)
             e.set_labels(lab0)
-            torch.cuda.synchronize()
+            sync()
             # step rate from a pass without kernel events, the product's
             # launch time from a second pass of the same steps with them
             t0 = time.perf_counter()
             e.run(steps, check_converged=False)
-            torch.cuda.synchronize()
+            sync()
             wall = time.perf_counter() - t0
             lab = e.labels()
             e.set_labels(lab0)
-            torch.cuda.synchronize()
+            sync()
             r = e.run(steps, check_converged=False, time_kernels=True)
-            torch.cuda.synchronize()
+            sync()
             assert np.array_equal(e.labels(), lab)
         its = int(r["iterations"])
         t_spmm = r["ms_spmm"] / 1e3 / max(1, int(r["spmm_launches"]))
@@ -283,8 +320,8 @@ def bench_batch(a):
     whole batch. value = problem-iterations per second of the headline form;
     FP32 beside it; a sample of problems run one by one gives the
     unbatched rate for comparison."""
-    import torch
     import kmc
+    sync = device_sync()
     steps = a.steps if a.steps is not None else 30
     log("generating 512 problems")
     probs = c4_problems()
@@ -295,10 +332,10 @@ def bench_batch(a):
     for prec in (a.prec, "fp32") if a.prec != "fp32" else ("fp32",):
         with kmc.Batch(probs, tol=0.0, precision=prec) as b:
             b.run(min(a.warmup, steps), check_converged=False)
-            torch.cuda.synchronize()
+            sync()
             t0 = time.perf_counter()
             res, ms = b.run(steps, check_converged=False)
-            torch.cuda.synchronize()
+            sync()
             wall = time.perf_counter() - t0
             its = sum(int(r["iterations"]) for r in res)
             forms[prec] = {"problem_iters_per_s": round(its / wall, 1),
@@ -316,10 +353,10 @@ def bench_batch(a):
             e.set_points(X)
             e.set_centroids(C0)
             e.run(1, check_converged=False)
-            torch.cuda.synchronize()
+            sync()
             t0 = time.perf_counter()
             e.run(steps, check_converged=False)
-            torch.cuda.synchronize()
+            sync()
             t_lone += time.perf_counter() - t0
     lone_rate = len(sample) * steps / t_lone
     head = forms[a.prec]
@@ -399,15 +436,12 @@ def main():
     if a.update is None:
         a.update = W["update"]
 
-    import torch
     import kmc
     from kmc import datagen
 
-    def sync():
-        torch.cuda.synchronize()
-
-    if not torch.cuda.is_available() or kmc.device_count() < 1:
+    if kmc.device_count() < 1:
         sys.exit("bench.py: no GPU visible")
+    sync = device_sync()
 
     log("generating N=%d D=%d K=%d blobs" % (N, D, K))
     X, _, _ = getattr(datagen, W["gen"])(N, D, K, DATA_SEED)

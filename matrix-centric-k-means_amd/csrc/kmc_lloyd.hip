@@ -620,11 +620,14 @@ void k_assign(AssignParams P) {
       load_xfrag<DP, PREC>(P, r, r < P.n, h, nxt);
     }
 
-    float b[MS][4], x2[MS];
+    float b[MS][4], x2[MS], xd[MS];
 #pragma unroll
     for (int m = 0; m < MS; ++m) {
       const long long r = row_of(lt, m);
       x2[m] = r < rlim ? P.xn2[rbase + r] : 0.0f;
+      // the measured operand error, needed after the sweep: loaded now so
+      // that its latency hides behind the sweep
+      xd[m] = (!BATCH && P.hop > 0.0f && r < rlim) ? P.xdn[rbase + r] : 0.0f;
 #pragma unroll
       for (int k = 0; k < 4; ++k) b[m][k] = kFar;
     }
@@ -727,7 +730,7 @@ void k_assign(AssignParams P) {
         float E = P.alpha * xn * cmax + Ecm + P.xpk * x2[m];
         if (!BATCH && P.hop > 0.0f) {
           // |stored(-2c).stored(x) - (-2c).x| <= ||dc|| ||x|| + ||-2c|| ||dx|| + ||dc|| ||dx||
-          const float dx = P.xdn[prow];
+          const float dx = xd[m];
           E += P.hop * (cdmax * xn + 2.0f * cmax * dx + cdmax * dx);
         }
         const float twoE = 2.0f * E;
