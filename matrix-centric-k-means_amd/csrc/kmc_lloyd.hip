@@ -2563,13 +2563,17 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   for (; i0 < hi; i0 += stride) {
     const bool more = i0 + stride < hi;
     if (more) load_step(i0 + stride, vn, labn, qn, pvn);
+    // a cluster's accumulator row is component-major (dimension 4 g + j
+    // at j * LPP + g), so the LPP lanes of one point add to consecutive
+    // 8-byte words in every atomic instruction: no LDS bank conflict among
+    // them
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      unsigned long long* a = acc + (size_t)lab[u] * ST + 4 * dg;
-      atomicAdd(a + 0, (unsigned long long)__float2ll_rn(v[u].x * fs));
-      atomicAdd(a + 1, (unsigned long long)__float2ll_rn(v[u].y * fs));
-      atomicAdd(a + 2, (unsigned long long)__float2ll_rn(v[u].z * fs));
-      atomicAdd(a + 3, (unsigned long long)__float2ll_rn(v[u].w * fs));
+      unsigned long long* a = acc + (size_t)lab[u] * ST + dg;
+      atomicAdd(a + 0 * LPP, (unsigned long long)__float2ll_rn(v[u].x * fs));
+      atomicAdd(a + 1 * LPP, (unsigned long long)__float2ll_rn(v[u].y * fs));
+      atomicAdd(a + 2 * LPP, (unsigned long long)__float2ll_rn(v[u].z * fs));
+      atomicAdd(a + 3 * LPP, (unsigned long long)__float2ll_rn(v[u].w * fs));
     }
     if (pt_lane) {
 #pragma unroll
@@ -2600,7 +2604,7 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   long long* out = P.fpart + ((size_t)slice * P.wgps + wg) * (size_t)P.Kp * DS;
   for (int t = threadIdx.x; t < P.Kp * DS; t += kFxThreads) {
     const int c = t / DS, d = t - c * DS;
-    out[t] = (long long)acc[(size_t)c * ST + d];
+    out[t] = (long long)acc[(size_t)c * ST + (d & 3) * LPP + (d >> 2)];
   }
   if (slice == 0) {
     for (int t = threadIdx.x; t < P.Kp; t += kFxThreads) {
