@@ -339,6 +339,16 @@ kmc_status kmc_memory_footprint(int device, size_t* used_bytes);
 /* Storage type of the n x n kernel matrix. */
 typedef enum { KMC_STORE_FP32 = 0, KMC_STORE_BF16 = 1 } kmc_kstore;
 
+/* Arithmetic of the Gram product X X^T that the kernel matrix is built from
+ * (the reference's cublasSgemm / cublasSsyrk, src/kernels/kernels.cuh:
+ * 567-649); every form computes only the tiles on and below the diagonal and
+ * mirrors them, as the reference's SYRK path does for n/d <= 100. */
+typedef enum {
+  KMC_KBUILD_FP32 = 0,   /* v_mfma_f32_32x32x2_f32                         */
+  KMC_KBUILD_BF16X3 = 1, /* hi+lo split bf16, 3 x v_mfma_f32_32x32x16_bf16 */
+  KMC_KBUILD_BF16 = 2    /* one bf16 product                               */
+} kmc_kbuild;
+
 #define KMC_RUN_STOP_STABLE 0x10u /* with KMC_RUN_CHECK_CONVERGED (kernel
                                      k-means): stop when no label changes,
                                      instead of the score test              */
@@ -361,6 +371,7 @@ typedef struct {
   float gamma;       /* RBF width; <= 0 selects 1/d                  */
   int32_t device;
   int32_t product;   /* kmc_kk_product                               */
+  int32_t build;     /* kmc_kbuild (0: fp32)                          */
 } kmc_kk_config;
 
 typedef struct {

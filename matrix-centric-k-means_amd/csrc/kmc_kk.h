@@ -10,6 +10,8 @@ namespace kmc {
 
 enum { KK_LINEAR = 0, KK_POLY = 1, KK_SIGMOID = 2, KK_RBF = 3 };
 enum { KS_FP32 = 0, KS_BF16 = 1 };
+// arithmetic of the Gram product X X^T in the kernel-matrix build
+enum { KB_FP32 = 0, KB_BF16X3 = 1, KB_BF16 = 2 };
 // S = K H products: dense one-hot MFMA product, or cluster-sorted row sums
 enum { KK_DENSE = 0, KK_SORTED = 1 };
 constexpr int KK_MAX_K = 1024;       // clusters (cluster-sorted product)
@@ -34,6 +36,7 @@ struct KKParams {
   long long Np;      // padded order of K (multiple of 64)
   void* Kmat;        // Np x Np, fp32 or bf16
   int store, kind;
+  int build;         // KB_*
   float gamma;
   float* xn2;        // n
   float* kdiag;      // n

@@ -86,6 +86,7 @@ kmc_status kmc_kk_create(const kmc_kk_config* cfg, kmc_kk_handle** out) {
     return KMC_ERR_UNSUPPORTED;
   if (cfg->kernel < KMC_KERNEL_LINEAR || cfg->kernel > KMC_KERNEL_RBF) return KMC_ERR_ARG;
   if (cfg->store != KMC_STORE_FP32 && cfg->store != KMC_STORE_BF16) return KMC_ERR_ARG;
+  if (cfg->build < KMC_KBUILD_FP32 || cfg->build > KMC_KBUILD_BF16) return KMC_ERR_ARG;
   if (cfg->n > (size_t)0x7fffffff) return KMC_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return KMC_ERR_NO_DEVICE;
@@ -102,6 +103,9 @@ kmc_status kmc_kk_create(const kmc_kk_config* cfg, kmc_kk_handle** out) {
   P.Np = (P.n + 63) / 64 * 64;
   P.store = cfg->store == KMC_STORE_FP32 ? kmc::KS_FP32 : kmc::KS_BF16;
   P.kind = cfg->kernel;
+  P.build = cfg->build == KMC_KBUILD_BF16X3 ? kmc::KB_BF16X3
+            : cfg->build == KMC_KBUILD_BF16 ? kmc::KB_BF16
+                                            : kmc::KB_FP32;
   P.gamma = cfg->gamma > 0.0f ? cfg->gamma : 1.0f / (float)cfg->d;
   P.K = (int)cfg->k;
   P.product = product;

@@ -2,8 +2,10 @@
 //
 // Set-up:
 //   k_kk_norms   ||x_i||^2 (for the RBF kernel)
-//   k_kk_build   K = f(X X^T) on MFMA (fp32 32x32x2), kernel function fused in
-//                the epilogue, stored fp32 or bf16, row stride Np (zero pad)
+//   k_kk_build   K = f(X X^T) on MFMA (fp32 32x32x2, or bf16 32x32x16 as one
+//                product or hi+lo split), lower-triangle tiles only, each
+//                stored with its transpose; kernel function fused in the
+//                epilogue, stored fp32 or bf16, row stride Np (zero pad)
 // Per iteration (labels -> labels):
 //   k_kk_onehot  H^T[c][j] = (label_j == c), fp32 or bf16 (exact 0/1)
 //   k_kk_count   cluster sizes
@@ -56,15 +58,38 @@ __device__ __forceinline__ float kernel_fn(int kind, float v, float xi2, float x
   }
 }
 
-// 64x64 output tile per workgroup, 4 waves of 32x32; X staged through LDS in
-// k-chunks of 32 (rows padded by 4 floats against bank conflicts).
-template <int STORE>
+// K = f(X X^T) is symmetric: only the tiles on and below the diagonal are
+// computed (a 1-D grid over the T(T+1)/2 pairs ti >= tj of 64 x 64 tiles),
+// and an off-diagonal tile is stored twice, as itself and transposed.
+// Workgroup of 4 waves, each a 32 x 32 block; X staged through LDS in
+// k-chunks of 32. The Gram product runs on MFMA in one of three forms:
+//   KB_FP32    v_mfma_f32_32x32x2_f32 on the fp32 values
+//   KB_BF16X3  x = hi + lo (bf16, round to nearest even), three
+//              v_mfma_f32_32x32x16_bf16 per k-step: hi.hi + hi.lo + lo.hi
+//   KB_BF16    one bf16 product
+// all accumulated in fp32; the kernel function is applied in the epilogue.
+__device__ __forceinline__ void tri_tile(long long b, long long& ti, long long& tj) {
+  long long t = (long long)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while (t * (t + 1) / 2 > b) --t;
+  while ((t + 1) * (t + 2) / 2 <= b) ++t;
+  ti = t;
+  tj = b - t * (t + 1) / 2;
+}
+
+template <int STORE, int BUILD>
 __global__ __launch_bounds__(256) void k_kk_build(KKParams P) {
-  constexpr int TK = 32, RS = TK + 4;
-  __shared__ __attribute__((aligned(16))) float As[64 * RS];
-  __shared__ __attribute__((aligned(16))) float Bs[64 * RS];
+  constexpr int TK = 32;
+  constexpr int RS = TK + 4;   // fp32 row stride (floats)
+  constexpr int RH = TK + 8;   // bf16 row stride (bf16 elements)
+  constexpr bool kF32 = BUILD == KB_FP32;
+  constexpr int kPl = BUILD == KB_BF16X3 ? 2 : 1;  // bf16 planes (hi, lo)
+  __shared__ __attribute__((aligned(16))) float As[kF32 ? 64 * RS : 1];
+  __shared__ __attribute__((aligned(16))) float Bs[kF32 ? 64 * RS : 1];
+  __shared__ __attribute__((aligned(16))) uint16_t Ah[kF32 ? 1 : kPl * 64 * RH];
+  __shared__ __attribute__((aligned(16))) uint16_t Bh[kF32 ? 1 : kPl * 64 * RH];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long ti = blockIdx.y, tj = blockIdx.x;
+  long long ti, tj;
+  tri_tile((long long)blockIdx.x, ti, tj);
   const long long r0 = ti * 64, c0 = tj * 64;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int p = lane & 31, h = lane >> 5;
@@ -77,34 +102,98 @@ __global__ __launch_bounds__(256) void k_kk_build(KKParams P) {
       const int r = v / TK, q = v - r * TK;
       const long long gi = r0 + r, gj = c0 + r;
       const int kk = k0 + q;
-      As[r * RS + q] = (gi < P.n && kk < P.d) ? P.X[(size_t)gi * P.ldx + kk] : 0.0f;
-      Bs[r * RS + q] = (gj < P.n && kk < P.d) ? P.X[(size_t)gj * P.ldx + kk] : 0.0f;
+      const float a = (gi < P.n && kk < P.d) ? P.X[(size_t)gi * P.ldx + kk] : 0.0f;
+      const float b = (gj < P.n && kk < P.d) ? P.X[(size_t)gj * P.ldx + kk] : 0.0f;
+      if constexpr (kF32) {
+        As[r * RS + q] = a;
+        Bs[r * RS + q] = b;
+      } else {
+        const uint16_t ah = bf16_rne(a), bh = bf16_rne(b);
+        Ah[r * RH + q] = ah;
+        Bh[r * RH + q] = bh;
+        if (kPl == 2) {
+          Ah[64 * RH + r * RH + q] = bf16_rne(a - bf16_to_f32(ah));
+          Bh[64 * RH + r * RH + q] = bf16_rne(b - bf16_to_f32(bh));
+        }
+      }
     }
     __syncthreads();
-    // A: rows of this wave's i-block, B: rows of its j-block; lane half h
-    // takes k = 2*step + h
+    if constexpr (kF32) {
+      // A: rows of this wave's i-block, B: rows of its j-block; lane half h
+      // takes k = 2*step + h
 #pragma unroll
-    for (int s = 0; s < TK / 2; ++s) {
-      const float a = As[(wr + p) * RS + 2 * s + h];
-      const float b = Bs[(wc + p) * RS + 2 * s + h];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+      for (int s = 0; s < TK / 2; ++s) {
+        const float a = As[(wr + p) * RS + 2 * s + h];
+        const float b = Bs[(wc + p) * RS + 2 * s + h];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+      }
+    } else {
+      // k-step s: dimensions 16 s + 8 h + [0, 8) of both rows
+#pragma unroll
+      for (int s = 0; s < TK / 16; ++s) {
+        const uint4 a = *reinterpret_cast<const uint4*>(&Ah[(wr + p) * RH + 16 * s + 8 * h]);
+        const uint4 b = *reinterpret_cast<const uint4*>(&Bh[(wc + p) * RH + 16 * s + 8 * h]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a),
+                                                      __builtin_bit_cast(kmc_bf16x8, b), acc,
+                                                      0, 0, 0);
+        if constexpr (kPl == 2) {
+          const uint4 al =
+              *reinterpret_cast<const uint4*>(&Ah[64 * RH + (wr + p) * RH + 16 * s + 8 * h]);
+          const uint4 bl =
+              *reinterpret_cast<const uint4*>(&Bh[64 * RH + (wc + p) * RH + 16 * s + 8 * h]);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a),
+                                                        __builtin_bit_cast(kmc_bf16x8, bl), acc,
+                                                        0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, al),
+                                                        __builtin_bit_cast(kmc_bf16x8, b), acc,
+                                                        0, 0, 0);
+        }
+      }
     }
   }
   // acc[r]: row (i) = 8*(r>>2) + 4*h + (r&3) of the A block, column (j) = p
   const long long gj = c0 + wc + p;
+  const float xj2 = (P.kind == KK_RBF && gj < P.n) ? P.xn2[gj] : 0.0f;
+  const bool mirror = ti != tj;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const long long gi = r0 + wr + 8 * (r >> 2) + 4 * h + (r & 3);
-    if (gi >= P.Np || gj >= P.Np) continue;
-    float v = 0.0f;
-    if (gi < P.n && gj < P.n) {
-      const float xi2 = P.kind == KK_RBF ? P.xn2[gi] : 0.0f;
-      const float xj2 = P.kind == KK_RBF ? P.xn2[gj] : 0.0f;
-      v = kernel_fn(P.kind, acc[r], xi2, xj2, P.gamma, gi == gj);
+  for (int g = 0; g < 4; ++g) {
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int r = 4 * g + c;
+      const long long gi = r0 + wr + 8 * g + 4 * h + c;
+      v[c] = 0.0f;
+      if (gi < P.n && gj < P.n) {
+        const float xi2 = P.kind == KK_RBF ? P.xn2[gi] : 0.0f;
+        v[c] = kernel_fn(P.kind, acc[r], xi2, xj2, P.gamma, gi == gj);
+      }
+      // a diagonal tile keeps its lower triangle and mirrors it (the hi+lo
+      // product is not symmetric in the order of its terms)
+      if (gi < P.Np && gj < P.Np && (mirror || gi >= gj)) {
+        const size_t off = (size_t)gi * P.Np + gj;
+        if (STORE == KS_FP32) reinterpret_cast<float*>(P.Kmat)[off] = v[c];
+        else reinterpret_cast<uint16_t*>(P.Kmat)[off] = bf16_rne(v[c]);
+        if (!mirror && gi > gj) {
+          const size_t moff = (size_t)gj * P.Np + gi;
+          if (STORE == KS_FP32) reinterpret_cast<float*>(P.Kmat)[moff] = v[c];
+          else reinterpret_cast<uint16_t*>(P.Kmat)[moff] = bf16_rne(v[c]);
+        }
+      }
     }
-    const size_t off = (size_t)gi * P.Np + gj;
-    if (STORE == KS_FP32) reinterpret_cast<float*>(P.Kmat)[off] = v;
-    else reinterpret_cast<uint16_t*>(P.Kmat)[off] = bf16_rne(v);
+    // the transposed copy: row gj, columns gi .. gi + 3 (one 16- or 8-byte
+    // store per lane; a row's pieces meet in the L2 as whole lines)
+    const long long gi0 = r0 + wr + 8 * g + 4 * h;
+    if (mirror && gj < P.Np && gi0 + 3 < P.Np) {
+      const size_t off = (size_t)gj * P.Np + gi0;
+      if (STORE == KS_FP32) {
+        *reinterpret_cast<float4*>(reinterpret_cast<float*>(P.Kmat) + off) =
+            make_float4(v[0], v[1], v[2], v[3]);
+      } else {
+        const uint32_t lo = (uint32_t)bf16_rne(v[0]) | ((uint32_t)bf16_rne(v[1]) << 16);
+        const uint32_t hi = (uint32_t)bf16_rne(v[2]) | ((uint32_t)bf16_rne(v[3]) << 16);
+        *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(P.Kmat) + off) = make_uint2(lo, hi);
+      }
+    }
   }
 }
 
@@ -481,14 +570,21 @@ __global__ void k_kk_zero(KKParams P) {
 // ---------------------------------------------------------------------------
 hipError_t kk_launch_setup(const KKParams& P, hipStream_t s) {
   hipLaunchKernelGGL(k_kk_norms, dim3((unsigned)((P.n + 3) / 4)), dim3(256), 0, s, P);
-  const unsigned T = (unsigned)((P.Np + 63) / 64);
-  if (P.store == KS_FP32) {
-    hipLaunchKernelGGL(k_kk_build<KS_FP32>, dim3(T, T), dim3(256), 0, s, P);
+  const long long T = (P.Np + 63) / 64;
+  const dim3 grid((unsigned)(T * (T + 1) / 2));  // tiles on and below the diagonal
+#define KK_BUILD_CASE(st, bd) \
+  if (P.store == st && P.build == bd) hipLaunchKernelGGL((k_kk_build<st, bd>), grid, dim3(256), 0, s, P);
+  KK_BUILD_CASE(KS_FP32, KB_FP32)
+  KK_BUILD_CASE(KS_FP32, KB_BF16X3)
+  KK_BUILD_CASE(KS_FP32, KB_BF16)
+  KK_BUILD_CASE(KS_BF16, KB_FP32)
+  KK_BUILD_CASE(KS_BF16, KB_BF16X3)
+  KK_BUILD_CASE(KS_BF16, KB_BF16)
+#undef KK_BUILD_CASE
+  if (P.store == KS_FP32)
     hipLaunchKernelGGL(k_kk_diag<KS_FP32>, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, s, P);
-  } else {
-    hipLaunchKernelGGL(k_kk_build<KS_BF16>, dim3(T, T), dim3(256), 0, s, P);
+  else
     hipLaunchKernelGGL(k_kk_diag<KS_BF16>, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, s, P);
-  }
   return hipGetLastError();
 }
 

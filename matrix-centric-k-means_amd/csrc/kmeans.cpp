@@ -113,6 +113,9 @@ void Kmeans::init(const int* seed) {
     kc.gamma = 0.0f;
     kc.device = deviceProps ? deviceProps->ordinal : 0;
     kc.product = KMC_KK_AUTO;
+    kc.build = opts.precision == Precision::fp32     ? KMC_KBUILD_FP32
+               : opts.precision == Precision::bf16x3 ? KMC_KBUILD_BF16X3
+                                                     : KMC_KBUILD_BF16;
     const kmc_status s = kmc_kk_create(&kc, &kh);
     if (s != KMC_OK) throw kmc::Error(s, std::string("kmc_kk_create: ") + kmc_status_string(s));
     std::vector<float> X(n * (size_t)d);

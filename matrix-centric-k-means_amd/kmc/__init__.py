@@ -84,7 +84,7 @@ class _KKConfig(ctypes.Structure):
     _fields_ = [("n", ctypes.c_size_t), ("d", ctypes.c_uint32), ("k", ctypes.c_uint32),
                 ("tol", ctypes.c_float), ("kernel", ctypes.c_int32), ("store", ctypes.c_int32),
                 ("gamma", ctypes.c_float), ("device", ctypes.c_int32),
-                ("product", ctypes.c_int32)]
+                ("product", ctypes.c_int32), ("build", ctypes.c_int32)]
 
 
 class _KKResult(ctypes.Structure):
@@ -120,6 +120,7 @@ RUN_EXCLUDE_EMPTY = 0x8
 RUN_STOP_STABLE = 0x10
 RUN_GRAPH = 0x20
 KK_PRODUCTS = {"auto": 0, "dense": 1, "sorted": 2}
+KBUILDS = {"fp32": 0, "bf16x3": 1, "bf16": 2}
 RUN_CONVERGE_SHIFT_SUM = 0x40
 RUN_COST_RATIO = 0x80
 RUN_KEEP_EMPTY = 0x100
@@ -407,11 +408,11 @@ class KernelEngine:
     iteration (src/kmeans.cu:71-305, 772-1122)."""
 
     def __init__(self, n, d, k, kernel="polynomial", store="fp32", gamma=0.0, tol=1e-4,
-                 device=0, product="auto"):
+                 device=0, product="auto", build="fp32"):
         self._lib = load_library()
         self.n, self.d, self.k = int(n), int(d), int(k)
         cfg = _KKConfig(self.n, self.d, self.k, float(tol), KERNELS[kernel], STORES[store],
-                        float(gamma), int(device), KK_PRODUCTS[product])
+                        float(gamma), int(device), KK_PRODUCTS[product], KBUILDS[build])
         h = ctypes.c_void_p()
         st = self._lib.kmc_kk_create(ctypes.byref(cfg), ctypes.byref(h))
         if st != OK:

@@ -59,7 +59,9 @@ def test_c3_linear_kernel_is_fp64_lloyd_step(kmc_mod, oracle, c3_data, store):
     scale = float((X.astype(np.float64) ** 2).sum(1).max())
     safe = margin > 2 * _dist_tol(n, store, scale)
     _say("C3 linear", store, "safe", safe.mean())
-    with kmc_mod.KernelEngine(n, d, k, kernel="linear", store=store) as e:
+    # bf16 storage takes the bf16 hi+lo Gram build
+    build = "fp32" if store == "fp32" else "bf16x3"
+    with kmc_mod.KernelEngine(n, d, k, kernel="linear", store=store, build=build) as e:
         e.set_points(X)
         e.set_labels(lab0)
         res = e.run(1)

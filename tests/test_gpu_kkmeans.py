@@ -78,6 +78,75 @@ def test_kernel_matrix_and_distances(kmc_mod, oracle, kernel, store, product):
         assert np.all(np.abs(D[fin] - D64[fin]) <= dtol), np.max(np.abs(D[fin] - D64[fin]))
 
 
+def _bf16(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _gram_as_built(X, build):
+    """The Gram values each build form computes exactly (before fp32
+    accumulation): fp32 x.y; bf16x3 hi.hi + hi.lo + lo.hi with x = hi + lo;
+    bf16 hi.hi."""
+    X64 = X.astype(np.float64)
+    if build == "fp32":
+        return X64 @ X64.T
+    hi = _bf16(X)
+    if build == "bf16":
+        return hi @ hi.T
+    lo = _bf16((X64 - hi).astype(np.float32))
+    return hi @ hi.T + hi @ lo.T + lo @ hi.T
+
+
+def _kernel_of(V, kind, xn2, gamma):
+    if kind == "linear":
+        return V.copy()
+    if kind == "polynomial":
+        return (V + 1.0) ** 2
+    if kind == "sigmoid":
+        return np.tanh(V + 1.0)
+    K = np.exp(-gamma * np.maximum(xn2[:, None] + xn2[None, :] - 2.0 * V, 0.0))
+    np.fill_diagonal(K, 1.0)
+    return K
+
+
+@pytest.mark.parametrize("build", ["fp32", "bf16x3", "bf16"])
+@pytest.mark.parametrize("store", ["fp32", "bf16"])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_kernel_matrix_build_forms(kmc_mod, kernel, store, build):
+    """Every build form: K is exactly symmetric (the lower-triangle tiles are
+    mirrored), and K equals the kernel of the Gram values the form computes
+    (its operand rounding restated in FP64 here) within fp32 accumulation
+    and the storage rounding. 700 points: off-diagonal and diagonal tiles,
+    a padded last tile."""
+    n, d, k = 700, 45, 4
+    X = _data(n, d, k, 7 + KIND[kernel])
+    gamma = 1.0 / d
+    V = _gram_as_built(X, build)
+    xn2 = (X.astype(np.float64) ** 2).sum(1)
+    Kref = _kernel_of(V, kernel, xn2, gamma)
+    scale = np.abs(Kref).max()
+    _say("kk build", kernel, store, build)
+    with kmc_mod.KernelEngine(n, d, k, kernel=kernel, store=store, build=build) as e:
+        e.set_points(X)
+        Kg = e.kernel_rows(0, n)
+    assert np.array_equal(Kg, Kg.T)
+    # accumulation: gamma_{3d} of sum |x||y| <= 3d 2^-24 (||x||^2 + ||y||^2)/2,
+    # propagated through the kernel function (|f'| <= 2(|v|+1) for the
+    # polynomial, 2 gamma K for the RBF), plus float evaluation of f
+    xa = np.abs(X.astype(np.float64))
+    sabs = xa @ xa.T
+    acc = 3 * d * 2.0 ** -24 * sabs * 2.0
+    fprime = {"linear": 1.0, "sigmoid": 1.0, "polynomial": 2.0 * (np.abs(V) + 1.0),
+              "rbf": 2.0 * gamma * np.abs(Kref)}[kernel]
+    tol = fprime * acc + 4e-7 * (np.abs(Kref) + scale)
+    if store == "bf16":
+        tol = tol + 2.0 ** -8 * np.abs(Kref)
+    err = np.abs(Kg - Kref)
+    assert np.all(err <= tol), float(np.max(err - tol))
+
+
 @pytest.mark.parametrize("product", PRODUCTS)
 @pytest.mark.parametrize("store", ["fp32", "bf16"])
 @pytest.mark.parametrize("kernel", ["polynomial", "rbf"])
