@@ -37,6 +37,11 @@ struct KKParams {
   void* Kmat;        // Np x Np, fp32 or bf16
   int store, kind;
   int build;         // KB_*
+  // bf16 builds: X split once into bf16 hi (and lo) planes, n x dpad
+  // (dpad = d rounded up to 32, zero pad), the operands the build stages
+  const uint16_t* Xh;
+  const uint16_t* Xl;
+  int dpad;
   float gamma;
   float* xn2;        // n
   float* kdiag;      // n

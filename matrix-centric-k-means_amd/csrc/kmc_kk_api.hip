@@ -203,10 +203,22 @@ kmc_status kmc_kk_set_points(kmc_kk_handle* h, const float* X, size_t ld) {
   float mc = 0.f;
   KK_CK(hipEventElapsedTime(&mc, e0, e1));
   h->setup.memcpy_ms = mc;
+  // the bf16 builds read X as bf16 planes, split once here
+  uint16_t* planes = nullptr;
+  P.dpad = (P.d + 31) / 32 * 32;
+  if (P.build != kmc::KB_FP32) {
+    const size_t np = P.build == kmc::KB_BF16X3 ? 2 : 1;
+    KK_CK(dalloc(&planes, np * (size_t)P.n * (size_t)P.dpad));
+    P.Xh = planes;
+    P.Xl = np == 2 ? planes + (size_t)P.n * (size_t)P.dpad : nullptr;
+  }
   KK_CK(hipEventRecord(e0, h->stream));
-  KK_CK(kmc::kk_launch_setup(P, h->stream));
-  KK_CK(hipEventRecord(e1, h->stream));
-  KK_CK(hipEventSynchronize(e1));
+  hipError_t be = kmc::kk_launch_setup(P, h->stream);
+  if (be == hipSuccess) be = hipEventRecord(e1, h->stream);
+  if (be == hipSuccess) be = hipEventSynchronize(e1);
+  if (planes) (void)hipFree(planes);
+  P.Xh = P.Xl = nullptr;
+  KK_CK(be);
   KK_CK(hipEventElapsedTime(&h->ms_build, e0, e1));
   h->setup.operands_ms = h->ms_build;
   (void)hipEventDestroy(e0);

@@ -31,6 +31,8 @@
 
 namespace kmc {
 
+constexpr int kKBTile = 128;  // kernel-matrix build: output tile edge
+
 __global__ __launch_bounds__(256) void k_kk_norms(KKParams P) {
   const int lane = threadIdx.x & 63;
   const long long i = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -59,10 +61,11 @@ __device__ __forceinline__ float kernel_fn(int kind, float v, float xi2, float x
 }
 
 // K = f(X X^T) is symmetric: only the tiles on and below the diagonal are
-// computed (a 1-D grid over the T(T+1)/2 pairs ti >= tj of 64 x 64 tiles),
+// computed (a 1-D grid over the T(T+1)/2 pairs ti >= tj of tiles),
 // and an off-diagonal tile is stored twice, as itself and transposed.
-// Workgroup of 4 waves, each a 32 x 32 block; X staged through LDS in
-// k-chunks of 32. The Gram product runs on MFMA in one of three forms:
+// Workgroup of 4 waves over a 128 x 128 tile, each wave a 64 x 64 block (2 x 2
+// MFMA blocks: every operand fragment read from LDS feeds two MFMAs); X
+// staged through LDS in k-chunks of 32. The Gram product runs on MFMA in one of three forms:
 //   KB_FP32    v_mfma_f32_32x32x2_f32 on the fp32 values
 //   KB_BF16X3  x = hi + lo (bf16, round to nearest even), three
 //              v_mfma_f32_32x32x16_bf16 per k-step: hi.hi + hi.lo + lo.hi
@@ -76,124 +79,203 @@ __device__ __forceinline__ void tri_tile(long long b, long long& ti, long long& 
   tj = b - t * (t + 1) / 2;
 }
 
+// bf16 hi (and lo) planes of X, n x dpad, zero padded: x = hi + lo with hi
+// = bf16(x), lo = bf16(x - hi) (round to nearest even)
+__global__ __launch_bounds__(256) void k_kk_split(KKParams P) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= P.n * (long long)P.dpad) return;
+  const long long i = t / P.dpad;
+  const int k = (int)(t - i * P.dpad);
+  const float x = k < P.d ? P.X[(size_t)i * P.ldx + k] : 0.0f;
+  const uint16_t hi = bf16_rne(x);
+  const_cast<uint16_t*>(P.Xh)[t] = hi;
+  if (P.Xl) const_cast<uint16_t*>(P.Xl)[t] = bf16_rne(x - bf16_to_f32(hi));
+}
+
 template <int STORE, int BUILD>
 __global__ __launch_bounds__(256) void k_kk_build(KKParams P) {
+  constexpr int TT = kKBTile;  // output tile (points) per workgroup
   constexpr int TK = 32;
   constexpr int RS = TK + 4;   // fp32 row stride (floats)
   constexpr int RH = TK + 8;   // bf16 row stride (bf16 elements)
   constexpr bool kF32 = BUILD == KB_FP32;
   constexpr int kPl = BUILD == KB_BF16X3 ? 2 : 1;  // bf16 planes (hi, lo)
-  __shared__ __attribute__((aligned(16))) float As[kF32 ? 64 * RS : 1];
-  __shared__ __attribute__((aligned(16))) float Bs[kF32 ? 64 * RS : 1];
-  __shared__ __attribute__((aligned(16))) uint16_t Ah[kF32 ? 1 : kPl * 64 * RH];
-  __shared__ __attribute__((aligned(16))) uint16_t Bh[kF32 ? 1 : kPl * 64 * RH];
+  __shared__ __attribute__((aligned(16))) float As[kF32 ? TT * RS : 1];
+  __shared__ __attribute__((aligned(16))) float Bs[kF32 ? TT * RS : 1];
+  __shared__ __attribute__((aligned(16))) uint16_t Ah[kF32 ? 1 : kPl * TT * RH];
+  __shared__ __attribute__((aligned(16))) uint16_t Bh[kF32 ? 1 : kPl * TT * RH];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   long long ti, tj;
   tri_tile((long long)blockIdx.x, ti, tj);
-  const long long r0 = ti * 64, c0 = tj * 64;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const long long r0 = ti * TT, c0 = tj * TT;
+  // wave: a 64 x 64 block of the tile, as 2 x 2 MFMA blocks of 32 x 32
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
   const int p = lane & 31, h = lane >> 5;
-  kmc_f32x16 acc;
+  kmc_f32x16 acc[2][2];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+  // staging: each thread moves its share of each operand per k-chunk (fp32:
+  // kSt floats; bf16: kSv 16-byte pieces per plane of the pre-split planes);
+  // the next chunk is loaded into registers while the current one feeds the
+  // MFMAs
+  constexpr int kSt = kF32 ? TT * TK / 256 : 1;
+  constexpr int kSv = kF32 ? 1 : TT * TK / 8 / 256;
+  float ra[kSt], rb[kSt];
+  uint4 va[kPl][kSv], vb[kPl][kSv];
+  auto load_chunk = [&](int k0) {
+    if constexpr (kF32) {
+#pragma unroll
+      for (int u = 0; u < kSt; ++u) {
+        const int v = threadIdx.x + 256 * u;
+        const int r = v / TK, q = v - r * TK;
+        const long long gi = r0 + r, gj = c0 + r;
+        const int kk = k0 + q;
+        ra[u] = (gi < P.n && kk < P.d) ? P.X[(size_t)gi * P.ldx + kk] : 0.0f;
+        rb[u] = (gj < P.n && kk < P.d) ? P.X[(size_t)gj * P.ldx + kk] : 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kSv; ++u) {
+        const int v = threadIdx.x + 256 * u;
+        const int r = v / (TK / 8), q = 8 * (v - r * (TK / 8));
+        const long long gi = r0 + r, gj = c0 + r;
+#pragma unroll
+        for (int pl = 0; pl < kPl; ++pl) {
+          const uint16_t* src = pl == 0 ? P.Xh : P.Xl;
+          va[pl][u] = gi < P.n ? *reinterpret_cast<const uint4*>(src + (size_t)gi * P.dpad + k0 + q)
+                               : make_uint4(0u, 0u, 0u, 0u);
+          vb[pl][u] = gj < P.n ? *reinterpret_cast<const uint4*>(src + (size_t)gj * P.dpad + k0 + q)
+                               : make_uint4(0u, 0u, 0u, 0u);
+        }
+      }
+    }
+  };
+  load_chunk(0);
   for (int k0 = 0; k0 < P.d; k0 += TK) {
     __syncthreads();
-    for (int v = threadIdx.x; v < 64 * TK; v += 256) {
-      const int r = v / TK, q = v - r * TK;
-      const long long gi = r0 + r, gj = c0 + r;
-      const int kk = k0 + q;
-      const float a = (gi < P.n && kk < P.d) ? P.X[(size_t)gi * P.ldx + kk] : 0.0f;
-      const float b = (gj < P.n && kk < P.d) ? P.X[(size_t)gj * P.ldx + kk] : 0.0f;
-      if constexpr (kF32) {
-        As[r * RS + q] = a;
-        Bs[r * RS + q] = b;
-      } else {
-        const uint16_t ah = bf16_rne(a), bh = bf16_rne(b);
-        Ah[r * RH + q] = ah;
-        Bh[r * RH + q] = bh;
-        if (kPl == 2) {
-          Ah[64 * RH + r * RH + q] = bf16_rne(a - bf16_to_f32(ah));
-          Bh[64 * RH + r * RH + q] = bf16_rne(b - bf16_to_f32(bh));
+    if constexpr (kF32) {
+#pragma unroll
+      for (int u = 0; u < kSt; ++u) {
+        const int v = threadIdx.x + 256 * u;
+        const int r = v / TK, q = v - r * TK;
+        As[r * RS + q] = ra[u];
+        Bs[r * RS + q] = rb[u];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kSv; ++u) {
+        const int v = threadIdx.x + 256 * u;
+        const int r = v / (TK / 8), q = 8 * (v - r * (TK / 8));
+#pragma unroll
+        for (int pl = 0; pl < kPl; ++pl) {
+          *reinterpret_cast<uint4*>(&Ah[pl * TT * RH + r * RH + q]) = va[pl][u];
+          *reinterpret_cast<uint4*>(&Bh[pl * TT * RH + r * RH + q]) = vb[pl][u];
         }
       }
     }
     __syncthreads();
+    if (k0 + TK < P.d) load_chunk(k0 + TK);
     if constexpr (kF32) {
-      // A: rows of this wave's i-block, B: rows of its j-block; lane half h
-      // takes k = 2*step + h
-#pragma unroll
+      // lane half h takes k = 2*step + h
+#pragma unroll 4
       for (int s = 0; s < TK / 2; ++s) {
-        const float a = As[(wr + p) * RS + 2 * s + h];
-        const float b = Bs[(wc + p) * RS + 2 * s + h];
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+        float a[2], b[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          a[u] = As[(wr + 32 * u + p) * RS + 2 * s + h];
+          b[u] = Bs[(wc + 32 * u + p) * RS + 2 * s + h];
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y)
+            acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[x], b[y], acc[x][y], 0, 0, 0);
       }
     } else {
-      // k-step s: dimensions 16 s + 8 h + [0, 8) of both rows
+      // k-step s: dimensions 16 s + 8 h + [0, 8) of every row
 #pragma unroll
       for (int s = 0; s < TK / 16; ++s) {
-        const uint4 a = *reinterpret_cast<const uint4*>(&Ah[(wr + p) * RH + 16 * s + 8 * h]);
-        const uint4 b = *reinterpret_cast<const uint4*>(&Bh[(wc + p) * RH + 16 * s + 8 * h]);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a),
-                                                      __builtin_bit_cast(kmc_bf16x8, b), acc,
-                                                      0, 0, 0);
-        if constexpr (kPl == 2) {
-          const uint4 al =
-              *reinterpret_cast<const uint4*>(&Ah[64 * RH + (wr + p) * RH + 16 * s + 8 * h]);
-          const uint4 bl =
-              *reinterpret_cast<const uint4*>(&Bh[64 * RH + (wc + p) * RH + 16 * s + 8 * h]);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, a),
-                                                        __builtin_bit_cast(kmc_bf16x8, bl), acc,
-                                                        0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(kmc_bf16x8, al),
-                                                        __builtin_bit_cast(kmc_bf16x8, b), acc,
-                                                        0, 0, 0);
-        }
+        uint4 a[2][kPl], b[2][kPl];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int pl = 0; pl < kPl; ++pl) {
+            a[u][pl] = *reinterpret_cast<const uint4*>(
+                &Ah[pl * TT * RH + (wr + 32 * u + p) * RH + 16 * s + 8 * h]);
+            b[u][pl] = *reinterpret_cast<const uint4*>(
+                &Bh[pl * TT * RH + (wc + 32 * u + p) * RH + 16 * s + 8 * h]);
+          }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y) {
+            acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                __builtin_bit_cast(kmc_bf16x8, a[x][0]), __builtin_bit_cast(kmc_bf16x8, b[y][0]),
+                acc[x][y], 0, 0, 0);
+            if constexpr (kPl == 2) {
+              acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                  __builtin_bit_cast(kmc_bf16x8, a[x][0]),
+                  __builtin_bit_cast(kmc_bf16x8, b[y][kPl - 1]), acc[x][y], 0, 0, 0);
+              acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                  __builtin_bit_cast(kmc_bf16x8, a[x][kPl - 1]),
+                  __builtin_bit_cast(kmc_bf16x8, b[y][0]), acc[x][y], 0, 0, 0);
+            }
+          }
       }
     }
   }
-  // acc[r]: row (i) = 8*(r>>2) + 4*h + (r&3) of the A block, column (j) = p
-  const long long gj = c0 + wc + p;
-  const float xj2 = (P.kind == KK_RBF && gj < P.n) ? P.xn2[gj] : 0.0f;
+  // acc[x][y][r]: row (i) = 32 x + 8*(r>>2) + 4*h + (r&3), column (j) = 32 y + p
   const bool mirror = ti != tj;
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    float v[4];
+  for (int y = 0; y < 2; ++y) {
+    const long long gj = c0 + wc + 32 * y + p;
+    const float xj2 = (P.kind == KK_RBF && gj < P.n) ? P.xn2[gj] : 0.0f;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int r = 4 * g + c;
-      const long long gi = r0 + wr + 8 * g + 4 * h + c;
-      v[c] = 0.0f;
-      if (gi < P.n && gj < P.n) {
-        const float xi2 = P.kind == KK_RBF ? P.xn2[gi] : 0.0f;
-        v[c] = kernel_fn(P.kind, acc[r], xi2, xj2, P.gamma, gi == gj);
-      }
-      // a diagonal tile keeps its lower triangle and mirrors it (the hi+lo
-      // product is not symmetric in the order of its terms)
-      if (gi < P.Np && gj < P.Np && (mirror || gi >= gj)) {
-        const size_t off = (size_t)gi * P.Np + gj;
-        if (STORE == KS_FP32) reinterpret_cast<float*>(P.Kmat)[off] = v[c];
-        else reinterpret_cast<uint16_t*>(P.Kmat)[off] = bf16_rne(v[c]);
-        if (!mirror && gi > gj) {
-          const size_t moff = (size_t)gj * P.Np + gi;
-          if (STORE == KS_FP32) reinterpret_cast<float*>(P.Kmat)[moff] = v[c];
-          else reinterpret_cast<uint16_t*>(P.Kmat)[moff] = bf16_rne(v[c]);
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        float v[4];
+        const long long gi0 = r0 + wr + 32 * x + 8 * g + 4 * h;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const long long gi = gi0 + c;
+          v[c] = 0.0f;
+          if (gi < P.n && gj < P.n) {
+            const float xi2 = P.kind == KK_RBF ? P.xn2[gi] : 0.0f;
+            v[c] = kernel_fn(P.kind, acc[x][y][4 * g + c], xi2, xj2, P.gamma, gi == gj);
+          }
+          // a diagonal tile keeps its lower triangle and mirrors it (the
+          // hi+lo product is not symmetric in the order of its terms)
+          if (gi < P.Np && gj < P.Np && (mirror || gi >= gj)) {
+            const size_t off = (size_t)gi * P.Np + gj;
+            if (STORE == KS_FP32) reinterpret_cast<float*>(P.Kmat)[off] = v[c];
+            else reinterpret_cast<uint16_t*>(P.Kmat)[off] = bf16_rne(v[c]);
+            if (!mirror && gi > gj) {
+              const size_t moff = (size_t)gj * P.Np + gi;
+              if (STORE == KS_FP32) reinterpret_cast<float*>(P.Kmat)[moff] = v[c];
+              else reinterpret_cast<uint16_t*>(P.Kmat)[moff] = bf16_rne(v[c]);
+            }
+          }
+        }
+        // the transposed copy: row gj, columns gi0 .. gi0 + 3 (one 16- or
+        // 8-byte store per lane; a row's pieces meet in the L2 as whole lines)
+        if (mirror && gj < P.Np && gi0 + 3 < P.Np) {
+          const size_t off = (size_t)gj * P.Np + gi0;
+          if (STORE == KS_FP32) {
+            *reinterpret_cast<float4*>(reinterpret_cast<float*>(P.Kmat) + off) =
+                make_float4(v[0], v[1], v[2], v[3]);
+          } else {
+            const uint32_t lo = (uint32_t)bf16_rne(v[0]) | ((uint32_t)bf16_rne(v[1]) << 16);
+            const uint32_t hi = (uint32_t)bf16_rne(v[2]) | ((uint32_t)bf16_rne(v[3]) << 16);
+            *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(P.Kmat) + off) =
+                make_uint2(lo, hi);
+          }
         }
       }
-    }
-    // the transposed copy: row gj, columns gi .. gi + 3 (one 16- or 8-byte
-    // store per lane; a row's pieces meet in the L2 as whole lines)
-    const long long gi0 = r0 + wr + 8 * g + 4 * h;
-    if (mirror && gj < P.Np && gi0 + 3 < P.Np) {
-      const size_t off = (size_t)gj * P.Np + gi0;
-      if (STORE == KS_FP32) {
-        *reinterpret_cast<float4*>(reinterpret_cast<float*>(P.Kmat) + off) =
-            make_float4(v[0], v[1], v[2], v[3]);
-      } else {
-        const uint32_t lo = (uint32_t)bf16_rne(v[0]) | ((uint32_t)bf16_rne(v[1]) << 16);
-        const uint32_t hi = (uint32_t)bf16_rne(v[2]) | ((uint32_t)bf16_rne(v[3]) << 16);
-        *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(P.Kmat) + off) = make_uint2(lo, hi);
-      }
-    }
   }
 }
 
@@ -570,7 +652,10 @@ __global__ void k_kk_zero(KKParams P) {
 // ---------------------------------------------------------------------------
 hipError_t kk_launch_setup(const KKParams& P, hipStream_t s) {
   hipLaunchKernelGGL(k_kk_norms, dim3((unsigned)((P.n + 3) / 4)), dim3(256), 0, s, P);
-  const long long T = (P.Np + 63) / 64;
+  if (P.build != KB_FP32)
+    hipLaunchKernelGGL(k_kk_split, dim3((unsigned)((P.n * (long long)P.dpad + 255) / 256)),
+                       dim3(256), 0, s, P);
+  const long long T = (P.Np + kKBTile - 1) / kKBTile;
   const dim3 grid((unsigned)(T * (T + 1) / 2));  // tiles on and below the diagonal
 #define KK_BUILD_CASE(st, bd) \
   if (P.store == st && P.build == bd) hipLaunchKernelGGL((k_kk_build<st, bd>), grid, dim3(256), 0, s, P);
