@@ -113,11 +113,14 @@ def run_form(kmc, X, C0, prec, steps, warmup, sync, update, graph):
     with kmc.Engine(N, D, K, tol=0.0, precision=prec, update=update) as e:
         e.set_points(X)
         e.set_centroids(C0)
+        if warmup:
+            e.run(warmup, check_converged=False, graph=graph)
+        # the variants are timed on the centroids after the warm-up
+        # iterations: the first assignments from the seeded rows leave many
+        # more band points to refine than the rest of a run does
         tuning = e.autotune()
         log("%s autotune: variant %d, ms %s" % (prec, tuning["chosen"],
                                                [round(v, 4) for v in tuning["ms"]]))
-        if warmup:
-            e.run(warmup, check_converged=False, graph=graph)
         # every timed pass runs the first `steps` iterations from the initial
         # centroids, as a run of the engine would. Capture the timed chunks
         # once (the capture is set-up, not a step): run each, then restore
@@ -180,16 +183,23 @@ def cpu_baseline(X, C0, n_sample=100_000, iters=2):
 
 # Launch shapes of the distance kernel's variants in the autotuner's order:
 # (waves per workgroup, point sets per wave, register prefetch, waves-per-SIMD
-# target), the kVariants table of csrc/kmc_lloyd.hip (a CPU test compares them).
-ASSIGN_VARIANTS = [(8, 1, True, 0), (8, 1, False, 0), (12, 1, False, 0), (16, 1, False, 0),
-                   (12, 1, False, 6), (4, 2, False, 0), (8, 2, False, 0), (8, 1, False, 8)]
+# target, two-set kernel, values kept per lane), the kVariants table of
+# csrc/kmc_lloyd.hip (a CPU test compares them).
+ASSIGN_VARIANTS = [(8, 1, True, 0, 0, 4), (8, 1, False, 0, 0, 4), (12, 1, False, 0, 0, 4),
+                   (16, 1, False, 0, 0, 4), (12, 1, False, 6, 0, 4), (4, 2, False, 0, 0, 4),
+                   (8, 2, False, 0, 0, 4), (8, 1, False, 8, 0, 4), (8, 2, False, 0, 1, 4),
+                   (8, 2, False, 0, 1, 3), (4, 2, True, 0, 1, 4), (4, 2, True, 0, 1, 3),
+                   (4, 2, False, 0, 1, 4), (8, 2, False, 0, 1, 2)]
 FORM_CODE = {"fp32": 0, "bf16x3": 1, "bf16": 2, "fp8": 3}
 
 
 def assign_kernel_name(dp, prec, variant):
     """The distance kernel's instantiation for a form and variant, as the
     profiler names it (without the return type and parameters)."""
-    nw, ms, pf, wpe = ASSIGN_VARIANTS[variant]
+    nw, ms, pf, wpe, two, topk = ASSIGN_VARIANTS[variant]
+    if two:
+        return "k_assign2<%d, %d, %d, %d, %s>" % (dp, FORM_CODE[prec], nw, topk,
+                                                 "true" if pf else "false")
     return "k_assign<%d, %d, %d, %d, %s, %d, false>" % (dp, FORM_CODE[prec], nw, ms,
                                                         "true" if pf else "false", wpe)
 

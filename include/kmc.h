@@ -277,13 +277,17 @@ kmc_status kmc_probe_mfma(int32_t precision, uint32_t d, const float* A, const f
 
 /* Launch-variant autotuner for the fused distance kernel (no counterpart in
  * the reference, whose stages are fixed library calls): times every variant
- * (waves per workgroup, register prefetch) on the handle's points and
- * current centroids and keeps the fastest for later kmc_run/kmc_assign. */
-#define KMC_MAX_VARIANTS 8
+ * (waves per workgroup, point sets per wave, register prefetch, values kept
+ * per lane) on the handle's points and current centroids -- the distance
+ * kernel plus the refinement of the points its band leaves -- and keeps the
+ * fastest for later kmc_run/kmc_assign. Variants whose registers spill, or
+ * that need every centroid in LDS where they do not fit, are not offered
+ * (ms = -1). */
+#define KMC_MAX_VARIANTS 16
 typedef struct {
   int32_t nvariants;
   int32_t chosen;
-  float ms[KMC_MAX_VARIANTS];     /* mean kernel time per variant */
+  float ms[KMC_MAX_VARIANTS];     /* mean assignment time per variant, -1 if not offered */
   int32_t threads[KMC_MAX_VARIANTS];
   int32_t tile[KMC_MAX_VARIANTS]; /* points per workgroup tile */
 } kmc_tuning;

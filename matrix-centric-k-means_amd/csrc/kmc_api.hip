@@ -84,8 +84,8 @@ struct kmc_handle {
   size_t assign_lds = 0;
   int assign_grid = 0;
   int variant = 0;  // k_assign launch variant (kmc_autotune)
-  int vgrid[8] = {0};
-  long long vqstride[8] = {0};
+  int vgrid[KMC_MAX_VARIANTS] = {0};
+  long long vqstride[KMC_MAX_VARIANTS] = {0};
   int max_regions = 0;
   float alpha = 0, beta = 0, gref = 0, xpk = 0;
   float alpha3 = 0, beta3 = 0, gref3 = 0;  // bf16x3 constants (pruned all-k pass)
@@ -840,6 +840,15 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     for (int v = 0; v < kmc::assign_variants() && ok; ++v) {
       const void* fn = kmc::assign_kernel_ptr(DP, h->prec, v);
       if (!fn) continue;  // variant not built for this (DP, precision)
+      if (kmc::assign_variant_resident_only(v) && h->kchunk < h->Kp) continue;
+      // a variant whose registers spill to scratch memory is not offered
+      hipFuncAttributes fa{};
+      if (hipFuncGetAttributes(&fa, fn) != hipSuccess) {
+        st = fail(h, KMC_ERR_HIP, "hipFuncGetAttributes(assign)");
+        ok = false;
+        break;
+      }
+      if (fa.localSizeBytes > 0) continue;
       if (kmc::raise_lds_limit(fn, h->assign_lds) != hipSuccess) {
         st = fail(h, KMC_ERR_HIP, "hipFuncSetAttribute(assign)");
         ok = false;
@@ -859,9 +868,14 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
       h->max_regions = std::max(h->max_regions, h->vgrid[v]);
     }
     if (!ok) break;
-    h->variant = 0;
-    h->assign_grid = h->vgrid[0];
-    h->qstride = h->vqstride[0];
+    h->variant = 0;  // the first variant offered for this shape
+    while (h->variant + 1 < kmc::assign_variants() && h->vgrid[h->variant] == 0) ++h->variant;
+    if (h->vgrid[h->variant] == 0) {
+      st = fail(h, KMC_ERR_UNSUPPORTED, "no distance-kernel variant for this shape");
+      break;
+    }
+    h->assign_grid = h->vgrid[h->variant];
+    h->qstride = h->vqstride[h->variant];
     h->idx_bits = 0;
     while ((1 << h->idx_bits) < h->Kp) ++h->idx_bits;
     kmc::band_constants(h->prec, h->D, h->idx_bits, &h->alpha, &h->beta, &h->gref,
@@ -1161,10 +1175,12 @@ kmc_status kmc_autotune(kmc_handle* h, kmc_tuning* out) {
     }
     st = kmc_set_assign_variant(h, v);
     if (st != KMC_OK) break;
-    // one untimed launch, then two timed ones
+    // one untimed assignment, then two timed ones; a timed assignment is the
+    // distance kernel and the refinement of its band points, since variants
+    // that keep fewer values per lane leave more points to refine
     st = enqueue_assign(h, nullptr, nullptr);
-    if (st == KMC_OK) st = enqueue_assign(h, e[0], e[1], false);
-    if (st == KMC_OK) st = enqueue_assign(h, e[2], e[3], false);
+    if (st == KMC_OK) st = enqueue_assign(h, e[0], nullptr, false, e[1]);
+    if (st == KMC_OK) st = enqueue_assign(h, e[2], nullptr, false, e[3]);
     if (st != KMC_OK) break;
     KMC_CK(hipEventSynchronize(e[3]));
     float a = 0.f, b = 0.f;

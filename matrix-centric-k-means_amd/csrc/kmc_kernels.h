@@ -291,6 +291,8 @@ hipError_t launch_assign(int DP, int prec, int variant, const AssignParams& P, i
                          size_t lds, hipStream_t s);
 size_t assign_lds(int DP, int prec, int kchunk);
 int assign_variants();
+// the variant stages every centroid in LDS (no chunked form)
+bool assign_variant_resident_only(int variant);
 int assign_threads(int variant);
 int assign_tile(int variant);
 const void* assign_kernel_ptr(int DP, int prec, int variant);

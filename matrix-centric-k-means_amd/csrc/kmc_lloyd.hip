@@ -791,6 +791,409 @@ void k_assign(AssignParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// Two-set distance kernel for the single-product forms (bf16, fp8)
+// ---------------------------------------------------------------------------
+// Insert v into a running list of the T smallest values b0 <= ... <= b(T-1)
+// (T med3 operations; the k-th smallest of {b, v} is med3(b(k-1), b(k), v)).
+template <int T>
+__device__ __forceinline__ void topT_push(float v, float (&b)[T], float ninf) {
+#pragma unroll
+  for (int k = T - 1; k >= 1; --k) b[k] = __builtin_amdgcn_fmed3f(b[k - 1], b[k], v);
+  b[0] = __builtin_amdgcn_fmed3f(ninf, b[0], v);
+}
+
+template <int T>
+__device__ __forceinline__ void push_blockT(const kmc_f32x16& acc, uint32_t base, uint32_t keepv,
+                                            float (&b)[T], float ninf) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+    topT_push<T>(__uint_as_float((__float_as_uint(acc[r]) & keepv) |
+                                 (base + (uint32_t)(8 * (r >> 2) + (r & 3)))),
+                 b, ninf);
+}
+
+// The A operand of one 32-centroid block as a lane reads it from LDS (its
+// k-slices of -2c for centroid row cb + (lane & 31), and the block's
+// ||c||^2 row as the accumulator start).
+template <int DP, int PREC>
+struct ABlock {
+  static constexpr int NA = PREC == PREC_FP8 ? DP / 32 : DP / 16;
+  uint4 a[NA];
+  int sa;
+  kmc_f32x16 c;
+};
+
+template <int DP, int PREC>
+__device__ __forceinline__ void load_ablock(const unsigned char* rows, const float* cn_s, int cb,
+                                            int p, int h, ABlock<DP, PREC>& ab) {
+  using T = AssignTraits<DP, PREC>;
+  const unsigned char* arow = rows + (size_t)(cb + p) * T::kRowBytes;
+  ab.c = init_acc<false>(cn_s, cb, h, 0.0f);
+  if constexpr (PREC == PREC_FP8) {
+    ab.sa = *reinterpret_cast<const int*>(arow + DP);
+#pragma unroll
+    for (int s = 0; s < DP / 64; ++s) {
+      ab.a[2 * s] = *reinterpret_cast<const uint4*>(arow + 64 * s + 32 * h);
+      ab.a[2 * s + 1] = *reinterpret_cast<const uint4*>(arow + 64 * s + 32 * h + 16);
+    }
+  } else {
+    ab.sa = 0;
+    const uint16_t* ah = reinterpret_cast<const uint16_t*>(arow);
+#pragma unroll
+    for (int s = 0; s < DP / 16; ++s)
+      ab.a[s] = *reinterpret_cast<const uint4*>(ah + 16 * s + 8 * h);
+  }
+}
+
+// The MFMA chains of one block against two point sets: every A fragment
+// feeds both chains; both start from the block's ||c||^2 row.
+template <int DP, int PREC>
+__device__ __forceinline__ void mfma_chain2(const ABlock<DP, PREC>& ab,
+                                            const XFrag<DP, PREC>& x0,
+                                            const XFrag<DP, PREC>& x1, kmc_f32x16& a0,
+                                            kmc_f32x16& a1) {
+  if constexpr (PREC == PREC_FP8) {
+#pragma unroll
+    for (int s = 0; s < DP / 64; ++s) {
+      a0 = mfma_fp8_step(ab.a[2 * s], ab.a[2 * s + 1], ab.sa, x0.q[2 * s], x0.q[2 * s + 1],
+                         x0.sc, s == 0 ? ab.c : a0);
+      a1 = mfma_fp8_step(ab.a[2 * s], ab.a[2 * s + 1], ab.sa, x1.q[2 * s], x1.q[2 * s + 1],
+                         x1.sc, s == 0 ? ab.c : a1);
+    }
+  } else {
+    static_assert(PREC == PREC_BF16, "two-set chains: single-product forms only");
+#pragma unroll
+    for (int s = 0; s < DP / 16; ++s) {
+      const kmc_bf16x8 av = __builtin_bit_cast(kmc_bf16x8, ab.a[s]);
+      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(kmc_bf16x8, x0.hi[s]),
+                                                   s == 0 ? ab.c : a0, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(kmc_bf16x8, x1.hi[s]),
+                                                   s == 0 ? ab.c : a1, 0, 0, 0);
+    }
+  }
+}
+
+// Band test and output of one point set after its sweep. bl holds this
+// lane's T smallest packed values of its k-half; the two halves' lists are
+// merged into the four smallest known values m1..m4, and t = the smaller of
+// the two halves' T-th values bounds every value that was not kept, so the
+// k-th smallest distance satisfies b_k >= min(m_k, t) (exact for k <= T).
+// The band test uses these bounds: a point is only sent to a refinement with
+// fewer candidates when no other centroid can win. With T = 4 this is the
+// plain top-4 test. Every lane of the wave calls it (ballots, shuffles).
+template <int T, bool kAddX2, bool kMeas>
+__device__ __forceinline__ void band_out(const AssignParams& P, const float (&bl)[T], float x2,
+                                         float xd, bool valid, long long prow, float cmax,
+                                         float Ecm, float cdmax, float ninf, int h, int lane,
+                                         int* q2, int* qa, unsigned* q2c_s, unsigned* qac_s,
+                                         int Kv, int cbase) {
+  float v[4], o[T];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    v[k] = k < T ? __uint_as_float(__float_as_uint(bl[k < T ? k : 0]) | (uint32_t)(4 * h))
+                 : kFar;
+#pragma unroll
+  for (int k = 0; k < T; ++k) o[k] = __shfl_xor(v[k], 32, 64);
+  const float t = __builtin_amdgcn_fmed3f(ninf, v[T - 1], o[T - 1]);
+#pragma unroll
+  for (int k = 0; k < T; ++k) top4_push(o[k], v, ninf);
+  float b1 = v[0];
+  float b2 = T >= 2 ? v[1] : __builtin_amdgcn_fmed3f(ninf, v[1], t);
+  float b3 = T >= 3 ? v[2] : __builtin_amdgcn_fmed3f(ninf, v[2], t);
+  float b4 = T >= 4 ? v[3] : __builtin_amdgcn_fmed3f(ninf, v[3], t);
+  const int i1 = (int)(__float_as_uint(v[0]) & P.idx_mask);
+  const int i2 = (int)(__float_as_uint(v[1]) & P.idx_mask);
+  const int i3 = (int)(__float_as_uint(v[2]) & P.idx_mask);
+  if (!kAddX2) {
+    b1 += x2;
+    b2 += x2;
+    b3 += x2;
+    b4 += x2;
+  }
+  // b1 <= b2 <= b3 <= b4 bound the full squared distances (plus the packed
+  // index bits) from below; E bounds the MFMA error of each, gref the
+  // reference's own float error plus the packing perturbation, both relative
+  // to the distances.
+  bool flag2 = false, flagA = false, few3 = false;
+  if (h == 0 && valid) {
+    const float xn = sqrtf(x2);
+    float E = P.alpha * xn * cmax + Ecm + P.xpk * x2;
+    if (kMeas && P.hop > 0.0f) {
+      // |stored(-2c).stored(x) - (-2c).x| <= ||dc|| ||x|| + ||-2c|| ||dx|| + ||dc|| ||dx||
+      E += P.hop * (cdmax * xn + 2.0f * cmax * xd + cdmax * xd);
+    }
+    const float twoE = 2.0f * E;
+    const float b1p = fmaxf(b1 + E, 0.0f);
+    // clear(b): every centroid ranked at b or later loses to the first
+    auto clear = [&](float bb) {
+      return (bb >= kFarTest) || (bb - b1 > twoE + P.gref * (fmaxf(bb + E, 0.0f) + b1p));
+    };
+    const bool uniq = clear(b2);
+    const bool two = clear(b3);
+    const bool three = clear(b4);
+    // a point without any finite distance goes to the all-k pass, which
+    // applies the first-min rule
+    const bool sane = (b1 < kFarTest) && (i1 < Kv);
+    P.labels[prow] = cbase + (sane ? i1 : 0);
+    flag2 = sane && !uniq && (two || three);
+    flagA = !sane || (!uniq && !two && !three);
+    few3 = !two;
+  }
+  const uint64_t m2 = __ballot(flag2);
+  const uint64_t mA = __ballot(flagA);
+  if (m2) {
+    const int leader = __ffsll((unsigned long long)m2) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(q2c_s, (unsigned)__popcll(m2));
+    base = __shfl(base, leader, 64);
+    if (flag2) {
+      const unsigned pos = base + mbcnt64(m2);
+      *reinterpret_cast<int4*>(q2 + 4 * (size_t)pos) =
+          make_int4((int)prow, cbase + i1, cbase + i2, few3 ? cbase + i3 : -1);
+    }
+  }
+  if (mA) {
+    const int leader = __ffsll((unsigned long long)mA) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(qac_s, (unsigned)__popcll(mA));
+    base = __shfl(base, leader, 64);
+    if (flagA) qa[base + mbcnt64(mA)] = (int)prow;
+  }
+}
+
+// Band test and output of both point sets of a two-set sweep in one pass:
+// lane half h serves set h. Each lane keeps its own k-half's list of set h
+// and receives the other half's list of the same set (one exchange of T
+// values), so the merge and the band test run once per lane instead of once
+// per set on every lane.
+template <int T>
+__device__ __forceinline__ void band_out2(const AssignParams& P, const float (&la)[T],
+                                          const float (&lb)[T], float x2, float xd, bool valid,
+                                          long long prow, float cmax, float Ecm, float cdmax,
+                                          float ninf, int h, int lane, int* q2, int* qa,
+                                          unsigned* q2c_s, unsigned* qac_s) {
+  float v[4], o[T];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float own = k < T ? (h ? lb[k < T ? k : 0] : la[k < T ? k : 0]) : kFar;
+    v[k] = k < T ? __uint_as_float(__float_as_uint(own) | (uint32_t)(4 * h)) : kFar;
+  }
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    const float send = __uint_as_float(__float_as_uint(h ? la[k] : lb[k]) | (uint32_t)(4 * h));
+    o[k] = __shfl_xor(send, 32, 64);
+  }
+  const float t = __builtin_amdgcn_fmed3f(ninf, v[T - 1], o[T - 1]);
+#pragma unroll
+  for (int k = 0; k < T; ++k) top4_push(o[k], v, ninf);
+  float b1 = v[0];
+  float b2 = T >= 2 ? v[1] : __builtin_amdgcn_fmed3f(ninf, v[1], t);
+  float b3 = T >= 3 ? v[2] : __builtin_amdgcn_fmed3f(ninf, v[2], t);
+  float b4 = T >= 4 ? v[3] : __builtin_amdgcn_fmed3f(ninf, v[3], t);
+  const int i1 = (int)(__float_as_uint(v[0]) & P.idx_mask);
+  const int i2 = (int)(__float_as_uint(v[1]) & P.idx_mask);
+  const int i3 = (int)(__float_as_uint(v[2]) & P.idx_mask);
+  // single-product forms: ||x||^2 is added after the ranking
+  b1 += x2;
+  b2 += x2;
+  b3 += x2;
+  b4 += x2;
+  bool flag2 = false, flagA = false, few3 = false;
+  if (valid) {
+    const float xn = sqrtf(x2);
+    float E = P.alpha * xn * cmax + Ecm + P.xpk * x2;
+    if (P.hop > 0.0f) E += P.hop * (cdmax * xn + 2.0f * cmax * xd + cdmax * xd);
+    const float twoE = 2.0f * E;
+    const float b1p = fmaxf(b1 + E, 0.0f);
+    auto clear = [&](float bb) {
+      return (bb >= kFarTest) || (bb - b1 > twoE + P.gref * (fmaxf(bb + E, 0.0f) + b1p));
+    };
+    const bool uniq = clear(b2);
+    const bool two = clear(b3);
+    const bool three = clear(b4);
+    const bool sane = (b1 < kFarTest) && (i1 < P.K);
+    P.labels[prow] = sane ? i1 : 0;
+    flag2 = sane && !uniq && (two || three);
+    flagA = !sane || (!uniq && !two && !three);
+    few3 = !two;
+  }
+  const uint64_t m2 = __ballot(flag2);
+  const uint64_t mA = __ballot(flagA);
+  if (m2) {
+    const int leader = __ffsll((unsigned long long)m2) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(q2c_s, (unsigned)__popcll(m2));
+    base = __shfl(base, leader, 64);
+    if (flag2) {
+      const unsigned pos = base + mbcnt64(m2);
+      *reinterpret_cast<int4*>(q2 + 4 * (size_t)pos) =
+          make_int4((int)prow, i1, i2, few3 ? i3 : -1);
+    }
+  }
+  if (mA) {
+    const int leader = __ffsll((unsigned long long)mA) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(qac_s, (unsigned)__popcll(mA));
+    base = __shfl(base, leader, 64);
+    if (flagA) qa[base + mbcnt64(mA)] = (int)prow;
+  }
+}
+
+// Work split: the centroids are resident in LDS (one copy per workgroup);
+// workgroup g takes the contiguous 32-point units [U g / G, U (g+1) / G) and
+// splits them into contiguous runs, one per wave. A wave sweeps two units
+// (64 points) at a time: each A-operand fragment and each block's ||c||^2
+// row read from LDS feed both units' MFMA chains, which halves the LDS
+// traffic per distance against one unit per wave, and the chains of block s
+// are issued before the epilogue of block s-1 (two accumulator pairs
+// alternate). The epilogue keeps the T smallest packed values per lane
+// (1 + T VALU operations per value; band_out turns the lists into bounds).
+// The A fragments are read from LDS step by step, so the point fragments of
+// both units, two accumulator pairs and one A step fit the register budget
+// of two or three waves per SIMD.
+template <int DP, int PREC, int NW, int T, bool PF>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1)))
+void k_assign2(AssignParams P) {
+  static_assert(PREC == PREC_BF16 || PREC == PREC_FP8, "single-product forms only");
+  if (*P.done) return;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ float red_s[NW];
+  __shared__ unsigned q2c_s, qac_s;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int p = lane & 31;
+  const int h = lane >> 5;
+  if (threadIdx.x == 0) {
+    q2c_s = 0u;
+    qac_s = 0u;
+  }
+  int* const q2 = P.q2 + 4 * (size_t)blockIdx.x * P.qstride;
+  int* const qa = P.qa + (size_t)blockIdx.x * P.qstride;
+
+  const long long U = (P.n + 31) / 32;
+  const long long g0 = U * blockIdx.x / gridDim.x;
+  const long long g1 = U * (blockIdx.x + 1) / gridDim.x;
+  const long long w0 = g0 + (g1 - g0) * wave / NW;
+  const long long w1 = g0 + (g1 - g0) * (wave + 1) / NW;
+
+  // The fragments of a pair of units; lane half h also carries the row,
+  // norm and measured operand error of its set h's point (the band test of
+  // set h runs on half h). With PF the next pair is requested a whole pair
+  // ahead into a second set of registers.
+  XFrag<DP, PREC> x0, x1, n0, n1;
+  long long prow = 0, nrow = 0;
+  bool pvalid = false, nvalid = false;
+  float px2 = 0.0f, pxd = 0.0f, nx2 = 0.0f, nxd = 0.0f;
+  auto load_pair = [&](long long uu, XFrag<DP, PREC>& f0, XFrag<DP, PREC>& f1, long long& row,
+                       bool& valid, float& x2, float& xd) {
+    const long long ra = uu * 32 + p;
+    const long long rb = ra + 32;
+    const bool va = ra < P.n;
+    const bool vb = uu + 1 < w1 && rb < P.n;
+    load_xfrag<DP, PREC>(P, ra, va, h, f0);
+    load_xfrag<DP, PREC>(P, rb, vb, h, f1);
+    row = h ? rb : ra;
+    valid = h ? vb : va;
+    x2 = valid ? P.xn2[row] : 0.0f;
+    xd = (P.hop > 0.0f && valid) ? P.xdn[row] : 0.0f;
+  };
+  // the first pairs' rows are requested before the centroids are staged
+  if (w0 < w1) load_pair(w0, x0, x1, prow, pvalid, px2, pxd);
+  if (PF && w0 + 2 < w1) load_pair(w0 + 2, n0, n1, nrow, nvalid, nx2, nxd);
+
+  // max ||c|| over the live operand rows and max measured operand error of
+  // the centroid rows (for the error bound)
+  float cm = 0.0f, cd = 0.0f;
+  for (int c = threadIdx.x; c < P.Kp; c += blockDim.x) {
+    const float v = P.cn[c];
+    if (c < P.K && v < kFarTest) cm = fmaxf(cm, v);  // excluded clusters carry kFar
+    if (P.hop > 0.0f) cd = fmaxf(cd, P.cdn[c]);
+  }
+  cm = wave_max_f32(cm);
+  cd = wave_max_f32(cd);
+  if (lane == 0) red_s[wave] = cm;
+  __syncthreads();
+  float cmax = 0.0f;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) cmax = fmaxf(cmax, red_s[w]);
+  cmax = sqrtf(cmax);
+  __syncthreads();  // red_s reused
+  if (lane == 0) red_s[wave] = cd;
+  stage_centroids<DP, PREC>(P, smem, 0, P.Kp);
+  __syncthreads();
+  float cdmax = 0.0f;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) cdmax = fmaxf(cdmax, red_s[w]);
+  const float Ecm = P.beta * cmax * cmax;
+  const uint32_t keepv = opaque_vgpr(~P.idx_mask);
+  const float ninf = __uint_as_float(opaque_vgpr(0xff800000u));
+
+  const float* cn_s = reinterpret_cast<const float*>(smem);
+  const unsigned char* rows = smem + (size_t)P.Kp * 4;
+  const int nblk = P.Kp / 32;  // even: Kp is a multiple of 64
+
+  for (long long u = w0; u < w1; u += 2) {
+    float ba[T], bb[T];
+#pragma unroll
+    for (int k = 0; k < T; ++k) {
+      ba[k] = kFar;
+      bb[k] = kFar;
+    }
+    // Block step: the block's LDS reads are issued first; the first half of
+    // the previous block's epilogue (set 0's values) runs while they are in
+    // flight; then the block's MFMA chains, interleaved with the second half
+    // (set 1's values). Scheduling barriers keep the compiler from moving the
+    // reads next to their use or the epilogue behind the wait for them.
+    auto step = [&](int blk, kmc_f32x16& c0, kmc_f32x16& c1, const kmc_f32x16& p0,
+                    const kmc_f32x16& p1) {
+      ABlock<DP, PREC> ab;
+      load_ablock<DP, PREC>(rows, cn_s, 32 * blk, p, h, ab);
+      __builtin_amdgcn_sched_barrier(0);
+      if (blk > 0) push_blockT<T>(p0, (uint32_t)(32 * (blk - 1)), keepv, ba, ninf);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_chain2<DP, PREC>(ab, x0, x1, c0, c1);
+      if (blk > 0) push_blockT<T>(p1, (uint32_t)(32 * (blk - 1)), keepv, bb, ninf);
+    };
+    kmc_f32x16 pA0, pA1, pB0, pB1;
+    step(0, pA0, pA1, pB0, pB1);
+    int blk = 1;
+    for (; blk + 1 < nblk; blk += 2) {
+      step(blk, pB0, pB1, pA0, pA1);
+      step(blk + 1, pA0, pA1, pB0, pB1);
+    }
+    step(blk, pB0, pB1, pA0, pA1);  // blk == nblk - 1: the last use of this pair's fragments
+    const long long crow = prow;
+    const bool cvalid = pvalid;
+    const float cx2 = px2, cxd = pxd;
+    if constexpr (PF) {
+      // the pair after next is requested now; the next one is already in
+      // flight or here
+      x0 = n0;
+      x1 = n1;
+      prow = nrow;
+      pvalid = nvalid;
+      px2 = nx2;
+      pxd = nxd;
+      if (u + 4 < w1) load_pair(u + 4, n0, n1, nrow, nvalid, nx2, nxd);
+    } else {
+      // the next pair's rows are requested now, so that their memory latency
+      // passes while the last epilogues and the band test of this pair run
+      if (u + 2 < w1) load_pair(u + 2, x0, x1, prow, pvalid, px2, pxd);
+    }
+    push_blockT<T>(pB0, (uint32_t)(32 * blk), keepv, ba, ninf);
+    push_blockT<T>(pB1, (uint32_t)(32 * blk), keepv, bb, ninf);
+    band_out2<T>(P, ba, bb, cx2, cxd, cvalid, crow, cmax, Ecm, cdmax, ninf, h, lane, q2, qa,
+                 &q2c_s, &qac_s);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    P.q2cnt[blockIdx.x] = q2c_s;
+    P.qacnt[blockIdx.x] = qac_s;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Refinement: reference float arithmetic, first-min rule.
 //
 // The assign workgroups leave their band points in per-workgroup queue
@@ -3334,33 +3737,49 @@ size_t assign_lds(int DP, int prec, int kchunk) {
 #undef KMC_LDS_CASE
 }
 
-// Launch variants of k_assign: (waves per workgroup, point sets per wave,
-// prefetch, waves per SIMD the register allocation targets; 0 = none).
-// Two point sets per wave need both fragment sets in registers; where they
-// do not fit (kFragRegs > 64) the variant is not built. Targets above four
-// waves per SIMD select the register-lean loop and let two workgroups share
-// a CU when their centroid copies fit the LDS together.
+// Launch variants of the distance kernel: (waves per workgroup, point sets
+// per wave, prefetch, waves per SIMD the register allocation targets (0 =
+// none), kernel (0 = k_assign, 1 = the two-set k_assign2), values kept per
+// lane). Two point sets per wave need both fragment sets in registers; where
+// they do not fit (kFragRegs > 64) the variant is not built. Targets above
+// four waves per SIMD select the register-lean loop and let two workgroups
+// share a CU when their centroid copies fit the LDS together. k_assign2 is
+// built for the single-product forms only (bf16 up to DP = 128, fp8) and
+// needs every centroid resident in LDS.
 struct VariantShape {
   int nw;
   int ms;
   bool pf;
   int wpe;
+  int two;
+  int topk;
 };
-constexpr VariantShape kVariants[] = {{8, 1, true, 0},   {8, 1, false, 0}, {12, 1, false, 0},
-                                      {16, 1, false, 0}, {12, 1, false, 6}, {4, 2, false, 0},
-                                      {8, 2, false, 0},  {8, 1, false, 8}};
+constexpr VariantShape kVariants[] = {
+    {8, 1, true, 0, 0, 4},  {8, 1, false, 0, 0, 4},  {12, 1, false, 0, 0, 4},
+    {16, 1, false, 0, 0, 4}, {12, 1, false, 6, 0, 4}, {4, 2, false, 0, 0, 4},
+    {8, 2, false, 0, 0, 4},  {8, 1, false, 8, 0, 4},  {8, 2, false, 0, 1, 4},
+    {8, 2, false, 0, 1, 3},  {4, 2, true, 0, 1, 4},   {4, 2, true, 0, 1, 3},
+    {4, 2, false, 0, 1, 4},  {8, 2, false, 0, 1, 2}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static_assert(kNumVariants <= KMC_MAX_VARIANTS, "variant table exceeds kmc_tuning");
 
 template <int DP, int PREC, int V>
 static const void* kernel_of() {
   constexpr VariantShape S = kVariants[V];
-  // a register target leaves room for the point fragments plus ~40 others
-  constexpr bool kFits = S.wpe == 0 || frag_regs<DP, PREC>() * S.ms + 40 <= 512 / S.wpe;
-  if constexpr ((S.ms > 1 && frag_regs<DP, PREC>() > 64) || !kFits) {
-    return nullptr;
+  if constexpr (S.two) {
+    constexpr bool kBuilt = (PREC == PREC_BF16 && DP <= 128) || PREC == PREC_FP8;
+    if constexpr (kBuilt)
+      return reinterpret_cast<const void*>(k_assign2<DP, PREC, S.nw, S.topk, S.pf>);
+    else
+      return nullptr;
   } else {
-    return reinterpret_cast<const void*>(k_assign<DP, PREC, S.nw, S.ms, S.pf, S.wpe>);
+    // a register target leaves room for the point fragments plus ~40 others
+    constexpr bool kFits = S.wpe == 0 || frag_regs<DP, PREC>() * S.ms + 40 <= 512 / S.wpe;
+    if constexpr ((S.ms > 1 && frag_regs<DP, PREC>() > 64) || !kFits) {
+      return nullptr;
+    } else {
+      return reinterpret_cast<const void*>(k_assign<DP, PREC, S.nw, S.ms, S.pf, S.wpe>);
+    }
   }
 }
 
@@ -3375,6 +3794,12 @@ static const void* kernel_of_v(int v) {
     case 5: return kernel_of<DP, PREC, 5>();
     case 6: return kernel_of<DP, PREC, 6>();
     case 7: return kernel_of<DP, PREC, 7>();
+    case 8: return kernel_of<DP, PREC, 8>();
+    case 9: return kernel_of<DP, PREC, 9>();
+    case 10: return kernel_of<DP, PREC, 10>();
+    case 11: return kernel_of<DP, PREC, 11>();
+    case 12: return kernel_of<DP, PREC, 12>();
+    case 13: return kernel_of<DP, PREC, 13>();
     default: return nullptr;
   }
 }
@@ -3425,6 +3850,7 @@ hipError_t raise_lds_limit(const void* fn, size_t bytes) {
 }
 
 int assign_variants() { return kNumVariants; }
+bool assign_variant_resident_only(int v) { return kVariants[v].two != 0; }
 int assign_threads(int v) { return 64 * kVariants[v].nw; }
 int assign_tile(int v) { return 32 * kVariants[v].ms * kVariants[v].nw; }
 

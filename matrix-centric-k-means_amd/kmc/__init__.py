@@ -71,7 +71,7 @@ def _stage_rows(fn, h, check):
     return [(a.distances_ms, a.argmin_ms, a.centroids_ms) for a in arr[:cnt.value]]
 
 
-MAX_VARIANTS = 8
+MAX_VARIANTS = 16
 
 
 class _Tuning(ctypes.Structure):

@@ -132,10 +132,13 @@ def test_bench_variant_table_matches_the_kernel_source():
     source (same shapes, same order as kmc_autotune reports them)."""
     src = open(os.path.join(ROOT, "matrix-centric-k-means_amd", "csrc", "kmc_lloyd.hip")).read()
     body = re.search(r"kVariants\[\]\s*=\s*\{(.*?)\};", src, flags=re.S).group(1)
-    shapes = [(int(a), int(b), c == "true", int(d))
-              for a, b, c, d in re.findall(r"\{\s*(\d+),\s*(\d+),\s*(true|false),\s*(\d+)\s*\}", body)]
+    shapes = [(int(a), int(b), c == "true", int(d), int(e), int(f))
+              for a, b, c, d, e, f in re.findall(
+                  r"\{\s*(\d+),\s*(\d+),\s*(true|false),\s*(\d+),\s*(\d+),\s*(\d+)\s*\}",
+                  body)]
     import sys
     sys.path.insert(0, ROOT)
     import bench
     assert shapes == bench.ASSIGN_VARIANTS
     assert bench.assign_kernel_name(128, "bf16", 1) == "k_assign<128, 2, 8, 1, false, 0, false>"
+    assert bench.assign_kernel_name(128, "bf16", 13) == "k_assign2<128, 2, 8, 2, false>"
