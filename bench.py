@@ -188,8 +188,8 @@ def cpu_baseline(X, C0, n_sample=100_000, iters=2):
 ASSIGN_VARIANTS = [(8, 1, True, 0, 0, 4), (8, 1, False, 0, 0, 4), (12, 1, False, 0, 0, 4),
                    (16, 1, False, 0, 0, 4), (12, 1, False, 6, 0, 4), (4, 2, False, 0, 0, 4),
                    (8, 2, False, 0, 0, 4), (8, 1, False, 8, 0, 4), (8, 2, False, 0, 1, 4),
-                   (8, 2, False, 0, 1, 3), (4, 2, True, 0, 1, 4), (4, 2, True, 0, 1, 3),
-                   (4, 2, False, 0, 1, 4), (8, 2, False, 0, 1, 2)]
+                   (8, 2, False, 0, 1, 3), (6, 2, False, 3, 2, 4), (6, 2, False, 3, 1, 4),
+                   (4, 2, False, 0, 1, 4), (4, 2, False, 0, 2, 4)]
 FORM_CODE = {"fp32": 0, "bf16x3": 1, "bf16": 2, "fp8": 3}
 
 
@@ -198,8 +198,9 @@ def assign_kernel_name(dp, prec, variant):
     profiler names it (without the return type and parameters)."""
     nw, ms, pf, wpe, two, topk = ASSIGN_VARIANTS[variant]
     if two:
-        return "k_assign2<%d, %d, %d, %d, %s>" % (dp, FORM_CODE[prec], nw, topk,
-                                                 "true" if pf else "false")
+        return "k_assign2<%d, %d, %d, %d, %s, %s, %d>" % (
+            dp, FORM_CODE[prec], nw, topk, "true" if pf else "false",
+            "true" if two == 1 else "false", wpe)
     return "k_assign<%d, %d, %d, %d, %s, %d, false>" % (dp, FORM_CODE[prec], nw, ms,
                                                         "true" if pf else "false", wpe)
 
@@ -344,10 +345,11 @@ def bench_batch(a):
     forms = {}
     for prec in (a.prec, "fp32") if a.prec != "fp32" else ("fp32",):
         with kmc.Batch(probs, tol=0.0, precision=prec) as b:
-            b.run(min(a.warmup, steps), check_converged=False)
+            graph = not a.eager
+            b.run(min(a.warmup, steps), check_converged=False, graph=graph)
             sync()
             t0 = time.perf_counter()
-            res, ms = b.run(steps, check_converged=False)
+            res, ms = b.run(steps, check_converged=False, graph=graph)
             sync()
             wall = time.perf_counter() - t0
             its = sum(int(r["iterations"]) for r in res)
@@ -355,7 +357,15 @@ def bench_batch(a):
                            "batch_iters_per_s": round(steps / wall, 3),
                            "ms_per_batch_iteration": round(wall * 1e3 / steps, 4),
                            "device_ms": round(ms, 3),
-                           "distance_tflops": round(flops * steps / wall / 1e12, 2)}
+                           "distance_tflops": round(flops * steps / wall / 1e12, 2),
+                           "graph": graph}
+            if graph:
+                # the same steps launched eagerly, for comparison
+                t0 = time.perf_counter()
+                b.run(steps, check_converged=False)
+                sync()
+                forms[prec]["eager_ms_per_batch_iteration"] = round(
+                    (time.perf_counter() - t0) * 1e3 / steps, 4)
             log("batch", prec, forms[prec])
     # unbatched comparison: the first 16 problems, one engine each
     sample = probs[:16]

@@ -50,6 +50,13 @@ struct DeviceInfo {
 // Fills *info for device `ordinal`; returns false when no device exists.
 bool query_device(int ordinal, DeviceInfo* info);
 
+// The reference CPU k-means' initial centroids: k distinct rows of the
+// points drawn with mt19937(seed_seq{*seed}) (random_device when seed is
+// null), comparisons/CPU_Kmeans/src/kmeans.cpp:57-64,84-119. X is the n x d
+// row-major copy of the points; returns k x d floats.
+std::vector<float> seeded_rows(Point<DATA_TYPE>** points, const std::vector<float>& X, size_t n,
+                               uint32_t d, uint32_t k, const int* seed);
+
 }  // namespace kmc
 
 class Kmeans {

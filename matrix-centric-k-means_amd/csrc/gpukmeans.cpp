@@ -15,6 +15,12 @@
 //   --no-stage-times                  omit the [PERFORMANCE] stage-time lines
 //                                     (printed by default, as the reference's
 //                                     harness expects them)
+//   --batch-runs                      run the -r repetitions together as one
+//                                     batch (kmc_batch_*: one launch per stage
+//                                     for all of them, each chunk of iterations
+//                                     replayed as a HIP graph); needs
+//                                     --cpu-semantics, -b random and the linear
+//                                     kernel
 #include <cfloat>
 #include <chrono>
 #include <cstdio>
@@ -50,7 +56,7 @@ const Flag kFlags[] = {
     {"t", "tolerance", true},    {"c", "check", true},        {"b", "init", true},
     {"f", "kernel", true},       {"l", "level", true},        {"p", "precision", true},
     {nullptr, "label-col", true}, {nullptr, "cpu-semantics", false},
-    {nullptr, "no-stage-times", false}};
+    {nullptr, "no-stage-times", false}, {nullptr, "batch-runs", false}};
 
 void usage() {
   std::cout
@@ -73,7 +79,8 @@ void usage() {
          "  -p, --precision arg      fp32 | bf16x3 | bf16 | fp8 (default: fp32)\n"
          "      --label-col arg      first | last | none (default: first)\n"
          "      --cpu-semantics      reference CPU k-means semantics\n"
-         "      --no-stage-times     omit the [PERFORMANCE] stage-time lines\n";
+         "      --no-stage-times     omit the [PERFORMANCE] stage-time lines\n"
+         "      --batch-runs         run the -r repetitions as one batch\n";
 }
 
 [[noreturn]] void missing(const char* arg) {
@@ -206,6 +213,82 @@ void write_csv(std::ostream& o, const std::vector<Point<float>*>& pts, int d) {
   }
 }
 
+// The -r repetitions as one batch (the reference's loop is src/main.cu:87-115:
+// one Kmeans object and one run per repetition, the first a warm-up). Every
+// repetition is one problem of a kmc_batch with the same points and the same
+// seeded rows, so all of them run in one launch per stage; the batch is run
+// once untimed (the warm-up), then timed, and each repetition is charged an
+// equal share of the timed run. The batch follows the reference CPU k-means'
+// rules (shift test, empty cluster -> zero vector) with FP64 centroid sums
+// rounded once. Prints the sequential loop's report lines.
+int batch_runs(std::vector<Point<float>*>& pts, int d, uint32_t k, float tol, int seed,
+               uint32_t runs, uint64_t maxiter, bool check_converged, const std::string& prec,
+               const kmc::DeviceInfo& dev) {
+  const size_t n = pts.size();
+  std::vector<float> X(n * (size_t)d);
+  for (size_t i = 0; i < n; ++i)
+    for (int j = 0; j < d; ++j) X[i * d + j] = pts[i]->get(j);
+  std::vector<float> C0;
+  try {
+    C0 = kmc::seeded_rows(pts.data(), X, n, (uint32_t)d, k, &seed);
+  } catch (const kmc::Error& e) {
+    std::cerr << e.what() << std::endl;
+    return 1;
+  }
+  double sum_x2 = 0.0;  // the score is the inertia less sum ||x||^2
+  for (float v : X) sum_x2 += (double)v * (double)v;
+  std::vector<kmc_problem> probs(runs);
+  for (uint32_t r = 0; r < runs; ++r) probs[r] = {n, (uint32_t)d, k, X.data(), (size_t)d, C0.data()};
+  kmc_batch_config cfg{};
+  cfg.tol = tol;
+  cfg.precision = prec == "bf16x3" ? KMC_PREC_BF16X3 : prec == "bf16" ? KMC_PREC_BF16
+                  : prec == "fp8"  ? KMC_PREC_FP8 : KMC_PREC_FP32;
+  cfg.device = dev.ordinal;
+  kmc_batch* b = nullptr;
+  const auto i0 = std::chrono::high_resolution_clock::now();
+  kmc_status st = kmc_batch_create(probs.data(), runs, &cfg, &b);
+  const auto i1 = std::chrono::high_resolution_clock::now();
+  if (st != KMC_OK) {
+    std::cerr << "kmc_batch_create: " << kmc_status_string(st) << std::endl;
+    return st == KMC_ERR_NO_DEVICE ? EXIT_DEV : 1;
+  }
+  const uint32_t flags = (check_converged ? KMC_RUN_CHECK_CONVERGED : 0u) | KMC_RUN_GRAPH;
+  std::vector<kmc_batch_result> res(runs);
+  st = kmc_batch_run(b, maxiter, flags, res.data(), nullptr);  // warm-up
+  const auto t0 = std::chrono::high_resolution_clock::now();
+  if (st == KMC_OK) st = kmc_batch_run(b, maxiter, flags, res.data(), nullptr);
+  const auto t1 = std::chrono::high_resolution_clock::now();
+  std::vector<int32_t> lab(n);
+  if (st == KMC_OK) st = kmc_batch_get_labels(b, 0, lab.data());
+  if (st != KMC_OK) {
+    const char* m = kmc_batch_last_error(b);
+    std::cerr << "kmc_batch_run: " << (m ? m : kmc_status_string(st)) << std::endl;
+    kmc_batch_destroy(b);
+    return 1;
+  }
+  kmc_batch_destroy(b);
+  const double per_run = std::chrono::duration<double>(t1 - t0).count() / runs;
+  const double init_per_run = std::chrono::duration<double>(i1 - i0).count() / runs;
+  printf("%s", BOLDBLUE);
+  double score = 0;
+  for (uint32_t r = 0; r < runs; ++r) {
+    if (res[r].converged_at < maxiter)
+      printf("K-means converged at iteration %lu - ", (unsigned long)res[r].converged_at);
+    else
+      printf("K-means did NOT converge - ");
+    printf("Time: %lf\n", per_run);
+    const float s = (float)(res[r].inertia - sum_x2);
+    printf("Objective score: %lf\n", s);
+    score += s;
+  }
+  printf("GPU_Kmeans: %lfs (%u runs)\n", per_run, runs);
+  printf("Init_time: %lfs \n", init_per_run);
+  printf("Score: %lf\n", score / runs);
+  printf("%s", RESET);
+  for (size_t i = 0; i < n; ++i) pts[i]->setCluster((unsigned int)lab[i]);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -290,6 +373,22 @@ int main(int argc, char** argv) {
   else if (prec == "fp8") opts.precision = Kmeans::Precision::fp8;
   else if (prec == "bf16") opts.precision = Kmeans::Precision::bf16;
   else missing("precision");
+
+  if (args.count("batch-runs")) {
+    if (kf != Kmeans::Kernel::linear || opts.reference_gpu || im != Kmeans::InitMethod::random) {
+      std::cerr << "--batch-runs needs the linear kernel, --cpu-semantics and -b random"
+                << std::endl;
+      return EXIT_ARGS;
+    }
+    const int st = batch_runs(pts, d, k, tol, seed, runs, maxiter, check_converged, prec, dev);
+    if (st != 0) return st;
+    if (out_file != "None") {
+      std::ofstream fout(out_file);
+      write_csv(fout, pts, d);
+    }
+    for (Point<float>* p : pts) delete p;
+    return 0;
+  }
 
   printf("%s", BOLDBLUE);
   double tot_time = 0, init_time = 0, score = 0;

@@ -379,14 +379,24 @@ struct BatchGroup {
   int *item_prob = nullptr, *blk_prob = nullptr, *clus_prob = nullptr, *row_prob = nullptr;
 };
 
+// A captured chunk of batch iterations: every group's iterations as one
+// branch of the graph (the groups are independent).
+struct BatchGraph {
+  uint64_t iters = 0, maxiter = 0;
+  int check = 0;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct kmc_batch {
   kmc_batch_config cfg{};
   int dev = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr;  // capture: the groups' branches start here
   std::string err;
   uint32_t nprob = 0;
   std::vector<BatchGroup*> groups;
   std::vector<int> grp_of, loc_of;  // group of each problem, index inside it
+  std::vector<BatchGraph> graphs;
 };
 
 namespace {
@@ -433,6 +443,11 @@ void gfree_all(BatchGroup* g) {
 }
 
 void bfree_all(kmc_batch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (BatchGraph& gr : b->graphs) (void)hipGraphExecDestroy(gr.exec);
+  b->graphs.clear();
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  b->fork = nullptr;
   for (BatchGroup* g : b->groups) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     gfree_all(g);
@@ -593,6 +608,53 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   return KMC_OK;
 }
 
+
+// Replays (capturing it on first use) a graph of m iterations of every group
+// on the batch stream: the capture forks each group's stream from the batch
+// stream and joins it back, so the groups' iterations are parallel branches.
+// A group whose device stop flag is set makes its remaining kernels no-ops,
+// as in the eager launches. KMC_ERR_UNSUPPORTED when the runtime refuses the
+// capture (the caller then launches eagerly).
+kmc_status launch_batch_graph(kmc_batch* b, uint64_t m, uint64_t maxiter, int check) {
+  for (const BatchGraph& gr : b->graphs) {
+    if (gr.iters == m && gr.maxiter == maxiter && gr.check == check) {
+      KMC_BCK(hipGraphLaunch(gr.exec, b->stream));
+      return KMC_OK;
+    }
+  }
+  if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return KMC_ERR_UNSUPPORTED;
+  kmc_status st = KMC_OK;
+  if (hipEventRecord(b->fork, b->stream) != hipSuccess) st = KMC_ERR_UNSUPPORTED;
+  for (BatchGroup* g : b->groups) {
+    if (st != KMC_OK) break;
+    if (hipStreamWaitEvent(g->stream, b->fork, 0) != hipSuccess) {
+      st = KMC_ERR_UNSUPPORTED;
+      break;
+    }
+    for (uint64_t i = 0; i < m && st == KMC_OK; ++i) st = enqueue_iteration(b, g, maxiter, check);
+    if (st == KMC_OK && (hipEventRecord(g->end, g->stream) != hipSuccess ||
+                         hipStreamWaitEvent(b->stream, g->end, 0) != hipSuccess))
+      st = KMC_ERR_UNSUPPORTED;
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->stream, &graph);
+  if (st != KMC_OK || ec != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return KMC_ERR_UNSUPPORTED;
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    (void)hipGetLastError();
+    return KMC_ERR_UNSUPPORTED;
+  }
+  b->graphs.push_back({m, maxiter, check, exec});
+  KMC_BCK(hipGraphLaunch(exec, b->stream));
+  return KMC_OK;
+}
 
 // Packs one width class: tables, launch shape, buffers, points and initial
 // centroids, norms and bf16 planes.
@@ -804,6 +866,7 @@ kmc_status kmc_batch_create(const kmc_problem* probs, uint32_t count,
   auto body = [&]() -> kmc_status {
     KMC_BCK(hipSetDevice(b->dev));
     KMC_BCK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    KMC_BCK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     // one group per padded width, problems in batch order inside a group
     const uint32_t classes[] = {16, 32, 64, 128, 256};
     for (uint32_t dp : classes) {
@@ -870,6 +933,14 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
   }
   kmc_status st = KMC_OK;
   const size_t ng = b->groups.size();
+  bool use_graph = (flags & KMC_RUN_GRAPH) != 0;
+  if (use_graph) {
+    // the graph runs on the batch stream: it starts after every group's set-up
+    for (BatchGroup* g : b->groups) {
+      KMC_BCK(hipEventRecord(g->end, g->stream));
+      KMC_BCK(hipStreamWaitEvent(b->stream, g->end, 0));
+    }
+  }
   if (maxiter > 0) {
     // chunks of iterations between host looks at the stop flags (a group's
     // device flag makes its queued iterations after the stop no-ops)
@@ -878,17 +949,37 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
     uint64_t launched = 0;
     while (launched < maxiter && st == KMC_OK) {
       const uint64_t m = std::min<uint64_t>(16, maxiter - launched);
-      for (uint64_t i = 0; i < m && st == KMC_OK; ++i)
-        for (size_t gi = 0; gi < ng && st == KMC_OK; ++gi)
-          if (!gdone[gi]) st = enqueue_iteration(b, b->groups[gi], maxiter, check);
+      if (use_graph) {
+        st = launch_batch_graph(b, m, maxiter, check);
+        if (st == KMC_ERR_UNSUPPORTED) {
+          // capture refused: launch eagerly from here on, each group on its
+          // own stream after the batch stream's work so far
+          use_graph = false;
+          st = KMC_OK;
+          KMC_BCK(hipEventRecord(b->fork, b->stream));
+          for (BatchGroup* g : b->groups) KMC_BCK(hipStreamWaitEvent(g->stream, b->fork, 0));
+        }
+        if (st != KMC_OK) break;
+        if (use_graph) {
+          for (size_t gi = 0; gi < ng; ++gi)
+            KMC_BCK(hipMemcpyAsync(&hs[gi], b->groups[gi]->state, sizeof(DevState),
+                                   hipMemcpyDeviceToHost, b->stream));
+          KMC_BCK(hipStreamSynchronize(b->stream));
+        }
+      }
+      if (!use_graph) {
+        for (uint64_t i = 0; i < m && st == KMC_OK; ++i)
+          for (size_t gi = 0; gi < ng && st == KMC_OK; ++gi)
+            if (!gdone[gi]) st = enqueue_iteration(b, b->groups[gi], maxiter, check);
+        if (st != KMC_OK) break;
+        for (size_t gi = 0; gi < ng; ++gi)
+          if (!gdone[gi])
+            KMC_BCK(hipMemcpyAsync(&hs[gi], b->groups[gi]->state, sizeof(DevState),
+                                   hipMemcpyDeviceToHost, b->groups[gi]->stream));
+        for (size_t gi = 0; gi < ng; ++gi)
+          if (!gdone[gi]) KMC_BCK(hipStreamSynchronize(b->groups[gi]->stream));
+      }
       launched += m;
-      if (st != KMC_OK) break;
-      for (size_t gi = 0; gi < ng; ++gi)
-        if (!gdone[gi])
-          KMC_BCK(hipMemcpyAsync(&hs[gi], b->groups[gi]->state, sizeof(DevState),
-                                 hipMemcpyDeviceToHost, b->groups[gi]->stream));
-      for (size_t gi = 0; gi < ng; ++gi)
-        if (!gdone[gi]) KMC_BCK(hipStreamSynchronize(b->groups[gi]->stream));
       bool all = true;
       for (size_t gi = 0; gi < ng; ++gi) {
         if (!gdone[gi] && hs[gi].done) gdone[gi] = 1;
@@ -896,6 +987,11 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
       }
       if (all) break;
     }
+  }
+  if (st == KMC_OK && use_graph) {
+    // the per-group tails below run on the groups' streams after the graphs
+    KMC_BCK(hipEventRecord(b->fork, b->stream));
+    for (BatchGroup* g : b->groups) KMC_BCK(hipStreamWaitEvent(g->stream, b->fork, 0));
   }
   if (st == KMC_OK) {
     for (BatchGroup* g : b->groups) {

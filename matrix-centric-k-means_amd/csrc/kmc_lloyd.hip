@@ -1051,8 +1051,8 @@ __device__ __forceinline__ void band_out2(const AssignParams& P, const float (&l
 // The A fragments are read from LDS step by step, so the point fragments of
 // both units, two accumulator pairs and one A step fit the register budget
 // of two or three waves per SIMD.
-template <int DP, int PREC, int NW, int T, bool PF>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1)))
+template <int DP, int PREC, int NW, int T, bool PF, bool PIPE, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_assign2(AssignParams P) {
   static_assert(PREC == PREC_BF16 || PREC == PREC_FP8, "single-product forms only");
   if (*P.done) return;
@@ -1156,13 +1156,28 @@ void k_assign2(AssignParams P) {
       if (blk > 0) push_blockT<T>(p1, (uint32_t)(32 * (blk - 1)), keepv, bb, ninf);
     };
     kmc_f32x16 pA0, pA1, pB0, pB1;
-    step(0, pA0, pA1, pB0, pB1);
     int blk = 1;
-    for (; blk + 1 < nblk; blk += 2) {
-      step(blk, pB0, pB1, pA0, pA1);
-      step(blk + 1, pA0, pA1, pB0, pB1);
+    if constexpr (PIPE) {
+      step(0, pA0, pA1, pB0, pB1);
+      for (; blk + 1 < nblk; blk += 2) {
+        step(blk, pB0, pB1, pA0, pA1);
+        step(blk + 1, pA0, pA1, pB0, pB1);
+      }
+      step(blk, pB0, pB1, pA0, pA1);  // blk == nblk - 1: the last use of this pair's fragments
+    } else {
+      // register-lean form: each block's values are ranked right after its
+      // chains; the other waves of the SIMD fill the gaps
+      for (blk = 0; blk + 1 < nblk; ++blk) {
+        ABlock<DP, PREC> ab;
+        load_ablock<DP, PREC>(rows, cn_s, 32 * blk, p, h, ab);
+        mfma_chain2<DP, PREC>(ab, x0, x1, pA0, pA1);
+        push_blockT<T>(pA0, (uint32_t)(32 * blk), keepv, ba, ninf);
+        push_blockT<T>(pA1, (uint32_t)(32 * blk), keepv, bb, ninf);
+      }
+      ABlock<DP, PREC> ab;  // blk == nblk - 1: the last use of this pair's fragments
+      load_ablock<DP, PREC>(rows, cn_s, 32 * blk, p, h, ab);
+      mfma_chain2<DP, PREC>(ab, x0, x1, pB0, pB1);
     }
-    step(blk, pB0, pB1, pA0, pA1);  // blk == nblk - 1: the last use of this pair's fragments
     const long long crow = prow;
     const bool cvalid = pvalid;
     const float cx2 = px2, cxd = pxd;
@@ -3739,8 +3754,8 @@ size_t assign_lds(int DP, int prec, int kchunk) {
 
 // Launch variants of the distance kernel: (waves per workgroup, point sets
 // per wave, prefetch, waves per SIMD the register allocation targets (0 =
-// none), kernel (0 = k_assign, 1 = the two-set k_assign2), values kept per
-// lane). Two point sets per wave need both fragment sets in registers; where
+// none), kernel (0 = k_assign, 1 = the two-set k_assign2, 2 = its form
+// without the block software pipeline), values kept per lane). Two point sets per wave need both fragment sets in registers; where
 // they do not fit (kFragRegs > 64) the variant is not built. Targets above
 // four waves per SIMD select the register-lean loop and let two workgroups
 // share a CU when their centroid copies fit the LDS together. k_assign2 is
@@ -3758,8 +3773,8 @@ constexpr VariantShape kVariants[] = {
     {8, 1, true, 0, 0, 4},  {8, 1, false, 0, 0, 4},  {12, 1, false, 0, 0, 4},
     {16, 1, false, 0, 0, 4}, {12, 1, false, 6, 0, 4}, {4, 2, false, 0, 0, 4},
     {8, 2, false, 0, 0, 4},  {8, 1, false, 8, 0, 4},  {8, 2, false, 0, 1, 4},
-    {8, 2, false, 0, 1, 3},  {4, 2, true, 0, 1, 4},   {4, 2, true, 0, 1, 3},
-    {4, 2, false, 0, 1, 4},  {8, 2, false, 0, 1, 2}};
+    {8, 2, false, 0, 1, 3},  {6, 2, false, 3, 2, 4},  {6, 2, false, 3, 1, 4},
+    {4, 2, false, 0, 1, 4},  {4, 2, false, 0, 2, 4}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static_assert(kNumVariants <= KMC_MAX_VARIANTS, "variant table exceeds kmc_tuning");
 
@@ -3769,7 +3784,8 @@ static const void* kernel_of() {
   if constexpr (S.two) {
     constexpr bool kBuilt = (PREC == PREC_BF16 && DP <= 128) || PREC == PREC_FP8;
     if constexpr (kBuilt)
-      return reinterpret_cast<const void*>(k_assign2<DP, PREC, S.nw, S.topk, S.pf>);
+      return reinterpret_cast<const void*>(
+          k_assign2<DP, PREC, S.nw, S.topk, S.pf, S.two == 1, S.wpe>);
     else
       return nullptr;
   } else {

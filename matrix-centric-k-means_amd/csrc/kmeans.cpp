@@ -167,33 +167,43 @@ void Kmeans::init(const int* seed) {
     for (size_t i = 0; i < n; ++i) lab[i] = (int32_t)(i % k);
     check(h, kmc_set_labels(h, lab.data()), "kmc_set_labels");
   } else {
-    // k distinct rows drawn with mt19937(seed_seq{seed}) as the reference
-    // CPU k-means does (comparisons/CPU_Kmeans/src/kmeans.cpp:57-64,84-119).
-    std::mt19937 gen;
-    if (seed) {
-      std::seed_seq ss{*seed};
-      gen.seed(ss);
-    } else {
-      std::random_device rd;
-      gen.seed(rd());
-    }
-    std::uniform_int_distribution<int> pick(0, (int)n - 1);
-    std::vector<size_t> used;
-    std::vector<float> C0((size_t)k * d);
-    uint64_t draws = 0;
-    const uint64_t max_draws = 64ull * n + 1024ull * k;
-    while (used.size() < k) {
-      if (draws++ > max_draws) throw kmc::Error(KMC_ERR_ARG, "Kmeans: fewer than k distinct points");
-      const size_t r = (size_t)pick(gen);
-      bool dup = false;
-      for (size_t u : used)
-        if (*points[u] == *points[r]) { dup = true; break; }
-      if (dup) continue;
-      for (uint32_t j = 0; j < d; ++j) C0[used.size() * d + j] = X[r * d + j];
-      used.push_back(r);
-    }
+    // k distinct rows drawn as the reference CPU k-means does
+    const std::vector<float> C0 = kmc::seeded_rows(points, X, n, d, k, seed);
     check(h, kmc_set_centroids(h, C0.data()), "kmc_set_centroids");
   }
+}
+
+std::vector<float> kmc::seeded_rows(Point<DATA_TYPE>** points, const std::vector<float>& X,
+                                    size_t n, uint32_t d, uint32_t k, const int* seed) {
+  // mt19937(seed_seq{seed}) and distinct rows, as
+  // comparisons/CPU_Kmeans/src/kmeans.cpp:57-64,84-119
+  std::mt19937 gen;
+  if (seed) {
+    std::seed_seq ss{*seed};
+    gen.seed(ss);
+  } else {
+    std::random_device rd;
+    gen.seed(rd());
+  }
+  std::uniform_int_distribution<int> pick(0, (int)n - 1);
+  std::vector<size_t> used;
+  std::vector<float> C0((size_t)k * d);
+  uint64_t draws = 0;
+  const uint64_t max_draws = 64ull * n + 1024ull * k;
+  while (used.size() < k) {
+    if (draws++ > max_draws) throw kmc::Error(KMC_ERR_ARG, "Kmeans: fewer than k distinct points");
+    const size_t r = (size_t)pick(gen);
+    bool dup = false;
+    for (size_t u : used)
+      if (*points[u] == *points[r]) {
+        dup = true;
+        break;
+      }
+    if (dup) continue;
+    for (uint32_t j = 0; j < d; ++j) C0[used.size() * d + j] = X[r * d + j];
+    used.push_back(r);
+  }
+  return C0;
 }
 
 Kmeans::~Kmeans() {

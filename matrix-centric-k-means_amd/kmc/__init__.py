@@ -535,12 +535,14 @@ class Batch:
     def __exit__(self, *exc):
         self.close()
 
-    def run(self, maxiter, check_converged=True):
+    def run(self, maxiter, check_converged=True, graph=False):
+        """Run every problem from its initial centroids; graph=True replays
+        each chunk of iterations (all width groups) as one captured HIP graph."""
         res = (_BatchResult * len(self.shapes))()
         ms = ctypes.c_double(0.0)
-        self._check(self._lib.kmc_batch_run(self._h, int(maxiter),
-                                            RUN_CHECK_CONVERGED if check_converged else 0,
-                                            res, ctypes.byref(ms)), "batch_run")
+        flags = (RUN_CHECK_CONVERGED if check_converged else 0) | (RUN_GRAPH if graph else 0)
+        self._check(self._lib.kmc_batch_run(self._h, int(maxiter), flags, res,
+                                            ctypes.byref(ms)), "batch_run")
         return [{"iterations": r.iterations, "converged_at": r.converged_at,
                  "inertia": r.inertia} for r in res], ms.value
 

@@ -141,4 +141,4 @@ def test_bench_variant_table_matches_the_kernel_source():
     import bench
     assert shapes == bench.ASSIGN_VARIANTS
     assert bench.assign_kernel_name(128, "bf16", 1) == "k_assign<128, 2, 8, 1, false, 0, false>"
-    assert bench.assign_kernel_name(128, "bf16", 13) == "k_assign2<128, 2, 8, 2, false>"
+    assert bench.assign_kernel_name(128, "bf16", 8) == "k_assign2<128, 2, 8, 4, false, true, 0>"

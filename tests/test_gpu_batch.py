@@ -74,3 +74,26 @@ def test_batch_inertia_matches_fp64(kmc_mod, oracle):
             assert res[p]["iterations"] == 5
             want = oracle.f64_inertia(X, b.centroids(p), b.labels(p).astype(np.uint32))
             assert abs(res[p]["inertia"] - want) <= 1e-9 * max(want, 1e-30), (p, res[p], want)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_batch_graph_replays_equal_eager(kmc_mod, prec):
+    """KMC_RUN_GRAPH: each chunk of batch iterations (every width group as a
+    branch of one captured graph) ends exactly where the eager launches end;
+    the graphs are replayed several times with eager runs in between, with
+    and without the convergence test, and with a chunk shorter than 16."""
+    probs = _problems(21, count=9)
+    with kmc_mod.Batch(probs, tol=1e-4, precision=prec) as b:
+        for maxiter, check in ((40, True), (19, False)):
+            eager, _ = b.run(maxiter, check_converged=check)
+            lab_e = [b.labels(p) for p in range(len(probs))]
+            C_e = [b.centroids(p) for p in range(len(probs))]
+            for rep in range(3):
+                res, ms = b.run(maxiter, check_converged=check, graph=True)
+                _say("graph", prec, maxiter, check, rep, ms, [r["iterations"] for r in res])
+                assert res == eager
+                for p in range(len(probs)):
+                    assert np.array_equal(b.labels(p), lab_e[p]), (rep, p)
+                    assert np.array_equal(b.centroids(p), C_e[p]), (rep, p)
+                again, _ = b.run(maxiter, check_converged=check)  # eager between replays
+                assert again == eager

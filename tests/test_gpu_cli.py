@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "matrix-centric-k-means_amd", "bin", "gpukmeans")
 
 
-def _run_cli(tmp_path, X, k, init, seed, tol, maxiter):
+def _run_cli(tmp_path, X, k, init, seed, tol, maxiter, extra=(), runs=2):
     n, d = X.shape
     inp = tmp_path / "in.csv"
     out = tmp_path / "out.csv"
@@ -25,8 +25,9 @@ def _run_cli(tmp_path, X, k, init, seed, tol, maxiter):
         for i in range(n):
             f.write("0," + ",".join("%.9g" % v for v in X[i]) + "\n")
     cmd = [EXE, "-d", str(d), "-n", str(n), "-k", str(k), "-m", str(maxiter), "-t", repr(tol),
-           "-c", "1", "-b", init, "-f", "linear", "-l", "2", "-s", str(seed), "-r", "2",
+           "-c", "1", "-b", init, "-f", "linear", "-l", "2", "-s", str(seed), "-r", str(runs),
            "-i", str(inp), "-o", str(out), "--label-col", "first", "--cpu-semantics"]
+    cmd += list(extra)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stdout
@@ -184,3 +185,39 @@ def test_cli_reads_libsvm(tmp_path, oracle):
     conv, want, C, cnt, ran = oracle.lloyd(X, C0, tol, maxiter)
     assert np.array_equal(lab, want.astype(np.int64))
     assert ("K-means converged at iteration %d" % conv) in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_cli_batch_runs_match_reference(tmp_path, oracle, prec):
+    """--batch-runs -r 5: the five repetitions run as one batch (one launch
+    per stage, graph replays) and each reports the reference CPU k-means'
+    convergence iteration; the written labels are the reference's; every
+    repetition's objective score equals the FP64 score of the oracle's final
+    state to float rounding (the batch forms FP64 centroid sums)."""
+    g = load_golden("ref_c1_blobs.npz")
+    k, seed, tol, maxiter, runs = 10, 42, 1e-4, 300, 5
+    lab, X, text = _run_cli(tmp_path, g["X"], k, "random", seed, tol, maxiter,
+                            extra=["--batch-runs", "-p", prec], runs=runs)
+    C0 = oracle.init_centroids(X, k, seed)
+    conv, want, C, cnt, ran = oracle.lloyd(X, C0, tol, maxiter)
+    assert np.array_equal(lab, want.astype(np.int64))
+    assert text.count("K-means converged at iteration %d - " % conv) == runs, text
+    assert ("GPU_Kmeans:" in text) and ("(%d runs)" % runs in text), text
+    inertia = oracle.f64_inertia(X, C, want.astype(np.uint32))
+    score = inertia - float(np.sum(X.astype(np.float64) ** 2))
+    got = [float(line.split(":")[1]) for line in text.splitlines()
+           if line.startswith("Objective score:")]
+    assert len(got) == runs
+    for v in got:
+        assert abs(v - score) <= 1e-5 * abs(score) + 1e-3, (v, score)
+
+
+def test_cli_batch_runs_refuses_gpu_semantics(tmp_path):
+    """--batch-runs runs the batch engine, which follows the reference CPU
+    k-means' rules: without --cpu-semantics it is refused."""
+    cmd = [EXE, "-d", "4", "-n", "100", "-k", "3", "-m", "10", "-b", "random", "-f", "linear",
+           "-l", "2", "-r", "2", "--batch-runs"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "--batch-runs needs" in r.stdout
