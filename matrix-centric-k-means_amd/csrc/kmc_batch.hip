@@ -351,6 +351,7 @@ using kmc::DevState;
 // a problem of d=16 streams 64-byte rows, not the widest problem's.
 struct BatchGroup {
   hipStream_t stream = nullptr;  // the group's iterations run on their own stream
+  bool own_stream = true;        // false: the batch stream (the largest group)
   hipEvent_t end = nullptr;
   uint32_t nprob = 0;           // problems in this group
   std::vector<uint32_t> ids;    // their indices in the batch
@@ -452,7 +453,7 @@ void bfree_all(kmc_batch* b) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     gfree_all(g);
     if (g->end) (void)hipEventDestroy(g->end);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->stream && g->own_stream) (void)hipStreamDestroy(g->stream);
     delete g;
   }
   b->groups.clear();
@@ -626,6 +627,8 @@ kmc_status launch_batch_graph(kmc_batch* b, uint64_t m, uint64_t maxiter, int ch
     return KMC_ERR_UNSUPPORTED;
   kmc_status st = KMC_OK;
   if (hipEventRecord(b->fork, b->stream) != hipSuccess) st = KMC_ERR_UNSUPPORTED;
+  // every branch is forked and filled before any is joined (one group may
+  // run on the batch stream itself)
   for (BatchGroup* g : b->groups) {
     if (st != KMC_OK) break;
     if (hipStreamWaitEvent(g->stream, b->fork, 0) != hipSuccess) {
@@ -633,8 +636,11 @@ kmc_status launch_batch_graph(kmc_batch* b, uint64_t m, uint64_t maxiter, int ch
       break;
     }
     for (uint64_t i = 0; i < m && st == KMC_OK; ++i) st = enqueue_iteration(b, g, maxiter, check);
-    if (st == KMC_OK && (hipEventRecord(g->end, g->stream) != hipSuccess ||
-                         hipStreamWaitEvent(b->stream, g->end, 0) != hipSuccess))
+  }
+  for (BatchGroup* g : b->groups) {
+    if (st != KMC_OK) break;
+    if (hipEventRecord(g->end, g->stream) != hipSuccess ||
+        hipStreamWaitEvent(b->stream, g->end, 0) != hipSuccess)
       st = KMC_ERR_UNSUPPORTED;
   }
   hipGraph_t graph = nullptr;
@@ -891,6 +897,19 @@ kmc_status kmc_batch_create(const kmc_problem* probs, uint32_t count,
       const kmc_status s = create_group(b, g, sub);
       if (s != KMC_OK) return s;
     }
+    // The device runs a few hardware queues per process (streams beyond
+    // that share one, and a shared queue runs its groups one after the
+    // other): the group that streams the most bytes runs on the batch stream
+    // instead of a stream of its own.
+    BatchGroup* big = nullptr;
+    for (BatchGroup* g : b->groups)
+      if (!big || g->npad * g->DP > big->npad * big->DP) big = g;
+    if (big && b->groups.size() > 1) {
+      KMC_BCK(hipStreamSynchronize(big->stream));
+      KMC_BCK(hipStreamDestroy(big->stream));
+      big->stream = b->stream;
+      big->own_stream = false;
+    }
     return KMC_OK;
   };
   const kmc_status st = body();
@@ -1011,6 +1030,8 @@ kmc_status kmc_batch_run(kmc_batch* b, uint64_t maxiter, uint32_t flags, kmc_bat
       hipLaunchKernelGGL(kmc::k_binertia_sum, dim3((g->nprob + 3) / 4), dim3(256), 0,
                          g->stream, g->bpart, g->bp, (int)g->nprob, g->inertia);
       KMC_BCK(hipGetLastError());
+    }
+    for (BatchGroup* g : b->groups) {
       KMC_BCK(hipEventRecord(g->end, g->stream));
       KMC_BCK(hipStreamWaitEvent(b->stream, g->end, 0));
     }
