@@ -188,8 +188,7 @@ def cpu_baseline(X, C0, n_sample=100_000, iters=2):
 ASSIGN_VARIANTS = [(8, 1, True, 0, 0, 4), (8, 1, False, 0, 0, 4), (12, 1, False, 0, 0, 4),
                    (16, 1, False, 0, 0, 4), (12, 1, False, 6, 0, 4), (4, 2, False, 0, 0, 4),
                    (8, 2, False, 0, 0, 4), (8, 1, False, 8, 0, 4), (8, 2, False, 0, 1, 4),
-                   (8, 2, False, 0, 1, 3), (6, 2, False, 3, 2, 4), (6, 2, False, 3, 1, 4),
-                   (4, 2, False, 0, 1, 4), (4, 2, False, 0, 2, 4)]
+                   (8, 2, False, 0, 1, 3), (4, 2, False, 0, 1, 4), (4, 2, False, 0, 2, 4)]
 FORM_CODE = {"fp32": 0, "bf16x3": 1, "bf16": 2, "fp8": 3}
 
 

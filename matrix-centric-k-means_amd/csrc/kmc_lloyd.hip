@@ -3773,8 +3773,7 @@ constexpr VariantShape kVariants[] = {
     {8, 1, true, 0, 0, 4},  {8, 1, false, 0, 0, 4},  {12, 1, false, 0, 0, 4},
     {16, 1, false, 0, 0, 4}, {12, 1, false, 6, 0, 4}, {4, 2, false, 0, 0, 4},
     {8, 2, false, 0, 0, 4},  {8, 1, false, 8, 0, 4},  {8, 2, false, 0, 1, 4},
-    {8, 2, false, 0, 1, 3},  {6, 2, false, 3, 2, 4},  {6, 2, false, 3, 1, 4},
-    {4, 2, false, 0, 1, 4},  {4, 2, false, 0, 2, 4}};
+    {8, 2, false, 0, 1, 3},  {4, 2, false, 0, 1, 4},  {4, 2, false, 0, 2, 4}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static_assert(kNumVariants <= KMC_MAX_VARIANTS, "variant table exceeds kmc_tuning");
 
@@ -3814,8 +3813,6 @@ static const void* kernel_of_v(int v) {
     case 9: return kernel_of<DP, PREC, 9>();
     case 10: return kernel_of<DP, PREC, 10>();
     case 11: return kernel_of<DP, PREC, 11>();
-    case 12: return kernel_of<DP, PREC, 12>();
-    case 13: return kernel_of<DP, PREC, 13>();
     default: return nullptr;
   }
 }
