@@ -441,8 +441,9 @@ kmc_status kmc_kk_destroy(kmc_kk_handle* h);
  * (problem, tile) queue. Each problem follows the reference CPU k-means
  * (comparisons/CPU_Kmeans/src/kmeans.cpp:86-200): exact first-min labels,
  * its own shift test from iteration 2 (a converged problem is frozen while
- * the others go on); centroids are FP64 sums rounded once, as
- * KMC_UPDATE_TREE. Limits: d <= 256, k <= 256, and the centroid operands of
+ * the others go on); centroids are exact fixed-point sums rounded once, as
+ * KMC_UPDATE_FIXED with the problem's own scale (a full pass over the sorted
+ * members, then the moved points only while at most an eighth move). Limits: d <= 256, k <= 256, and the centroid operands of
  * the largest problem must fit the LDS (k x d <= 256 x 128 in fp32 or
  * bf16x3). */
 typedef struct kmc_problem {

@@ -219,8 +219,8 @@ void write_csv(std::ostream& o, const std::vector<Point<float>*>& pts, int d) {
 // seeded rows, so all of them run in one launch per stage; the batch is run
 // once untimed (the warm-up), then timed, and each repetition is charged an
 // equal share of the timed run. The batch follows the reference CPU k-means'
-// rules (shift test, empty cluster -> zero vector) with FP64 centroid sums
-// rounded once. Prints the sequential loop's report lines.
+// rules (shift test, empty cluster -> zero vector) with exact fixed-point
+// centroid sums rounded once. Prints the sequential loop's report lines.
 int batch_runs(std::vector<Point<float>*>& pts, int d, uint32_t k, float tol, int seed,
                uint32_t runs, uint64_t maxiter, bool check_converged, const std::string& prec,
                const kmc::DeviceInfo& dev) {

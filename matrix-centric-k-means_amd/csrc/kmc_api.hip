@@ -703,8 +703,7 @@ kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
       double qm;
       std::memcpy(&qm, &hv[1], sizeof(qm));
       const double nn = (double)std::max<size_t>(h->n, 1);
-      const int s1 = std::max(-120, std::min(120, (int)std::floor(
-                                                      61.0 - std::log2(nn * std::max<double>(am, 1e-30)))));
+      const int s1 = kmc::fx_scale_exp(h->n, am);
       const int s2 = std::max(-900, std::min(900, (int)std::floor(
                                                       61.0 - std::log2(nn * std::max(qm, 1e-300)))));
       h->fx_scale = std::ldexp(1.0f, s1);

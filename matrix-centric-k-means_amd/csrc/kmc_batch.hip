@@ -22,9 +22,15 @@
 //                to the point's problem)
 //   k_bhist / k_bscan_blocks / k_bscatter   stable counting sort, blocks of
 //                kBatchAlign points never straddle problems
-//   k_update_tree, k_finish    FP64 tree update and shift (shared kernels)
+//   k_update_tree, k_finish    exact update and shift (shared kernels): the
+//                64-bit integer sums of round(x * scale) at each problem's
+//                fixed-point scale over the sorted members, or, when at most
+//                an eighth of the points moved, the moved points' rows
+//                subtracted from and added to the kept cluster totals (the
+//                sort kernels then return at once)
 //   k_bstats     per-problem shift test and iteration counter, stop flags
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -92,12 +98,16 @@ struct BSortParams {
   const int* clus_prob;  // problem of each master cluster
   const int* done;
   int nblk, KT, lbits;
+  // the sort serves the full passes of the exact update only
+  const DevState* fx_state;
+  long long fx_n;
 };
 
 // Block histograms: one wave per kBatchAlign points, local labels, the counts
 // stored cluster-major per problem.
 __global__ __launch_bounds__(64) void k_bhist(BSortParams P) {
   if (*P.done) return;
+  if (!fx_full_pass(P.fx_state, P.fx_n)) return;
   extern __shared__ int hist_s[];
   const int b = blockIdx.x;
   const BatchProblem bpr = P.bp[P.blk_prob[b]];
@@ -120,6 +130,7 @@ __global__ __launch_bounds__(64) void k_bhist(BSortParams P) {
 // wave per cluster, 8 counts per lane per round).
 __global__ __launch_bounds__(256) void k_bscan_blocks(BSortParams P) {
   if (*P.done) return;
+  if (!fx_full_pass(P.fx_state, P.fx_n)) return;
   const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (g >= P.KT) return;
@@ -164,6 +175,7 @@ __global__ __launch_bounds__(256) void k_bscan_blocks(BSortParams P) {
 // by one ballot per label bit.
 __global__ __launch_bounds__(64) void k_bscatter(BSortParams P) {
   if (*P.done) return;
+  if (!fx_full_pass(P.fx_state, P.fx_n)) return;
   extern __shared__ int cur_s[];
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
@@ -223,6 +235,7 @@ struct BStatsParams {
   const int* done;
   int nprob, check;
   unsigned long long maxiter;
+  long long n;  // points of the packed space (the full-pass threshold)
 };
 
 // Per-problem iteration counter and shift test (reference: tested from
@@ -233,7 +246,18 @@ __global__ __launch_bounds__(kBStatsThreads) void k_bstats(BStatsParams P) {
   if (*P.done) return;
   __shared__ int active_s;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) active_s = 0;
+  if (threadIdx.x == 0) {
+    active_s = 0;
+    // the iteration's update is complete: its moved count decides the next
+    // one, and the cluster totals now stand for its labels
+    if (fx_full_pass(P.state, P.n))
+      P.state->full_passes += 1;
+    else
+      P.state->moved_total += P.state->moved_acc;
+    P.state->moved = P.state->moved_acc;
+    P.state->moved_acc = 0ull;
+    P.state->sums_valid = 1;
+  }
   __syncthreads();
   int active = 0;
   for (int pb = wave; pb < P.nprob; pb += kBStatsThreads / 64) {
@@ -378,6 +402,11 @@ struct BatchGroup {
   unsigned *piter = nullptr, *pconv = nullptr;
   BatchProblem* bp = nullptr;
   int *item_prob = nullptr, *blk_prob = nullptr, *clus_prob = nullptr, *row_prob = nullptr;
+  // exact update: cluster totals, labels they stand for, per-cluster scales
+  long long* Stot = nullptr;
+  int *Ntot = nullptr, *prev = nullptr;
+  float* cl_scale = nullptr;
+  double* cl_inv_scale = nullptr;
 };
 
 // A captured chunk of batch iterations: every group's iterations as one
@@ -438,7 +467,8 @@ void gfree_all(BatchGroup* g) {
                   g->qa,    g->q2cnt, g->qacnt,   g->bcount,  g->count,   g->start,  g->members,
                   g->tpart, g->ipart, g->shift2,  g->inertia, g->conv,    g->alive,  g->state,
                   g->pdone, g->piter, g->pconv,   g->bp,      g->item_prob, g->blk_prob,
-                  g->clus_prob, g->row_prob, g->bpart};
+                  g->clus_prob, g->row_prob, g->bpart, g->Stot, g->Ntot,
+                  g->prev,  g->cl_scale, g->cl_inv_scale};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -550,6 +580,8 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   sp.nblk = g->nblk;
   sp.KT = g->KT;
   sp.lbits = g->lbits;
+  sp.fx_state = g->state;
+  sp.fx_n = g->npad;
   const size_t hl = (size_t)(1 << g->lbits) * sizeof(int);
   hipLaunchKernelGGL(kmc::k_bhist, dim3(g->nblk), dim3(64), hl, g->stream, sp);
   hipLaunchKernelGGL(kmc::k_bscan_blocks, dim3((g->KT + 3) / 4), dim3(256), 0, g->stream, sp);
@@ -571,6 +603,14 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   up.K = g->KT;
   up.D = g->DP;
   up.DP = g->DP;
+  // exact form: full passes over the sorted members, otherwise the moved
+  // points only
+  up.cl_scale = g->cl_scale;
+  up.Stot = g->Stot;
+  up.Ntot = g->Ntot;
+  up.Qtot = nullptr;
+  up.prev = g->prev;
+  up.state = g->state;
   KMC_BCK(kmc::launch_update(up, kmc::UPDATE_TREE, g->stream));
 
   kmc::FinishParams fp{};
@@ -578,7 +618,12 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   fp.Cnext = g->Cnext;
   fp.tpart = g->tpart;
   fp.start = g->start;
-  fp.tree = 1;
+  fp.tree = 2;  // exact integer sums (full pass) or the cluster totals
+  fp.Stot = g->Stot;
+  fp.Ntot = g->Ntot;
+  fp.cl_inv_scale = g->cl_inv_scale;
+  fp.fx_state = g->state;
+  fp.n = g->npad;
   fp.conv = g->conv;
   fp.shift2 = g->shift2;
   fp.count = g->count;
@@ -604,6 +649,7 @@ kmc_status enqueue_iteration(kmc_batch* b, BatchGroup* g, uint64_t maxiter, int 
   st.nprob = (int)g->nprob;
   st.check = check;
   st.maxiter = maxiter;
+  st.n = g->npad;
   hipLaunchKernelGGL(kmc::k_bstats, dim3(1), dim3(kmc::kBStatsThreads), 0, g->stream, st);
   KMC_BCK(hipGetLastError());
   return KMC_OK;
@@ -723,6 +769,22 @@ kmc_status create_group(kmc_batch* b, BatchGroup* g, const std::vector<kmc_probl
   g->nblk = (int)(poff / kmc::kBatchAlign);
   g->nbcount = bcoff;
   std::vector<int> item_prob(g->nitems), blk_prob(g->nblk), clus_prob(g->KT), row_prob(g->nops);
+  // each problem's fixed-point scale, from its own n and max|x| exactly as a
+  // handle of that problem alone chooses it (UPDATE_FIXED), so the exact sums
+  // and the centroids are those of its lone run
+  std::vector<float> cl_scale(g->KT);
+  std::vector<double> cl_inv(g->KT);
+  for (uint32_t p = 0; p < count; ++p) {
+    const kmc_problem& q = probs[p];
+    float am = 0.0f;
+    for (uint64_t i = 0; i < q.n; ++i)
+      for (uint32_t j = 0; j < q.d; ++j) am = std::max(am, std::fabs(q.X[i * q.ld + j]));
+    const float sc = std::ldexp(1.0f, kmc::fx_scale_exp((size_t)q.n, am));
+    for (int c = 0; c <= g->hp[p].k; ++c) {
+      cl_scale[g->hp[p].coff + c] = sc;
+      cl_inv[g->hp[p].coff + c] = 1.0 / (double)sc;
+    }
+  }
   for (uint32_t p = 0; p < count; ++p) {
     const BatchProblem& e = g->hp[p];
     const int nt = (e.n + kmc::kBatchTile - 1) / kmc::kBatchTile;
@@ -784,6 +846,11 @@ kmc_status create_group(kmc_batch* b, BatchGroup* g, const std::vector<kmc_probl
   KMC_BCK(upload(&g->blk_prob, blk_prob, b->stream));
   KMC_BCK(upload(&g->clus_prob, clus_prob, b->stream));
   KMC_BCK(upload(&g->row_prob, row_prob, b->stream));
+  KMC_BCK(upload(&g->cl_scale, cl_scale, b->stream));
+  KMC_BCK(upload(&g->cl_inv_scale, cl_inv, b->stream));
+  KMC_BCK(balloc(&g->Stot, (size_t)g->KT * DP));
+  KMC_BCK(balloc(&g->Ntot, (size_t)g->KT));
+  KMC_BCK(balloc(&g->prev, np));
   KMC_BCK(kmc::launch_fill_i32(g->alive, (long long)g->KT, 1, b->stream));
   KMC_BCK(hipMemsetAsync(g->X, 0, np * DP * sizeof(float), b->stream));
   if (g->Xhi) KMC_BCK(hipMemsetAsync(g->Xhi, 0, np * DP * sizeof(uint16_t), b->stream));

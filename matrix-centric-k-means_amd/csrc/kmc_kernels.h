@@ -4,6 +4,9 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+
 namespace kmc {
 
 enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_BF16 = 2, PREC_FP8 = 3 };
@@ -45,6 +48,14 @@ struct DevState {
   unsigned long long full_passes;  // iterations of this run with a full pass
   unsigned long long moved_total;  // points moved in the other iterations
 };
+
+// Exponent of the fixed-point scale 2^e of x -> round(x * 2^e): the largest
+// that keeps n * max|x| * 2^e below 2^61, so no cluster sum can overflow.
+inline int fx_scale_exp(size_t n, float amax) {
+  const double nn = (double)(n > 0 ? n : 1);
+  const int e = (int)std::floor(61.0 - std::log2(nn * std::max<double>(amax, 1e-30)));
+  return std::max(-120, std::min(120, e));
+}
 
 // The fixed-point update of an iteration is a full streaming pass (and it
 // rebuilds the cluster totals) unless the totals are valid and at most an
@@ -208,6 +219,12 @@ struct UpdateParams {
   long long* Qtot;
   int* Ntot;
   DevState* state;
+  // exact tree form (batches): per master cluster the fixed-point scale of
+  // its problem. The tree update then sums the converted values as 64-bit
+  // integers per segment (tpart holds them), keeps the labels in prev and
+  // counts the moved points; iterations that are not a full pass take the
+  // moved points only (Qtot may be null: no fixed-point ||x||^2).
+  const float* cl_scale;
 };
 
 
@@ -258,6 +275,11 @@ struct FinishParams {
   long long* Qtot;
   int* Ntot;
   long long n;
+  // exact tree form (tree == 2): integer tree partials of a full pass or the
+  // cluster totals, each cluster's inverse fixed-point scale, and the state
+  // that chooses between them
+  const double* cl_inv_scale;
+  const DevState* fx_state;
   PrepParams prep;
   StatsParams stats;  // k_stats, launched after k_finish
 };

@@ -2662,9 +2662,15 @@ __global__ __launch_bounds__(256) void k_update(UpdateParams P) {
 // ---------------------------------------------------------------------------
 constexpr int kTreeWaves = 4;
 
-// stage member ids and labels of local rows [0, L) of a chunk
-__device__ __forceinline__ void tree_stage(const UpdateParams& P, long long p0, int L,
-                                           int* mem_s, int* lab_s, int lane) {
+// the moved points of an incremental iteration (defined with k_update_fx)
+__device__ void update_moved_points(const UpdateParams& P);
+
+// stage member ids and labels of local rows [0, L) of a chunk; the exact
+// form also keeps each point's label in prev and returns how many of the
+// lane's staged points changed cluster since the last update
+template <bool EX>
+__device__ __forceinline__ unsigned tree_stage(const UpdateParams& P, long long p0, int L,
+                                               int* mem_s, int* lab_s, int lane) {
   int id[kTreeRows / 64], lab[kTreeRows / 64];
 #pragma unroll
   for (int m = 0; m < kTreeRows / 64; ++m) {
@@ -2676,11 +2682,25 @@ __device__ __forceinline__ void tree_stage(const UpdateParams& P, long long p0, 
     const int i = m * 64 + lane;
     lab[m] = i < L ? P.labels[id[m]] : -1;
   }
+  unsigned moved = 0;
+  if constexpr (EX) {
+    int old[kTreeRows / 64];
+#pragma unroll
+    for (int m = 0; m < kTreeRows / 64; ++m) old[m] = m * 64 + lane < L ? P.prev[id[m]] : 0;
+#pragma unroll
+    for (int m = 0; m < kTreeRows / 64; ++m) {
+      if (m * 64 + lane < L) {
+        moved += old[m] != lab[m] ? 1u : 0u;
+        P.prev[id[m]] = lab[m];
+      }
+    }
+  }
 #pragma unroll
   for (int m = 0; m < kTreeRows / 64; ++m) {
     mem_s[m * 64 + lane] = id[m];
     lab_s[m * 64 + lane] = lab[m];
   }
+  return moved;
 }
 
 // first local index in (r, L) whose label differs from c, or L
@@ -2695,18 +2715,31 @@ __device__ __forceinline__ int tree_seg_end(const int* lab_s, int r, int L, int 
 
 // Wide rows (DP = 128 with VW = 2, DP = 256 with VW = 4): lane owns dimensions
 // [VW*lane, VW*lane + VW) of every row, one contiguous 256*VW-byte read per
-// row, batches of 16 rows.
-template <int VW>
+// row, batches of 16 rows. EX: the exact form (UpdateParams::cl_scale): a
+// segment's sum is the 64-bit integer sum of round(x * scale), its cluster's
+// fixed-point scale; the iterations that are not a full pass update the
+// totals from the moved points only.
+template <int VW, bool EX>
 __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree(UpdateParams P) {
   if (*P.done) return;
+  if (EX && !fx_full_pass(P.state, P.n)) {
+    update_moved_points(P);
+    return;
+  }
   __shared__ int mem_s[kTreeWaves][kTreeRows], lab_s[kTreeWaves][kTreeRows];
+  __shared__ unsigned moved_s;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (EX && threadIdx.x == 0) moved_s = 0u;
+  __syncthreads();
   const long long q = (long long)blockIdx.x * kTreeWaves + wv;
   const long long nchunks = (P.n + kTreeRows - 1) / kTreeRows;
   const long long p0 = q * kTreeRows;
   const int L = q < nchunks ? (int)min((long long)kTreeRows, P.n - p0) : 0;
-  tree_stage(P, p0, L, mem_s[wv], lab_s[wv], lane);
+  const unsigned mv = tree_stage<EX>(P, p0, L, mem_s[wv], lab_s[wv], lane);
+  if (EX && mv) atomicAdd(&moved_s, mv);
   __syncthreads();
+  if (EX && threadIdx.x == 0 && moved_s)
+    atomicAdd(&P.state->moved_acc, (unsigned long long)moved_s);
   if (L == 0) return;
   const int* ms = mem_s[wv];
   const int j0 = VW * lane;
@@ -2718,10 +2751,13 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree(UpdateParams P)
     const int c = lab_s[wv][r];
     const int e = tree_seg_end(lab_s[wv], r, L, c, lane);
     double cold[VW], sum[VW];
+    long long isum[VW];
+    const float fs = EX ? P.cl_scale[c] : 0.0f;
 #pragma unroll
     for (int v = 0; v < VW; ++v) {
       cold[v] = act ? (double)P.C[(size_t)c * P.DP + j0 + v] : 0.0;
       sum[v] = 0.0;
+      isum[v] = 0;
     }
     // batches of U rows; the last, partial batch of a segment loads a valid
     // row in place of the rows past the end and masks them out
@@ -2745,10 +2781,14 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree(UpdateParams P)
         const bool ok = !partial || t + u < e;  // wave-uniform
 #pragma unroll
         for (int v = 0; v < VW; ++v) {
-          const double xd = (double)xv[u][v];
-          const double df = xd - cold[v];
-          sum[v] = ok ? sum[v] + xd : sum[v];
-          in = ok ? fma(df, df, in) : in;
+          if constexpr (EX) {
+            isum[v] += ok ? __float2ll_rn(xv[u][v] * fs) : 0ll;
+          } else {
+            const double xd = (double)xv[u][v];
+            const double df = xd - cold[v];
+            sum[v] = ok ? sum[v] + xd : sum[v];
+            in = ok ? fma(df, df, in) : in;
+          }
         }
       }
     };
@@ -2757,7 +2797,11 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree(UpdateParams P)
     if (t < e) batch(t, true);
     if (act) {
 #pragma unroll
-      for (int v = 0; v < VW; ++v) P.tpart[(size_t)(c + q) * P.DP + j0 + v] = sum[v];
+      for (int v = 0; v < VW; ++v) {
+        const size_t o = (size_t)(c + q) * P.DP + j0 + v;
+        if constexpr (EX) reinterpret_cast<long long*>(P.tpart)[o] = isum[v];
+        else P.tpart[o] = sum[v];
+      }
     }
     r = e;
   }
@@ -2770,21 +2814,31 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree(UpdateParams P)
 // load), so one wave load covers G = 256 / DP rows and a batch of U loads 64
 // rows. Lane group g sums the rows t + u*G + g of a segment; at the segment's
 // end the G group sums are added by a fixed butterfly, so the result is
-// deterministic.
-template <int DP>
+// deterministic. EX as for k_update_tree.
+template <int DP, bool EX>
 __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree_rows(UpdateParams P) {
   if (*P.done) return;
+  if (EX && !fx_full_pass(P.state, P.n)) {
+    update_moved_points(P);
+    return;
+  }
   constexpr int LPR = DP / 4;    // lanes per row
   constexpr int G = 64 / LPR;    // rows per wave load
   constexpr int U = G >= 4 ? 64 / G : 16;  // loads per batch
   __shared__ int mem_s[kTreeWaves][kTreeRows], lab_s[kTreeWaves][kTreeRows];
+  __shared__ unsigned moved_s;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (EX && threadIdx.x == 0) moved_s = 0u;
+  __syncthreads();
   const long long q = (long long)blockIdx.x * kTreeWaves + wv;
   const long long nchunks = (P.n + kTreeRows - 1) / kTreeRows;
   const long long p0 = q * kTreeRows;
   const int L = q < nchunks ? (int)min((long long)kTreeRows, P.n - p0) : 0;
-  tree_stage(P, p0, L, mem_s[wv], lab_s[wv], lane);
+  const unsigned mv = tree_stage<EX>(P, p0, L, mem_s[wv], lab_s[wv], lane);
+  if (EX && mv) atomicAdd(&moved_s, mv);
   __syncthreads();
+  if (EX && threadIdx.x == 0 && moved_s)
+    atomicAdd(&P.state->moved_acc, (unsigned long long)moved_s);
   if (L == 0) return;
   const int* ms = mem_s[wv];
   const int g = lane / LPR, j0 = 4 * (lane % LPR);
@@ -2797,6 +2851,8 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree_rows(UpdatePara
     const float4 cf = *reinterpret_cast<const float4*>(P.C + (size_t)c * DP + j0);
     const double cold[4] = {(double)cf.x, (double)cf.y, (double)cf.z, (double)cf.w};
     double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    long long isum[4] = {0, 0, 0, 0};
+    const float fs = EX ? P.cl_scale[c] : 0.0f;
     auto batch = [&](int t, bool partial) {
       float4 xv[U];
 #pragma unroll
@@ -2808,13 +2864,17 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree_rows(UpdatePara
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const bool ok = !partial || t + u * G + g < e;
-        const double xd[4] = {(double)xv[u].x, (double)xv[u].y, (double)xv[u].z,
-                              (double)xv[u].w};
+        const float xf[4] = {xv[u].x, xv[u].y, xv[u].z, xv[u].w};
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const double df = xd[v] - cold[v];
-          sum[v] = ok ? sum[v] + xd[v] : sum[v];
-          in = ok ? fma(df, df, in) : in;
+          if constexpr (EX) {
+            isum[v] += ok ? __float2ll_rn(xf[v] * fs) : 0ll;
+          } else {
+            const double xd = (double)xf[v];
+            const double df = xd - cold[v];
+            sum[v] = ok ? sum[v] + xd : sum[v];
+            in = ok ? fma(df, df, in) : in;
+          }
         }
       }
     };
@@ -2824,11 +2884,18 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree_rows(UpdatePara
 #pragma unroll
     for (int o = LPR; o < 64; o <<= 1) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) sum[v] += __shfl_xor(sum[v], o, 64);
+      for (int v = 0; v < 4; ++v) {
+        if constexpr (EX) isum[v] += __shfl_xor(isum[v], o, 64);
+        else sum[v] += __shfl_xor(sum[v], o, 64);
+      }
     }
     if (g == 0) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) P.tpart[(size_t)(c + q) * DP + j0 + v] = sum[v];
+      for (int v = 0; v < 4; ++v) {
+        const size_t o = (size_t)(c + q) * DP + j0 + v;
+        if constexpr (EX) reinterpret_cast<long long*>(P.tpart)[o] = isum[v];
+        else P.tpart[o] = sum[v];
+      }
     }
     r = e;
   }
@@ -2872,10 +2939,11 @@ __device__ void update_moved_points(const UpdateParams& P) {
   const int lane = threadIdx.x & 63;
   const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
   const long long w0 = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const float fs = P.fscale;
   // one moved point: its converted row out of the old cluster's totals and
-  // into the new one's, a lane per dimension
+  // into the new one's, a lane per dimension (both clusters are of the
+  // point's problem: one scale)
   auto move_point = [&](long long pi, int nl, int ol) {
+    const float fs = P.cl_scale ? P.cl_scale[nl] : P.fscale;
     const float* xr = P.X + (size_t)pi * P.DP;
     unsigned long long* sn = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)nl * P.DP);
     unsigned long long* so = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)ol * P.DP);
@@ -2885,9 +2953,11 @@ __device__ void update_moved_points(const UpdateParams& P) {
       atomicAdd(so + j, (unsigned long long)(-q));
     }
     if (lane == 0) {
-      const long long qq = __double2ll_rn(P.xn64[pi] * P.qscale);
-      atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + nl), (unsigned long long)qq);
-      atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + ol), (unsigned long long)(-qq));
+      if (P.Qtot) {
+        const long long qq = __double2ll_rn(P.xn64[pi] * P.qscale);
+        atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + nl), (unsigned long long)qq);
+        atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + ol), (unsigned long long)(-qq));
+      }
       atomicAdd(P.Ntot + nl, 1);
       atomicAdd(P.Ntot + ol, -1);
     }
@@ -3541,6 +3611,54 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
     }
     __syncthreads();
     cnt = cnt_s;
+  } else if (P.tree == 2) {
+    // exact tree form: the 64-bit integer segment sums of a full pass (the
+    // cluster's chunks of the sorted members, any order), stored as the
+    // cluster totals, or the totals the moved points kept up to date; the
+    // means rounded once, as the fixed-point update's
+    long long(*is_s)[kMaxDP] = reinterpret_cast<long long(*)[kMaxDP]>(part_s);
+    const bool from_totals = !fx_full_pass(P.fx_state, P.n);
+    if (from_totals) cnt = P.Ntot[c];
+    long long q0 = 0, q1 = -1;
+    if (!from_totals && cnt > 0) {
+      q0 = (long long)P.start[c] / kTreeRows;
+      q1 = ((long long)P.start[c] + cnt - 1) / kTreeRows;
+    }
+    constexpr int S = kFinishWaves;
+    const long long* tp64 = reinterpret_cast<const long long*>(P.tpart);
+    for (int j = lane; j < P.D && !from_totals; j += 64) {
+      const long long* tp = tp64 + (size_t)c * P.DP + j;
+      long long sum = 0;
+      long long q = q0 + wave;
+      for (; q + 3 * S <= q1; q += 4 * S) {
+        long long v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = tp[(size_t)(q + u * S) * P.DP];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sum += v[u];
+      }
+      for (; q <= q1; q += S) sum += tp[(size_t)q * P.DP];
+      is_s[wave][j] = sum;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const double inv = P.cl_inv_scale[c];
+      for (int j = lane; j < P.D; j += 64) {
+        long long sj = 0;
+        if (from_totals) {
+          sj = P.Stot[(size_t)c * P.DP + j];
+        } else {
+#pragma unroll
+          for (int w = 0; w < S; ++w) sj += is_s[w][j];
+          P.Stot[(size_t)c * P.DP + j] = sj;
+        }
+        const double Sd = (double)sj * inv;
+        const float v = (float)(Sd / (double)(cnt > 1 ? cnt : 1));
+        ns[j] = v;
+        nrow[j] = v;
+      }
+      if (!from_totals && lane == 0) P.Ntot[c] = cnt;
+    }
   } else if (P.tree) {
     long long q0 = 0, q1 = -1;
     if (cnt > 0) {
@@ -3990,12 +4108,25 @@ hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
     const long long waves = (P.n + kTreeRows - 1) / kTreeRows;
     const dim3 g((unsigned)((waves + kTreeWaves - 1) / kTreeWaves));
     const dim3 b(64 * kTreeWaves);
-    if (P.DP >= 256) hipLaunchKernelGGL(k_update_tree<4>, g, b, 0, s, P);
-    else if (P.DP >= 128) hipLaunchKernelGGL(k_update_tree_rows<128>, g, b, 0, s, P);
-    else if (P.DP == 64) hipLaunchKernelGGL(k_update_tree_rows<64>, g, b, 0, s, P);
-    else if (P.DP == 32) hipLaunchKernelGGL(k_update_tree_rows<32>, g, b, 0, s, P);
-    else if (P.DP == 16) hipLaunchKernelGGL(k_update_tree_rows<16>, g, b, 0, s, P);
-    else return hipErrorInvalidValue;
+    const bool ex = P.cl_scale != nullptr;
+    if (P.DP >= 256) {
+      if (ex) hipLaunchKernelGGL((k_update_tree<4, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((k_update_tree<4, false>), g, b, 0, s, P);
+    } else if (P.DP >= 128) {
+      if (ex) hipLaunchKernelGGL((k_update_tree_rows<128, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((k_update_tree_rows<128, false>), g, b, 0, s, P);
+    } else if (P.DP == 64) {
+      if (ex) hipLaunchKernelGGL((k_update_tree_rows<64, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((k_update_tree_rows<64, false>), g, b, 0, s, P);
+    } else if (P.DP == 32) {
+      if (ex) hipLaunchKernelGGL((k_update_tree_rows<32, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((k_update_tree_rows<32, false>), g, b, 0, s, P);
+    } else if (P.DP == 16) {
+      if (ex) hipLaunchKernelGGL((k_update_tree_rows<16, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((k_update_tree_rows<16, false>), g, b, 0, s, P);
+    } else {
+      return hipErrorInvalidValue;
+    }
   } else {
     hipLaunchKernelGGL(k_update, dim3((P.K * S + 3) / 4), dim3(256), 0, s, P);
   }

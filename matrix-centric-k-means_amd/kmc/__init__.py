@@ -488,7 +488,7 @@ class Batch:
     (kmc_batch_*): every stage of an iteration is one launch for the whole
     batch. problems: sequence of (X n x d, C0 k x d) float32 arrays. Each
     problem follows the reference CPU k-means with its own shift test; the
-    centroids are FP64 sums rounded once (as update="tree")."""
+    centroids are exact fixed-point sums rounded once (as update="fixed")."""
 
     def __init__(self, problems, tol=1e-4, precision="fp32", device=0):
         self._lib = load_library()

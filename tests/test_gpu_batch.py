@@ -2,12 +2,14 @@
 many independent clusterings run as one batch).
 
 Bar: every problem of a batch of uneven n, d and k ends exactly where a run of
-that problem alone ends (Engine with update="tree": identical labels,
+that problem alone ends (Engine with update="fixed": identical labels,
 identical centroids, identical iteration count and convergence iteration) —
-the batch packs each problem so that its sort order and its FP64 update
-partials are the ones a lone run forms. A batch member that is the C1
-golden problem reproduces the reference CPU k-means labels (its centroids are
-FP64 sums rounded once, so they match the reference's float sums to 1e-6).
+the batch sums each cluster exactly, as 64-bit integers at the fixed-point
+scale a lone run of the problem chooses (full passes over the sorted members,
+then the moved points only), and integer sums do not depend on the order. A
+batch member that is the C1 golden problem reproduces the reference CPU
+k-means labels (its centroids are exact sums rounded once, so they match the
+reference's float sums to 1e-6).
 """
 import sys
 
@@ -51,7 +53,7 @@ def test_batch_matches_lone_runs(kmc_mod, prec):
         assert res2 == res
         for p, (X, C0) in enumerate(probs):
             with kmc_mod.Engine(X.shape[0], X.shape[1], C0.shape[0], tol=tol, precision=prec,
-                                update="tree") as e:
+                                update="fixed") as e:
                 e.set_points(X)
                 e.set_centroids(C0)
                 r = e.run(maxiter, check_converged=True)

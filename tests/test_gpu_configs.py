@@ -160,7 +160,7 @@ def test_c4_batch_of_512(kmc_mod, oracle, c4_batch):
         for p in range(0, 512, 32):
             X, C0 = probs[p]
             with kmc_mod.Engine(X.shape[0], X.shape[1], C0.shape[0], tol=0.0, precision="bf16",
-                                update="tree") as e:
+                                update="fixed") as e:
                 e.set_points(X)
                 e.set_centroids(C0)
                 e.run(steps, check_converged=False)
