@@ -3513,6 +3513,62 @@ __device__ void stats_reduce(const StatsParams& P) {
 // C <- C_new lane-parallel, and the next iteration's distance operands for
 // the centroid (prep_row).
 // ---------------------------------------------------------------------------
+// The end of a centroid's update, run by one wave: the empty-cluster rule,
+// the float shift in the reference's order (squared terms lane-parallel,
+// summed on one lane in dimension order), the FP64 shift, C <- C_new and the
+// next iteration's distance operands. ns: the new centroid (LDS), sq_s: LDS
+// scratch of the wave.
+__device__ __forceinline__ void finish_tail(const FinishParams& P, int c, int cnt, float* ns,
+                                            float* sq_s, int lane) {
+  float* cr = P.C + (size_t)c * P.DP;
+  float* nrow = P.Cnext + (size_t)c * P.DP;
+  if (P.keep_empty && cnt == 0) {
+    // an empty cluster keeps its centroid (the explicit-Lloyd rule of
+    // experiments/raft-bench/src/kmeans.cu:158-166)
+    for (int j = lane; j < P.D; j += 64) {
+      ns[j] = cr[j];
+      nrow[j] = cr[j];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  double s64 = 0.0;
+  for (int j = lane; j < P.D; j += 64) {
+    const float o = cr[j], nv = ns[j];
+    const double td = (double)nv - (double)o;
+    s64 = fma(td, td, s64);
+    {
+#pragma clang fp contract(off)
+      const float t = nv - o;
+      sq_s[j] = t * t;
+    }
+  }
+  for (int j = P.D + lane; j < ((P.D + 3) & ~3); j += 64) sq_s[j] = 0.0f;
+  s64 = wave_sum_f64(s64);
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) {
+    float s = 0.0f;
+    for (int j = 0; j < P.D; j += 4) {
+#pragma clang fp contract(off)
+      const float4 v = *reinterpret_cast<const float4*>(&sq_s[j]);
+      s = s + v.x;
+      if (j + 1 < P.D) s = s + v.y;
+      if (j + 2 < P.D) s = s + v.z;
+      if (j + 3 < P.D) s = s + v.w;
+    }
+    P.conv[c] = (__fsqrt_rn(s) > P.tol) ? 0 : 1;
+    P.shift2[c] = s64;
+    if (P.exclude_empty && cnt == 0) P.alive[c] = 0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (int j = lane; j < P.D; j += 64) cr[j] = ns[j];
+  // the next iteration's distance operands for this centroid (alive[c] is
+  // final for it now)
+  if (P.do_prep) {
+    __builtin_amdgcn_wave_barrier();
+    prep_row(P.prep, c, lane, [&](int j) { return j < P.D ? ns[j] : 0.0f; });
+  }
+}
+
 constexpr int kFinishWaves = 8;
 __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
   if (*P.done) return;
@@ -3694,56 +3750,49 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
   } else if (wave == 0) {
     for (int j = lane; j < P.D; j += 64) ns[j] = nrow[j];
   }
-  if (wave == 0) {
-    if (P.keep_empty && cnt == 0) {
-      // an empty cluster keeps its centroid (the explicit-Lloyd rule of
-      // experiments/raft-bench/src/kmeans.cu:158-166)
-      for (int j = lane; j < P.D; j += 64) {
-        ns[j] = cr[j];
-        nrow[j] = cr[j];
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    // the squared terms of the float shift are independent: form them
-    // lane-parallel (each is one rounded difference and one rounded product,
-    // as in the reference), then add them in dimension order on lane 0
-    double s64 = 0.0;
-    for (int j = lane; j < P.D; j += 64) {
-      const float o = cr[j], nv = ns[j];
-      const double td = (double)nv - (double)o;
-      s64 = fma(td, td, s64);
-      {
-#pragma clang fp contract(off)
-        const float t = nv - o;
-        sq_s[j] = t * t;
-      }
-    }
-    for (int j = P.D + lane; j < ((P.D + 3) & ~3); j += 64) sq_s[j] = 0.0f;
-    s64 = wave_sum_f64(s64);
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) {
-      float s = 0.0f;
-      for (int j = 0; j < P.D; j += 4) {
-#pragma clang fp contract(off)
-        const float4 v = *reinterpret_cast<const float4*>(&sq_s[j]);
-        s = s + v.x;
-        if (j + 1 < P.D) s = s + v.y;
-        if (j + 2 < P.D) s = s + v.z;
-        if (j + 3 < P.D) s = s + v.w;
-      }
-      P.conv[c] = (__fsqrt_rn(s) > P.tol) ? 0 : 1;
-      P.shift2[c] = s64;
-      if (P.exclude_empty && cnt == 0) P.alive[c] = 0;
-    }
-    __builtin_amdgcn_wave_barrier();
-    for (int j = lane; j < P.D; j += 64) cr[j] = ns[j];
-    // the next iteration's distance operands for this centroid (alive[c]
-    // is final for it now)
-    if (P.do_prep) {
-      __builtin_amdgcn_wave_barrier();
-      prep_row(P.prep, c, lane, [&](int j) { return j < P.D ? ns[j] : 0.0f; });
-    }
+  if (wave == 0) finish_tail(P, c, cnt, ns, sq_s, lane);
+}
+
+// k_finish for the exact tree form with many small clusters (batches): one
+// wave per cluster, four clusters per workgroup. The wave adds the integer
+// partials of the cluster's chunks itself (a batch cluster spans few chunks)
+// or reads the totals, rounds the means as k_finish does, and runs the same
+// tail. No distance operands are written (the batch prepares its own).
+constexpr int kFinishClustersPerBlock = 4;
+__global__ __launch_bounds__(64 * kFinishClustersPerBlock) void k_finish_waves(FinishParams P) {
+  if (*P.done) return;
+  __shared__ float ns_s[kFinishClustersPerBlock][kMaxDP];
+  __shared__ __attribute__((aligned(16))) float sq_s[kFinishClustersPerBlock][kMaxDP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x * kFinishClustersPerBlock + wave;
+  if (c >= P.K) return;
+  float* ns = ns_s[wave];
+  float* nrow = P.Cnext + (size_t)c * P.DP;
+  const bool from_totals = !fx_full_pass(P.fx_state, P.n);
+  const int cnt = from_totals ? P.Ntot[c] : P.count[c];
+  long long q0 = 0, q1 = -1;
+  if (!from_totals && cnt > 0) {
+    q0 = (long long)P.start[c] / kTreeRows;
+    q1 = ((long long)P.start[c] + cnt - 1) / kTreeRows;
   }
+  const long long* tp64 = reinterpret_cast<const long long*>(P.tpart);
+  const double inv = P.cl_inv_scale[c];
+  for (int j = lane; j < P.D; j += 64) {
+    long long sj = 0;
+    if (from_totals) {
+      sj = P.Stot[(size_t)c * P.DP + j];
+    } else {
+      const long long* tp = tp64 + (size_t)c * P.DP + j;
+      for (long long q = q0; q <= q1; ++q) sj += tp[(size_t)q * P.DP];
+      P.Stot[(size_t)c * P.DP + j] = sj;
+    }
+    const double Sd = (double)sj * inv;
+    const float v = (float)(Sd / (double)(cnt > 1 ? cnt : 1));
+    ns[j] = v;
+    nrow[j] = v;
+  }
+  if (!from_totals && lane == 0) P.Ntot[c] = cnt;
+  finish_tail(P, c, cnt, ns, sq_s[wave], lane);
 }
 
 // k_stats: one workgroup.
@@ -4133,7 +4182,12 @@ hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_finish(const FinishParams& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_finish, dim3(F.K), dim3(64 * kFinishWaves), 0, s, F);
+  if (F.tree == 2 && !F.do_prep)
+    hipLaunchKernelGGL(k_finish_waves,
+                       dim3((F.K + kFinishClustersPerBlock - 1) / kFinishClustersPerBlock),
+                       dim3(64 * kFinishClustersPerBlock), 0, s, F);
+  else
+    hipLaunchKernelGGL(k_finish, dim3(F.K), dim3(64 * kFinishWaves), 0, s, F);
   if (F.stats.state) hipLaunchKernelGGL(k_stats, dim3(1), dim3(kStatsThreads), 0, s, F.stats);
   return hipGetLastError();
 }
