@@ -826,7 +826,7 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
                     : h->update_mode == kmc::UPDATE_FIXED ? h->K : (int)(h->K * S);
 
     // assign launch shape: all centroids resident in LDS when they fit
-    const size_t kResidentMax = 144 * 1024, kChunkMax = 64 * 1024;
+    const size_t kResidentMax = 144 * 1024, kChunkMax = 144 * 1024;
     h->kchunk = h->Kp;
     h->assign_lds = kmc::assign_lds(DP, h->prec, h->kchunk);
     if (h->assign_lds > kResidentMax) {
