@@ -185,21 +185,24 @@ def cpu_baseline(X, C0, n_sample=100_000, iters=2):
 # (waves per workgroup, point sets per wave, register prefetch, waves-per-SIMD
 # target, two-set kernel, values kept per lane), the kVariants table of
 # csrc/kmc_lloyd.hip (a CPU test compares them).
-ASSIGN_VARIANTS = [(8, 1, True, 0, 0, 4), (8, 1, False, 0, 0, 4), (12, 1, False, 0, 0, 4),
-                   (16, 1, False, 0, 0, 4), (12, 1, False, 6, 0, 4), (4, 2, False, 0, 0, 4),
-                   (8, 2, False, 0, 0, 4), (8, 1, False, 8, 0, 4), (8, 2, False, 0, 1, 4),
-                   (8, 2, False, 0, 1, 3), (4, 2, False, 0, 1, 4), (4, 2, False, 0, 2, 4)]
+ASSIGN_VARIANTS = [(8, 1, True, 0, 0, 4, 1), (8, 1, False, 0, 0, 4, 1),
+                   (12, 1, False, 0, 0, 4, 1), (16, 1, False, 0, 0, 4, 1),
+                   (12, 1, False, 6, 0, 4, 1), (4, 2, False, 0, 0, 4, 1),
+                   (8, 2, False, 0, 0, 4, 1), (8, 1, False, 8, 0, 4, 1),
+                   (8, 2, False, 0, 1, 4, 1), (8, 2, False, 0, 1, 3, 2),
+                   (4, 2, False, 0, 1, 4, 1), (4, 2, False, 0, 2, 4, 1),
+                   (8, 2, False, 0, 1, 3, 4), (4, 2, False, 0, 2, 3, 2)]
 FORM_CODE = {"fp32": 0, "bf16x3": 1, "bf16": 2, "fp8": 3}
 
 
 def assign_kernel_name(dp, prec, variant):
     """The distance kernel's instantiation for a form and variant, as the
     profiler names it (without the return type and parameters)."""
-    nw, ms, pf, wpe, two, topk = ASSIGN_VARIANTS[variant]
+    nw, ms, pf, wpe, two, topk, lists = ASSIGN_VARIANTS[variant]
     if two:
-        return "k_assign2<%d, %d, %d, %d, %s, %s, %d>" % (
+        return "k_assign2<%d, %d, %d, %d, %s, %s, %d, %d>" % (
             dp, FORM_CODE[prec], nw, topk, "true" if pf else "false",
-            "true" if two == 1 else "false", wpe)
+            "true" if two == 1 else "false", wpe, lists)
     return "k_assign<%d, %d, %d, %d, %s, %d, false>" % (dp, FORM_CODE[prec], nw, ms,
                                                         "true" if pf else "false", wpe)
 

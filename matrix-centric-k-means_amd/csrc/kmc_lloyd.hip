@@ -270,6 +270,29 @@ __device__ __forceinline__ void load_xfrag(const AssignParams& P, long long prow
   }
 }
 
+// The same for the single-product forms without the zero fill: the row is
+// clamped to row 0 instead, so the loads carry no lane mask and need no
+// cleared destination (the values of an invalid point are never used).
+template <int DP, int PREC>
+__device__ __forceinline__ void load_xfrag_any(const AssignParams& P, long long prow, bool valid,
+                                               int h, XFrag<DP, PREC>& x) {
+  static_assert(PREC == PREC_FP8 || PREC == PREC_BF16, "single-product forms only");
+  const long long r = valid ? prow : 0;
+  if constexpr (PREC == PREC_FP8) {
+    const uint8_t* xr = P.Xq + (size_t)r * DP;
+#pragma unroll
+    for (int s = 0; s < DP / 64; ++s) {
+      x.q[2 * s] = *reinterpret_cast<const uint4*>(xr + 64 * s + 32 * h);
+      x.q[2 * s + 1] = *reinterpret_cast<const uint4*>(xr + 64 * s + 32 * h + 16);
+    }
+    x.sc = P.xsc[r];
+  } else {
+    const uint16_t* xr = P.Xhi + (size_t)r * DP;
+#pragma unroll
+    for (int s = 0; s < DP / 16; ++s) x.hi[s] = *reinterpret_cast<const uint4*>(xr + 16 * s + 8 * h);
+  }
+}
+
 // Registers per lane of one point set's fragments.
 template <int DP, int PREC>
 constexpr int frag_regs() {
@@ -812,6 +835,37 @@ __device__ __forceinline__ void push_blockT(const kmc_f32x16& acc, uint32_t base
                  b, ninf);
 }
 
+// The same over L lists per lane: value r of the block goes to list r % L
+// (each list keeps its own T smallest; consecutive values update different
+// lists, so their dependency chains interleave). With T = 2 a value costs
+// three VALU operations instead of five for one list of four.
+template <int T, int L>
+__device__ __forceinline__ void push_blockL(const kmc_f32x16& acc, uint32_t base, uint32_t keepv,
+                                            float (&b)[L][T], float ninf) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+    topT_push<T>(__uint_as_float((__float_as_uint(acc[r]) & keepv) |
+                                 (base + (uint32_t)(8 * (r >> 2) + (r & 3)))),
+                 b[r % L], ninf);
+}
+
+// The four smallest values kept by a lane's L lists (kFar where fewer) and
+// t = the smallest of the lists' T-th values: every value a list dropped is
+// at least t.
+template <int T, int L>
+__device__ __forceinline__ void merge_lists(const float (&l)[L][T], float ninf, float (&v)[4],
+                                            float& t) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = k < T ? l[0][k < T ? k : 0] : kFar;
+  t = l[0][T - 1];
+#pragma unroll
+  for (int j = 1; j < L; ++j) {
+    t = __builtin_amdgcn_fmed3f(ninf, t, l[j][T - 1]);
+#pragma unroll
+    for (int k = 0; k < T; ++k) top4_push(l[j][k], v, ninf);
+  }
+}
+
 // The A operand of one 32-centroid block as a lane reads it from LDS (its
 // k-slices of -2c for centroid row cb + (lane & 31), and the block's
 // ||c||^2 row as the accumulator start).
@@ -962,30 +1016,34 @@ __device__ __forceinline__ void band_out(const AssignParams& P, const float (&bl
 }
 
 // Band test and output of both point sets of a two-set sweep in one pass:
-// lane half h serves set h. Each lane keeps its own k-half's list of set h
-// and receives the other half's list of the same set (one exchange of T
-// values), so the merge and the band test run once per lane instead of once
-// per set on every lane.
-template <int T>
-__device__ __forceinline__ void band_out2(const AssignParams& P, const float (&la)[T],
-                                          const float (&lb)[T], float x2, float xd, bool valid,
-                                          long long prow, float cmax, float Ecm, float cdmax,
-                                          float ninf, int h, int lane, int* q2, int* qa,
-                                          unsigned* q2c_s, unsigned* qac_s) {
-  float v[4], o[T];
+// lane half h serves set h. Each lane merges its own k-half's lists of set h
+// and receives the other half's merged lists of the same set (one exchange
+// of up to four values and their bound t), so the merge and the band test
+// run once per lane instead of once per set on every lane.
+template <int T, int L>
+__device__ __forceinline__ void band_out2(const AssignParams& P, const float (&la)[L][T],
+                                          const float (&lb)[L][T], float x2, float xd,
+                                          bool valid, long long prow, float cmax, float Ecm,
+                                          float cdmax, float ninf, int h, int lane, int* q2,
+                                          int* qa, unsigned* q2c_s, unsigned* qac_s) {
+  // M values of each lane's merged lists cross to the other half
+  constexpr int M = L * T < 4 ? L * T : 4;
+  float v[4], s4[4], o[M], t, ts;
+  merge_lists<T, L>(h ? lb : la, ninf, v, t);
+  merge_lists<T, L>(h ? la : lb, ninf, s4, ts);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const float own = k < T ? (h ? lb[k < T ? k : 0] : la[k < T ? k : 0]) : kFar;
-    v[k] = k < T ? __uint_as_float(__float_as_uint(own) | (uint32_t)(4 * h)) : kFar;
-  }
+  for (int k = 0; k < 4; ++k)
+    v[k] = k < M ? __uint_as_float(__float_as_uint(v[k]) | (uint32_t)(4 * h)) : kFar;
 #pragma unroll
-  for (int k = 0; k < T; ++k) {
-    const float send = __uint_as_float(__float_as_uint(h ? la[k] : lb[k]) | (uint32_t)(4 * h));
-    o[k] = __shfl_xor(send, 32, 64);
-  }
-  const float t = __builtin_amdgcn_fmed3f(ninf, v[T - 1], o[T - 1]);
+  for (int k = 0; k < M; ++k)
+    o[k] = __shfl_xor(__uint_as_float(__float_as_uint(s4[k]) | (uint32_t)(4 * h)), 32, 64);
+  // dropped values of either half are at least t (bit 2 of a packed value
+  // is free before the half is added, so adding it keeps the order)
+  t = __builtin_amdgcn_fmed3f(
+      ninf, __uint_as_float(__float_as_uint(t) | (uint32_t)(4 * h)),
+      __shfl_xor(__uint_as_float(__float_as_uint(ts) | (uint32_t)(4 * h)), 32, 64));
 #pragma unroll
-  for (int k = 0; k < T; ++k) top4_push(o[k], v, ninf);
+  for (int k = 0; k < M; ++k) top4_push(o[k], v, ninf);
   float b1 = v[0];
   float b2 = T >= 2 ? v[1] : __builtin_amdgcn_fmed3f(ninf, v[1], t);
   float b3 = T >= 3 ? v[2] : __builtin_amdgcn_fmed3f(ninf, v[2], t);
@@ -1046,12 +1104,14 @@ __device__ __forceinline__ void band_out2(const AssignParams& P, const float (&l
 // row read from LDS feed both units' MFMA chains, which halves the LDS
 // traffic per distance against one unit per wave, and the chains of block s
 // are issued before the epilogue of block s-1 (two accumulator pairs
-// alternate). The epilogue keeps the T smallest packed values per lane
-// (1 + T VALU operations per value; band_out turns the lists into bounds).
+// alternate). The epilogue keeps the T smallest packed values of each of L
+// lists per lane (1 + T VALU operations per value; band_out2 turns the lists
+// into bounds: the two smallest distances stay exact for T >= 2, the third
+// and fourth are exact unless one list held three of the four smallest).
 // The A fragments are read from LDS step by step, so the point fragments of
 // both units, two accumulator pairs and one A step fit the register budget
 // of two or three waves per SIMD.
-template <int DP, int PREC, int NW, int T, bool PF, bool PIPE, int WPE>
+template <int DP, int PREC, int NW, int T, bool PF, bool PIPE, int WPE, int L>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_assign2(AssignParams P) {
   static_assert(PREC == PREC_BF16 || PREC == PREC_FP8, "single-product forms only");
@@ -1091,8 +1151,8 @@ void k_assign2(AssignParams P) {
     const long long rb = ra + 32;
     const bool va = ra < P.n;
     const bool vb = uu + 1 < w1 && rb < P.n;
-    load_xfrag<DP, PREC>(P, ra, va, h, f0);
-    load_xfrag<DP, PREC>(P, rb, vb, h, f1);
+    load_xfrag_any<DP, PREC>(P, ra, va, h, f0);
+    load_xfrag_any<DP, PREC>(P, rb, vb, h, f1);
     row = h ? rb : ra;
     valid = h ? vb : va;
     x2 = valid ? P.xn2[row] : 0.0f;
@@ -1134,12 +1194,14 @@ void k_assign2(AssignParams P) {
   const int nblk = P.Kp / 32;  // even: Kp is a multiple of 64
 
   for (long long u = w0; u < w1; u += 2) {
-    float ba[T], bb[T];
+    float ba[L][T], bb[L][T];
 #pragma unroll
-    for (int k = 0; k < T; ++k) {
-      ba[k] = kFar;
-      bb[k] = kFar;
-    }
+    for (int j = 0; j < L; ++j)
+#pragma unroll
+      for (int k = 0; k < T; ++k) {
+        ba[j][k] = kFar;
+        bb[j][k] = kFar;
+      }
     // Block step: the block's LDS reads are issued first; the first half of
     // the previous block's epilogue (set 0's values) runs while they are in
     // flight; then the block's MFMA chains, interleaved with the second half
@@ -1150,10 +1212,10 @@ void k_assign2(AssignParams P) {
       ABlock<DP, PREC> ab;
       load_ablock<DP, PREC>(rows, cn_s, 32 * blk, p, h, ab);
       __builtin_amdgcn_sched_barrier(0);
-      if (blk > 0) push_blockT<T>(p0, (uint32_t)(32 * (blk - 1)), keepv, ba, ninf);
+      if (blk > 0) push_blockL<T, L>(p0, (uint32_t)(32 * (blk - 1)), keepv, ba, ninf);
       __builtin_amdgcn_sched_barrier(0);
       mfma_chain2<DP, PREC>(ab, x0, x1, c0, c1);
-      if (blk > 0) push_blockT<T>(p1, (uint32_t)(32 * (blk - 1)), keepv, bb, ninf);
+      if (blk > 0) push_blockL<T, L>(p1, (uint32_t)(32 * (blk - 1)), keepv, bb, ninf);
     };
     kmc_f32x16 pA0, pA1, pB0, pB1;
     int blk = 1;
@@ -1171,8 +1233,8 @@ void k_assign2(AssignParams P) {
         ABlock<DP, PREC> ab;
         load_ablock<DP, PREC>(rows, cn_s, 32 * blk, p, h, ab);
         mfma_chain2<DP, PREC>(ab, x0, x1, pA0, pA1);
-        push_blockT<T>(pA0, (uint32_t)(32 * blk), keepv, ba, ninf);
-        push_blockT<T>(pA1, (uint32_t)(32 * blk), keepv, bb, ninf);
+        push_blockL<T, L>(pA0, (uint32_t)(32 * blk), keepv, ba, ninf);
+        push_blockL<T, L>(pA1, (uint32_t)(32 * blk), keepv, bb, ninf);
       }
       ABlock<DP, PREC> ab;  // blk == nblk - 1: the last use of this pair's fragments
       load_ablock<DP, PREC>(rows, cn_s, 32 * blk, p, h, ab);
@@ -1196,9 +1258,9 @@ void k_assign2(AssignParams P) {
       // passes while the last epilogues and the band test of this pair run
       if (u + 2 < w1) load_pair(u + 2, x0, x1, prow, pvalid, px2, pxd);
     }
-    push_blockT<T>(pB0, (uint32_t)(32 * blk), keepv, ba, ninf);
-    push_blockT<T>(pB1, (uint32_t)(32 * blk), keepv, bb, ninf);
-    band_out2<T>(P, ba, bb, cx2, cxd, cvalid, crow, cmax, Ecm, cdmax, ninf, h, lane, q2, qa,
+    push_blockL<T, L>(pB0, (uint32_t)(32 * blk), keepv, ba, ninf);
+    push_blockL<T, L>(pB1, (uint32_t)(32 * blk), keepv, bb, ninf);
+    band_out2<T, L>(P, ba, bb, cx2, cxd, cvalid, crow, cmax, Ecm, cdmax, ninf, h, lane, q2, qa,
                  &q2c_s, &qac_s);
   }
   __syncthreads();
@@ -3922,8 +3984,10 @@ size_t assign_lds(int DP, int prec, int kchunk) {
 // Launch variants of the distance kernel: (waves per workgroup, point sets
 // per wave, prefetch, waves per SIMD the register allocation targets (0 =
 // none), kernel (0 = k_assign, 1 = the two-set k_assign2, 2 = its form
-// without the block software pipeline), values kept per lane). Two point sets per wave need both fragment sets in registers; where
-// they do not fit (kFragRegs > 64) the variant is not built. Targets above
+// without the block software pipeline), values kept per list, lists per
+// lane (k_assign2 only)). Two point sets per wave need both fragment sets in
+// registers; where they do not fit (kFragRegs > 64) the variant is not
+// built. Targets above
 // four waves per SIMD select the register-lean loop and let two workgroups
 // share a CU when their centroid copies fit the LDS together. k_assign2 is
 // built for the single-product forms only (bf16 up to DP = 128, fp8) and
@@ -3935,12 +3999,14 @@ struct VariantShape {
   int wpe;
   int two;
   int topk;
+  int lists = 1;
 };
 constexpr VariantShape kVariants[] = {
-    {8, 1, true, 0, 0, 4},  {8, 1, false, 0, 0, 4},  {12, 1, false, 0, 0, 4},
-    {16, 1, false, 0, 0, 4}, {12, 1, false, 6, 0, 4}, {4, 2, false, 0, 0, 4},
-    {8, 2, false, 0, 0, 4},  {8, 1, false, 8, 0, 4},  {8, 2, false, 0, 1, 4},
-    {8, 2, false, 0, 1, 3},  {4, 2, false, 0, 1, 4},  {4, 2, false, 0, 2, 4}};
+    {8, 1, true, 0, 0, 4},     {8, 1, false, 0, 0, 4},    {12, 1, false, 0, 0, 4},
+    {16, 1, false, 0, 0, 4},   {12, 1, false, 6, 0, 4},   {4, 2, false, 0, 0, 4},
+    {8, 2, false, 0, 0, 4},    {8, 1, false, 8, 0, 4},    {8, 2, false, 0, 1, 4},
+    {8, 2, false, 0, 1, 3, 2}, {4, 2, false, 0, 1, 4},    {4, 2, false, 0, 2, 4},
+    {8, 2, false, 0, 1, 3, 4}, {4, 2, false, 0, 2, 3, 2}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static_assert(kNumVariants <= KMC_MAX_VARIANTS, "variant table exceeds kmc_tuning");
 
@@ -3951,7 +4017,7 @@ static const void* kernel_of() {
     constexpr bool kBuilt = (PREC == PREC_BF16 && DP <= 128) || PREC == PREC_FP8;
     if constexpr (kBuilt)
       return reinterpret_cast<const void*>(
-          k_assign2<DP, PREC, S.nw, S.topk, S.pf, S.two == 1, S.wpe>);
+          k_assign2<DP, PREC, S.nw, S.topk, S.pf, S.two == 1, S.wpe, S.lists>);
     else
       return nullptr;
   } else {
@@ -3980,6 +4046,8 @@ static const void* kernel_of_v(int v) {
     case 9: return kernel_of<DP, PREC, 9>();
     case 10: return kernel_of<DP, PREC, 10>();
     case 11: return kernel_of<DP, PREC, 11>();
+    case 12: return kernel_of<DP, PREC, 12>();
+    case 13: return kernel_of<DP, PREC, 13>();
     default: return nullptr;
   }
 }

@@ -132,13 +132,14 @@ def test_bench_variant_table_matches_the_kernel_source():
     source (same shapes, same order as kmc_autotune reports them)."""
     src = open(os.path.join(ROOT, "matrix-centric-k-means_amd", "csrc", "kmc_lloyd.hip")).read()
     body = re.search(r"kVariants\[\]\s*=\s*\{(.*?)\};", src, flags=re.S).group(1)
-    shapes = [(int(a), int(b), c == "true", int(d), int(e), int(f))
-              for a, b, c, d, e, f in re.findall(
-                  r"\{\s*(\d+),\s*(\d+),\s*(true|false),\s*(\d+),\s*(\d+),\s*(\d+)\s*\}",
-                  body)]
+    shapes = [(int(a), int(b), c == "true", int(d), int(e), int(f), int(g) if g else 1)
+              for a, b, c, d, e, f, g in re.findall(
+                  r"\{\s*(\d+),\s*(\d+),\s*(true|false),\s*(\d+),\s*(\d+),\s*(\d+)"
+                  r"(?:,\s*(\d+))?\s*\}", body)]
     import sys
     sys.path.insert(0, ROOT)
     import bench
     assert shapes == bench.ASSIGN_VARIANTS
+    assert bench.assign_kernel_name(128, "bf16", 9) == "k_assign2<128, 2, 8, 3, false, true, 0, 2>"
     assert bench.assign_kernel_name(128, "bf16", 1) == "k_assign<128, 2, 8, 1, false, 0, false>"
-    assert bench.assign_kernel_name(128, "bf16", 8) == "k_assign2<128, 2, 8, 4, false, true, 0>"
+    assert bench.assign_kernel_name(128, "bf16", 8) == "k_assign2<128, 2, 8, 4, false, true, 0, 1>"
