@@ -71,6 +71,10 @@ struct kmc_handle {
   int* Ntot = nullptr;
   long long* qpart = nullptr;
   int* cpart = nullptr;
+  // the moved points' changes to the totals, kDeltaCopies copies
+  long long* Sdel = nullptr;
+  long long* Qdel = nullptr;
+  int* Ndel = nullptr;
   int fx_ds = 0, fx_nslice = 0, fx_wgps = 0;
   float fx_scale = 1.0f;
   double fx_qscale = 1.0;
@@ -287,6 +291,13 @@ kmc_status alloc_all(kmc_handle* h) {
     KMC_CK(dalloc(&h->Ntot, Kp));
     KMC_CK(dalloc(&h->qpart, ws * Kp));
     KMC_CK(dalloc(&h->cpart, ws * Kp));
+    const size_t nc = (size_t)kmc::kDeltaCopies;
+    KMC_CK(dalloc(&h->Sdel, nc * Kp * DP));
+    KMC_CK(dalloc(&h->Qdel, nc * Kp));
+    KMC_CK(dalloc(&h->Ndel, nc * Kp));
+    KMC_CK(hipMemsetAsync(h->Sdel, 0, nc * Kp * DP * sizeof(long long), h->stream));
+    KMC_CK(hipMemsetAsync(h->Qdel, 0, nc * Kp * sizeof(long long), h->stream));
+    KMC_CK(hipMemsetAsync(h->Ndel, 0, nc * Kp * sizeof(int), h->stream));
   }
   KMC_CK(dalloc(&h->conv, K));
   KMC_CK(dalloc(&h->shift2, K));
@@ -309,7 +320,8 @@ void free_all(kmc_handle* h) {
                   h->count, h->start, h->members, h->ipart, h->tpart, h->conv,
                   h->shift2, h->alive, h->state, h->xn64, h->fpart, h->qpart, h->cpart,
                   h->prev, h->Stot, h->Qtot, h->Ntot,
-                  h->Xq, h->xsc, h->Cq, h->xdn, h->cdn, h->Phi, h->Plo};
+                  h->Xq, h->xsc, h->Cq, h->xdn, h->cdn, h->Phi, h->Plo,
+                  h->Sdel, h->Qdel, h->Ndel};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -501,6 +513,9 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nu
   up.nslice = h->fx_nslice;
   up.wgps = h->fx_wgps;
   up.ds = h->fx_ds;
+  up.Sdel = h->Sdel;
+  up.Qdel = h->Qdel;
+  up.Ndel = h->Ndel;
   if (u0) KMC_CK(record_event(h->stream, u0));
   KMC_CK(kmc::launch_update(up, h->update_mode, h->stream));
   if (u1) KMC_CK(record_event(h->stream, u1));
@@ -540,6 +555,9 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nu
   fp.ds = h->fx_ds;
   fp.inv_fscale = 1.0 / (double)h->fx_scale;
   fp.inv_qscale = 1.0 / h->fx_qscale;
+  fp.Sdel = h->Sdel;
+  fp.Qdel = h->Qdel;
+  fp.Ndel = h->Ndel;
   kmc::StatsParams& st = fp.stats;
   st.ipart = h->ipart;
   st.nparts = h->nparts;

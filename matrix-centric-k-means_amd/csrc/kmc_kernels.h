@@ -225,7 +225,16 @@ struct UpdateParams {
   // counts the moved points; iterations that are not a full pass take the
   // moved points only (Qtot may be null: no fixed-point ||x||^2).
   const float* cl_scale;
+  // incremental fixed-point update: kDeltaCopies copies of the totals'
+  // changes (Kp x DP, Kp, Kp each; null: the moved rows go to the totals
+  // directly); a wave adds to copy (its index) mod kDeltaCopies, so the
+  // atomics on a much-moved cluster's row spread over the copies, and
+  // k_finish folds them into the totals and clears them
+  long long* Sdel;
+  long long* Qdel;
+  int* Ndel;
 };
+constexpr int kDeltaCopies = 8;
 
 
 struct StatsParams {
@@ -282,6 +291,11 @@ struct FinishParams {
   const DevState* fx_state;
   PrepParams prep;
   StatsParams stats;  // k_stats, launched after k_finish
+  // the incremental update's delta copies (null: none), folded into the
+  // totals and cleared here
+  long long* Sdel;
+  long long* Qdel;
+  int* Ndel;
 };
 
 struct LoadParams {

@@ -3001,27 +3001,32 @@ __device__ void update_moved_points(const UpdateParams& P) {
   const int lane = threadIdx.x & 63;
   const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
   const long long w0 = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // the totals this wave adds to: its delta copy, or the totals themselves
+  const int copy = (int)(w0 % kDeltaCopies);
+  long long* const St = P.Sdel ? P.Sdel + (size_t)copy * P.Kp * P.DP : P.Stot;
+  long long* const Qt = P.Sdel ? (P.Qtot ? P.Qdel + (size_t)copy * P.Kp : nullptr) : P.Qtot;
+  int* const Nt = P.Sdel ? P.Ndel + (size_t)copy * P.Kp : P.Ntot;
   // one moved point: its converted row out of the old cluster's totals and
   // into the new one's, a lane per dimension (both clusters are of the
   // point's problem: one scale)
   auto move_point = [&](long long pi, int nl, int ol) {
     const float fs = P.cl_scale ? P.cl_scale[nl] : P.fscale;
     const float* xr = P.X + (size_t)pi * P.DP;
-    unsigned long long* sn = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)nl * P.DP);
-    unsigned long long* so = reinterpret_cast<unsigned long long*>(P.Stot + (size_t)ol * P.DP);
+    unsigned long long* sn = reinterpret_cast<unsigned long long*>(St + (size_t)nl * P.DP);
+    unsigned long long* so = reinterpret_cast<unsigned long long*>(St + (size_t)ol * P.DP);
     for (int j = lane; j < P.D; j += 64) {
       const long long q = __float2ll_rn(xr[j] * fs);
       atomicAdd(sn + j, (unsigned long long)q);
       atomicAdd(so + j, (unsigned long long)(-q));
     }
     if (lane == 0) {
-      if (P.Qtot) {
+      if (Qt) {
         const long long qq = __double2ll_rn(P.xn64[pi] * P.qscale);
-        atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + nl), (unsigned long long)qq);
-        atomicAdd(reinterpret_cast<unsigned long long*>(P.Qtot + ol), (unsigned long long)(-qq));
+        atomicAdd(reinterpret_cast<unsigned long long*>(Qt + nl), (unsigned long long)qq);
+        atomicAdd(reinterpret_cast<unsigned long long*>(Qt + ol), (unsigned long long)(-qq));
       }
-      atomicAdd(P.Ntot + nl, 1);
-      atomicAdd(P.Ntot + ol, -1);
+      atomicAdd(Nt + nl, 1);
+      atomicAdd(Nt + ol, -1);
     }
   };
   // a wave takes points base + u * nw * 64 + lane, u < UD: the labels and
@@ -3655,6 +3660,24 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
     const bool from_totals = P.Stot && !fx_full_pass(P.stats.state, P.n);
     long long qsum = 0;
     int nsum = 0;
+    // moved points of an incremental iteration: wave w takes delta copy w
+    // (one per wave) and clears it for the next iteration
+    static_assert(kDeltaCopies == kFinishWaves, "one delta copy per wave");
+    const bool deltas = from_totals && P.Sdel;
+    if (deltas) {
+      long long* sd = P.Sdel + ((size_t)wave * P.Kp + c) * P.DP;
+      for (int j = lane; j < P.D; j += 64) {
+        fs_s[wave][j] = sd[j];
+        sd[j] = 0;
+      }
+      if (lane == 0) {
+        const size_t o = (size_t)wave * P.Kp + c;
+        qsum = P.Qdel ? P.Qdel[o] : 0;
+        nsum = P.Ndel[o];
+        if (P.Qdel) P.Qdel[o] = 0;
+        P.Ndel[o] = 0;
+      }
+    }
     // wave w adds the partials of workgroups w, w+8, ...; four loads in
     // flight per dimension (integer sums: the order does not matter)
     for (int j = lane; j < P.D && !from_totals; j += 64) {
@@ -3690,6 +3713,17 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
       if (from_totals) {
         qt = P.Qtot[c];
         nt = P.Ntot[c];
+        if (deltas) {
+#pragma unroll
+          for (int w = 0; w < kFinishWaves; ++w) {
+            qt += q_s[w];
+            nt += n_s[w];
+          }
+          if (lane == 0) {
+            P.Qtot[c] = qt;
+            P.Ntot[c] = nt;
+          }
+        }
       } else {
 #pragma unroll
         for (int w = 0; w < kFinishWaves; ++w) {
@@ -3706,6 +3740,11 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
         long long sj = 0;
         if (from_totals) {
           sj = P.Stot[(size_t)c * P.DP + j];
+          if (deltas) {
+#pragma unroll
+            for (int w = 0; w < kFinishWaves; ++w) sj += fs_s[w][j];
+            P.Stot[(size_t)c * P.DP + j] = sj;
+          }
         } else {
 #pragma unroll
           for (int w = 0; w < kFinishWaves; ++w) sj += fs_s[w][j];
