@@ -277,8 +277,9 @@ kmc_status kmc_probe_mfma(int32_t precision, uint32_t d, const float* A, const f
 
 /* Launch-variant autotuner for the fused distance kernel (no counterpart in
  * the reference, whose stages are fixed library calls): times every variant
- * (waves per workgroup, point sets per wave, register prefetch, values kept
- * per lane) on the handle's points and current centroids -- the distance
+ * (waves per workgroup, point sets per wave, register prefetch, ranking
+ * lists per lane and values kept per list) on the handle's points and
+ * current centroids -- the distance
  * kernel plus the refinement of the points its band leaves -- and keeps the
  * fastest for later kmc_run/kmc_assign. Variants whose registers spill, or
  * that need every centroid in LDS where they do not fit, are not offered
