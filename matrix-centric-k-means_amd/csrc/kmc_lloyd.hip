@@ -818,6 +818,8 @@ void k_assign(AssignParams P) {
 // ---------------------------------------------------------------------------
 // Insert v into a running list of the T smallest values b0 <= ... <= b(T-1)
 // (T med3 operations; the k-th smallest of {b, v} is med3(b(k-1), b(k), v)).
+// Insert packed value v into the ascending list b of the T smallest (T
+// v_med3 operations; the value that falls off the end is dropped).
 template <int T>
 __device__ __forceinline__ void topT_push(float v, float (&b)[T], float ninf) {
 #pragma unroll
@@ -825,20 +827,11 @@ __device__ __forceinline__ void topT_push(float v, float (&b)[T], float ninf) {
   b[0] = __builtin_amdgcn_fmed3f(ninf, b[0], v);
 }
 
-template <int T>
-__device__ __forceinline__ void push_blockT(const kmc_f32x16& acc, uint32_t base, uint32_t keepv,
-                                            float (&b)[T], float ninf) {
-#pragma unroll
-  for (int r = 0; r < 16; ++r)
-    topT_push<T>(__uint_as_float((__float_as_uint(acc[r]) & keepv) |
-                                 (base + (uint32_t)(8 * (r >> 2) + (r & 3)))),
-                 b, ninf);
-}
-
-// The same over L lists per lane: value r of the block goes to list r % L
-// (each list keeps its own T smallest; consecutive values update different
-// lists, so their dependency chains interleave). With T = 2 a value costs
-// three VALU operations instead of five for one list of four.
+// Rank one block's 16 accumulator values of a lane (each with its centroid
+// index packed into the low mantissa bits) into L lists: value r goes to
+// list r % L, each list keeps its own T smallest, and consecutive values
+// update different lists, so their dependency chains interleave. A value
+// costs 1 + T VALU operations (five for one list of four, four with T = 3).
 template <int T, int L>
 __device__ __forceinline__ void push_blockL(const kmc_f32x16& acc, uint32_t base, uint32_t keepv,
                                             float (&b)[L][T], float ninf) {
@@ -924,94 +917,6 @@ __device__ __forceinline__ void mfma_chain2(const ABlock<DP, PREC>& ab,
       a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(kmc_bf16x8, x1.hi[s]),
                                                    s == 0 ? ab.c : a1, 0, 0, 0);
     }
-  }
-}
-
-// Band test and output of one point set after its sweep. bl holds this
-// lane's T smallest packed values of its k-half; the two halves' lists are
-// merged into the four smallest known values m1..m4, and t = the smaller of
-// the two halves' T-th values bounds every value that was not kept, so the
-// k-th smallest distance satisfies b_k >= min(m_k, t) (exact for k <= T).
-// The band test uses these bounds: a point is only sent to a refinement with
-// fewer candidates when no other centroid can win. With T = 4 this is the
-// plain top-4 test. Every lane of the wave calls it (ballots, shuffles).
-template <int T, bool kAddX2, bool kMeas>
-__device__ __forceinline__ void band_out(const AssignParams& P, const float (&bl)[T], float x2,
-                                         float xd, bool valid, long long prow, float cmax,
-                                         float Ecm, float cdmax, float ninf, int h, int lane,
-                                         int* q2, int* qa, unsigned* q2c_s, unsigned* qac_s,
-                                         int Kv, int cbase) {
-  float v[4], o[T];
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    v[k] = k < T ? __uint_as_float(__float_as_uint(bl[k < T ? k : 0]) | (uint32_t)(4 * h))
-                 : kFar;
-#pragma unroll
-  for (int k = 0; k < T; ++k) o[k] = __shfl_xor(v[k], 32, 64);
-  const float t = __builtin_amdgcn_fmed3f(ninf, v[T - 1], o[T - 1]);
-#pragma unroll
-  for (int k = 0; k < T; ++k) top4_push(o[k], v, ninf);
-  float b1 = v[0];
-  float b2 = T >= 2 ? v[1] : __builtin_amdgcn_fmed3f(ninf, v[1], t);
-  float b3 = T >= 3 ? v[2] : __builtin_amdgcn_fmed3f(ninf, v[2], t);
-  float b4 = T >= 4 ? v[3] : __builtin_amdgcn_fmed3f(ninf, v[3], t);
-  const int i1 = (int)(__float_as_uint(v[0]) & P.idx_mask);
-  const int i2 = (int)(__float_as_uint(v[1]) & P.idx_mask);
-  const int i3 = (int)(__float_as_uint(v[2]) & P.idx_mask);
-  if (!kAddX2) {
-    b1 += x2;
-    b2 += x2;
-    b3 += x2;
-    b4 += x2;
-  }
-  // b1 <= b2 <= b3 <= b4 bound the full squared distances (plus the packed
-  // index bits) from below; E bounds the MFMA error of each, gref the
-  // reference's own float error plus the packing perturbation, both relative
-  // to the distances.
-  bool flag2 = false, flagA = false, few3 = false;
-  if (h == 0 && valid) {
-    const float xn = sqrtf(x2);
-    float E = P.alpha * xn * cmax + Ecm + P.xpk * x2;
-    if (kMeas && P.hop > 0.0f) {
-      // |stored(-2c).stored(x) - (-2c).x| <= ||dc|| ||x|| + ||-2c|| ||dx|| + ||dc|| ||dx||
-      E += P.hop * (cdmax * xn + 2.0f * cmax * xd + cdmax * xd);
-    }
-    const float twoE = 2.0f * E;
-    const float b1p = fmaxf(b1 + E, 0.0f);
-    // clear(b): every centroid ranked at b or later loses to the first
-    auto clear = [&](float bb) {
-      return (bb >= kFarTest) || (bb - b1 > twoE + P.gref * (fmaxf(bb + E, 0.0f) + b1p));
-    };
-    const bool uniq = clear(b2);
-    const bool two = clear(b3);
-    const bool three = clear(b4);
-    // a point without any finite distance goes to the all-k pass, which
-    // applies the first-min rule
-    const bool sane = (b1 < kFarTest) && (i1 < Kv);
-    P.labels[prow] = cbase + (sane ? i1 : 0);
-    flag2 = sane && !uniq && (two || three);
-    flagA = !sane || (!uniq && !two && !three);
-    few3 = !two;
-  }
-  const uint64_t m2 = __ballot(flag2);
-  const uint64_t mA = __ballot(flagA);
-  if (m2) {
-    const int leader = __ffsll((unsigned long long)m2) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(q2c_s, (unsigned)__popcll(m2));
-    base = __shfl(base, leader, 64);
-    if (flag2) {
-      const unsigned pos = base + mbcnt64(m2);
-      *reinterpret_cast<int4*>(q2 + 4 * (size_t)pos) =
-          make_int4((int)prow, cbase + i1, cbase + i2, few3 ? cbase + i3 : -1);
-    }
-  }
-  if (mA) {
-    const int leader = __ffsll((unsigned long long)mA) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(qac_s, (unsigned)__popcll(mA));
-    base = __shfl(base, leader, 64);
-    if (flagA) qa[base + mbcnt64(mA)] = (int)prow;
   }
 }
 
