@@ -188,6 +188,32 @@ def test_every_launch_variant_gives_the_same_labels(kmc_mod, oracle, c2_data, pr
             assert np.array_equal(lab, want), v
 
 
+@pytest.mark.parametrize("prec", ("bf16", "fp8"))
+@pytest.mark.parametrize("d,k", [(20, 40), (64, 130), (100, 300)])
+def test_every_launch_variant_on_other_shapes(kmc_mod, oracle, prec, d, k):
+    """Every launch variant offered for a shape -- the two-set kernel's
+    ranking-list forms included, over padded widths (DP 32/64/128, fp8 at
+    least 64) and centroid counts that are not a multiple of 64 -- gives the
+    reference's labels; half the centroids are pulled towards a neighbour so
+    that many points fall in the margin band."""
+    from kmc import datagen
+    X, _, _ = datagen.blobs(20_000, d, k, 20261204 + d, -4.0, 4.0, 0.6, 1.4)
+    C = datagen.init_rows(X, k, 11).astype(np.float32)
+    C[1::2] = (0.8 * C[1::2] + 0.2 * C[0::2]).astype(np.float32)
+    want, nclose = _reference_labels(oracle, X, C)
+    with kmc_mod.Engine(X.shape[0], d, k, precision=prec) as e:
+        e.set_points(X)
+        e.set_centroids(C)
+        t = e.autotune()
+        _say("shape", d, k, prec, "close", nclose, t["ms"])
+        offered = [v for v in range(len(t["ms"])) if t["ms"][v] >= 0]
+        assert offered
+        for v in offered:
+            e.set_variant(v)
+            lab, info = e.assign(C)
+            assert np.array_equal(lab, want), (v, int(np.sum(lab != want)))
+
+
 @pytest.mark.parametrize("prec", PRECS)
 def test_pruned_all_k_pass_gives_the_same_labels(kmc_mod, oracle, c2_data, prec):
     """The all-k refinement with and without its bf16x3 MFMA pruning stage
