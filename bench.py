@@ -20,10 +20,9 @@ the line reports the headline's iterations/s as ``value`` and the other forms
 beside it, checks that every form ends with bit-identical centroids and
 labels, and measures the inertia error of the headline form against an FP64
 oracle. ``--update fixed`` (default) sums each cluster exactly in 64-bit
-fixed point and keeps the totals across the run: an iteration streams every
-point only at the start of the run or after more than an eighth of the points
-moved, and otherwise subtracts and adds the rows of the moved points (the
-totals equal those of a full pass, bit for bit); ``--update tree`` sorts the
+fixed point and keeps the totals across the run: the first iteration streams
+every point, the others subtract and add the rows of the moved points only
+(the totals equal those of a full pass, bit for bit); ``--update tree`` sorts the
 members and sums each cluster in FP64 every iteration; ``--update ordered``
 reproduces the reference's float summation order.
 
@@ -562,7 +561,7 @@ def main():
                    "update": {"ordered": "ordered (reference summation order)",
                               "tree": "tree (FP64 chunk partials)",
                               "fixed": "fixed (exact fixed-point totals, moved points "
-                                       "only once at most N/8 move)"}[a.update]},
+                                       "only after the first iteration)"}[a.update]},
         "roofline": roof,
         "roofline_second_kernel": roof_other,
         "assign_variant": {"chosen": head["tuning"]["chosen"],

@@ -136,8 +136,9 @@ typedef struct {
   double ms_update;      /* sum of centroid-update kernel times (same flag)   */
   uint64_t update_launches; /* update-kernel launches timed                   */
   /* KMC_UPDATE_FIXED keeps exact cluster totals across the iterations of a
-     run and, once at most an eighth of the points moved in the previous
-     iteration, updates them from the moved points only: */
+     run: the first iteration streams every point, the others update the
+     totals from the moved points only (the batch engine's exact tree form:
+     once at most an eighth of the points moved in the previous iteration): */
   uint64_t full_passes;  /* iterations whose update streamed every point      */
   uint64_t moved_points; /* points moved, summed over the other iterations    */
 } kmc_result;
@@ -288,7 +289,8 @@ kmc_status kmc_probe_mfma(int32_t precision, uint32_t d, const float* A, const f
 typedef struct {
   int32_t nvariants;
   int32_t chosen;
-  float ms[KMC_MAX_VARIANTS];     /* mean assignment time per variant, -1 if not offered */
+  float ms[KMC_MAX_VARIANTS];     /* assignment time per variant (best of two rounds' means
+                                     of three), -1 if not offered */
   int32_t threads[KMC_MAX_VARIANTS];
   int32_t tile[KMC_MAX_VARIANTS]; /* points per workgroup tile */
 } kmc_tuning;

@@ -558,6 +558,8 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nu
   fp.Sdel = h->Sdel;
   fp.Qdel = h->Qdel;
   fp.Ndel = h->Ndel;
+  fp.labels = h->labels;
+  fp.prev = h->prev;
   kmc::StatsParams& st = fp.stats;
   st.ipart = h->ipart;
   st.nparts = h->nparts;
@@ -574,6 +576,7 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nu
   st.cost_ratio = (rules & kmc::RULE_COST_RATIO) ? 1 : 0;
   st.incremental = h->Stot ? 1 : 0;
   st.n = (long long)h->n;
+  st.fx_modes = fixed ? 1 : 0;
   KMC_CK(kmc::launch_finish(fp, h->stream));
   if (f1) KMC_CK(record_event(h->stream, f1));
   return KMC_OK;
@@ -824,7 +827,7 @@ kmc_status kmc_create(const kmc_config* cfg, kmc_handle** out) {
     if (h->update_mode == kmc::UPDATE_FIXED) {
       // slice width: the widest whose per-workgroup accumulators fit the
       // LDS; none (very large K): the tree update instead
-      const size_t kFxLdsMax = 150 * 1024;
+      const size_t kFxLdsMax = 160 * 1024 - 256;  // the kernel's static LDS beside it
       h->fx_ds = 0;
       for (int ds : {64, 32, 16})
         if (ds <= DP && kmc::update_fx_lds(h->Kp, ds) <= kFxLdsMax) {
@@ -1178,41 +1181,49 @@ kmc_status kmc_autotune(kmc_handle* h, kmc_tuning* out) {
   KMC_CK(hipSetDevice(h->dev));
   kmc_status st = reset_state(h, 1);
   if (st != KMC_OK) return st;
-  hipEvent_t e[4];
-  for (int i = 0; i < 4; ++i) KMC_CK(hipEventCreate(&e[i]));
+  constexpr int kTimed = 3, kRounds = 2;
+  hipEvent_t e[2 * kTimed];
+  for (int i = 0; i < 2 * kTimed; ++i) KMC_CK(hipEventCreate(&e[i]));
   const int nv = std::min(kmc::assign_variants(), KMC_MAX_VARIANTS);
-  float best = 1e30f;
-  int best_v = 0;
   kmc_tuning t{};
   t.nvariants = nv;
-  for (int v = 0; v < nv; ++v) {
-    if (h->vgrid[v] == 0) {  // not built for this (DP, precision)
-      t.ms[v] = -1.0f;
-      continue;
+  for (int v = 0; v < nv; ++v) t.ms[v] = h->vgrid[v] == 0 ? -1.0f : 1e30f;
+  // every variant is timed in each of kRounds rounds over the variants (so a
+  // change of the clock during the tuning does not favour one variant); a
+  // variant's time is its best round's mean of kTimed assignments after an
+  // untimed one. A timed assignment is the distance kernel and the
+  // refinement of its band points, since variants that keep fewer values per
+  // lane leave more points to refine.
+  for (int r = 0; r < kRounds && st == KMC_OK; ++r) {
+    for (int v = 0; v < nv; ++v) {
+      if (h->vgrid[v] == 0) continue;  // not built for this (DP, precision)
+      st = kmc_set_assign_variant(h, v);
+      if (st != KMC_OK) break;
+      st = enqueue_assign(h, nullptr, nullptr);
+      for (int i = 0; i < kTimed && st == KMC_OK; ++i)
+        st = enqueue_assign(h, e[2 * i], nullptr, false, e[2 * i + 1]);
+      if (st != KMC_OK) break;
+      KMC_CK(hipEventSynchronize(e[2 * kTimed - 1]));
+      float sum = 0.f;
+      for (int i = 0; i < kTimed; ++i) {
+        float a = 0.f;
+        KMC_CK(hipEventElapsedTime(&a, e[2 * i], e[2 * i + 1]));
+        sum += a;
+      }
+      t.ms[v] = std::min(t.ms[v], sum / kTimed);
+      t.threads[v] = kmc::assign_threads(v);
+      t.tile[v] = kmc::assign_tile(v);
     }
-    st = kmc_set_assign_variant(h, v);
-    if (st != KMC_OK) break;
-    // one untimed assignment, then two timed ones; a timed assignment is the
-    // distance kernel and the refinement of its band points, since variants
-    // that keep fewer values per lane leave more points to refine
-    st = enqueue_assign(h, nullptr, nullptr);
-    if (st == KMC_OK) st = enqueue_assign(h, e[0], nullptr, false, e[1]);
-    if (st == KMC_OK) st = enqueue_assign(h, e[2], nullptr, false, e[3]);
-    if (st != KMC_OK) break;
-    KMC_CK(hipEventSynchronize(e[3]));
-    float a = 0.f, b = 0.f;
-    KMC_CK(hipEventElapsedTime(&a, e[0], e[1]));
-    KMC_CK(hipEventElapsedTime(&b, e[2], e[3]));
-    t.ms[v] = 0.5f * (a + b);
-    t.threads[v] = kmc::assign_threads(v);
-    t.tile[v] = kmc::assign_tile(v);
-    if (t.ms[v] < best) {
+  }
+  for (int i = 0; i < 2 * kTimed; ++i) (void)hipEventDestroy(e[i]);
+  if (st != KMC_OK) return st;
+  int best_v = 0;
+  float best = 1e30f;
+  for (int v = 0; v < nv; ++v)
+    if (t.ms[v] >= 0.f && t.ms[v] < best) {
       best = t.ms[v];
       best_v = v;
     }
-  }
-  for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
-  if (st != KMC_OK) return st;
   t.chosen = best_v;
   st = kmc_set_assign_variant(h, best_v);
   if (out) *out = t;

@@ -57,12 +57,28 @@ inline int fx_scale_exp(size_t n, float amax) {
   return std::max(-120, std::min(120, e));
 }
 
-// The fixed-point update of an iteration is a full streaming pass (and it
-// rebuilds the cluster totals) unless the totals are valid and at most an
-// eighth of the points changed cluster in the last iteration; then only the
-// moved points' rows are subtracted from and added to the totals.
+// The batch engine's exact tree update of an iteration is a full pass over
+// the sorted members (and it rebuilds the cluster totals) unless the totals
+// are valid and at most an eighth of the points changed cluster in the last
+// iteration; then only the moved points' rows are subtracted from and added
+// to the totals.
 __host__ __device__ inline bool fx_full_pass(const DevState* s, long long n) {
   return !s->sums_valid || s->moved > (unsigned long long)(n / 8);
+}
+
+// The single-problem fixed-point update (k_update_fx) chooses among three
+// forms: a full streaming pass while the cluster totals are not valid (the
+// start of a run); otherwise only the points whose label changed are taken
+// out of their old cluster's totals and added to the new one's -- through
+// the workgroups' LDS accumulators and partials when more than
+// n / kFxMovedDiv points moved in the last iteration (a pass that reads
+// every label but only the moved points' rows), else by global atomics on
+// the totals' delta copies.
+enum { FX_FULL = 0, FX_MOVED = 1, FX_ATOMIC = 2 };
+constexpr long long kFxMovedDiv = 64;
+__host__ __device__ inline int fx_mode(const DevState* s, long long n) {
+  if (!s->sums_valid) return FX_FULL;
+  return s->moved > (unsigned long long)(n / kFxMovedDiv) ? FX_MOVED : FX_ATOMIC;
 }
 
 // convergence tests of the k_stats kernel
@@ -252,6 +268,7 @@ struct StatsParams {
   int cost_ratio;  // also stop when cost_t / cost_{t-1} > 1 - tol (t >= 2)
   int incremental;  // fixed-point update keeps cluster totals across iterations
   long long n;      // points (the full-pass test)
+  int fx_modes;     // the update is k_update_fx: fx_mode decides what a full pass is
 };
 
 struct FinishParams {
@@ -296,6 +313,10 @@ struct FinishParams {
   long long* Sdel;
   long long* Qdel;
   int* Ndel;
+  // labels of the iteration and the labels the totals stand for: after an
+  // update of the FX_MOVED form k_finish copies the former into the latter
+  const int* labels;
+  int* prev;
 };
 
 struct LoadParams {

@@ -2886,8 +2886,13 @@ constexpr int kFxThreads = 512;
 // LDS row stride of a cluster's accumulators: one extra word so that lanes
 // of different points (different clusters) fall in different banks
 __host__ __device__ constexpr int fx_stride(int ds) { return ds + 1; }
+// moved-point form (FX_MOVED): labels a lane scans per step, and the LDS
+// lists of the waves' moved points (point offset, new and old label)
+constexpr int kFxScan = 4;
+constexpr int kFxRound = kFxThreads * kFxScan;
 size_t update_fx_lds(int Kp, int ds) {
-  return (size_t)Kp * fx_stride(ds) * 8 + (size_t)Kp * 8 + (size_t)Kp * 4;
+  return (size_t)Kp * fx_stride(ds) * 8 + (size_t)Kp * 8 + (size_t)Kp * 4 +
+         (size_t)kFxRound * sizeof(int2);
 }
 
 // Incremental form of the fixed-point update (the iterations that are not a
@@ -2964,10 +2969,116 @@ __device__ void update_moved_points(const UpdateParams& P) {
   if (threadIdx.x == 0 && moved_s) atomicAdd(&P.state->moved_acc, (unsigned long long)moved_s);
 }
 
+// The moved points only (FX_MOVED), run by a k_update_fx workgroup over its
+// points [lo, hi) and its dimension slice: a point whose label differs from
+// the one the totals stand for adds its converted slice to the new cluster's
+// LDS accumulators and subtracts it from the old one's (on slice 0 its
+// fixed-point ||x||^2 and count likewise), so the workgroup partials are
+// changes of the totals. Each wave works on its own: a step scans 64 *
+// kFxScan labels and previous labels (the next step's load while this one's
+// rows are read), lists the moved points in the wave's LDS list, and reads
+// their rows with every load of a batch issued before any is used; no
+// workgroup barrier. prev is left as it is: the workgroups of every slice
+// read it, and k_finish copies the labels into it.
+template <int DS>
+__device__ __forceinline__ void fx_moved_rows(const UpdateParams& P, unsigned long long* acc,
+                                              unsigned long long* qacc, unsigned* cacc,
+                                              int2* lists, int slice, long long lo, long long hi,
+                                              unsigned& mv_lane) {
+  constexpr int ST = fx_stride(DS);
+  constexpr int LPP = DS / 4;    // lanes per point (a float4 each)
+  constexpr int PPW = 64 / LPP;  // points per wave load
+  constexpr int UM = 16;         // loads of a batch
+  constexpr int NW = kFxThreads / 64;
+  constexpr int WSTEP = 64 * kFxScan;  // points a wave scans per step
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int pp = lane / LPP, dg = lane % LPP;
+  const int col = slice * DS + 4 * dg;
+  const bool pt_lane = slice == 0 && dg == 0;
+  const float fs = P.fscale;
+  int2* const list = lists + wave * WSTEP;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int la[kFxScan], pa[kFxScan];
+  auto load_labels = [&](long long s0) {
+#pragma unroll
+    for (int u = 0; u < kFxScan; ++u) {
+      const long long i = s0 + u * 64 + lane;
+      la[u] = i < hi ? P.labels[i] : 0;
+      pa[u] = i < hi ? P.prev[i] : 0;  // equal past the range: not moved
+    }
+  };
+  const long long wstride = (long long)NW * WSTEP;
+  long long s0 = lo + (long long)wave * WSTEP;
+  load_labels(s0);
+  for (; s0 < hi; s0 += wstride) {
+    // the step's moved points into the wave's list, ranked by ballot
+    // (labels < 2^16: k is limited by the LDS)
+    unsigned cnt = 0;
+#pragma unroll
+    for (int u = 0; u < kFxScan; ++u) {
+      const bool mv = la[u] != pa[u];
+      const unsigned long long m = __ballot(mv);
+      if (mv)
+        list[cnt + (unsigned)__popcll(m & below)] =
+            make_int2((int)(s0 + u * 64 + lane - lo), (la[u] << 16) | pa[u]);
+      cnt += (unsigned)__popcll(m);
+    }
+    if (slice == 0 && lane == 0) mv_lane += cnt;
+    load_labels(s0 + wstride);
+    __builtin_amdgcn_wave_barrier();
+    for (unsigned e0 = 0; e0 < cnt; e0 += UM * PPW) {
+      float4 v[UM];
+      double q[UM];
+      int ln[UM], lp[UM];
+#pragma unroll
+      for (int u = 0; u < UM; ++u) {
+        const unsigned e = e0 + u * PPW + pp;
+        v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        q[u] = 0.0;
+        ln[u] = -1;
+        lp[u] = 0;
+        if (e < cnt) {
+          const int2 t = list[e];
+          const long long i = lo + t.x;
+          ln[u] = t.y >> 16;
+          lp[u] = t.y & 0xffff;
+          v[u] = *reinterpret_cast<const float4*>(P.X + (size_t)i * P.DP + col);
+          if (pt_lane) q[u] = P.xn64[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UM; ++u) {
+        if (ln[u] < 0) continue;
+        unsigned long long* an = acc + (size_t)ln[u] * ST + dg;
+        unsigned long long* ao = acc + (size_t)lp[u] * ST + dg;
+        const long long x0 = __float2ll_rn(v[u].x * fs), x1 = __float2ll_rn(v[u].y * fs);
+        const long long x2 = __float2ll_rn(v[u].z * fs), x3 = __float2ll_rn(v[u].w * fs);
+        atomicAdd(an + 0 * LPP, (unsigned long long)x0);
+        atomicAdd(an + 1 * LPP, (unsigned long long)x1);
+        atomicAdd(an + 2 * LPP, (unsigned long long)x2);
+        atomicAdd(an + 3 * LPP, (unsigned long long)x3);
+        atomicAdd(ao + 0 * LPP, (unsigned long long)(-x0));
+        atomicAdd(ao + 1 * LPP, (unsigned long long)(-x1));
+        atomicAdd(ao + 2 * LPP, (unsigned long long)(-x2));
+        atomicAdd(ao + 3 * LPP, (unsigned long long)(-x3));
+        if (pt_lane) {
+          const long long qq = __double2ll_rn(q[u] * P.qscale);
+          atomicAdd(&qacc[ln[u]], (unsigned long long)qq);
+          atomicAdd(&qacc[lp[u]], (unsigned long long)(-qq));
+          atomicAdd(&cacc[ln[u]], 1u);
+          atomicAdd(&cacc[lp[u]], ~0u);  // -1 modulo 2^32
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // the list is rewritten by the next step
+  }
+}
+
 template <int DS>
 __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   if (*P.done) return;
-  if (P.Stot && !fx_full_pass(P.state, P.n)) {  // the moved points only
+  const int mode = P.Stot ? fx_mode(P.state, P.n) : FX_FULL;
+  if (mode == FX_ATOMIC) {  // few moved points: global atomics
     update_moved_points(P);
     return;
   }
@@ -3019,7 +3130,13 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   };
   unsigned mv_lane = 0;  // this lane's moved points, added to moved_s once
   long long i0 = lo + (long long)wave * PPW * U;
-  if (i0 < hi) load_step(i0, v, lab, q, pv);
+  if (mode == FX_MOVED) {
+    fx_moved_rows<DS>(P, acc, qacc, cacc, reinterpret_cast<int2*>(cacc + P.Kp), slice, lo, hi,
+                      mv_lane);
+    i0 = hi;  // the full-pass loop below is skipped
+  } else if (i0 < hi) {
+    load_step(i0, v, lab, q, pv);
+  }
   for (; i0 < hi; i0 += stride) {
     const bool more = i0 + stride < hi;
     if (more) load_step(i0 + stride, vn, labn, qn, pvn);
@@ -3338,7 +3455,7 @@ __device__ void stats_reduce(const StatsParams& P) {
   if (P.incremental && threadIdx.x == 0) {
     // the iteration's update is complete: its moved count decides the next
     // one, and the cluster totals now stand for its labels
-    if (fx_full_pass(P.state, P.n))
+    if (P.fx_modes ? fx_mode(P.state, P.n) == FX_FULL : fx_full_pass(P.state, P.n))
       P.state->full_passes += 1;
     else
       P.state->moved_total += P.state->moved_acc;
@@ -3557,18 +3674,42 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
     // exact integer sums of the workgroup partials (any order), then the
     // means rounded once and the FP64 inertia of the assignment:
     //   sum ||x - c||^2 = Q - 2 c.S + n ||c||^2
-    // In the incremental form an iteration without a full pass reads the
-    // cluster totals instead, and a full pass stores them.
+    // In the incremental form a full pass stores the sums as the cluster
+    // totals; the other iterations add the changes (workgroup partials or
+    // delta copies) to the totals.
     long long(*fs_s)[kMaxDP] = reinterpret_cast<long long(*)[kMaxDP]>(part_s);
     __shared__ long long q_s[kFinishWaves];
     __shared__ int n_s[kFinishWaves];
-    const bool from_totals = P.Stot && !fx_full_pass(P.stats.state, P.n);
+    const int fxm = P.Stot ? fx_mode(P.stats.state, P.n) : FX_FULL;
+    const bool from_totals = fxm != FX_FULL;
+    // the workgroup partials: the sums of a full pass, or (FX_MOVED) the
+    // changes of the totals
+    const bool parts = fxm != FX_ATOMIC;
+    if (fxm == FX_MOVED && P.prev) {
+      // the totals now stand for this iteration's labels: prev <- labels
+      // (the update left prev alone), 16 bytes a lane, two loads in flight
+      const long long n4 = P.n >> 2, nt = (long long)gridDim.x * blockDim.x;
+      const int4* src = reinterpret_cast<const int4*>(P.labels);
+      int4* dst = reinterpret_cast<int4*>(P.prev);
+      for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += 2 * nt) {
+        const int4 a = src[i];
+        const bool two = i + nt < n4;
+        int4 b = a;
+        if (two) b = src[i + nt];
+        dst[i] = a;
+        if (two) dst[i + nt] = b;
+      }
+      if (blockIdx.x == 0 && threadIdx.x < (P.n & 3)) {
+        const long long i = (n4 << 2) + threadIdx.x;
+        P.prev[i] = P.labels[i];
+      }
+    }
     long long qsum = 0;
     int nsum = 0;
     // moved points of an incremental iteration: wave w takes delta copy w
     // (one per wave) and clears it for the next iteration
     static_assert(kDeltaCopies == kFinishWaves, "one delta copy per wave");
-    const bool deltas = from_totals && P.Sdel;
+    const bool deltas = fxm == FX_ATOMIC && P.Sdel;
     if (deltas) {
       long long* sd = P.Sdel + ((size_t)wave * P.Kp + c) * P.DP;
       for (int j = lane; j < P.D; j += 64) {
@@ -3585,7 +3726,7 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
     }
     // wave w adds the partials of workgroups w, w+8, ...; four loads in
     // flight per dimension (integer sums: the order does not matter)
-    for (int j = lane; j < P.D && !from_totals; j += 64) {
+    for (int j = lane; j < P.D && parts; j += 64) {
       const int sl = j / P.ds, dj = j - sl * P.ds;
       const long long* fp = P.fpart + ((size_t)sl * P.wgps * P.Kp + c) * P.ds + dj;
       const size_t wst = (size_t)P.Kp * P.ds;  // one workgroup further
@@ -3601,7 +3742,7 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
       for (; w < P.wgps; w += kFinishWaves) acc += fp[(size_t)w * wst];
       fs_s[wave][j] = acc;
     }
-    if (lane == 0 && !from_totals) {
+    if (lane == 0 && parts) {
       for (int w = wave; w < P.wgps; w += kFinishWaves) {
         qsum += P.qpart[(size_t)w * P.Kp + c];
         nsum += P.cpart[(size_t)w * P.Kp + c];
@@ -3613,23 +3754,12 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
     }
     __syncthreads();
     if (wave == 0) {
-      long long qt = 0;
-      int nt = 0;
-      if (from_totals) {
-        qt = P.Qtot[c];
-        nt = P.Ntot[c];
-        if (deltas) {
-#pragma unroll
-          for (int w = 0; w < kFinishWaves; ++w) {
-            qt += q_s[w];
-            nt += n_s[w];
-          }
-          if (lane == 0) {
-            P.Qtot[c] = qt;
-            P.Ntot[c] = nt;
-          }
-        }
-      } else {
+      // totals (when they are kept and valid) plus the waves' delta copies
+      // or partials; stored back as the new totals
+      const bool adds = deltas || parts;
+      long long qt = from_totals ? P.Qtot[c] : 0;
+      int nt = from_totals ? P.Ntot[c] : 0;
+      if (adds) {
 #pragma unroll
         for (int w = 0; w < kFinishWaves; ++w) {
           qt += q_s[w];
@@ -3642,15 +3772,8 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
       }
       double cs = 0.0, cc = 0.0;
       for (int j = lane; j < P.D; j += 64) {
-        long long sj = 0;
-        if (from_totals) {
-          sj = P.Stot[(size_t)c * P.DP + j];
-          if (deltas) {
-#pragma unroll
-            for (int w = 0; w < kFinishWaves; ++w) sj += fs_s[w][j];
-            P.Stot[(size_t)c * P.DP + j] = sj;
-          }
-        } else {
+        long long sj = from_totals ? P.Stot[(size_t)c * P.DP + j] : 0;
+        if (adds) {
 #pragma unroll
           for (int w = 0; w < kFinishWaves; ++w) sj += fs_s[w][j];
           if (P.Stot) P.Stot[(size_t)c * P.DP + j] = sj;

@@ -245,22 +245,35 @@ def test_fixed_update_from_moved_points_equals_full_passes(kmc_mod, c2_data, pre
     iterations of a run and, once few points move, adds and subtracts only
     the moved points' rows. Single-iteration runs (each starts with a full
     pass) must give the same labels, centroid bits and inertia as one run
-    over the same iterations."""
+    over the same iterations. Only the first iteration of the run streams
+    every point; an iteration after one in which more than n/64 points moved
+    (kFxMovedDiv) takes the moved points through the workgroups' LDS
+    accumulators, the others by global atomics -- both forms must occur
+    here."""
     X, C0 = c2_data
     Xs = np.ascontiguousarray(X[:200_000])
+    n = Xs.shape[0]
     iters = 12
-    with kmc_mod.Engine(Xs.shape[0], 128, 256, tol=0.0, precision=prec, update="fixed") as e:
+    with kmc_mod.Engine(n, 128, 256, tol=0.0, precision=prec, update="fixed") as e:
         e.set_points(Xs)
         e.set_centroids(C0)
         r = e.run(iters, check_converged=False, graph=graph)
         C_run, L_run = e.centroids(), e.labels()
         e.set_centroids(C0)
+        labs = []
         for _ in range(iters):
             r1 = e.run(1, check_converged=False)
-        _say("fixed", prec, graph, r["inertia"], r1["inertia"])
+            labs.append(e.labels())
+        # moved[t]: points whose label changed from iteration t to t + 1; the
+        # update of iteration t + 2 chooses its form from it
+        moved = [int(np.count_nonzero(labs[t] != labs[t - 1])) for t in range(1, iters - 1)]
+        _say("fixed", prec, graph, r["inertia"], r1["inertia"], "moved", moved,
+             "full passes", r["full_passes"], "moved points", r["moved_points"])
         assert np.array_equal(e.labels(), L_run)
         assert np.array_equal(e.centroids(), C_run)
         assert r1["inertia"] == r["inertia"]
+        assert r["full_passes"] == 1
+        assert any(m > n // 64 for m in moved) and any(0 < m <= n // 64 for m in moved)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
