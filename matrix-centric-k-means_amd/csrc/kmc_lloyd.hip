@@ -41,7 +41,8 @@ namespace kmc {
 // this, so the operands are the same bits whichever kernel forms them.
 // ---------------------------------------------------------------------------
 template <class Get>
-__device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, Get get) {
+__device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, bool alive,
+                                         Get get) {
   const size_t row = (size_t)c * P.DP;
   if (P.prec == PREC_FP8) {
     // -2c quantised with the row's own power-of-two scale (lane owns
@@ -83,7 +84,7 @@ __device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, G
     d2 = wave_sum_f64(d2);
     if (lane == 0) {
       *reinterpret_cast<int*>(q + P.DP) = e + 127;
-      P.cn[c] = (c < P.K && P.alive[c]) ? s : kFar;
+      P.cn[c] = alive ? s : kFar;
       if (P.cdn) P.cdn[c] = norm_up(d2);
     }
     return;
@@ -114,7 +115,7 @@ __device__ __forceinline__ void prep_row(const PrepParams& P, int c, int lane, G
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (P.cdn) d2 = wave_sum_f64(d2);
   if (lane == 0) {
-    P.cn[c] = (c < P.K && P.alive[c]) ? s : kFar;
+    P.cn[c] = alive ? s : kFar;
     if (P.cdn) P.cdn[c] = norm_up(d2);
   }
 }
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams P) {
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= P.Kp) return;
   const float* cr = P.C + (size_t)c * P.DP;
-  prep_row(P, c, lane, [&](int j) { return cr[j]; });
+  prep_row(P, c, lane, c < P.K && P.alive[c], [&](int j) { return cr[j]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -3591,38 +3592,35 @@ __device__ void stats_reduce(const StatsParams& P) {
 }
 
 // ---------------------------------------------------------------------------
-// k_finish: one workgroup of kFinishWaves waves per centroid. With the tree
-// update the waves first add the cluster's chunk partials: wave w takes
-// partials q0+w, q0+w+kFinishWaves, ... in order, and the wave sums are added
-// in wave order, so the result depends only on the partials (a cluster of
-// any size costs the same few load rounds per wave). c_new = sum / count,
-// rounded once (an empty cluster becomes the zero vector). Then, on wave 0,
-// the float shift in the reference's order on lane 0
-// (comparisons/CPU_Kmeans/src/kmeans.cpp:225-234), the FP64 shift,
-// C <- C_new lane-parallel, and the next iteration's distance operands for
-// the centroid (prep_row).
+// Finish kernels: the new centroid from the update's sums (k_finish_fx for
+// the fixed-point update, k_finish for the tree and ordered updates,
+// k_finish_waves for batches), then, on one wave, the float shift in the
+// reference's order on lane 0 (comparisons/CPU_Kmeans/src/kmeans.cpp:225-234),
+// the FP64 shift, C <- C_new lane-parallel, and the next iteration's distance
+// operands for the centroid (prep_row).
 // ---------------------------------------------------------------------------
 // The end of a centroid's update, run by one wave: the empty-cluster rule,
 // the float shift in the reference's order (squared terms lane-parallel,
 // summed on one lane in dimension order), the FP64 shift, C <- C_new and the
 // next iteration's distance operands. ns: the new centroid (LDS), sq_s: LDS
-// scratch of the wave.
+// scratch of the wave, os: the old centroid (LDS or the row of C), alive:
+// alive[c] as the iteration found it.
 __device__ __forceinline__ void finish_tail(const FinishParams& P, int c, int cnt, float* ns,
-                                            float* sq_s, int lane) {
+                                            float* sq_s, int lane, const float* os, bool alive) {
   float* cr = P.C + (size_t)c * P.DP;
   float* nrow = P.Cnext + (size_t)c * P.DP;
   if (P.keep_empty && cnt == 0) {
     // an empty cluster keeps its centroid (the explicit-Lloyd rule of
     // experiments/raft-bench/src/kmeans.cu:158-166)
     for (int j = lane; j < P.D; j += 64) {
-      ns[j] = cr[j];
-      nrow[j] = cr[j];
+      ns[j] = os[j];
+      nrow[j] = os[j];
     }
   }
   __builtin_amdgcn_wave_barrier();
   double s64 = 0.0;
   for (int j = lane; j < P.D; j += 64) {
-    const float o = cr[j], nv = ns[j];
+    const float o = os[j], nv = ns[j];
     const double td = (double)nv - (double)o;
     s64 = fma(td, td, s64);
     {
@@ -3634,99 +3632,157 @@ __device__ __forceinline__ void finish_tail(const FinishParams& P, int c, int cn
   for (int j = P.D + lane; j < ((P.D + 3) & ~3); j += 64) sq_s[j] = 0.0f;
   s64 = wave_sum_f64(s64);
   __builtin_amdgcn_wave_barrier();
+  if (P.exclude_empty && cnt == 0) alive = false;
   if (lane == 0) {
+    // the squared terms in dimension order, 32 of them loaded before they
+    // are added (the padding up to a multiple of 4 holds zeros: s + 0 = s
+    // for s >= 0)
     float s = 0.0f;
-    for (int j = 0; j < P.D; j += 4) {
+    for (int j0 = 0; j0 < P.D; j0 += 32) {
+      float4 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        v[u] = j0 + 4 * u < P.D ? *reinterpret_cast<const float4*>(&sq_s[j0 + 4 * u])
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
 #pragma clang fp contract(off)
-      const float4 v = *reinterpret_cast<const float4*>(&sq_s[j]);
-      s = s + v.x;
-      if (j + 1 < P.D) s = s + v.y;
-      if (j + 2 < P.D) s = s + v.z;
-      if (j + 3 < P.D) s = s + v.w;
+        s = s + v[u].x;
+        s = s + v[u].y;
+        s = s + v[u].z;
+        s = s + v[u].w;
+      }
     }
     P.conv[c] = (__fsqrt_rn(s) > P.tol) ? 0 : 1;
     P.shift2[c] = s64;
-    if (P.exclude_empty && cnt == 0) P.alive[c] = 0;
+    if (!alive) P.alive[c] = 0;
   }
   __builtin_amdgcn_wave_barrier();
   for (int j = lane; j < P.D; j += 64) cr[j] = ns[j];
-  // the next iteration's distance operands for this centroid (alive[c] is
-  // final for it now)
+  // the next iteration's distance operands for this centroid
   if (P.do_prep) {
     __builtin_amdgcn_wave_barrier();
-    prep_row(P.prep, c, lane, [&](int j) { return j < P.D ? ns[j] : 0.0f; });
+    prep_row(P.prep, c, lane, alive, [&](int j) { return j < P.D ? ns[j] : 0.0f; });
   }
 }
 
 constexpr int kFinishWaves = 8;
-__global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
-  if (*P.done) return;
-  __shared__ double part_s[kFinishWaves][kMaxDP];
+
+// k_finish_fx: k_finish for the fixed-point update (one workgroup of
+// kFinishWaves waves per centroid). The exact integer sums -- a full pass's
+// workgroup partials, or the totals plus the changes of an incremental
+// iteration (workgroup partials of the moved points, or the delta copies of
+// the global atomics, wave w taking copy w) -- are rounded once into the new
+// centroid, and the FP64 inertia of the assignment is formed from them:
+//   sum ||x - c||^2 = Q - 2 c.S + n ||c||^2
+// Every load that depends neither on the stop flag nor on the update's form
+// (the delta copy, the totals, the old centroid, alive[c]) is issued at the
+// start, beside the loads of the stop flag and the state, and parked in
+// LDS, so an iteration of the incremental form waits on memory about once.
+__global__ __launch_bounds__(64 * kFinishWaves) void k_finish_fx(FinishParams P) {
+  __shared__ long long fs_s[kFinishWaves][kMaxDP];
+  __shared__ long long tot_s[kMaxDP];
+  __shared__ float os_s[kMaxDP];
   __shared__ float ns[kMaxDP];
   __shared__ __attribute__((aligned(16))) float sq_s[kMaxDP];
+  __shared__ long long q_s[kFinishWaves];
+  __shared__ int n_s[kFinishWaves];
+  __shared__ long long qtot_s;
+  __shared__ int ntot_s, alive_s;
+  constexpr int JT = kMaxDP / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = blockIdx.x;
   float* cr = P.C + (size_t)c * P.DP;
   float* nrow = P.Cnext + (size_t)c * P.DP;
-  __shared__ int cnt_s;
-  int cnt = P.fixed ? 0 : P.count[c];
-  if (P.fixed) {
-    // exact integer sums of the workgroup partials (any order), then the
-    // means rounded once and the FP64 inertia of the assignment:
-    //   sum ||x - c||^2 = Q - 2 c.S + n ||c||^2
-    // In the incremental form a full pass stores the sums as the cluster
-    // totals; the other iterations add the changes (workgroup partials or
-    // delta copies) to the totals.
-    long long(*fs_s)[kMaxDP] = reinterpret_cast<long long(*)[kMaxDP]>(part_s);
-    __shared__ long long q_s[kFinishWaves];
-    __shared__ int n_s[kFinishWaves];
-    const int fxm = P.Stot ? fx_mode(P.stats.state, P.n) : FX_FULL;
-    const bool from_totals = fxm != FX_FULL;
-    // the workgroup partials: the sums of a full pass, or (FX_MOVED) the
-    // changes of the totals
-    const bool parts = fxm != FX_ATOMIC;
-    if (fxm == FX_MOVED && P.prev) {
-      // the totals now stand for this iteration's labels: prev <- labels
-      // (the update left prev alone), 16 bytes a lane, two loads in flight
-      const long long n4 = P.n >> 2, nt = (long long)gridDim.x * blockDim.x;
-      const int4* src = reinterpret_cast<const int4*>(P.labels);
-      int4* dst = reinterpret_cast<int4*>(P.prev);
-      for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += 2 * nt) {
-        const int4 a = src[i];
-        const bool two = i + nt < n4;
-        int4 b = a;
-        if (two) b = src[i + nt];
-        dst[i] = a;
-        if (two) dst[i + nt] = b;
+  const size_t dco = (size_t)wave * P.Kp + c;  // this wave's delta copy row
+  const int stop = *P.done;
+  const int fxm = P.Stot ? fx_mode(P.stats.state, P.n) : FX_FULL;
+  {
+    long long dv[JT], tv[JT];
+    float ov[JT];
+#pragma unroll
+    for (int t = 0; t < JT; ++t) {
+      const int j = lane + 64 * t;
+      const bool in = j < P.D;
+      dv[t] = (P.Sdel && in) ? P.Sdel[dco * P.DP + j] : 0;
+      tv[t] = (wave == 0 && P.Stot && in) ? P.Stot[(size_t)c * P.DP + j] : 0;
+      ov[t] = (wave == 0 && in) ? cr[j] : 0.0f;
+    }
+    long long qd = 0, qt0 = 0;
+    int nd = 0, nt0 = 0, al = 1;
+    if (lane == 0) {
+      if (P.Sdel) {
+        qd = P.Qdel ? P.Qdel[dco] : 0;
+        nd = P.Ndel[dco];
       }
-      if (blockIdx.x == 0 && threadIdx.x < (P.n & 3)) {
-        const long long i = (n4 << 2) + threadIdx.x;
-        P.prev[i] = P.labels[i];
+      if (wave == 0) {
+        if (P.Stot) {
+          qt0 = P.Qtot[c];
+          nt0 = P.Ntot[c];
+        }
+        al = P.alive[c];
       }
     }
-    long long qsum = 0;
-    int nsum = 0;
-    // moved points of an incremental iteration: wave w takes delta copy w
-    // (one per wave) and clears it for the next iteration
-    static_assert(kDeltaCopies == kFinishWaves, "one delta copy per wave");
-    const bool deltas = fxm == FX_ATOMIC && P.Sdel;
-    if (deltas) {
-      long long* sd = P.Sdel + ((size_t)wave * P.Kp + c) * P.DP;
-      for (int j = lane; j < P.D; j += 64) {
-        fs_s[wave][j] = sd[j];
-        sd[j] = 0;
-      }
-      if (lane == 0) {
-        const size_t o = (size_t)wave * P.Kp + c;
-        qsum = P.Qdel ? P.Qdel[o] : 0;
-        nsum = P.Ndel[o];
-        if (P.Qdel) P.Qdel[o] = 0;
-        P.Ndel[o] = 0;
+#pragma unroll
+    for (int t = 0; t < JT; ++t) {
+      const int j = lane + 64 * t;
+      if (j < P.D) {
+        fs_s[wave][j] = dv[t];
+        if (wave == 0) {
+          tot_s[j] = tv[t];
+          os_s[j] = ov[t];
+        }
       }
     }
+    if (lane == 0) {
+      q_s[wave] = qd;
+      n_s[wave] = nd;
+      if (wave == 0) {
+        qtot_s = qt0;
+        ntot_s = nt0;
+        alive_s = al;
+      }
+    }
+  }
+  if (stop) return;
+  const bool from_totals = fxm != FX_FULL;
+  // the workgroup partials: the sums of a full pass, or (FX_MOVED) the
+  // changes of the totals
+  const bool parts = fxm != FX_ATOMIC;
+  // the delta copies hold this iteration's moves (FX_ATOMIC); cleared for
+  // the next iteration
+  static_assert(kDeltaCopies == kFinishWaves, "one delta copy per wave");
+  const bool deltas = fxm == FX_ATOMIC && P.Sdel;
+  if (deltas) {
+    for (int j = lane; j < P.D; j += 64) P.Sdel[dco * P.DP + j] = 0;
+    if (lane == 0) {
+      if (P.Qdel) P.Qdel[dco] = 0;
+      P.Ndel[dco] = 0;
+    }
+  }
+  if (fxm == FX_MOVED && P.prev) {
+    // the totals now stand for this iteration's labels: prev <- labels
+    // (the update left prev alone), 16 bytes a lane, two loads in flight
+    const long long n4 = P.n >> 2, nt = (long long)gridDim.x * blockDim.x;
+    const int4* src = reinterpret_cast<const int4*>(P.labels);
+    int4* dst = reinterpret_cast<int4*>(P.prev);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += 2 * nt) {
+      const int4 a = src[i];
+      const bool two = i + nt < n4;
+      int4 b = a;
+      if (two) b = src[i + nt];
+      dst[i] = a;
+      if (two) dst[i + nt] = b;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (P.n & 3)) {
+      const long long i = (n4 << 2) + threadIdx.x;
+      P.prev[i] = P.labels[i];
+    }
+  }
+  if (parts) {
     // wave w adds the partials of workgroups w, w+8, ...; four loads in
     // flight per dimension (integer sums: the order does not matter)
-    for (int j = lane; j < P.D && parts; j += 64) {
+    for (int j = lane; j < P.D; j += 64) {
       const int sl = j / P.ds, dj = j - sl * P.ds;
       const long long* fp = P.fpart + ((size_t)sl * P.wgps * P.Kp + c) * P.ds + dj;
       const size_t wst = (size_t)P.Kp * P.ds;  // one workgroup further
@@ -3742,61 +3798,78 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
       for (; w < P.wgps; w += kFinishWaves) acc += fp[(size_t)w * wst];
       fs_s[wave][j] = acc;
     }
-    if (lane == 0 && parts) {
+    if (lane == 0) {
+      long long qsum = 0;
+      int nsum = 0;
       for (int w = wave; w < P.wgps; w += kFinishWaves) {
         qsum += P.qpart[(size_t)w * P.Kp + c];
         nsum += P.cpart[(size_t)w * P.Kp + c];
       }
-    }
-    if (lane == 0) {
       q_s[wave] = qsum;
       n_s[wave] = nsum;
     }
-    __syncthreads();
-    if (wave == 0) {
-      // totals (when they are kept and valid) plus the waves' delta copies
-      // or partials; stored back as the new totals
-      const bool adds = deltas || parts;
-      long long qt = from_totals ? P.Qtot[c] : 0;
-      int nt = from_totals ? P.Ntot[c] : 0;
-      if (adds) {
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  // totals (when they are kept and valid) plus the waves' delta copies or
+  // partials, stored back as the new totals
+  const bool adds = deltas || parts;
+  long long qt = from_totals ? qtot_s : 0;
+  int nt = from_totals ? ntot_s : 0;
+  if (adds) {
 #pragma unroll
-        for (int w = 0; w < kFinishWaves; ++w) {
-          qt += q_s[w];
-          nt += n_s[w];
-        }
-        if (P.Stot && lane == 0) {
-          P.Qtot[c] = qt;
-          P.Ntot[c] = nt;
-        }
-      }
-      double cs = 0.0, cc = 0.0;
-      for (int j = lane; j < P.D; j += 64) {
-        long long sj = from_totals ? P.Stot[(size_t)c * P.DP + j] : 0;
-        if (adds) {
-#pragma unroll
-          for (int w = 0; w < kFinishWaves; ++w) sj += fs_s[w][j];
-          if (P.Stot) P.Stot[(size_t)c * P.DP + j] = sj;
-        }
-        const double S = (double)sj * P.inv_fscale;
-        const float v = (float)(S / (double)(nt > 1 ? nt : 1));
-        ns[j] = v;
-        nrow[j] = v;
-        const double co = (double)cr[j];
-        cs = fma(co, S, cs);
-        cc = fma(co, co, cc);
-      }
-      cs = wave_sum_f64(cs);
-      cc = wave_sum_f64(cc);
-      if (lane == 0) {
-        cnt_s = nt;
-        P.count_out[c] = nt;
-        P.ipart_out[c] = (double)qt * P.inv_qscale - 2.0 * cs + (double)nt * cc;
-      }
+    for (int w = 0; w < kFinishWaves; ++w) {
+      qt += q_s[w];
+      nt += n_s[w];
     }
-    __syncthreads();
-    cnt = cnt_s;
-  } else if (P.tree == 2) {
+    if (P.Stot && lane == 0) {
+      P.Qtot[c] = qt;
+      P.Ntot[c] = nt;
+    }
+  }
+  double cs = 0.0, cc = 0.0;
+  for (int j = lane; j < P.D; j += 64) {
+    long long sj = from_totals ? tot_s[j] : 0;
+    if (adds) {
+#pragma unroll
+      for (int w = 0; w < kFinishWaves; ++w) sj += fs_s[w][j];
+      if (P.Stot) P.Stot[(size_t)c * P.DP + j] = sj;
+    }
+    const double S = (double)sj * P.inv_fscale;
+    const float v = (float)(S / (double)(nt > 1 ? nt : 1));
+    ns[j] = v;
+    nrow[j] = v;
+    const double co = (double)os_s[j];
+    cs = fma(co, S, cs);
+    cc = fma(co, co, cc);
+  }
+  cs = wave_sum_f64(cs);
+  cc = wave_sum_f64(cc);
+  if (lane == 0) {
+    P.count_out[c] = nt;
+    P.ipart_out[c] = (double)qt * P.inv_qscale - 2.0 * cs + (double)nt * cc;
+  }
+  finish_tail(P, c, nt, ns, sq_s, lane, os_s, alive_s != 0);
+}
+
+// k_finish for the tree and ordered updates: one workgroup of kFinishWaves
+// waves per centroid. With the tree update the waves first add the
+// cluster's chunk partials: wave w takes partials q0+w, q0+w+kFinishWaves,
+// ... in order, and the wave sums are added in wave order, so the result
+// depends only on the partials (a cluster of any size costs the same few
+// load rounds per wave). c_new = sum / count, rounded once (an empty cluster
+// becomes the zero vector); then finish_tail on wave 0.
+__global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
+  if (*P.done) return;
+  __shared__ double part_s[kFinishWaves][kMaxDP];
+  __shared__ float ns[kMaxDP];
+  __shared__ __attribute__((aligned(16))) float sq_s[kMaxDP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x;
+  float* cr = P.C + (size_t)c * P.DP;
+  float* nrow = P.Cnext + (size_t)c * P.DP;
+  int cnt = P.count[c];
+  if (P.tree == 2) {
     // exact tree form: the 64-bit integer segment sums of a full pass (the
     // cluster's chunks of the sorted members, any order), stored as the
     // cluster totals, or the totals the moved points kept up to date; the
@@ -3879,7 +3952,7 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(FinishParams P) {
   } else if (wave == 0) {
     for (int j = lane; j < P.D; j += 64) ns[j] = nrow[j];
   }
-  if (wave == 0) finish_tail(P, c, cnt, ns, sq_s, lane);
+  if (wave == 0) finish_tail(P, c, cnt, ns, sq_s, lane, cr, P.alive[c] != 0);
 }
 
 // k_finish for the exact tree form with many small clusters (batches): one
@@ -3921,7 +3994,7 @@ __global__ __launch_bounds__(64 * kFinishClustersPerBlock) void k_finish_waves(F
     nrow[j] = v;
   }
   if (!from_totals && lane == 0) P.Ntot[c] = cnt;
-  finish_tail(P, c, cnt, ns, sq_s[wave], lane);
+  finish_tail(P, c, cnt, ns, sq_s[wave], lane, P.C + (size_t)c * P.DP, P.alive[c] != 0);
 }
 
 // k_stats: one workgroup.
@@ -4321,6 +4394,8 @@ hipError_t launch_finish(const FinishParams& F, hipStream_t s) {
     hipLaunchKernelGGL(k_finish_waves,
                        dim3((F.K + kFinishClustersPerBlock - 1) / kFinishClustersPerBlock),
                        dim3(64 * kFinishClustersPerBlock), 0, s, F);
+  else if (F.fixed)
+    hipLaunchKernelGGL(k_finish_fx, dim3(F.K), dim3(64 * kFinishWaves), 0, s, F);
   else
     hipLaunchKernelGGL(k_finish, dim3(F.K), dim3(64 * kFinishWaves), 0, s, F);
   if (F.stats.state) hipLaunchKernelGGL(k_stats, dim3(1), dim3(kStatsThreads), 0, s, F.stats);
