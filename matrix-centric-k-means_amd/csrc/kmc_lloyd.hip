@@ -3453,27 +3453,48 @@ hipError_t launch_plusplus(const float* X, long long n, int DP, float* C, int K,
 // ---------------------------------------------------------------------------
 constexpr int kStatsThreads = 512;
 __device__ void stats_reduce(const StatsParams& P) {
-  if (P.incremental && threadIdx.x == 0) {
-    // the iteration's update is complete: its moved count decides the next
-    // one, and the cluster totals now stand for its labels
-    if (P.fx_modes ? fx_mode(P.state, P.n) == FX_FULL : fx_full_pass(P.state, P.n))
-      P.state->full_passes += 1;
-    else
-      P.state->moved_total += P.state->moved_acc;
-    P.state->moved = P.state->moved_acc;
-    P.state->moved_acc = 0ull;
-    P.state->sums_valid = 1;
-  }
   __shared__ double sd[kStatsThreads];
   __shared__ double sm[kStatsThreads];
+  __shared__ double ss[kStatsThreads];
   __shared__ int sc[kStatsThreads];
   __shared__ unsigned long long sq2[kStatsThreads], sqa[kStatsThreads];
+  __shared__ DevState st_s;
   const int t = threadIdx.x;
+  // every load is issued before the stop flag is known (they read memory
+  // that is valid either way) and parked in LDS, so the kernel waits on
+  // memory about once
+  const int stop = *P.done;
+  if (t == 0) st_s = *P.state;
   double in = 0.0, mx = 0.0, sh = 0.0;
   int all = 1;
   unsigned long long r2 = 0, ra = 0;
-  // eight loads in flight per thread and round (fixed order: deterministic)
-  for (int r0 = 0; r0 < P.nregions; r0 += 8 * kStatsThreads) {
+  {
+    // the first eight entries per thread of every array (all of them up to
+    // 8 * kStatsThreads); fixed order: deterministic
+    unsigned a[8], b[8];
+    double v[8], w[8];
+    int cv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = u * kStatsThreads + t;
+      a[u] = r < P.nregions ? P.q2cnt[r] : 0u;
+      b[u] = r < P.nregions ? P.qacnt[r] : 0u;
+      v[u] = r < P.nparts ? P.ipart[r] : 0.0;
+      w[u] = r < P.K ? P.shift2[r] : 0.0;
+      cv[u] = r < P.K ? P.conv[r] : 1;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      r2 += a[u];
+      ra += b[u];
+      in += v[u];
+      mx = fmax(mx, w[u]);
+      sh += w[u];
+      all &= cv[u];
+    }
+  }
+  // the rest, eight loads in flight per thread and round
+  for (int r0 = 8 * kStatsThreads; r0 < P.nregions; r0 += 8 * kStatsThreads) {
     unsigned a[8], b[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -3487,7 +3508,7 @@ __device__ void stats_reduce(const StatsParams& P) {
       ra += b[u];
     }
   }
-  for (int i0 = 0; i0 < P.nparts; i0 += 8 * kStatsThreads) {
+  for (int i0 = 8 * kStatsThreads; i0 < P.nparts; i0 += 8 * kStatsThreads) {
     double v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -3497,7 +3518,7 @@ __device__ void stats_reduce(const StatsParams& P) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) in += v[u];
   }
-  for (int c0 = 0; c0 < P.K; c0 += 8 * kStatsThreads) {
+  for (int c0 = 8 * kStatsThreads; c0 < P.K; c0 += 8 * kStatsThreads) {
     double v[8];
     int cv[8];
 #pragma unroll
@@ -3513,13 +3534,13 @@ __device__ void stats_reduce(const StatsParams& P) {
       all &= cv[u];
     }
   }
-  __shared__ double ss[kStatsThreads];
   sd[t] = in;
   sm[t] = mx;
   ss[t] = sh;
   sc[t] = all;
   sq2[t] = r2;
   sqa[t] = ra;
+  if (stop) return;
   __syncthreads();
   if (t >= 64) return;
   // The FP64 sums keep the order of the halving tree over the 512 values
@@ -3551,7 +3572,19 @@ __device__ void stats_reduce(const StatsParams& P) {
     va += __shfl_down(va, o, 64);
   }
   if (t == 0) {
-    DevState* s = P.state;
+    DevState sv = st_s;  // the state as the kernel found it
+    DevState* s = &sv;
+    if (P.incremental) {
+      // the iteration's update is complete: its moved count decides the
+      // next one, and the cluster totals now stand for its labels
+      if (P.fx_modes ? fx_mode(s, P.n) == FX_FULL : fx_full_pass(s, P.n))
+        s->full_passes += 1;
+      else
+        s->moved_total += s->moved_acc;
+      s->moved = s->moved_acc;
+      s->moved_acc = 0ull;
+      s->sums_valid = 1;
+    }
     const unsigned long long it = s->iter + 1;
     s->iter = it;
     s->inertia = vd;
@@ -3588,6 +3621,7 @@ __device__ void stats_reduce(const StatsParams& P) {
     }
     s->prior_cost = vd;
     if (it >= s->maxiter) s->done = 1;
+    *P.state = sv;
   }
 }
 
@@ -3999,8 +4033,7 @@ __global__ __launch_bounds__(64 * kFinishClustersPerBlock) void k_finish_waves(F
 
 // k_stats: one workgroup.
 __global__ __launch_bounds__(kStatsThreads) void k_stats(StatsParams P) {
-  if (*P.done) return;
-  stats_reduce(P);
+  stats_reduce(P);  // returns at once (after its loads) when the stop flag is set
 }
 
 
