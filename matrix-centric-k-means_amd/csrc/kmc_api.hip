@@ -77,6 +77,7 @@ struct kmc_handle {
   int* Ndel = nullptr;
   int fx_ds = 0, fx_nslice = 0, fx_wgps = 0;
   float fx_scale = 1.0f;
+  int fx_magic = 0;  // max |x| * fx_scale < 2^50 (k_update_fx's fast conversion)
   double fx_qscale = 1.0;
   int* conv = nullptr;
   double* shift2 = nullptr;
@@ -508,6 +509,7 @@ kmc_status enqueue_update(kmc_handle* h, int mode, int rules, hipEvent_t u0 = nu
   up.qpart = h->qpart;
   up.cpart = h->cpart;
   up.fscale = h->fx_scale;
+  up.magic = h->fx_magic;
   up.qscale = h->fx_qscale;
   up.Kp = h->Kp;
   up.nslice = h->fx_nslice;
@@ -729,6 +731,7 @@ kmc_status load_points(kmc_handle* h, const float* dsrc, size_t ld) {
                                                       61.0 - std::log2(nn * std::max(qm, 1e-300)))));
       h->fx_scale = std::ldexp(1.0f, s1);
       h->fx_qscale = std::ldexp(1.0, s2);
+      h->fx_magic = std::ldexp((double)am, s1) < std::ldexp(1.0, 50) ? 1 : 0;
     }
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

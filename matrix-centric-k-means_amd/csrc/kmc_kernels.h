@@ -225,6 +225,7 @@ struct UpdateParams {
   long long* qpart;    // [w][Kp]
   int* cpart;          // [w][Kp]
   float fscale;        // x -> round(x * fscale), a power of two
+  int magic;           // max |x| * fscale < 2^50: the conversion by the double-add form
   double qscale;       // ||x||^2 -> round(||x||^2 * qscale), a power of two
   int Kp, nslice, wgps, ds;
   // incremental form (null Stot: every iteration is a full pass): labels of

@@ -2884,6 +2884,20 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_update_tree_rows(UpdatePara
 // partials exactly and rounds the means once.
 // ---------------------------------------------------------------------------
 constexpr int kFxThreads = 512;
+// round(v) as a 64-bit integer (ties to even, as __float2ll_rn). With M
+// (|v| < 2^50): v + 1.5 * 2^52 in double rounds v to an integer and leaves it
+// in the low mantissa bits, so the bits of the sum less those of 1.5 * 2^52
+// are round(v) -- three operations instead of the general conversion's
+// eleven.
+template <bool M>
+__device__ __forceinline__ unsigned long long fx_int(float v) {
+  if constexpr (M) {
+    return (unsigned long long)__double_as_longlong((double)v + 0x1.8p52) -
+           0x4338000000000000ull;
+  } else {
+    return (unsigned long long)__float2ll_rn(v);
+  }
+}
 // LDS row stride of a cluster's accumulators: one extra word so that lanes
 // of different points (different clusters) fall in different banks
 __host__ __device__ constexpr int fx_stride(int ds) { return ds + 1; }
@@ -2981,7 +2995,7 @@ __device__ void update_moved_points(const UpdateParams& P) {
 // their rows with every load of a batch issued before any is used; no
 // workgroup barrier. prev is left as it is: the workgroups of every slice
 // read it, and k_finish copies the labels into it.
-template <int DS>
+template <int DS, bool M>
 __device__ __forceinline__ void fx_moved_rows(const UpdateParams& P, unsigned long long* acc,
                                               unsigned long long* qacc, unsigned* cacc,
                                               int2* lists, int slice, long long lo, long long hi,
@@ -3052,8 +3066,8 @@ __device__ __forceinline__ void fx_moved_rows(const UpdateParams& P, unsigned lo
         if (ln[u] < 0) continue;
         unsigned long long* an = acc + (size_t)ln[u] * ST + dg;
         unsigned long long* ao = acc + (size_t)lp[u] * ST + dg;
-        const long long x0 = __float2ll_rn(v[u].x * fs), x1 = __float2ll_rn(v[u].y * fs);
-        const long long x2 = __float2ll_rn(v[u].z * fs), x3 = __float2ll_rn(v[u].w * fs);
+        const unsigned long long x0 = fx_int<M>(v[u].x * fs), x1 = fx_int<M>(v[u].y * fs);
+        const unsigned long long x2 = fx_int<M>(v[u].z * fs), x3 = fx_int<M>(v[u].w * fs);
         atomicAdd(an + 0 * LPP, (unsigned long long)x0);
         atomicAdd(an + 1 * LPP, (unsigned long long)x1);
         atomicAdd(an + 2 * LPP, (unsigned long long)x2);
@@ -3075,7 +3089,7 @@ __device__ __forceinline__ void fx_moved_rows(const UpdateParams& P, unsigned lo
   }
 }
 
-template <int DS>
+template <int DS, bool M>
 __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   if (*P.done) return;
   const int mode = P.Stot ? fx_mode(P.state, P.n) : FX_FULL;
@@ -3132,8 +3146,8 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   unsigned mv_lane = 0;  // this lane's moved points, added to moved_s once
   long long i0 = lo + (long long)wave * PPW * U;
   if (mode == FX_MOVED) {
-    fx_moved_rows<DS>(P, acc, qacc, cacc, reinterpret_cast<int2*>(cacc + P.Kp), slice, lo, hi,
-                      mv_lane);
+    fx_moved_rows<DS, M>(P, acc, qacc, cacc, reinterpret_cast<int2*>(cacc + P.Kp), slice, lo,
+                         hi, mv_lane);
     i0 = hi;  // the full-pass loop below is skipped
   } else if (i0 < hi) {
     load_step(i0, v, lab, q, pv);
@@ -3148,10 +3162,10 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       unsigned long long* a = acc + (size_t)lab[u] * ST + dg;
-      atomicAdd(a + 0 * LPP, (unsigned long long)__float2ll_rn(v[u].x * fs));
-      atomicAdd(a + 1 * LPP, (unsigned long long)__float2ll_rn(v[u].y * fs));
-      atomicAdd(a + 2 * LPP, (unsigned long long)__float2ll_rn(v[u].z * fs));
-      atomicAdd(a + 3 * LPP, (unsigned long long)__float2ll_rn(v[u].w * fs));
+      atomicAdd(a + 0 * LPP, fx_int<M>(v[u].x * fs));
+      atomicAdd(a + 1 * LPP, fx_int<M>(v[u].y * fs));
+      atomicAdd(a + 2 * LPP, fx_int<M>(v[u].z * fs));
+      atomicAdd(a + 3 * LPP, fx_int<M>(v[u].w * fs));
     }
     if (pt_lane) {
 #pragma unroll
@@ -4387,9 +4401,13 @@ hipError_t launch_update(const UpdateParams& P, int mode, hipStream_t s) {
   const int S = (P.D + 63) / 64;
   if (mode == UPDATE_FIXED) {
     const size_t lds = update_fx_lds(P.Kp, P.ds);
-    const void* fn = P.ds == 64   ? reinterpret_cast<const void*>(k_update_fx<64>)
-                     : P.ds == 32 ? reinterpret_cast<const void*>(k_update_fx<32>)
-                                  : reinterpret_cast<const void*>(k_update_fx<16>);
+    const void* fn =
+        P.magic ? (P.ds == 64   ? reinterpret_cast<const void*>(k_update_fx<64, true>)
+                   : P.ds == 32 ? reinterpret_cast<const void*>(k_update_fx<32, true>)
+                                : reinterpret_cast<const void*>(k_update_fx<16, true>))
+                : (P.ds == 64   ? reinterpret_cast<const void*>(k_update_fx<64, false>)
+                   : P.ds == 32 ? reinterpret_cast<const void*>(k_update_fx<32, false>)
+                                : reinterpret_cast<const void*>(k_update_fx<16, false>));
     if (P.ds != 64 && P.ds != 32 && P.ds != 16) return hipErrorInvalidValue;
     void* args[] = {const_cast<UpdateParams*>(&P)};
     return hipLaunchKernel(fn, dim3(P.nslice * P.wgps), dim3(kFxThreads), args, lds, s);
