@@ -143,3 +143,21 @@ def test_bench_variant_table_matches_the_kernel_source():
     assert bench.assign_kernel_name(128, "bf16", 9) == "k_assign2<128, 2, 8, 3, false, true, 0, 2>"
     assert bench.assign_kernel_name(128, "bf16", 1) == "k_assign<128, 2, 8, 1, false, 0, false>"
     assert bench.assign_kernel_name(128, "bf16", 8) == "k_assign2<128, 2, 8, 4, false, true, 0, 1>"
+
+
+def test_fixed_point_double_add_conversion():
+    """The fixed-point update rounds v = x * scale (|v| < 2^50) to a 64-bit
+    integer as the bits of the double v + 1.5 * 2^52 less those of 1.5 * 2^52
+    (fx_int in kmc_lloyd.hip); it must equal round-half-even of v, the
+    general conversion (__float2ll_rn) the moved-point atomics use."""
+    rng = np.random.default_rng(11)
+    e = rng.integers(-30, 50, size=200_000)
+    v = (rng.standard_normal(200_000) * np.exp2(e.astype(np.float64))).astype(np.float32)
+    v = v[np.abs(v.astype(np.float64)) < 2.0 ** 50]
+    ties = np.array([0.5, 1.5, 2.5, -0.5, -1.5, -2.5, 0.0, -0.0, 2.0 ** 49, -(2.0 ** 49),
+                     8388607.5, -8388607.5, 2.0 ** 50 - 2.0 ** 27], dtype=np.float32)
+    v = np.concatenate([v, ties])
+    magic = np.float64(1.5 * 2.0 ** 52)
+    got = (v.astype(np.float64) + magic).view(np.int64) - np.int64(0x4338000000000000)
+    want = np.rint(v.astype(np.float64)).astype(np.int64)  # half to even
+    assert np.array_equal(got, want)
