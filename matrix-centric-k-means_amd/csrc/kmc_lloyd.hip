@@ -3068,14 +3068,14 @@ __device__ __forceinline__ void fx_moved_rows(const UpdateParams& P, unsigned lo
         unsigned long long* ao = acc + (size_t)lp[u] * ST + dg;
         const unsigned long long x0 = fx_int<M>(v[u].x * fs), x1 = fx_int<M>(v[u].y * fs);
         const unsigned long long x2 = fx_int<M>(v[u].z * fs), x3 = fx_int<M>(v[u].w * fs);
-        atomicAdd(an + 0 * LPP, (unsigned long long)x0);
-        atomicAdd(an + 1 * LPP, (unsigned long long)x1);
-        atomicAdd(an + 2 * LPP, (unsigned long long)x2);
-        atomicAdd(an + 3 * LPP, (unsigned long long)x3);
-        atomicAdd(ao + 0 * LPP, (unsigned long long)(-x0));
-        atomicAdd(ao + 1 * LPP, (unsigned long long)(-x1));
-        atomicAdd(ao + 2 * LPP, (unsigned long long)(-x2));
-        atomicAdd(ao + 3 * LPP, (unsigned long long)(-x3));
+        atomicAdd(an + 0 * LPP, x0);
+        atomicAdd(an + 1 * LPP, x1);
+        atomicAdd(an + 2 * LPP, x2);
+        atomicAdd(an + 3 * LPP, x3);
+        atomicAdd(ao + 0 * LPP, 0ull - x0);  // subtraction modulo 2^64
+        atomicAdd(ao + 1 * LPP, 0ull - x1);
+        atomicAdd(ao + 2 * LPP, 0ull - x2);
+        atomicAdd(ao + 3 * LPP, 0ull - x3);
         if (pt_lane) {
           const long long qq = __double2ll_rn(q[u] * P.qscale);
           atomicAdd(&qacc[ln[u]], (unsigned long long)qq);
