@@ -1021,7 +1021,9 @@ template <int DP, int PREC, int NW, int T, bool PF, bool PIPE, int WPE, int L>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_assign2(AssignParams P) {
   static_assert(PREC == PREC_BF16 || PREC == PREC_FP8, "single-product forms only");
-  if (*P.done) return;
+  // the stop flag is read beside the first rows and the centroid norms
+  // (valid memory either way; nothing global is written before the test)
+  const int stop = *P.done;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ float red_s[NW];
   __shared__ unsigned q2c_s, qac_s;
@@ -1088,6 +1090,7 @@ void k_assign2(AssignParams P) {
   if (lane == 0) red_s[wave] = cd;
   stage_centroids<DP, PREC>(P, smem, 0, P.Kp);
   __syncthreads();
+  if (stop) return;
   float cdmax = 0.0f;
 #pragma unroll
   for (int w = 0; w < NW; ++w) cdmax = fmaxf(cdmax, red_s[w]);
@@ -2322,11 +2325,14 @@ __device__ void pq_body(const RefineParams& P, unsigned char* rsm, unsigned bid,
 template <int DP, int NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_refine_fused(RefineParams P) {
-  if (*P.done) return;
+  // the stop flag is read beside the queue counts of the region prefix (valid
+  // memory either way; the prefix only writes LDS), so its latency overlaps
+  const int stop = *P.done;
   extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
   unsigned* const pre_s = reinterpret_cast<unsigned*>(rsm);
   unsigned ta = 0;
   const unsigned t2 = region_prefix2(P.q2cnt, P.qacnt, P.nregions, pre_s, &ta);
+  if (stop) return;
   const unsigned ngA = (ta + 31) / 32;
   unsigned G = 0;
   if (ta > 0 && !(prune_active(P) && ta >= (unsigned)P.prune_min)) {
@@ -3091,8 +3097,12 @@ __device__ __forceinline__ void fx_moved_rows(const UpdateParams& P, unsigned lo
 
 template <int DS, bool M>
 __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
-  if (*P.done) return;
-  const int mode = P.Stot ? fx_mode(P.state, P.n) : FX_FULL;
+  // the stop flag and the state are read together (one memory latency); the
+  // early return tests both, so neither load waits behind the other
+  const int stop = *P.done;
+  const int mode0 = P.Stot ? fx_mode(P.state, P.n) : FX_FULL;
+  const int mode = stop ? -1 : mode0;
+  if (mode < 0) return;
   if (mode == FX_ATOMIC) {  // few moved points: global atomics
     update_moved_points(P);
     return;
