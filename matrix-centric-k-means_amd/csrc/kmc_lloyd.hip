@@ -3838,33 +3838,53 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish_fx(FinishParams P)
     }
   }
   if (parts) {
-    // wave w adds the partials of workgroups w, w+8, ...; four loads in
-    // flight per dimension (integer sums: the order does not matter)
-    for (int j = lane; j < P.D; j += 64) {
-      const int sl = j / P.ds, dj = j - sl * P.ds;
-      const long long* fp = P.fpart + ((size_t)sl * P.wgps * P.Kp + c) * P.ds + dj;
-      const size_t wst = (size_t)P.Kp * P.ds;  // one workgroup further
-      long long acc = 0;
-      int w = wave;
-      for (; w + 3 * kFinishWaves < P.wgps; w += 4 * kFinishWaves) {
-        long long t[4];
+    // wave w adds the partials of workgroups w, w+8, ...: every dimension's
+    // loads of a round (up to 8 partials) are issued before any is added
+    // (integer sums: the order does not matter)
+    constexpr int JT = kMaxDP / 64, UF = 8;
+    long long accj[JT];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) t[u] = fp[(size_t)(w + u * kFinishWaves) * wst];
+    for (int jt = 0; jt < JT; ++jt) accj[jt] = 0;
+    const size_t wst = (size_t)P.Kp * P.ds;  // one workgroup further
+    for (int w0 = wave; w0 < P.wgps; w0 += UF * kFinishWaves) {
+      long long t[JT][UF];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc += t[u];
+      for (int jt = 0; jt < JT; ++jt) {
+        const int j = lane + 64 * jt;
+        const int sl = j / P.ds, dj = j - sl * P.ds;
+        const long long* fp = P.fpart + ((size_t)sl * P.wgps * P.Kp + c) * P.ds + dj;
+#pragma unroll
+        for (int u = 0; u < UF; ++u) {
+          const int w = w0 + u * kFinishWaves;
+          t[jt][u] = (j < P.D && w < P.wgps) ? fp[(size_t)w * wst] : 0;
+        }
       }
-      for (; w < P.wgps; w += kFinishWaves) acc += fp[(size_t)w * wst];
-      fs_s[wave][j] = acc;
+#pragma unroll
+      for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+        for (int u = 0; u < UF; ++u) accj[jt] += t[jt][u];
     }
-    if (lane == 0) {
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt)
+      if (lane + 64 * jt < P.D) fs_s[wave][lane + 64 * jt] = accj[jt];
+    {
+      // the fixed-point ||x||^2 and counts: a lane per partial, then the
+      // wave's sum
       long long qsum = 0;
       int nsum = 0;
-      for (int w = wave; w < P.wgps; w += kFinishWaves) {
+      for (int w = wave + kFinishWaves * lane; w < P.wgps; w += 64 * kFinishWaves) {
         qsum += P.qpart[(size_t)w * P.Kp + c];
         nsum += P.cpart[(size_t)w * P.Kp + c];
       }
-      q_s[wave] = qsum;
-      n_s[wave] = nsum;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        qsum += __shfl_xor(qsum, o, 64);
+        nsum += __shfl_xor(nsum, o, 64);
+      }
+      if (lane == 0) {
+        q_s[wave] = qsum;
+        n_s[wave] = nsum;
+      }
     }
   }
   __syncthreads();
