@@ -2945,14 +2945,28 @@ __device__ void update_moved_points(const UpdateParams& P) {
     const float* xr = P.X + (size_t)pi * P.DP;
     unsigned long long* sn = reinterpret_cast<unsigned long long*>(St + (size_t)nl * P.DP);
     unsigned long long* so = reinterpret_cast<unsigned long long*>(St + (size_t)ol * P.DP);
-    for (int j = lane; j < P.D; j += 64) {
-      const long long q = __float2ll_rn(xr[j] * fs);
-      atomicAdd(sn + j, (unsigned long long)q);
-      atomicAdd(so + j, (unsigned long long)(-q));
+    // the row (a lane per dimension) and ||x||^2 are requested before any
+    // atomic is issued: one memory latency per moved point
+    constexpr int JT = kMaxDP / 64;
+    float xv[JT];
+#pragma unroll
+    for (int t = 0; t < JT; ++t) {
+      const int j = lane + 64 * t;
+      xv[t] = j < P.D ? xr[j] : 0.0f;
+    }
+    const double xq = (lane == 0 && Qt) ? P.xn64[pi] : 0.0;
+#pragma unroll
+    for (int t = 0; t < JT; ++t) {
+      const int j = lane + 64 * t;
+      if (j < P.D) {
+        const long long q = __float2ll_rn(xv[t] * fs);
+        atomicAdd(sn + j, (unsigned long long)q);
+        atomicAdd(so + j, (unsigned long long)(-q));
+      }
     }
     if (lane == 0) {
       if (Qt) {
-        const long long qq = __double2ll_rn(P.xn64[pi] * P.qscale);
+        const long long qq = __double2ll_rn(xq * P.qscale);
         atomicAdd(reinterpret_cast<unsigned long long*>(Qt + nl), (unsigned long long)qq);
         atomicAdd(reinterpret_cast<unsigned long long*>(Qt + ol), (unsigned long long)(-qq));
       }
