@@ -40,7 +40,7 @@ no events (``ms_per_step_with_kernel_events`` gives the second pass).
 
 The CPU baseline is the reference CPU k-means (comparisons/CPU_Kmeans,
 compiled from its own sources into oracle/_ref) timed single-threaded on this
-host on a bounded sample (first 100,000 points, 2 iterations), scaled to the
+host on a bounded sample (first 100,000 points, 3 iterations at N=1M), scaled to the
 full N; falls back to the oracle restatement when oracle/_ref is absent.
 """
 import argparse
@@ -513,7 +513,7 @@ def main():
                    "mean_launch_ms": round(t_assign * 1e3, 5), "launches_timed": launches}
     ubytes = N * D * 4 + N * 4 + K * D * 4  # X rows once, member ids, centroids
     uname = {"tree": "k_update_tree_rows<128, false>" if dp <= 128 else "k_update_tree<4, false>",
-             "ordered": "k_update", "fixed": "k_update_fx<64>"}[a.update]
+             "ordered": "k_update", "fixed": "k_update_fx<64, true>"}[a.update]
     update_note = None
     if a.update == "fixed" and int(r["full_passes"]) < iters:
         # incremental: full passes stream X, labels and the previous labels;
@@ -523,7 +523,7 @@ def main():
         total = full * N * (D * 4 + 12) + (iters - full) * N * 8 + \
             int(r["moved_points"]) * D * 4
         ubytes = total / max(1, iters)
-        uname = "k_update_fx<64> (incremental)"
+        uname = "k_update_fx<64, true> (incremental)"
         update_note = ("%d of %d iterations streamed all points; the others updated the "
                        "cluster totals from %d moved points in all" %
                        (full, iters, int(r["moved_points"])))
