@@ -3109,6 +3109,114 @@ __device__ __forceinline__ void fx_moved_rows(const UpdateParams& P, unsigned lo
   }
 }
 
+// The moved points only by global atomics (FX_ATOMIC), run by k_update_fx's
+// grid (its waves interleaved over the points 64 at a time, as
+// update_moved_points): a wave lists a round's moved points (256 scanned) in
+// its LDS list and takes them eight at a time, a lane per dimension, with
+// the eight rows and norms requested before any atomic, so a wave with many
+// moved points pays one memory latency per eight instead of one each.
+template <bool M>
+__device__ void fx_atomic_rows(const UpdateParams& P, int2* lists) {
+  __shared__ unsigned moved_s;  // this workgroup's moved points (one global add)
+  if (threadIdx.x == 0) moved_s = 0u;
+  __syncthreads();
+  constexpr int UD = kFxScan;  // groups of 64 points a wave scans per round
+  constexpr int JT = kMaxDP / 64, UB = 8;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+  const long long w0 = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  // the totals this wave adds to: its delta copy, or the totals themselves
+  const int copy = (int)(w0 % kDeltaCopies);
+  long long* const St = P.Sdel ? P.Sdel + (size_t)copy * P.Kp * P.DP : P.Stot;
+  long long* const Qt = P.Sdel ? (P.Qtot ? P.Qdel + (size_t)copy * P.Kp : nullptr) : P.Qtot;
+  int* const Nt = P.Sdel ? P.Ndel + (size_t)copy * P.Kp : P.Ntot;
+  int2* const list = lists + wave * (64 * UD);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const float fs = P.fscale;
+  unsigned mv_lane = 0;
+  for (long long base = w0 * 64; base < P.n; base += nw * 64 * UD) {
+    int labv[UD], oldv[UD];
+#pragma unroll
+    for (int u = 0; u < UD; ++u) {
+      const long long i = base + u * nw * 64 + lane;
+      labv[u] = i < P.n ? P.labels[i] : 0;
+      oldv[u] = i < P.n ? P.prev[i] : 0;
+    }
+    // the round's moved points into the wave's list, ranked by ballot
+    // (labels < 2^16: k is limited by the LDS)
+    unsigned cnt = 0;
+#pragma unroll
+    for (int u = 0; u < UD; ++u) {
+      const long long i = base + u * nw * 64 + lane;
+      const bool mv = i < P.n && labv[u] != oldv[u];
+      const unsigned long long m = __ballot(mv);
+      if (mv) {
+        list[cnt + (unsigned)__popcll(m & below)] =
+            make_int2(u * 64 + lane, (labv[u] << 16) | oldv[u]);
+        P.prev[i] = labv[u];
+      }
+      cnt += (unsigned)__popcll(m);
+    }
+    if (lane == 0) mv_lane += cnt;
+    __builtin_amdgcn_wave_barrier();
+    for (unsigned e0 = 0; e0 < cnt; e0 += UB) {
+      float xv[UB][JT];
+      double xq[UB];
+      int nl[UB], ol[UB];
+#pragma unroll
+      for (int k = 0; k < UB; ++k) {
+        nl[k] = -1;
+        ol[k] = 0;
+        xq[k] = 0.0;
+#pragma unroll
+        for (int t = 0; t < JT; ++t) xv[k][t] = 0.0f;
+        if (e0 + k < cnt) {
+          const int2 en = list[e0 + k];
+          const long long i = base + (en.x >> 6) * nw * 64 + (en.x & 63);
+          nl[k] = en.y >> 16;
+          ol[k] = en.y & 0xffff;
+          const float* xr = P.X + (size_t)i * P.DP;
+#pragma unroll
+          for (int t = 0; t < JT; ++t) {
+            const int j = lane + 64 * t;
+            if (j < P.D) xv[k][t] = xr[j];
+          }
+          if (lane == 0 && Qt) xq[k] = P.xn64[i];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < UB; ++k) {
+        if (nl[k] < 0) continue;
+        unsigned long long* sn = reinterpret_cast<unsigned long long*>(St + (size_t)nl[k] * P.DP);
+        unsigned long long* so = reinterpret_cast<unsigned long long*>(St + (size_t)ol[k] * P.DP);
+#pragma unroll
+        for (int t = 0; t < JT; ++t) {
+          const int j = lane + 64 * t;
+          if (j < P.D) {
+            const unsigned long long q = fx_int<M>(xv[k][t] * fs);
+            atomicAdd(sn + j, q);
+            atomicAdd(so + j, 0ull - q);  // subtraction modulo 2^64
+          }
+        }
+        if (lane == 0) {
+          if (Qt) {
+            const long long qq = __double2ll_rn(xq[k] * P.qscale);
+            atomicAdd(reinterpret_cast<unsigned long long*>(Qt + nl[k]), (unsigned long long)qq);
+            atomicAdd(reinterpret_cast<unsigned long long*>(Qt + ol[k]),
+                      (unsigned long long)(-qq));
+          }
+          atomicAdd(Nt + nl[k], 1);
+          atomicAdd(Nt + ol[k], -1);
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // the list is rewritten by the next round
+  }
+  if (mv_lane) atomicAdd(&moved_s, mv_lane);
+  __syncthreads();
+  if (threadIdx.x == 0 && moved_s) atomicAdd(&P.state->moved_acc, (unsigned long long)moved_s);
+}
+
 template <int DS, bool M>
 __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   // the stop flag and the state are read together (one memory latency); the
@@ -3117,12 +3225,13 @@ __global__ __launch_bounds__(kFxThreads) void k_update_fx(UpdateParams P) {
   const int mode0 = P.Stot ? fx_mode(P.state, P.n) : FX_FULL;
   const int mode = stop ? -1 : mode0;
   if (mode < 0) return;
-  if (mode == FX_ATOMIC) {  // few moved points: global atomics
-    update_moved_points(P);
-    return;
-  }
   constexpr int ST = fx_stride(DS);
   extern __shared__ __attribute__((aligned(16))) unsigned char fsm[];
+  if (mode == FX_ATOMIC) {  // few moved points: global atomics
+    fx_atomic_rows<M>(P, reinterpret_cast<int2*>(fsm + (size_t)P.Kp * ST * 8 +
+                                                 (size_t)P.Kp * 8 + (size_t)P.Kp * 4));
+    return;
+  }
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(fsm);
   unsigned long long* qacc = acc + (size_t)P.Kp * ST;
   unsigned* cacc = reinterpret_cast<unsigned*>(qacc + P.Kp);
