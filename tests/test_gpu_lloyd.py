@@ -276,6 +276,32 @@ def test_fixed_update_from_moved_points_equals_full_passes(kmc_mod, c2_data, pre
         assert any(m > n // 64 for m in moved) and any(0 < m <= n // 64 for m in moved)
 
 
+@pytest.mark.parametrize("n,d,k", [(1001, 20, 7), (4099, 70, 40), (30_003, 130, 300)])
+def test_fixed_update_forms_on_ragged_shapes(kmc_mod, n, d, k):
+    """The three forms of the fixed-point update (full pass, moved points
+    through the LDS accumulators, moved points by global atomics) on shapes
+    with n not a multiple of four (the label copy's tail), narrow and wide
+    dimension slices and k beyond one accumulator block: a 10-iteration run
+    equals ten single-iteration runs (each a full pass) bit for bit."""
+    from kmc import datagen
+    X, _, _ = datagen.blobs(n, d, k, 777 + n, -3.0, 3.0, 0.8, 1.6)
+    C0 = datagen.init_rows(X, k, 5).astype(np.float32)
+    iters = 10
+    with kmc_mod.Engine(n, d, k, tol=0.0, precision="bf16", update="fixed") as e:
+        e.set_points(X)
+        e.set_centroids(C0)
+        r = e.run(iters, check_converged=False)
+        C_run, L_run = e.centroids(), e.labels()
+        e.set_centroids(C0)
+        for _ in range(iters):
+            r1 = e.run(1, check_converged=False)
+        _say("fixed forms", n, d, k, "full passes", r["full_passes"], "moved", r["moved_points"])
+        assert r["full_passes"] == 1
+        assert np.array_equal(e.labels(), L_run)
+        assert np.array_equal(e.centroids(), C_run)
+        assert r1["inertia"] == r["inertia"]
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("update", ["ordered", "tree", "fixed"])
 def test_graph_replays_match_eager(kmc_mod, prec, update):
